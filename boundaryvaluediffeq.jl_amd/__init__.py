@@ -1,0 +1,16 @@
+"""MI355X-native MIRK collocation solver behind the reference's interface.
+
+Host-side mirror of ``solve(prob, MIRK4()/MIRK6(); dt, abstol)`` from
+SciML/BoundaryValueDiffEq.jl; the compute runs in the HIP library
+``_build/libbvp_mirk.so`` through the C ABI of ``include/bvp_mirk.h``.
+"""
+from . import _abi
+from .problems import (BVPFunction, BVProblem, TwoPointBVProblem, DiffEqArray, InstanceSpec,
+                       LinearTanhFunction, PendulumFunction, BratuFunction, SingPertFunction,
+                       ScalarTUFunction, TurningFunction, extract, uniform_mesh, cld, instance_array)
+
+__all__ = [
+    "BVPFunction", "BVProblem", "TwoPointBVProblem", "DiffEqArray", "InstanceSpec",
+    "LinearTanhFunction", "PendulumFunction", "BratuFunction", "SingPertFunction",
+    "ScalarTUFunction", "TurningFunction", "extract", "uniform_mesh", "cld", "instance_array",
+]

@@ -1,0 +1,126 @@
+"""ctypes mirror of ``include/bvp_mirk.h`` (structs and constants).
+
+The constants are checked against the header by ``tests/test_abi.py``.
+"""
+import ctypes as C
+
+MIRK4 = 4
+MIRK6 = 6
+
+FAMILY_LINEAR_TANH = 0
+FAMILY_PENDULUM = 1
+FAMILY_BRATU = 2
+FAMILY_SINGPERT = 3
+FAMILY_SCALAR_TU = 4
+FAMILY_TURNING = 5
+
+BC_TWO_POINT = 0
+BC_STANDARD = 1
+
+RET_SUCCESS = 0
+RET_FAILURE = 1
+RET_MAXITERS = 2
+RET_UNSTABLE = 3
+RETCODE_NAMES = {0: "Success", 1: "Failure", 2: "MaxIters", 3: "Unstable"}
+
+PREC_FP64 = 0
+PREC_FP32_REFINE64 = 1
+
+TRANSFER_REFERENCE = 0
+TRANSFER_CLEAN = 1
+
+WARM_NONE = 0
+WARM_MLP = 1
+
+OK = 0
+EINVAL = -1
+EHIP = -2
+ENOMEM = -3
+ENOTSUP = -4
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Instance(C.Structure):
+    _fields_ = [
+        ("family", C.c_int32),
+        ("M", C.c_int32),
+        ("t0", C.c_double),
+        ("t1", C.c_double),
+        ("params", _dp),
+        ("n_params", C.c_int32),
+        ("bc_kind", C.c_int32),
+        ("n_bc", C.c_int32),
+        ("bc_t", _dp),
+        ("bc_comp", _ip),
+        ("bc_val", _dp),
+        ("N", C.c_int32),
+        ("mesh", _dp),
+        ("guess", _dp),
+        ("u0", _dp),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32),
+        ("abstol", C.c_double),
+        ("newton_abstol", C.c_double),
+        ("adaptive", C.c_int32),
+        ("defect_threshold", C.c_double),
+        ("max_num_subintervals", C.c_int32),
+        ("max_newton_iters", C.c_int32),
+        ("linesearch_fallback", C.c_int32),
+        ("transfer_mode", C.c_int32),
+        ("precision", C.c_int32),
+        ("refine_iters", C.c_int32),
+        ("warm_start", C.c_int32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("retcode", C.c_int32),
+        ("N_final", C.c_int32),
+        ("newton_iters", C.c_int32),
+        ("outer_iters", C.c_int32),
+        ("fallback_used", C.c_int32),
+        ("capacity", C.c_int32),
+        ("defect_norm", C.c_double),
+        ("resid_norm", C.c_double),
+        ("mesh", _dp),
+        ("u", _dp),
+    ]
+
+
+def default_opts(method=MIRK4):
+    """Defaults of the reference (MIRK/mirk.jl:49-53, MIRK/algorithms.jl:55-61,
+    Core/calc_errors.jl:54-60)."""
+    o = Opts()
+    o.method = method
+    o.abstol = 1e-6
+    o.newton_abstol = 0.0
+    o.adaptive = 1
+    o.defect_threshold = 0.1
+    o.max_num_subintervals = 3000
+    o.max_newton_iters = 1000
+    o.linesearch_fallback = 1
+    o.transfer_mode = TRANSFER_REFERENCE
+    o.precision = PREC_FP64
+    o.refine_iters = 0
+    o.warm_start = WARM_NONE
+    return o
+
+
+def dptr(a):
+    """Pointer to a contiguous float64 numpy array (or NULL for None)."""
+    if a is None:
+        return _dp()
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return _ip()
+    return a.ctypes.data_as(_ip)

@@ -1,0 +1,213 @@
+/*
+ * bvp_mirk.h -- C ABI of the MI355X (gfx950) MIRK collocation library.
+ *
+ * This is the drop-in boundary for the MIRK4/MIRK6 collocation solve that
+ * SciML/BoundaryValueDiffEq.jl runs behind
+ *     solve(prob::BVProblem, MIRK4()/MIRK6(); dt, abstol)
+ * Each entry point replaces one reference function (file:line cited on each
+ * declaration; paths are relative to the reference repository root, with
+ *   MIRK/ = lib/BoundaryValueDiffEqMIRK/src/  and
+ *   Core/ = lib/BoundaryValueDiffEqCore/src/ ).
+ *
+ * Conventions
+ *  - Plain C types only: int32_t, double, pointers and sizes.  No pointer is
+ *    retained after a call returns; all host buffers are caller-owned.
+ *  - Return value: 0 on success, a negative BVP_E* code on an API/argument or
+ *    HIP error (message via bvp_last_error).  Numerical outcomes of a solve
+ *    are reported per instance in bvp_result.retcode, as the reference
+ *    reports them in ReturnCode (MIRK/mirk.jl:374-385, MIRK/adaptivity.jl:51-66).
+ *  - State layout: node-major, component-minor ("point-major"): y[k*M + c]
+ *    is component c at mesh node k, k = 0..N.  This is the layout of the
+ *    reference's flattened unknown vector (Core/utils.jl:28-35, 59-79).
+ *  - Batched entry points take n instances.  Per-instance outputs are packed
+ *    back to back in instance order; the size of each slot is stated per
+ *    entry point.
+ *  - A context is not thread-safe: use one per host thread.
+ */
+#ifndef BVP_MIRK_H
+#define BVP_MIRK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- methods (MIRK/algorithms.jl:55-61; tableaus MIRK/mirk_tableaus.jl:62-87, 120-152) */
+#define BVP_MIRK4 4
+#define BVP_MIRK6 6
+
+/* ---- problem families: the right-hand side f(y, p, t) is compiled into the
+ * device code, since a host closure cannot run on the GPU.
+ *   LINEAR_TANH : f = A y + c tanh(y);   params = [c, A(M*M, row-major)]
+ *   PENDULUM    : f = [y2, -g sin(y1)];   params = [g]        (M = 2)
+ *   BRATU       : f = [y2, -lambda e^y1]; params = [lambda]   (M = 2)
+ *   SINGPERT    : f = [y2, (y1 - t y2)/eps]; params = [eps]   (M = 2)
+ *   SCALAR_TU   : f = t y;                params = []         (M = 1)
+ *   TURNING     : f = [y2, -t y2/eps - pi^2 cos(pi t) - pi t sin(pi t)/eps];
+ *                 params = [eps]  (M = 2; the turning-point problem of the
+ *                 reference's adaptivity test, test/misc/adaptivity_tests.jl:7-10) */
+#define BVP_FAMILY_LINEAR_TANH 0
+#define BVP_FAMILY_PENDULUM    1
+#define BVP_FAMILY_BRATU       2
+#define BVP_FAMILY_SINGPERT    3
+#define BVP_FAMILY_SCALAR_TU   4
+#define BVP_FAMILY_TURNING     5
+
+/* ---- boundary-condition kinds
+ *   TWO_POINT: TwoPointBVProblem.  Row k is y(bc_t[k])[bc_comp[k]] - bc_val[k]
+ *              with bc_t[k] equal to t0 (a "bca" row) or t1 (a "bcb" row);
+ *              residual layout [bca; Phi_1..Phi_N; bcb]  (MIRK/mirk.jl:509-521).
+ *   STANDARD : BVProblem with bc!(resid, sol, p, t).  Row k is
+ *              sol(bc_t[k])[bc_comp[k]] - bc_val[k], sol being the continuous
+ *              MIRK extension (MIRK/interpolation.jl:214-242);
+ *              residual layout [bc; Phi_1..Phi_N]  (MIRK/mirk.jl:471-482).
+ *              Its Jacobian rows hold the stages frozen, as the reference's
+ *              loss_bc does (MIRK/mirk.jl:565-572).                         */
+#define BVP_BC_TWO_POINT 0
+#define BVP_BC_STANDARD  1
+
+/* ---- per-instance return codes (SciMLBase.ReturnCode subset) */
+#define BVP_RET_SUCCESS  0
+#define BVP_RET_FAILURE  1
+#define BVP_RET_MAXITERS 2
+#define BVP_RET_UNSTABLE 3   /* non-finite residual */
+
+/* ---- linear-solve precision */
+#define BVP_PREC_FP64          0  /* block LU in FP64 */
+#define BVP_PREC_FP32_REFINE64 1  /* block LU in FP32 + FP64 iterative refinement */
+
+/* ---- mesh-transfer mode after a redistribution (MIRK/mirk.jl:364-372) */
+#define BVP_TRANSFER_REFERENCE 0  /* in place, as the reference writes y0 while reading it */
+#define BVP_TRANSFER_CLEAN     1  /* interpolate from an untouched copy of the old solution */
+
+/* ---- warm start */
+#define BVP_WARM_NONE 0
+#define BVP_WARM_MLP  1
+
+/* ---- API error codes */
+#define BVP_OK        0
+#define BVP_EINVAL   -1
+#define BVP_EHIP     -2
+#define BVP_ENOMEM   -3
+#define BVP_ENOTSUP  -4
+
+typedef struct bvp_ctx bvp_ctx;
+
+/* One problem instance.  Caller-owned, read-only during a call.
+ * Mirrors BVProblem/TwoPointBVProblem (f, bc, u0, tspan, p) plus the mesh
+ * that solve(...; dt) derives (Core/utils.jl:330-388, 694-704). */
+typedef struct {
+    int32_t family;          /* BVP_FAMILY_* */
+    int32_t M;               /* state dimension */
+    double t0, t1;           /* tspan */
+    const double *params;    /* family parameters (see above) */
+    int32_t n_params;
+    int32_t bc_kind;         /* BVP_BC_* */
+    int32_t n_bc;            /* number of BC rows; must equal M */
+    const double *bc_t;      /* [n_bc] evaluation point of each row */
+    const int32_t *bc_comp;  /* [n_bc] component (0-based) */
+    const double *bc_val;    /* [n_bc] target value */
+    int32_t N;               /* number of mesh intervals */
+    const double *mesh;      /* [N+1] nodes, or NULL for range(t0, t1, N+1) */
+    const double *guess;     /* [M*(N+1)] point-major initial guess, or NULL */
+    const double *u0;        /* [M] constant guess used when guess == NULL
+                                (Core/utils.jl:766-769) */
+} bvp_instance;
+
+/* Solver options.  Defaults (bvp_opts_default) follow the reference. */
+typedef struct {
+    int32_t method;               /* BVP_MIRK4 | BVP_MIRK6 */
+    double  abstol;               /* 1e-6            MIRK/mirk.jl:50            */
+    double  newton_abstol;        /* <=0 -> abstol   MIRK/mirk.jl:51            */
+    int32_t adaptive;             /* 1               MIRK/mirk.jl:50            */
+    double  defect_threshold;     /* 0.1             Core/calc_errors.jl:57     */
+    int32_t max_num_subintervals; /* 3000            MIRK/algorithms.jl:60      */
+    int32_t max_newton_iters;     /* 1000            (NonlinearSolve default)   */
+    int32_t linesearch_fallback;  /* 1: NR -> NR+backtracking, as the default
+                                     polyalgorithm Core/default_internal_solve.jl:31-45 */
+    int32_t transfer_mode;        /* BVP_TRANSFER_*                              */
+    int32_t precision;            /* BVP_PREC_*                                  */
+    int32_t refine_iters;         /* FP64 refinement steps per linear solve (FP32 LU) */
+    int32_t warm_start;           /* BVP_WARM_* (needs a loaded MLP)             */
+} bvp_opts;
+
+/* Per-instance result.  mesh/u are caller-allocated with room for
+ * `capacity` nodes (at least max_num_subintervals + 1). */
+typedef struct {
+    int32_t retcode;             /* BVP_RET_* */
+    int32_t N_final;             /* intervals of the returned mesh */
+    int32_t newton_iters;        /* Newton iterations, summed over mesh passes */
+    int32_t outer_iters;         /* mesh passes (MIRK/mirk.jl:299-307) */
+    int32_t fallback_used;       /* bit0: line-search fallback, bit1: warm-start fallback,
+                                    bit2: FP64 fallback after an FP32 refinement stall */
+    int32_t capacity;            /* in: node capacity of mesh and u */
+    double  defect_norm;         /* last defect estimate (MIRK/adaptivity.jl:410) */
+    double  resid_norm;          /* max-norm of the final collocation residual */
+    double *mesh;                /* out: [N_final+1] */
+    double *u;                   /* out: [M*(N_final+1)] point-major */
+} bvp_result;
+
+void bvp_opts_default(bvp_opts *opts);
+
+int  bvp_ctx_create(int device, bvp_ctx **out);
+void bvp_ctx_destroy(bvp_ctx *ctx);
+const char *bvp_last_error(const bvp_ctx *ctx);
+
+/* Full solve of n instances in one persistent launch on the GPU.
+ * Replaces SciMLBase.__solve -> __init + solve! for MIRK
+ * (Core/BoundaryValueDiffEqCore.jl:48-54; MIRK/mirk.jl:49-265, 286-388).
+ * All instances share opts and must share M. */
+int bvp_mirk_solve_batch(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
+                         const bvp_instance *inst, bvp_result *res);
+
+/* Collocation residual at (mesh, guess) of each instance, reference layout
+ * (TWO_POINT: [bca; Phi; bcb], STANDARD: [bc; Phi]).  Slot: M*(N+1) doubles.
+ * Replaces the loss functions MIRK/mirk.jl:471-534 with
+ * Phi! MIRK/collocation.jl:44-102 and the BC residual Core/utils.jl:157-209. */
+int bvp_mirk_residual(bvp_ctx *ctx, int32_t method, int32_t n,
+                      const bvp_instance *inst, double *resid);
+
+/* Jacobian blocks at (mesh, guess): S_i = dPhi_i/dy_i and R_i = dPhi_i/dy_{i+1}
+ * (M*M row-major each; slot N*M*M doubles for S and for R), computed with
+ * in-kernel forward-mode dual numbers; bc_node receives, per BC row, the mesh
+ * node whose component bc_comp[k] carries the unit Jacobian entry (slot n_bc).
+ * Replaces the sparse ForwardDiff Jacobian MIRK/mirk.jl:697-862, 927-1002 and
+ * MIRK/sparse_jacobians.jl:16-36. */
+int bvp_mirk_jacobian_blocks(bvp_ctx *ctx, int32_t method, int32_t n,
+                             const bvp_instance *inst, double *S, double *R,
+                             int32_t *bc_node);
+
+/* One Newton correction: solves J(y) delta = -F(y) at (mesh, guess) with the
+ * partitioned block elimination (slot M*(N+1) doubles, point-major).
+ * `precision` is BVP_PREC_*; refine_iters applies to FP32.
+ * Replaces the LinearSolve call on the AlmostBanded/Banded Jacobian handed
+ * over at Core/internal_problems.jl:107-169. */
+int bvp_mirk_newton_step(bvp_ctx *ctx, int32_t method, int32_t precision,
+                         int32_t refine_iters, int32_t n,
+                         const bvp_instance *inst, double *delta);
+
+/* Defect estimate at (mesh, guess): per-interval defect vectors (slot N*M)
+ * and defect norm (one per instance).  Includes the extra interpolation
+ * stages.  Replaces error_estimate!(::DefectControl) MIRK/adaptivity.jl:370-461
+ * with interp_setup! MIRK/interpolation.jl:300-372. */
+int bvp_mirk_defect(bvp_ctx *ctx, int32_t method, int32_t n,
+                    const bvp_instance *inst, double *errors,
+                    double *defect_norm);
+
+/* Mesh selection and solution transfer at (mesh, guess) after a converged
+ * solve: defect -> mesh_selector! (halving or redistribution) -> interp_eval!
+ * of the old continuous extension onto the new nodes.
+ * Outputs per instance: new_N (1 int32), info (BVP_RET_SUCCESS/FAILURE),
+ * new_mesh (slot `capacity` doubles), new_y (slot M*capacity doubles).
+ * Replaces MIRK/adaptivity.jl:6-75, 250-304 and MIRK/mirk.jl:361-373. */
+int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
+                         const bvp_instance *inst, int32_t capacity,
+                         int32_t *new_N, int32_t *info, double *new_mesh,
+                         double *new_y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BVP_MIRK_H */

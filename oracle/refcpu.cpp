@@ -1,0 +1,1077 @@
+// refcpu.cpp -- CPU restatement of the reference MIRK collocation solve.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: the GPU library is
+// checked against it, and the benchmark times it as the CPU baseline.  The
+// product never links, loads or calls it.
+//
+// It restates, in plain FP64 C++, the algorithm of SciML/BoundaryValueDiffEq.jl
+// on the MIRK path (paths relative to the reference root;
+// MIRK/ = lib/BoundaryValueDiffEqMIRK/src/, Core/ = lib/BoundaryValueDiffEqCore/src/):
+//   tableaus ............ MIRK/mirk_tableaus.jl:62-87, 120-152
+//   interp weights ...... MIRK/interpolation.jl:481-499, 528-575
+//   Phi (residual) ...... MIRK/collocation.jl:44-102
+//   BC residual ......... Core/utils.jl:157-209; MIRK/interpolation.jl:214-242
+//   residual layout ..... MIRK/mirk.jl:471-534; Core/utils.jl:28-52
+//   Jacobian ............ exact chain rule for the blocks the reference gets
+//                         from sparse ForwardDiff (MIRK/mirk.jl:697-862, 927-1002);
+//                         BC rows with frozen stages (MIRK/mirk.jl:565-572)
+//   linear solve ........ banded Gaussian elimination with partial pivoting
+//                         (the reference hands the banded / almost-banded
+//                         Jacobian to LinearSolve, Core/internal_problems.jl:107-169)
+//   Newton .............. NewtonRaphson, then NewtonRaphson + backtracking
+//                         (Core/default_internal_solve.jl:31-45); restated from
+//                         NonlinearSolveFirstOrder "1.2, 2" / NonlinearSolveBase
+//                         "2.11.2": undamped steps, stop when max|F| <= abstol
+//   interp_setup! ....... MIRK/interpolation.jl:300-372
+//   defect .............. MIRK/adaptivity.jl:370-461, 623-666
+//   mesh_selector! ...... MIRK/adaptivity.jl:23-75
+//   redistribute! ....... MIRK/adaptivity.jl:250-278
+//   half_mesh! .......... MIRK/adaptivity.jl:287-304
+//   transfer ............ MIRK/mirk.jl:361-373; MIRK/adaptivity.jl:6-13, 590-621
+//   outer loop .......... MIRK/mirk.jl:286-388
+//   mesh from dt ........ Core/utils.jl:354-362, 694-700
+#include "../include/bvp_mirk.h"
+
+#include <algorithm>
+#include <cfenv>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct Tab {
+    int s, sstar, order;
+    double c[5], v[5], b[5], x[5][5];
+    double cs[4], vs[4], xs[4][9];
+    double tau;
+};
+
+// MIRK/mirk_tableaus.jl:62-87
+Tab make_mirk4() {
+    Tab t{};
+    t.s = 3; t.sstar = 4; t.order = 4;
+    t.c[0] = 0; t.c[1] = 1; t.c[2] = 1.0 / 2;
+    t.v[0] = 0; t.v[1] = 1; t.v[2] = 1.0 / 2;
+    t.b[0] = 1.0 / 6; t.b[1] = 1.0 / 6; t.b[2] = 2.0 / 3;
+    t.x[2][0] = 1.0 / 8; t.x[2][1] = -1.0 / 8;
+    t.cs[0] = 3.0 / 4; t.vs[0] = 27.0 / 32;
+    t.xs[0][0] = 3.0 / 64; t.xs[0][1] = -9.0 / 64;
+    t.tau = 0.226;
+    return t;
+}
+
+// MIRK/mirk_tableaus.jl:120-152
+Tab make_mirk6() {
+    Tab t{};
+    t.s = 5; t.sstar = 9; t.order = 6;
+    const double c[5] = {0, 1, 1.0 / 4, 3.0 / 4, 1.0 / 2};
+    const double v[5] = {0, 1, 5.0 / 32, 27.0 / 32, 1.0 / 2};
+    const double b[5] = {7.0 / 90, 7.0 / 90, 16.0 / 45, 16.0 / 45, 2.0 / 15};
+    for (int i = 0; i < 5; ++i) { t.c[i] = c[i]; t.v[i] = v[i]; t.b[i] = b[i]; }
+    t.x[2][0] = 9.0 / 64;  t.x[2][1] = -3.0 / 64;
+    t.x[3][0] = 3.0 / 64;  t.x[3][1] = -9.0 / 64;
+    t.x[4][0] = -5.0 / 24; t.x[4][1] = 5.0 / 24; t.x[4][2] = 2.0 / 3; t.x[4][3] = -2.0 / 3;
+    const double cs[4] = {7.0 / 16, 3.0 / 8, 9.0 / 16, 1.0 / 8};
+    for (int i = 0; i < 4; ++i) { t.cs[i] = cs[i]; t.vs[i] = cs[i]; }
+    const double xs[4][9] = {
+        {1547.0 / 32768, -1225.0 / 32768, 749.0 / 4096, -287.0 / 2048, -861.0 / 16384, 0, 0, 0, 0},
+        {83.0 / 1536, -13.0 / 384, 283.0 / 1536, -167.0 / 1536, -49.0 / 512, 0, 0, 0, 0},
+        {1225.0 / 32768, -1547.0 / 32768, 287.0 / 2048, -749.0 / 4096, 861.0 / 16384, 0, 0, 0, 0},
+        {233.0 / 3456, -19.0 / 1152, 0, 0, 0, -5.0 / 72, 7.0 / 72, -17.0 / 216, 0}};
+    std::memcpy(t.xs, xs, sizeof(xs));
+    t.tau = 0.7156;
+    return t;
+}
+
+const Tab &tableau(int method) {
+    static const Tab t4 = make_mirk4();
+    static const Tab t6 = make_mirk6();
+    return method == BVP_MIRK6 ? t6 : t4;
+}
+
+// Continuous-extension weights w(tau), w'(tau): MIRK/interpolation.jl:481-499 (MIRK4),
+// :528-575 (MIRK6).  Rational coefficients are rounded to double once, as
+// Rational * Float64 does in the reference.
+void interp_weights(int method, double tau, double *w, double *wp) {
+    if (method == BVP_MIRK4) {
+        const double t2 = tau * tau, tm1 = tau - 1.0, t4m3 = tau * 4.0 - 3.0, t2m1 = tau * 2.0 - 1.0;
+        w[0] = -tau * (2.0 * tau - 3.0) * (2.0 * t2 - 3.0 * tau + 2.0) / 6.0;
+        w[1] = t2 * (12.0 * t2 - 20.0 * tau + 9.0) / 6.0;
+        w[2] = 2.0 * t2 * (6.0 * t2 - 14.0 * tau + 9.0) / 3.0;
+        w[3] = -16.0 * t2 * tm1 * tm1 / 3.0;
+        wp[0] = -tm1 * t4m3 * t2m1 / 3.0;
+        wp[1] = tau * t2m1 * t4m3;
+        wp[2] = 4.0 * tau * t4m3 * tm1;
+        wp[3] = -32.0 * tau * t2m1 * tm1 / 3.0;
+        return;
+    }
+    const double t = tau, t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t, t6 = t5 * t;
+    w[0] = t - 28607.0 / 7434 * t2 - 166210.0 / 33453 * t3 + 334780.0 / 11151 * t4 -
+           1911296.0 / 55755 * t5 + 406528.0 / 33453 * t6;
+    w[1] = 777.0 / 590 * t2 - 2534158.0 / 234171 * t3 + 2088580.0 / 78057 * t4 -
+           10479104.0 / 390285 * t5 + 11328512.0 / 1170855 * t6;
+    w[2] = -1008.0 / 59 * t2 + 222176.0 / 1593 * t3 - 180032.0 / 531 * t4 +
+           876544.0 / 2655 * t5 - 180224.0 / 1593 * t6;
+    w[3] = w[2];  // the reference lists the same polynomial for stages 3 and 4
+    w[4] = -378.0 / 59 * t2 + 27772.0 / 531 * t3 - 22504.0 / 177 * t4 +
+           109568.0 / 885 * t5 - 22528.0 / 531 * t6;
+    w[5] = -95232.0 / 413 * t2 + 62384128.0 / 33453 * t3 - 49429504.0 / 11151 * t4 +
+           46759936.0 / 11151 * t5 - 46661632.0 / 33453 * t6;
+    w[6] = 896.0 / 5 * t2 - 4352.0 / 3 * t3 + 3456.0 * t4 - 16384.0 / 5 * t5 + 16384.0 / 15 * t6;
+    w[7] = 50176.0 / 531 * t2 - 179554304.0 / 234171 * t3 + 143363072.0 / 78057 * t4 -
+           136675328.0 / 78057 * t5 + 137363456.0 / 234171 * t6;
+    w[8] = 16384.0 / 441 * t3 - 16384.0 / 147 * t4 + 16384.0 / 147 * t5 - 16384.0 / 441 * t6;
+    wp[0] = 1 - 28607.0 / 3717 * t - 166210.0 / 11151 * t2 + 1339120.0 / 11151 * t3 -
+            1911296.0 / 11151 * t4 + 813056.0 / 11151 * t5;
+    wp[1] = 777.0 / 295 * t - 2534158.0 / 78057 * t2 + 8354320.0 / 78057 * t3 -
+            10479104.0 / 78057 * t4 + 22657024.0 / 390285 * t5;
+    wp[2] = -2016.0 / 59 * t + 222176.0 / 531 * t2 - 720128.0 / 531 * t3 +
+            876544.0 / 531 * t4 - 360448.0 / 531 * t5;
+    wp[3] = wp[2];
+    wp[4] = -756.0 / 59 * t + 27772.0 / 177 * t2 - 90016.0 / 177 * t3 +
+            109568.0 / 177 * t4 - 45056.0 / 177 * t5;
+    wp[5] = -190464.0 / 413 * t + 62384128.0 / 11151 * t2 - 197718016.0 / 11151 * t3 +
+            233799680.0 / 11151 * t4 - 93323264.0 / 11151 * t5;
+    wp[6] = 1792.0 / 5 * t - 4352.0 * t2 + 13824.0 * t3 - 16384.0 * t4 + 32768.0 / 5 * t5;
+    wp[7] = 100352.0 / 531 * t - 179554304.0 / 78057 * t2 + 573452288.0 / 78057 * t3 -
+            683376640.0 / 78057 * t4 + 274726912.0 / 78057 * t5;
+    wp[8] = 16384.0 / 147 * t2 - 65536.0 / 147 * t3 + 81920.0 / 147 * t4 - 32768.0 / 147 * t5;
+}
+
+// ---------------------------------------------------------------- families
+void f_eval(const bvp_instance &P, double t, const double *y, double *out) {
+    const double *p = P.params;
+    switch (P.family) {
+    case BVP_FAMILY_LINEAR_TANH: {
+        const int M = P.M;
+        const double c = p[0];
+        const double *A = p + 1;
+        for (int i = 0; i < M; ++i) {
+            double acc = 0;
+            for (int j = 0; j < M; ++j) acc += A[i * M + j] * y[j];
+            out[i] = acc + c * std::tanh(y[i]);
+        }
+        break;
+    }
+    case BVP_FAMILY_PENDULUM:
+        out[0] = y[1];
+        out[1] = -p[0] * std::sin(y[0]);
+        break;
+    case BVP_FAMILY_BRATU:
+        out[0] = y[1];
+        out[1] = -p[0] * std::exp(y[0]);
+        break;
+    case BVP_FAMILY_SINGPERT:
+        out[0] = y[1];
+        out[1] = (y[0] - t * y[1]) / p[0];
+        break;
+    case BVP_FAMILY_SCALAR_TU:
+        out[0] = t * y[0];
+        break;
+    case BVP_FAMILY_TURNING: {
+        const double pi = 3.14159265358979323846;
+        out[0] = y[1];
+        out[1] = -t / p[0] * y[1] - pi * pi * std::cos(pi * t) - pi * t / p[0] * std::sin(pi * t);
+        break;
+    }
+    }
+}
+
+// df/dy, M*M row-major (analytic, family by family)
+void J_eval(const bvp_instance &P, double t, const double *y, double *J) {
+    const double *p = P.params;
+    const int M = P.M;
+    switch (P.family) {
+    case BVP_FAMILY_LINEAR_TANH: {
+        const double c = p[0];
+        for (int i = 0; i < M * M; ++i) J[i] = p[1 + i];
+        for (int i = 0; i < M; ++i) {
+            const double th = std::tanh(y[i]);
+            J[i * M + i] += c * (1.0 - th * th);
+        }
+        break;
+    }
+    case BVP_FAMILY_PENDULUM:
+        J[0] = 0; J[1] = 1; J[2] = -p[0] * std::cos(y[0]); J[3] = 0;
+        break;
+    case BVP_FAMILY_BRATU:
+        J[0] = 0; J[1] = 1; J[2] = -p[0] * std::exp(y[0]); J[3] = 0;
+        break;
+    case BVP_FAMILY_SINGPERT:
+        J[0] = 0; J[1] = 1; J[2] = 1.0 / p[0]; J[3] = -t / p[0];
+        break;
+    case BVP_FAMILY_SCALAR_TU:
+        J[0] = t;
+        break;
+    case BVP_FAMILY_TURNING:
+        J[0] = 0; J[1] = 1; J[2] = 0; J[3] = -t / p[0];
+        break;
+    }
+}
+
+bool family_ok(const bvp_instance &P) {
+    switch (P.family) {
+    case BVP_FAMILY_LINEAR_TANH: return P.M >= 1 && P.n_params >= 1 + P.M * P.M;
+    case BVP_FAMILY_PENDULUM:
+    case BVP_FAMILY_BRATU:
+    case BVP_FAMILY_SINGPERT:
+    case BVP_FAMILY_TURNING: return P.M == 2 && P.n_params >= 1;
+    case BVP_FAMILY_SCALAR_TU: return P.M == 1;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- mesh helpers
+// range(t0; stop=t1, length=N+1) (Core/utils.jl:694-700).  The node is the
+// correctly rounded value of (t0*(N-k) + t1*k)/N, computed in long double.
+void uniform_mesh(double t0, double t1, int N, double *mesh) {
+    for (int k = 0; k <= N; ++k) {
+        long double v = ((long double)t0 * (N - k) + (long double)t1 * k) / (long double)N;
+        mesh[k] = (double)v;
+    }
+    mesh[0] = t0;
+    mesh[N] = t1;
+}
+
+// interval(mesh, t) = clamp(searchsortedfirst(mesh, t) - 1, 1, N)  (Core/utils.jl:119-121),
+// returned 0-based.
+int interval_of(const double *mesh, int N, double t) {
+    const double *it = std::lower_bound(mesh, mesh + N + 1, t);
+    int j1 = (int)(it - mesh) + 1;        // 1-based searchsortedfirst
+    int iv = std::min(std::max(j1 - 1, 1), N);
+    return iv - 1;
+}
+
+// ---------------------------------------------------------------- workspace
+struct Ctx {
+    const bvp_instance &P;
+    const Tab &T;
+    int method;
+    int M;
+    Ctx(const bvp_instance &p, int m) : P(p), T(tableau(m)), method(m), M(p.M) {}
+};
+
+// Stages of one interval: Y_r = (1-v_r) y_i + v_r y_{i+1} + h sum_{j<r} x_rj K_j,
+// K_r = f(Y_r, t_i + c_r h)   (MIRK/collocation.jl:60-66)
+void interval_stages(const Ctx &C, double ti, double h, const double *yi, const double *yj,
+                     double *K, double *Y) {
+    const int M = C.M, s = C.T.s;
+    std::vector<double> acc(M);
+    for (int r = 0; r < s; ++r) {
+        double *Yr = Y + r * M;
+        for (int c = 0; c < M; ++c) Yr[c] = (1 - C.T.v[r]) * yi[c] + C.T.v[r] * yj[c];
+        if (r > 0) {
+            for (int c = 0; c < M; ++c) {
+                double a = 0;
+                for (int j = 0; j < r; ++j) a += K[j * M + c] * C.T.x[r][j];
+                Yr[c] += h * a;
+            }
+        }
+        f_eval(C.P, ti + C.T.c[r] * h, Yr, K + r * M);
+    }
+}
+
+// Phi_i = y_{i+1} - y_i - h sum_r b_r K_r   (MIRK/collocation.jl:69-70)
+void interval_phi(const Ctx &C, double h, const double *yi, const double *yj, const double *K,
+                  double *phi) {
+    const int M = C.M, s = C.T.s;
+    for (int c = 0; c < M; ++c) {
+        double a = 0;
+        for (int r = 0; r < s; ++r) a += K[r * M + c] * C.T.b[r];
+        phi[c] = (yj[c] - yi[c]) - h * a;
+    }
+}
+
+// Extra interpolation stages k*_r of one interval (MIRK/interpolation.jl:337-372)
+void interval_interp_stages(const Ctx &C, double ti, double h, const double *yi, const double *yj,
+                            const double *K, double *KI) {
+    const int M = C.M, s = C.T.s, ns = C.T.sstar - C.T.s;
+    std::vector<double> Y(M);
+    for (int r = 0; r < ns; ++r) {
+        for (int c = 0; c < M; ++c) {
+            double a = 0;
+            for (int j = 0; j < s; ++j) a += K[j * M + c] * C.T.xs[r][j];
+            for (int j = 0; j < r; ++j) a += KI[j * M + c] * C.T.xs[r][s + j];
+            Y[c] = a * h + (1 - C.T.vs[r]) * yi[c] + C.T.vs[r] * yj[c];
+        }
+        f_eval(C.P, ti + C.T.cs[r] * h, Y.data(), KI + r * M);
+    }
+}
+
+// z = y_i + h (K w[0:s] + KI w[s:]),  z' = K w' + KI w'   (MIRK/adaptivity.jl:623-666)
+void sum_stages(const Ctx &C, double h, const double *yi, const double *K, const double *KI,
+                const double *w, const double *wp, double *z, double *zp) {
+    const int M = C.M, s = C.T.s, ns = C.T.sstar - C.T.s;
+    for (int c = 0; c < M; ++c) {
+        double a = 0, ap = 0;
+        for (int r = 0; r < s; ++r) { a += K[r * M + c] * w[r]; if (zp) ap += K[r * M + c] * wp[r]; }
+        for (int r = 0; r < ns; ++r) { a += KI[r * M + c] * w[s + r]; if (zp) ap += KI[r * M + c] * wp[s + r]; }
+        z[c] = a * h + yi[c];
+        if (zp) zp[c] = ap;
+    }
+}
+
+// Exact Jacobian blocks of Phi_i by the chain rule through the stage recursion.
+//   dY_r/dy_i = (1-v_r) I + h sum_{j<r} x_rj dK_j/dy_i,  dK_r/dy_i = J_r dY_r/dy_i
+//   S = -I - h sum_r b_r dK_r/dy_i,  R = I - h sum_r b_r dK_r/dy_{i+1}
+void interval_jac(const Ctx &C, double ti, double h, const double *Y, double *S, double *R) {
+    const int M = C.M, s = C.T.s, MM = M * M;
+    std::vector<double> dKa(s * MM), dKb(s * MM), dYa(MM), dYb(MM), Jr(MM);
+    for (int r = 0; r < s; ++r) {
+        for (int i = 0; i < M; ++i)
+            for (int j = 0; j < M; ++j) {
+                double ea = (i == j) ? (1 - C.T.v[r]) : 0.0, eb = (i == j) ? C.T.v[r] : 0.0;
+                double aa = 0, ab = 0;
+                for (int q = 0; q < r; ++q) {
+                    aa += C.T.x[r][q] * dKa[q * MM + i * M + j];
+                    ab += C.T.x[r][q] * dKb[q * MM + i * M + j];
+                }
+                dYa[i * M + j] = ea + h * aa;
+                dYb[i * M + j] = eb + h * ab;
+            }
+        J_eval(C.P, ti + C.T.c[r] * h, Y + r * M, Jr.data());
+        for (int i = 0; i < M; ++i)
+            for (int j = 0; j < M; ++j) {
+                double aa = 0, ab = 0;
+                for (int q = 0; q < M; ++q) {
+                    aa += Jr[i * M + q] * dYa[q * M + j];
+                    ab += Jr[i * M + q] * dYb[q * M + j];
+                }
+                dKa[r * MM + i * M + j] = aa;
+                dKb[r * MM + i * M + j] = ab;
+            }
+    }
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) {
+            double aa = 0, ab = 0;
+            for (int r = 0; r < s; ++r) {
+                aa += C.T.b[r] * dKa[r * MM + i * M + j];
+                ab += C.T.b[r] * dKb[r * MM + i * M + j];
+            }
+            S[i * M + j] = (i == j ? -1.0 : 0.0) - h * aa;
+            R[i * M + j] = (i == j ? 1.0 : 0.0) - h * ab;
+        }
+}
+
+// ---------------------------------------------------------------- BC rows
+struct BcRow {
+    int node;      // mesh node carrying the unit Jacobian entry
+    int comp;
+    bool side_b;   // TWO_POINT: row belongs to bcb (after Phi)
+    bool direct;   // value read straight from a node (no interpolation)
+    int iv;        // interval for the continuous extension (STANDARD, !direct)
+    double tau;
+};
+
+// Resolve each BC row on the current mesh.  TWO_POINT: Core/utils.jl:157-209.
+// STANDARD: EvalSol shortcut at the ends, else interval + tau
+// (MIRK/interpolation.jl:214-226).
+int resolve_bc(const bvp_instance &P, int N, const double *mesh, std::vector<BcRow> &rows) {
+    rows.clear();
+    for (int k = 0; k < P.n_bc; ++k) {
+        BcRow r{};
+        r.comp = P.bc_comp[k];
+        if (r.comp < 0 || r.comp >= P.M) return BVP_EINVAL;
+        const double t = P.bc_t[k];
+        if (P.bc_kind == BVP_BC_TWO_POINT) {
+            if (t == P.t0) { r.node = 0; r.side_b = false; }
+            else if (t == P.t1) { r.node = N; r.side_b = true; }
+            else return BVP_EINVAL;
+            r.direct = true;
+        } else {
+            if (t == mesh[0]) { r.node = 0; r.direct = true; }
+            else if (t == mesh[N]) { r.node = N; r.direct = true; }
+            else {
+                r.iv = interval_of(mesh, N, t);
+                r.tau = (t - mesh[r.iv]) / (mesh[r.iv + 1] - mesh[r.iv]);
+                r.node = r.iv;
+                r.direct = false;
+            }
+        }
+        rows.push_back(r);
+    }
+    return 0;
+}
+
+// Residual of one instance in the reference layout.  Also returns the stages.
+struct Eval {
+    std::vector<double> K, Y, KI, phi;    // per interval
+    std::vector<double> bc;               // per BC row, in input order
+};
+
+void evaluate(const Ctx &C, int N, const double *mesh, const double *y,
+              const std::vector<BcRow> &rows, Eval &E, bool need_interp_all) {
+    const int M = C.M, s = C.T.s, ns = C.T.sstar - C.T.s;
+    E.K.assign((size_t)N * s * M, 0.0);
+    E.Y.assign((size_t)N * s * M, 0.0);
+    E.phi.assign((size_t)N * M, 0.0);
+    E.KI.assign((size_t)N * ns * M, 0.0);
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        interval_stages(C, mesh[i], h, y + i * M, y + (i + 1) * M, &E.K[(size_t)i * s * M],
+                        &E.Y[(size_t)i * s * M]);
+        interval_phi(C, h, y + i * M, y + (i + 1) * M, &E.K[(size_t)i * s * M], &E.phi[(size_t)i * M]);
+    }
+    if (need_interp_all)
+        for (int i = 0; i < N; ++i) {
+            const double h = mesh[i + 1] - mesh[i];
+            interval_interp_stages(C, mesh[i], h, y + i * M, y + (i + 1) * M,
+                                   &E.K[(size_t)i * s * M], &E.KI[(size_t)i * ns * M]);
+        }
+    E.bc.assign(rows.size(), 0.0);
+    std::vector<double> w(9), wp(9), z(M);
+    for (size_t k = 0; k < rows.size(); ++k) {
+        const BcRow &r = rows[k];
+        double val;
+        if (r.direct) {
+            val = y[r.node * M + r.comp];
+        } else {
+            const int i = r.iv;
+            const double h = mesh[i + 1] - mesh[i];
+            if (!need_interp_all)
+                interval_interp_stages(C, mesh[i], h, y + i * M, y + (i + 1) * M,
+                                       &E.K[(size_t)i * s * M], &E.KI[(size_t)i * ns * M]);
+            interp_weights(C.method, r.tau, w.data(), wp.data());
+            sum_stages(C, h, y + i * M, &E.K[(size_t)i * s * M], &E.KI[(size_t)i * ns * M],
+                       w.data(), nullptr, z.data(), nullptr);
+            val = z[r.comp];
+        }
+        E.bc[k] = val - C.P.bc_val[k];
+    }
+}
+
+// Flatten to the reference residual layout.
+void flatten_residual(const Ctx &C, int N, const std::vector<BcRow> &rows, const Eval &E,
+                      double *out) {
+    const int M = C.M;
+    size_t o = 0;
+    if (C.P.bc_kind == BVP_BC_TWO_POINT) {
+        for (size_t k = 0; k < rows.size(); ++k) if (!rows[k].side_b) out[o++] = E.bc[k];
+        for (size_t i = 0; i < (size_t)N * M; ++i) out[o++] = E.phi[i];
+        for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) out[o++] = E.bc[k];
+    } else {
+        for (size_t k = 0; k < rows.size(); ++k) out[o++] = E.bc[k];
+        for (size_t i = 0; i < (size_t)N * M; ++i) out[o++] = E.phi[i];
+    }
+}
+
+double max_abs_residual(const Eval &E) {
+    double m = 0;
+    for (double v : E.phi) { if (std::isnan(v)) return NAN; m = std::max(m, std::fabs(v)); }
+    for (double v : E.bc) { if (std::isnan(v)) return NAN; m = std::max(m, std::fabs(v)); }
+    return m;
+}
+
+// ---------------------------------------------------------------- banded LU
+// Gaussian elimination with partial pivoting on a band matrix (kl sub-, ku
+// super-diagonals), LAPACK-style column storage with kl extra rows for fill.
+struct Band {
+    int n, kl, ku, ld;
+    std::vector<double> ab;
+    std::vector<int> piv;
+    void init(int n_, int kl_, int ku_) {
+        n = n_; kl = kl_; ku = ku_; ld = 2 * kl + ku + 1;
+        ab.assign((size_t)ld * n, 0.0);
+        piv.assign(n, 0);
+    }
+    double &at(int i, int j) { return ab[(size_t)j * ld + (kl + ku + i - j)]; }
+    // returns 0, or 1 if a zero pivot was met
+    int factor() {
+        int sing = 0;
+        int ju = 0;
+        for (int j = 0; j < n; ++j) {
+            const int km = std::min(kl, n - 1 - j);
+            int p = j;
+            double best = std::fabs(at(j, j));
+            for (int i = 1; i <= km; ++i) {
+                double v = std::fabs(at(j + i, j));
+                if (v > best) { best = v; p = j + i; }
+            }
+            piv[j] = p;
+            if (at(p, j) == 0.0) { sing = 1; continue; }
+            ju = std::max(ju, std::min(p + ku, n - 1));
+            if (p != j)
+                for (int c = j; c <= ju; ++c) std::swap(at(j, c), at(p, c));
+            const double inv = 1.0 / at(j, j);
+            for (int i = 1; i <= km; ++i) at(j + i, j) *= inv;
+            for (int c = j + 1; c <= ju; ++c) {
+                const double a = at(j, c);
+                if (a != 0.0)
+                    for (int i = 1; i <= km; ++i) at(j + i, c) -= at(j + i, j) * a;
+            }
+        }
+        return sing;
+    }
+    void solve(double *b) {
+        for (int j = 0; j < n; ++j) {
+            const int km = std::min(kl, n - 1 - j);
+            if (piv[j] != j) std::swap(b[j], b[piv[j]]);
+            for (int i = 1; i <= km; ++i) b[j + i] -= at(j + i, j) * b[j];
+        }
+        const int kv = kl + ku;
+        for (int j = n - 1; j >= 0; --j) {
+            b[j] /= at(j, j);
+            const int lo = std::max(0, j - kv);
+            for (int i = lo; i < j; ++i) b[i] -= at(i, j) * b[j];
+        }
+    }
+};
+
+// Assemble the Newton matrix J (rows ordered node by node so that it is banded:
+// at node k, first the BC rows whose Jacobian entry sits on node k, then the
+// rows of Phi_k) and solve J delta = -F.  Row order does not change delta.
+int newton_solve(const Ctx &C, int N, const double *mesh, const double *y,
+                 const std::vector<BcRow> &rows, const Eval &E, double *delta, Band &B) {
+    const int M = C.M, MM = M * M, n = M * (N + 1);
+    // row descriptors
+    struct RowRef { int kind; int idx; };  // kind 0: bc row idx, 1: phi interval idx (M rows)
+    std::vector<int> order_bc(rows.size());
+    for (size_t k = 0; k < rows.size(); ++k) order_bc[k] = (int)k;
+    std::stable_sort(order_bc.begin(), order_bc.end(),
+                     [&](int a, int b) { return rows[a].node < rows[b].node; });
+    // band widths from the structure
+    int kl = 0, ku = 0;
+    {
+        int r = 0;
+        size_t q = 0;
+        for (int k = 0; k <= N; ++k) {
+            while (q < order_bc.size() && rows[order_bc[q]].node == k) {
+                int col = k * M + rows[order_bc[q]].comp;
+                kl = std::max(kl, r - col); ku = std::max(ku, col - r);
+                ++r; ++q;
+            }
+            if (k < N)
+                for (int c = 0; c < M; ++c) {
+                    kl = std::max(kl, r - k * M); ku = std::max(ku, (k + 2) * M - 1 - r);
+                    ++r;
+                }
+        }
+    }
+    B.init(n, kl, ku);
+    std::vector<double> rhs(n);
+    std::vector<double> S(MM), R(MM);
+    int r = 0;
+    size_t q = 0;
+    for (int k = 0; k <= N; ++k) {
+        while (q < order_bc.size() && rows[order_bc[q]].node == k) {
+            const int idx = order_bc[q];
+            B.at(r, k * M + rows[idx].comp) = 1.0;
+            rhs[r] = -E.bc[idx];
+            ++r; ++q;
+        }
+        if (k < N) {
+            const double h = mesh[k + 1] - mesh[k];
+            interval_jac(C, mesh[k], h, &E.Y[(size_t)k * C.T.s * M], S.data(), R.data());
+            for (int i = 0; i < M; ++i) {
+                for (int j = 0; j < M; ++j) {
+                    B.at(r, k * M + j) = S[i * M + j];
+                    B.at(r, (k + 1) * M + j) = R[i * M + j];
+                }
+                rhs[r] = -E.phi[(size_t)k * M + i];
+                ++r;
+            }
+        }
+    }
+    if (B.factor()) return 1;
+    B.solve(rhs.data());
+    for (int i = 0; i < n; ++i) delta[i] = rhs[i];
+    return 0;
+}
+
+// ---------------------------------------------------------------- Newton
+struct NlOut { int ret; int iters; double fnorm; int fallback; };
+
+// NewtonRaphson (undamped) followed, on failure, by NewtonRaphson with a
+// backtracking line search from the same start (Core/default_internal_solve.jl:37-43).
+// Termination: max|F| <= tol (checked before every step, so maxiters = 0
+// returns the guess untouched, MIRKt/Core/mirk_basic_tests.jl:678-720).
+NlOut newton(const Ctx &C, int N, const double *mesh, std::vector<double> &y,
+             const std::vector<BcRow> &rows, double tol, int maxiters, bool fallback) {
+    const int n = C.M * (N + 1);
+    std::vector<double> y0 = y, delta(n), yt(n);
+    Eval E, Et;
+    Band B;
+    NlOut out{BVP_RET_MAXITERS, 0, 0, 0};
+    const bool interp_all = false;
+    for (int attempt = 0; attempt < (fallback ? 2 : 1); ++attempt) {
+        if (attempt == 1) y = y0;
+        int ret = BVP_RET_MAXITERS;
+        double fn = 0;
+        for (int it = 0;; ++it) {
+            evaluate(C, N, mesh, y.data(), rows, E, interp_all);
+            fn = max_abs_residual(E);
+            if (!std::isfinite(fn)) { ret = BVP_RET_UNSTABLE; break; }
+            if (fn <= tol) { ret = BVP_RET_SUCCESS; break; }
+            if (it >= maxiters) { ret = BVP_RET_MAXITERS; break; }
+            if (newton_solve(C, N, mesh, y.data(), rows, E, delta.data(), B)) { ret = BVP_RET_FAILURE; break; }
+            out.iters++;
+            if (attempt == 0) {
+                for (int i = 0; i < n; ++i) y[i] += delta[i];
+            } else {
+                // Armijo backtracking on 0.5*||F||_2^2 with step halving
+                double phi0 = 0;
+                for (double v : E.phi) phi0 += v * v;
+                for (double v : E.bc) phi0 += v * v;
+                double alpha = 1.0;
+                for (int ls = 0; ls < 12; ++ls) {
+                    for (int i = 0; i < n; ++i) yt[i] = y[i] + alpha * delta[i];
+                    evaluate(C, N, mesh, yt.data(), rows, Et, interp_all);
+                    double phi = 0;
+                    for (double v : Et.phi) phi += v * v;
+                    for (double v : Et.bc) phi += v * v;
+                    if (std::isfinite(phi) && phi <= (1.0 - 2e-4 * alpha) * phi0) break;
+                    alpha *= 0.5;
+                }
+                for (int i = 0; i < n; ++i) y[i] += alpha * delta[i];
+            }
+        }
+        out.ret = ret;
+        out.fnorm = fn;
+        out.fallback = attempt;
+        if (ret == BVP_RET_SUCCESS) break;
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- defect
+// error_estimate!(::DefectControl)  MIRK/adaptivity.jl:416-461
+double defect(const Ctx &C, int N, const double *mesh, const double *y, Eval &E,
+              std::vector<double> &errors) {
+    const int M = C.M, s = C.T.s, ns = C.T.sstar - C.T.s;
+    std::vector<double> w1(9), w1p(9), w2(9), w2p(9), z(M), zp(M), f1(M), f2(M), d1(M), d2(M);
+    interp_weights(C.method, C.T.tau, w1.data(), w1p.data());
+    interp_weights(C.method, 1.0 - C.T.tau, w2.data(), w2p.data());
+    errors.assign((size_t)N * M, 0.0);
+    double dn = 0;
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        interval_interp_stages(C, mesh[i], h, y + i * M, y + (i + 1) * M, &E.K[(size_t)i * s * M],
+                               &E.KI[(size_t)i * ns * M]);
+    }
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        const double *K = &E.K[(size_t)i * s * M], *KI = &E.KI[(size_t)i * ns * M];
+        sum_stages(C, h, y + i * M, K, KI, w1.data(), w1p.data(), z.data(), zp.data());
+        f_eval(C.P, mesh[i] + C.T.tau * h, z.data(), f1.data());
+        double e1 = 0;
+        for (int c = 0; c < M; ++c) { d1[c] = (zp[c] - f1[c]) / (std::fabs(f1[c]) + 1.0); e1 = std::max(e1, std::fabs(d1[c])); }
+        sum_stages(C, h, y + i * M, K, KI, w2.data(), w2p.data(), z.data(), zp.data());
+        f_eval(C.P, mesh[i] + (1.0 - C.T.tau) * h, z.data(), f2.data());
+        double e2 = 0;
+        for (int c = 0; c < M; ++c) { d2[c] = (zp[c] - f2[c]) / (std::fabs(f2[c]) + 1.0); e2 = std::max(e2, std::fabs(d2[c])); }
+        const double *d = e1 > e2 ? d1.data() : d2.data();
+        double ei = 0;
+        for (int c = 0; c < M; ++c) { errors[(size_t)i * M + c] = d[c]; ei = std::max(ei, std::fabs(d[c])); }
+        if (!(ei <= dn)) dn = ei;   // NaN propagates
+    }
+    return dn;
+}
+
+// ---------------------------------------------------------------- mesh selection
+// half_mesh!  MIRK/adaptivity.jl:287-304
+void half_mesh(const std::vector<double> &mesh, std::vector<double> &out) {
+    const int n = (int)mesh.size() - 1;
+    std::vector<double> dt(n);
+    for (int i = 0; i < n; ++i) dt[i] = mesh[i + 1] - mesh[i];
+    out.assign(2 * n + 1, 0.0);
+    out[2 * n] = mesh[n];
+    for (int i = 0; i < n; ++i) out[2 * i] = mesh[i];
+    for (int i = 0; i < n; ++i) out[2 * i + 1] = (out[2 * i + 2] + out[2 * i]) / 2.0;
+}
+
+// Julia round(Int, x): ties to even
+long long round_even(double x) {
+    return (long long)std::nearbyint(x);  // default FE_TONEAREST
+}
+
+// mesh_selector!(::DefectControl) + redistribute!  MIRK/adaptivity.jl:23-75, 250-278.
+// Returns BVP_RET_SUCCESS/FAILURE; fills new_mesh.
+int mesh_select(int order, double abstol, int max_sub, const std::vector<double> &mesh,
+                const std::vector<double> &errors, int M, std::vector<double> &new_mesh) {
+    const int Nn = (int)mesh.size();  // length(mesh)
+    const int n = Nn - 1;
+    std::vector<double> dt(n), shat(n);
+    for (int i = 0; i < n; ++i) dt[i] = mesh[i + 1] - mesh[i];
+    for (int i = 0; i < n; ++i) {
+        double m = 0;
+        for (int c = 0; c < M; ++c) m = std::max(m, std::fabs(errors[(size_t)i * M + c]));
+        shat[i] = std::pow(m / abstol, 1.0 / (order + 1));
+    }
+    double r1 = shat[0], r2 = 0;
+    for (int i = 0; i < n; ++i) { r1 = std::max(r1, shat[i]); r2 += shat[i]; }
+    const double r3 = r2 / (Nn - 1);
+    long long n_predict = round_even(1.3 * r2 + 1);
+    const double n_ = 0.1 * n;
+    if (std::fabs((double)(n_predict - n)) < n_) n_predict = round_even(n + n_);
+    if (r1 <= 1.0 * r3) {
+        const long long Nsub = 2LL * n;
+        if (Nsub > max_sub) return BVP_RET_FAILURE;
+        half_mesh(mesh, new_mesh);
+        return BVP_RET_SUCCESS;
+    }
+    const long long lb = Nn / 2, ub = 4LL * (Nn - 1);
+    const long long Nsub = std::min(std::max(n_predict, lb), ub);
+    if (Nsub > max_sub) return BVP_RET_FAILURE;
+    for (int i = 0; i < n; ++i) shat[i] /= dt[i];
+    // redistribute!
+    double zsum = 0;
+    for (int i = 0; i < n; ++i) zsum += shat[i] * dt[i];
+    const double zeta = zsum / (double)Nsub;
+    new_mesh.assign((size_t)Nsub + 1, 0.0);
+    new_mesh[0] = mesh[0];
+    double t = mesh[0], integral = 0;
+    int k = 0;            // 0-based old interval
+    long long i = 0;      // nodes placed
+    while (k < n) {
+        const double next_piece = shat[k] * (mesh[k + 1] - t);
+        const double int_next = integral + next_piece;
+        if (int_next > zeta) {
+            const double tn = (zeta - integral) / shat[k] + t;
+            if (i + 1 <= Nsub) new_mesh[(size_t)(i + 1)] = tn;
+            t = tn;
+            ++i;
+            integral = 0;
+        } else {
+            integral = int_next;
+            t = mesh[k + 1];
+            ++k;
+        }
+    }
+    new_mesh[(size_t)Nsub] = mesh[n];
+    return BVP_RET_SUCCESS;
+}
+
+// Solution transfer: y_new(node) = continuous extension of the old solve.
+// transfer_mode REFERENCE reproduces the in-place update of the reference
+// (MIRK/mirk.jl:366-370 with sum_stages! MIRK/adaptivity.jl:606-621): the base
+// value y0.u[ii] is read from the array being written, so for ii < j it is the
+// already-updated entry of the new mesh.
+void transfer(const Ctx &C, int mode, const std::vector<double> &old_mesh, const std::vector<double> &yold,
+              const Eval &E, const std::vector<double> &new_mesh, std::vector<double> &ynew) {
+    const int M = C.M, s = C.T.s, ns = C.T.sstar - C.T.s;
+    const int Nold = (int)old_mesh.size() - 1;
+    const int Nnew = (int)new_mesh.size() - 1;
+    std::vector<double> work = yold;  // y0.u storage
+    work.resize((size_t)M * std::max(Nold + 1, Nnew + 1), 0.0);
+    std::vector<double> w(9), wp(9), z(M);
+    ynew.assign((size_t)M * (Nnew + 1), 0.0);
+    for (int j = 0; j <= Nnew; ++j) {
+        const double m = new_mesh[j];
+        const int ii = interval_of(old_mesh.data(), Nold, m);
+        const double dt = old_mesh[ii + 1] - old_mesh[ii];
+        const double tau = (m - old_mesh[ii]) / dt;
+        interp_weights(C.method, tau, w.data(), wp.data());
+        const double *base = (mode == BVP_TRANSFER_REFERENCE) ? &work[(size_t)ii * M] : &yold[(size_t)ii * M];
+        sum_stages(C, dt, base, &E.K[(size_t)ii * s * M], &E.KI[(size_t)ii * ns * M], w.data(), nullptr,
+                   z.data(), nullptr);
+        for (int c = 0; c < M; ++c) { work[(size_t)j * M + c] = z[c]; ynew[(size_t)j * M + c] = z[c]; }
+    }
+}
+
+// ---------------------------------------------------------------- full solve
+int instance_mesh_and_guess(const bvp_instance &P, std::vector<double> &mesh, std::vector<double> &y) {
+    if (P.N < 1) return BVP_EINVAL;
+    mesh.resize(P.N + 1);
+    if (P.mesh) std::memcpy(mesh.data(), P.mesh, sizeof(double) * (P.N + 1));
+    else uniform_mesh(P.t0, P.t1, P.N, mesh.data());
+    y.resize((size_t)P.M * (P.N + 1));
+    if (P.guess) std::memcpy(y.data(), P.guess, sizeof(double) * y.size());
+    else {
+        if (!P.u0) return BVP_EINVAL;
+        for (int k = 0; k <= P.N; ++k)
+            for (int c = 0; c < P.M; ++c) y[(size_t)k * P.M + c] = P.u0[c];
+    }
+    return 0;
+}
+
+int check_instance(const bvp_instance &P) {
+    if (!family_ok(P)) return BVP_EINVAL;
+    if (P.n_bc != P.M) return BVP_EINVAL;
+    if (P.N < 1) return BVP_EINVAL;
+    if (!(P.t1 > P.t0)) return BVP_EINVAL;
+    return 0;
+}
+
+int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
+    int rc = check_instance(P);
+    if (rc) return rc;
+    Ctx C(P, o.method);
+    std::vector<double> mesh, y;
+    rc = instance_mesh_and_guess(P, mesh, y);
+    if (rc) return rc;
+    const double tol = o.newton_abstol > 0 ? o.newton_abstol : o.abstol;
+    const int M = P.M;
+    int N = P.N;
+    int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
+    double error_norm = 2 * o.abstol, last_defect = NAN;
+    int outer = 0, newton_iters = 0, fb = 0;
+    std::vector<BcRow> rows;
+    std::vector<double> errors, new_mesh, ynew;
+    Eval E;
+    const int MAX_OUTER = 64;
+    for (;;) {
+        ++outer;
+        if (resolve_bc(P, N, mesh.data(), rows)) return BVP_EINVAL;
+        NlOut nl = newton(C, N, mesh.data(), y, rows, tol, o.max_newton_iters, o.linesearch_fallback != 0);
+        newton_iters += nl.iters;
+        if (nl.fallback) fb |= 1;
+        nl_ret = nl.ret;
+        error_norm = 2 * o.abstol;
+        if (!o.adaptive) { info = nl.ret; break; }
+        info = nl.ret;
+        if (info == BVP_RET_SUCCESS) {
+            evaluate(C, N, mesh.data(), y.data(), rows, E, false);
+            error_norm = defect(C, N, mesh.data(), y.data(), E, errors);
+            last_defect = error_norm;
+            info = (error_norm > o.defect_threshold || std::isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
+        }
+        if (info == BVP_RET_SUCCESS) {
+            if (error_norm > o.abstol) {
+                info = mesh_select(C.T.order, o.abstol, o.max_num_subintervals, mesh, errors, M, new_mesh);
+                if (info == BVP_RET_SUCCESS) {
+                    transfer(C, o.transfer_mode, mesh, y, E, new_mesh, ynew);
+                    mesh = new_mesh;
+                    y = ynew;
+                    N = (int)mesh.size() - 1;
+                }
+            }
+        } else {
+            if (2 * N > o.max_num_subintervals) {
+                info = BVP_RET_FAILURE;
+            } else {
+                std::vector<double> hm;
+                half_mesh(mesh, hm);
+                mesh = hm;
+                N = (int)mesh.size() - 1;
+                y.assign((size_t)M * (N + 1), 0.0);
+                info = BVP_RET_SUCCESS;
+            }
+        }
+        if (!(info == BVP_RET_SUCCESS && error_norm > o.abstol)) break;
+        if (outer >= MAX_OUTER) { info = BVP_RET_FAILURE; break; }
+    }
+    // __build_solution: Core/utils.jl:798-801
+    res.retcode = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
+    res.N_final = N;
+    res.newton_iters = newton_iters;
+    res.outer_iters = outer;
+    res.fallback_used = fb;
+    res.defect_norm = last_defect;
+    if (resolve_bc(P, N, mesh.data(), rows) == 0) {
+        evaluate(C, N, mesh.data(), y.data(), rows, E, false);
+        res.resid_norm = max_abs_residual(E);
+    }
+    if (res.capacity < N + 1) return BVP_EINVAL;
+    if (res.mesh) std::memcpy(res.mesh, mesh.data(), sizeof(double) * (N + 1));
+    if (res.u) std::memcpy(res.u, y.data(), sizeof(double) * M * (N + 1));
+    return 0;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI of the oracle (ctypes from tests/ and the benchmark's CPU leg)
+// ======================================================================
+extern "C" {
+
+void ref_opts_default(bvp_opts *o) {
+    o->method = BVP_MIRK4;
+    o->abstol = 1e-6;
+    o->newton_abstol = 0;
+    o->adaptive = 1;
+    o->defect_threshold = 0.1;
+    o->max_num_subintervals = 3000;
+    o->max_newton_iters = 1000;
+    o->linesearch_fallback = 1;
+    o->transfer_mode = BVP_TRANSFER_REFERENCE;
+    o->precision = BVP_PREC_FP64;
+    o->refine_iters = 0;
+    o->warm_start = BVP_WARM_NONE;
+}
+
+void ref_uniform_mesh(double t0, double t1, int N, double *mesh) { uniform_mesh(t0, t1, N, mesh); }
+
+int ref_interval(const double *mesh, int N, double t) { return interval_of(mesh, N, t); }
+
+void ref_interp_weights(int method, double tau, double *w, double *wp) { interp_weights(method, tau, w, wp); }
+
+// tableau export: s, sstar, order, c[5], v[5], b[5], x[25], cs[4], vs[4], xs[36], tau
+int ref_tableau(int method, int *ints, double *dbl) {
+    const Tab &T = tableau(method);
+    ints[0] = T.s; ints[1] = T.sstar; ints[2] = T.order;
+    int o = 0;
+    for (int i = 0; i < 5; ++i) dbl[o++] = T.c[i];
+    for (int i = 0; i < 5; ++i) dbl[o++] = T.v[i];
+    for (int i = 0; i < 5; ++i) dbl[o++] = T.b[i];
+    for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) dbl[o++] = T.x[i][j];
+    for (int i = 0; i < 4; ++i) dbl[o++] = T.cs[i];
+    for (int i = 0; i < 4; ++i) dbl[o++] = T.vs[i];
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 9; ++j) dbl[o++] = T.xs[i][j];
+    dbl[o++] = T.tau;
+    return o;
+}
+
+void ref_f(const bvp_instance *P, double t, const double *y, double *out) { f_eval(*P, t, y, out); }
+
+int ref_residual(int method, const bvp_instance *P, double *resid) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    Ctx C(*P, method);
+    std::vector<double> mesh, y;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    std::vector<BcRow> rows;
+    if ((rc = resolve_bc(*P, P->N, mesh.data(), rows))) return rc;
+    Eval E;
+    evaluate(C, P->N, mesh.data(), y.data(), rows, E, false);
+    flatten_residual(C, P->N, rows, E, resid);
+    return 0;
+}
+
+int ref_jacobian_blocks(int method, const bvp_instance *P, double *S, double *R, int32_t *bc_node) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    Ctx C(*P, method);
+    std::vector<double> mesh, y;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    std::vector<BcRow> rows;
+    if ((rc = resolve_bc(*P, P->N, mesh.data(), rows))) return rc;
+    Eval E;
+    evaluate(C, P->N, mesh.data(), y.data(), rows, E, false);
+    const int M = P->M, MM = M * M;
+    for (int i = 0; i < P->N; ++i)
+        interval_jac(C, mesh[i], mesh[i + 1] - mesh[i], &E.Y[(size_t)i * C.T.s * M], S + (size_t)i * MM,
+                     R + (size_t)i * MM);
+    for (size_t k = 0; k < rows.size(); ++k) bc_node[k] = rows[k].node;
+    return 0;
+}
+
+// One Newton correction delta solving J delta = -F (point-major, M*(N+1)).
+int ref_newton_step(int method, const bvp_instance *P, double *delta) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    Ctx C(*P, method);
+    std::vector<double> mesh, y;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    std::vector<BcRow> rows;
+    if ((rc = resolve_bc(*P, P->N, mesh.data(), rows))) return rc;
+    Eval E;
+    evaluate(C, P->N, mesh.data(), y.data(), rows, E, false);
+    Band B;
+    if (newton_solve(C, P->N, mesh.data(), y.data(), rows, E, delta, B)) return 1;
+    return 0;
+}
+
+// Dense check path: assemble the full Newton matrix (reference row layout) as
+// a dense n*n row-major array plus the residual.  For small N only.
+int ref_dense_system(int method, const bvp_instance *P, double *J, double *F) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    Ctx C(*P, method);
+    std::vector<double> mesh, y;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    std::vector<BcRow> rows;
+    if ((rc = resolve_bc(*P, P->N, mesh.data(), rows))) return rc;
+    Eval E;
+    evaluate(C, P->N, mesh.data(), y.data(), rows, E, false);
+    flatten_residual(C, P->N, rows, E, F);
+    const int M = P->M, MM = M * M, N = P->N, n = M * (N + 1);
+    std::memset(J, 0, sizeof(double) * (size_t)n * n);
+    std::vector<double> S(MM), R(MM);
+    int r = 0;
+    auto put_bc = [&](size_t k) { J[(size_t)r * n + rows[k].node * M + rows[k].comp] = 1.0; ++r; };
+    if (P->bc_kind == BVP_BC_TWO_POINT) { for (size_t k = 0; k < rows.size(); ++k) if (!rows[k].side_b) put_bc(k); }
+    else for (size_t k = 0; k < rows.size(); ++k) put_bc(k);
+    for (int i = 0; i < N; ++i) {
+        interval_jac(C, mesh[i], mesh[i + 1] - mesh[i], &E.Y[(size_t)i * C.T.s * M], S.data(), R.data());
+        for (int a = 0; a < M; ++a) {
+            for (int b = 0; b < M; ++b) {
+                J[(size_t)r * n + i * M + b] = S[a * M + b];
+                J[(size_t)r * n + (i + 1) * M + b] = R[a * M + b];
+            }
+            ++r;
+        }
+    }
+    if (P->bc_kind == BVP_BC_TWO_POINT) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) put_bc(k);
+    return 0;
+}
+
+int ref_defect(int method, const bvp_instance *P, double *errors, double *defect_norm) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    Ctx C(*P, method);
+    std::vector<double> mesh, y;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    std::vector<BcRow> rows;
+    if ((rc = resolve_bc(*P, P->N, mesh.data(), rows))) return rc;
+    Eval E;
+    evaluate(C, P->N, mesh.data(), y.data(), rows, E, false);
+    std::vector<double> err;
+    *defect_norm = defect(C, P->N, mesh.data(), y.data(), E, err);
+    std::memcpy(errors, err.data(), sizeof(double) * err.size());
+    return 0;
+}
+
+int ref_mesh_select(const bvp_opts *o, const bvp_instance *P, int capacity, int32_t *new_N, int32_t *info,
+                    double *new_mesh, double *new_y) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    Ctx C(*P, o->method);
+    std::vector<double> mesh, y;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    std::vector<BcRow> rows;
+    if ((rc = resolve_bc(*P, P->N, mesh.data(), rows))) return rc;
+    Eval E;
+    evaluate(C, P->N, mesh.data(), y.data(), rows, E, false);
+    std::vector<double> err, nm, ny;
+    defect(C, P->N, mesh.data(), y.data(), E, err);
+    *info = mesh_select(C.T.order, o->abstol, o->max_num_subintervals, mesh, err, P->M, nm);
+    if (*info != BVP_RET_SUCCESS) { *new_N = P->N; return 0; }
+    transfer(C, o->transfer_mode, mesh, y, E, nm, ny);
+    *new_N = (int)nm.size() - 1;
+    if ((int)nm.size() > capacity) return BVP_EINVAL;
+    std::memcpy(new_mesh, nm.data(), sizeof(double) * nm.size());
+    std::memcpy(new_y, ny.data(), sizeof(double) * ny.size());
+    return 0;
+}
+
+// mesh_selector! alone on given per-interval errors (for hand-computed cases)
+int ref_mesh_select_errors(int order, double abstol, int max_sub, int N, const double *mesh, int M,
+                           const double *errors, int capacity, double *new_mesh) {
+    std::vector<double> m(mesh, mesh + N + 1), e(errors, errors + (size_t)N * M), nm;
+    int info = mesh_select(order, abstol, max_sub, m, e, M, nm);
+    if (info != BVP_RET_SUCCESS) return -(int)info;  // negative: failure
+    if ((int)nm.size() > capacity) return BVP_EINVAL - 100;
+    std::memcpy(new_mesh, nm.data(), sizeof(double) * nm.size());
+    return (int)nm.size() - 1;
+}
+
+int ref_solve(const bvp_opts *o, const bvp_instance *P, bvp_result *res) { return solve_one(*P, *o, *res); }
+
+// EnsembleThreads analogue: one instance per thread, dynamic schedule.
+int ref_solve_batch(const bvp_opts *o, int n, const bvp_instance *inst, bvp_result *res, int nthreads) {
+    int bad = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+    for (int i = 0; i < n; ++i) {
+        int rc = solve_one(inst[i], *o, res[i]);
+        if (rc) bad |= 1;
+    }
+    return bad ? BVP_EINVAL : 0;
+}
+
+int ref_max_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"
