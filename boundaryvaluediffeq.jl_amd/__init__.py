@@ -8,9 +8,13 @@ from . import _abi
 from .problems import (BVPFunction, BVProblem, TwoPointBVProblem, DiffEqArray, InstanceSpec,
                        LinearTanhFunction, PendulumFunction, BratuFunction, SingPertFunction,
                        ScalarTUFunction, TurningFunction, extract, uniform_mesh, cld, instance_array)
+from .solve import (MIRK4, MIRK6, DefectControl, EnsembleProblem, EnsembleSolution, BVPSolution, solve,
+                    successful_retcode, make_opts)
 
 __all__ = [
     "BVPFunction", "BVProblem", "TwoPointBVProblem", "DiffEqArray", "InstanceSpec",
     "LinearTanhFunction", "PendulumFunction", "BratuFunction", "SingPertFunction",
     "ScalarTUFunction", "TurningFunction", "extract", "uniform_mesh", "cld", "instance_array",
+    "MIRK4", "MIRK6", "DefectControl", "EnsembleProblem", "EnsembleSolution", "BVPSolution", "solve",
+    "successful_retcode", "make_opts",
 ]

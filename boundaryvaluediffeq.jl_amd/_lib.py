@@ -1,0 +1,278 @@
+"""ctypes binding of the HIP library ``_build/libbvp_mirk.so`` (C ABI of
+``include/bvp_mirk.h``).  There is no CPU fallback: a missing library or a
+missing GPU raises."""
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+from . import _abi
+from .problems import instance_array
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "_build" / "libbvp_mirk.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_longlong)
+
+
+class BVPError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise BVPError(f"HIP library not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    Inst = C.POINTER(_abi.Instance)
+    Opts = C.POINTER(_abi.Opts)
+    Res = C.POINTER(_abi.Result)
+    sig = {
+        "bvp_opts_default": (None, [Opts]),
+        "bvp_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "bvp_ctx_destroy": (None, [vp]),
+        "bvp_last_error": (C.c_char_p, [vp]),
+        "bvp_mirk_solve_batch": (C.c_int, [vp, Opts, C.c_int32, Inst, Res]),
+        "bvp_mirk_residual": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp]),
+        "bvp_mirk_jacobian_blocks": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp, _ip]),
+        "bvp_mirk_newton_step": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, Inst, _dp]),
+        "bvp_mirk_defect": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp]),
+        "bvp_mirk_mesh_select": (C.c_int, [vp, Opts, C.c_int32, Inst, C.c_int32, _ip, _ip, _dp, _dp]),
+        "bvp_batch_create": (C.c_int, [vp, Opts, C.c_int32, Inst, C.POINTER(vp)]),
+        "bvp_batch_run": (C.c_int, [vp, vp]),
+        "bvp_batch_sync": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+        "bvp_batch_results": (C.c_int, [vp, vp, Res]),
+        "bvp_batch_counters": (C.c_int, [vp, vp, _lp]),
+        "bvp_batch_info": (C.c_int, [vp, _ip]),
+        "bvp_batch_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_error",
+            "bvp_mirk_solve_batch", "bvp_mirk_residual", "bvp_mirk_jacobian_blocks",
+            "bvp_mirk_newton_step", "bvp_mirk_defect", "bvp_mirk_mesh_select", "bvp_batch_create",
+            "bvp_batch_run", "bvp_batch_sync", "bvp_batch_results", "bvp_batch_counters",
+            "bvp_batch_info", "bvp_batch_destroy"]
+
+
+class Context:
+    """A device context (stream, events, device memory of the calls)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().bvp_ctx_create(device, C.byref(self._h))
+        if rc != 0:
+            raise BVPError(f"bvp_ctx_create(device={device}) failed with {rc} (no usable GPU?)")
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().bvp_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = lib().bvp_last_error(self._h)
+            raise BVPError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+_ctx = {}
+
+
+def context(device=0):
+    if device not in _ctx:
+        _ctx[device] = Context(device)
+    return _ctx[device]
+
+
+def _nodes(specs):
+    return [s.N + 1 for s in specs]
+
+
+def residual(ctx, method, specs):
+    arr = instance_array(specs)
+    M = specs[0].M
+    tot = sum(_nodes(specs)) * M
+    out = np.zeros(tot)
+    ctx.check(lib().bvp_mirk_residual(ctx.handle, method, len(specs), arr, out.ctypes.data_as(_dp)), "residual")
+    res, o = [], 0
+    for s in specs:
+        k = M * (s.N + 1)
+        res.append(out[o:o + k].copy())
+        o += k
+    return res
+
+
+def jacobian_blocks(ctx, method, specs):
+    arr = instance_array(specs)
+    M = specs[0].M
+    nint = sum(s.N for s in specs)
+    S = np.zeros(nint * M * M)
+    R = np.zeros(nint * M * M)
+    node = np.zeros(len(specs) * M, dtype=np.int32)
+    ctx.check(lib().bvp_mirk_jacobian_blocks(ctx.handle, method, len(specs), arr, S.ctypes.data_as(_dp),
+                                             R.ctypes.data_as(_dp), node.ctypes.data_as(_ip)), "jacobian_blocks")
+    out, o = [], 0
+    for i, s in enumerate(specs):
+        k = s.N * M * M
+        out.append((S[o:o + k].reshape(s.N, M, M).copy(), R[o:o + k].reshape(s.N, M, M).copy(),
+                    node[i * M:(i + 1) * M].copy()))
+        o += k
+    return out
+
+
+def newton_step(ctx, method, specs, precision=_abi.PREC_FP64, refine_iters=0):
+    arr = instance_array(specs)
+    M = specs[0].M
+    out = np.zeros(sum(_nodes(specs)) * M)
+    ctx.check(lib().bvp_mirk_newton_step(ctx.handle, method, precision, refine_iters, len(specs), arr,
+                                         out.ctypes.data_as(_dp)), "newton_step")
+    res, o = [], 0
+    for s in specs:
+        k = M * (s.N + 1)
+        res.append(out[o:o + k].copy())
+        o += k
+    return res
+
+
+def defect(ctx, method, specs):
+    arr = instance_array(specs)
+    M = specs[0].M
+    nint = sum(s.N for s in specs)
+    err = np.zeros(nint * M)
+    dn = np.zeros(len(specs))
+    ctx.check(lib().bvp_mirk_defect(ctx.handle, method, len(specs), arr, err.ctypes.data_as(_dp),
+                                    dn.ctypes.data_as(_dp)), "defect")
+    out, o = [], 0
+    for i, s in enumerate(specs):
+        out.append((err[o:o + s.N * M].reshape(s.N, M).copy(), float(dn[i])))
+        o += s.N * M
+    return out
+
+
+def mesh_select(ctx, opts, specs, capacity=None):
+    arr = instance_array(specs)
+    M = specs[0].M
+    n = len(specs)
+    cap = capacity or (opts.max_num_subintervals + 1)
+    nN = np.zeros(n, dtype=np.int32)
+    info = np.zeros(n, dtype=np.int32)
+    nm = np.zeros(n * cap)
+    ny = np.zeros(n * cap * M)
+    ctx.check(lib().bvp_mirk_mesh_select(ctx.handle, C.byref(opts), n, arr, cap, nN.ctypes.data_as(_ip),
+                                         info.ctypes.data_as(_ip), nm.ctypes.data_as(_dp),
+                                         ny.ctypes.data_as(_dp)), "mesh_select")
+    out = []
+    for i in range(n):
+        k = int(nN[i])
+        out.append((int(info[i]), k, nm[i * cap:i * cap + k + 1].copy(),
+                    ny[i * cap * M:i * cap * M + M * (k + 1)].copy()))
+    return out
+
+
+class RawSolution:
+    """Per-instance output of a batched solve."""
+
+    def __init__(self, r, M, mesh, u):
+        self.retcode = int(r.retcode)
+        self.N = int(r.N_final)
+        self.newton_iters = int(r.newton_iters)
+        self.outer_iters = int(r.outer_iters)
+        self.fallback_used = int(r.fallback_used)
+        self.defect_norm = float(r.defect_norm)
+        self.resid_norm = float(r.resid_norm)
+        self.t = mesh[:self.N + 1].copy()
+        self.u = u[:M * (self.N + 1)].reshape(self.N + 1, M).copy()
+
+
+def _result_array(specs, cap):
+    n = len(specs)
+    res = (_abi.Result * n)()
+    bufs = []
+    for i, s in enumerate(specs):
+        m, u = np.zeros(cap), np.zeros(cap * s.M)
+        bufs.append((m, u))
+        res[i].capacity = cap
+        res[i].mesh = m.ctypes.data_as(_dp)
+        res[i].u = u.ctypes.data_as(_dp)
+    return res, bufs
+
+
+def solve_batch(ctx, opts, specs):
+    arr = instance_array(specs)
+    cap = max(opts.max_num_subintervals, max(s.N for s in specs)) + 1
+    res, bufs = _result_array(specs, cap)
+    ctx.check(lib().bvp_mirk_solve_batch(ctx.handle, C.byref(opts), len(specs), arr, res), "solve_batch")
+    return [RawSolution(res[i], specs[i].M, *bufs[i]) for i in range(len(specs))]
+
+
+class Batch:
+    """Uploaded batch that can be solved repeatedly (bvp_batch_* plan API)."""
+
+    def __init__(self, ctx, opts, specs):
+        self.ctx = ctx
+        self.specs = specs
+        self.opts = opts
+        self._arr = instance_array(specs)
+        self._h = C.c_void_p()
+        ctx.check(lib().bvp_batch_create(ctx.handle, C.byref(opts), len(specs), self._arr, C.byref(self._h)),
+                  "batch_create")
+
+    def run(self):
+        self.ctx.check(lib().bvp_batch_run(self.ctx.handle, self._h), "batch_run")
+
+    def sync(self):
+        ms = C.c_float()
+        self.ctx.check(lib().bvp_batch_sync(self.ctx.handle, self._h, C.byref(ms)), "batch_sync")
+        return float(ms.value)
+
+    def results(self):
+        cap = max(self.opts.max_num_subintervals, max(s.N for s in self.specs)) + 1
+        res, bufs = _result_array(self.specs, cap)
+        self.ctx.check(lib().bvp_batch_results(self.ctx.handle, self._h, res), "batch_results")
+        return [RawSolution(res[i], self.specs[i].M, *bufs[i]) for i in range(len(self.specs))]
+
+    def counters(self):
+        c = (C.c_longlong * 4)()
+        self.ctx.check(lib().bvp_batch_counters(self.ctx.handle, self._h, c), "batch_counters")
+        return [int(x) for x in c]
+
+    def info(self):
+        c = (C.c_int32 * 4)()
+        lib().bvp_batch_info(self._h, c)
+        return [int(x) for x in c]
+
+    def close(self):
+        if self._h:
+            lib().bvp_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

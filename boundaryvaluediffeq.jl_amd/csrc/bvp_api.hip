@@ -1,0 +1,551 @@
+// bvp_api.hip -- C ABI of the MIRK collocation library (include/bvp_mirk.h).
+//
+// Host side of the drop-in boundary: validates and packs the caller's problem
+// instances, owns device memory and the stream, launches the gfx950 kernels and
+// copies results back.  No pointer passed in is retained after a call returns.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/bvp_mirk.h"
+#include "bvp_internal.h"
+#include "mirk_wave.h"
+
+namespace {
+
+void set_err(bvp_ctx *ctx, const char *fmt, ...) {
+    if (!ctx) return;
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+}
+
+#define HIP_TRY(ctx, expr)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            set_err((ctx), "%s failed: %s", #expr, hipGetErrorString(e_));                \
+            return BVP_EHIP;                                                               \
+        }                                                                                  \
+    } while (0)
+
+int params_needed(int family, int M) {
+    switch (family) {
+    case BVP_FAMILY_LINEAR_TANH: return 1 + M * M;
+    case BVP_FAMILY_SCALAR_TU: return 0;
+    default: return 1;
+    }
+}
+
+// range(t0; stop=t1, length=N+1) (Core/utils.jl:694-700), correctly rounded
+// through an extended-precision intermediate.
+void uniform_mesh(double t0, double t1, int N, double *m) {
+    for (int k = 0; k <= N; ++k)
+        m[k] = (double)(((long double)t0 * (N - k) + (long double)t1 * k) / (long double)N);
+    m[0] = t0;
+    m[N] = t1;
+}
+
+int validate(bvp_ctx *ctx, int32_t n, const bvp_instance *inst, int &M) {
+    if (n <= 0 || !inst) { set_err(ctx, "empty batch"); return BVP_EINVAL; }
+    M = inst[0].M;
+    for (int i = 0; i < n; ++i) {
+        const bvp_instance &P = inst[i];
+        if (P.M != M) { set_err(ctx, "instance %d: all instances of a batch must share M", i); return BVP_EINVAL; }
+        if (P.M < 1) { set_err(ctx, "instance %d: M must be >= 1", i); return BVP_EINVAL; }
+        if (!wave_supports(M, P.family) && !blockm_supports(M, P.family)) {
+            set_err(ctx, "instance %d: family %d with M=%d is not supported", i, P.family, M);
+            return BVP_ENOTSUP;
+        }
+        if (P.n_params < params_needed(P.family, M) || (params_needed(P.family, M) > 0 && !P.params)) {
+            set_err(ctx, "instance %d: family %d needs %d parameters", i, P.family, params_needed(P.family, M));
+            return BVP_EINVAL;
+        }
+        if (P.n_bc != M || !P.bc_t || !P.bc_comp || !P.bc_val) {
+            set_err(ctx, "instance %d: need exactly M boundary rows", i);
+            return BVP_EINVAL;
+        }
+        if (P.bc_kind != BVP_BC_TWO_POINT && P.bc_kind != BVP_BC_STANDARD) {
+            set_err(ctx, "instance %d: bad bc_kind", i);
+            return BVP_EINVAL;
+        }
+        for (int k = 0; k < M; ++k) {
+            if (P.bc_comp[k] < 0 || P.bc_comp[k] >= M) { set_err(ctx, "instance %d: bc_comp out of range", i); return BVP_EINVAL; }
+            if (P.bc_kind == BVP_BC_TWO_POINT && P.bc_t[k] != P.t0 && P.bc_t[k] != P.t1) {
+                set_err(ctx, "instance %d: two-point BC row %d not at t0 or t1", i, k);
+                return BVP_EINVAL;
+            }
+            if (!(P.bc_t[k] >= P.t0 && P.bc_t[k] <= P.t1)) { set_err(ctx, "instance %d: BC point outside tspan", i); return BVP_EINVAL; }
+        }
+        if (P.N < 1) { set_err(ctx, "instance %d: N must be >= 1", i); return BVP_EINVAL; }
+        if (!(P.t1 > P.t0)) { set_err(ctx, "instance %d: need t1 > t0", i); return BVP_EINVAL; }
+        if (!P.guess && !P.u0) { set_err(ctx, "instance %d: need a guess or u0", i); return BVP_EINVAL; }
+        if (P.mesh) {
+            for (int k = 0; k < P.N; ++k)
+                if (!(P.mesh[k + 1] > P.mesh[k])) { set_err(ctx, "instance %d: mesh not increasing", i); return BVP_EINVAL; }
+        }
+    }
+    return 0;
+}
+
+template <typename T>
+int dalloc(bvp_ctx *ctx, DevArena &arena, T *&p, size_t count) {
+    p = nullptr;
+    if (count == 0) count = 1;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_err(ctx, "hipMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
+        return BVP_ENOMEM;
+    }
+    arena.ptrs.push_back(q);
+    p = (T *)q;
+    return 0;
+}
+
+template <typename T>
+int upload(bvp_ctx *ctx, DevArena &arena, T *&p, const std::vector<T> &v) {
+    int rc = dalloc(ctx, arena, p, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIP_TRY(ctx, hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+}  // namespace
+
+DevArena::~DevArena() {
+    for (void *p : ptrs) (void)hipFree(p);
+}
+
+// ------------------------------------------------------------------ batch build
+int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *inst, int cap_req, bvp_batch **out) {
+    int M;
+    int rc = validate(ctx, n, inst, M);
+    if (rc) return rc;
+    if (!wave_supports(M, inst[0].family)) {
+        set_err(ctx, "M=%d is served by the block kernels, not yet wired into this entry point", M);
+        return BVP_ENOTSUP;
+    }
+    if (o->method != BVP_MIRK4 && o->method != BVP_MIRK6) { set_err(ctx, "method must be 4 or 6"); return BVP_EINVAL; }
+    if (!(o->abstol > 0)) { set_err(ctx, "abstol must be positive"); return BVP_EINVAL; }
+    int maxN0 = 0;
+    for (int i = 0; i < n; ++i) maxN0 = std::max(maxN0, inst[i].N);
+    int cap = std::max(cap_req, std::max(o->max_num_subintervals, maxN0) + 1);
+    auto *B = new bvp_batch();
+    B->n = n;
+    B->M = M;
+    B->method = o->method;
+    B->cap = cap;
+    B->opts = *o;
+    // ---- pack the problem description
+    std::vector<int> fam(n), poff(n), pcnt(n), bck(n), N0(n), bcc((size_t)n * M), order(n);
+    std::vector<double> t0(n), t1(n), params, bct((size_t)n * M), bcv((size_t)n * M);
+    std::vector<long long> ioff(n);
+    long long nodes = 0;
+    for (int i = 0; i < n; ++i) {
+        const bvp_instance &P = inst[i];
+        fam[i] = P.family;
+        t0[i] = P.t0;
+        t1[i] = P.t1;
+        poff[i] = (int)params.size();
+        pcnt[i] = P.n_params;
+        for (int k = 0; k < P.n_params; ++k) params.push_back(P.params[k]);
+        bck[i] = P.bc_kind;
+        for (int k = 0; k < M; ++k) {
+            bct[(size_t)i * M + k] = P.bc_t[k];
+            bcc[(size_t)i * M + k] = P.bc_comp[k];
+            bcv[(size_t)i * M + k] = P.bc_val[k];
+        }
+        N0[i] = P.N;
+        ioff[i] = nodes;
+        nodes += P.N + 1;
+    }
+    B->N0 = N0;
+    B->init_off = ioff;
+    B->total_nodes = nodes;
+    std::vector<double> imesh((size_t)nodes), iy((size_t)nodes * M);
+    for (int i = 0; i < n; ++i) {
+        const bvp_instance &P = inst[i];
+        double *m = &imesh[(size_t)ioff[i]];
+        if (P.mesh) std::memcpy(m, P.mesh, sizeof(double) * (P.N + 1));
+        else uniform_mesh(P.t0, P.t1, P.N, m);
+        double *y = &iy[(size_t)ioff[i] * M];
+        if (P.guess) std::memcpy(y, P.guess, sizeof(double) * (size_t)M * (P.N + 1));
+        else
+            for (int k = 0; k <= P.N; ++k)
+                for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = P.u0[c];
+    }
+    // queue order: largest meshes first (longest-processing-time first)
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return N0[a] > N0[b]; });
+
+    BatchDev &D = B->dev;
+    std::memset(&D, 0, sizeof D);
+    D.n = n;
+    D.M = M;
+    D.cap = cap;
+    DevArena &A = B->arena;
+    int *dfam, *dpoff, *dpcnt, *dbck, *dbcc, *dN0, *dorder;
+    double *dt0, *dt1, *dparams, *dbct, *dbcv, *dim, *diy;
+    long long *dioff;
+    if ((rc = upload(ctx, A, dfam, fam)) || (rc = upload(ctx, A, dpoff, poff)) || (rc = upload(ctx, A, dpcnt, pcnt)) ||
+        (rc = upload(ctx, A, dbck, bck)) || (rc = upload(ctx, A, dbcc, bcc)) || (rc = upload(ctx, A, dN0, N0)) ||
+        (rc = upload(ctx, A, dorder, order)) || (rc = upload(ctx, A, dt0, t0)) || (rc = upload(ctx, A, dt1, t1)) ||
+        (rc = upload(ctx, A, dparams, params)) || (rc = upload(ctx, A, dbct, bct)) || (rc = upload(ctx, A, dbcv, bcv)) ||
+        (rc = upload(ctx, A, dim, imesh)) || (rc = upload(ctx, A, diy, iy)) || (rc = upload(ctx, A, dioff, ioff))) {
+        delete B;
+        return rc;
+    }
+    D.family = dfam; D.param_off = dpoff; D.param_cnt = dpcnt; D.bc_kind = dbck; D.bc_comp = dbcc;
+    D.N0 = dN0; D.order = dorder; D.t0 = dt0; D.t1 = dt1; D.params = dparams; D.bc_t = dbct; D.bc_val = dbcv;
+    D.init_mesh = dim; D.init_y = diy; D.init_off = dioff;
+    // working state and results
+    if ((rc = dalloc(ctx, A, D.mesh, (size_t)n * 2 * cap)) || (rc = dalloc(ctx, A, D.y, (size_t)n * 2 * cap * M)) ||
+        (rc = dalloc(ctx, A, D.shat, (size_t)n * cap)) || (rc = dalloc(ctx, A, D.retcode, n)) ||
+        (rc = dalloc(ctx, A, D.N_final, n)) || (rc = dalloc(ctx, A, D.newton_iters, n)) ||
+        (rc = dalloc(ctx, A, D.outer_iters, n)) || (rc = dalloc(ctx, A, D.fallback, n)) ||
+        (rc = dalloc(ctx, A, D.cur_buf, n)) || (rc = dalloc(ctx, A, D.defect_norm, n)) ||
+        (rc = dalloc(ctx, A, D.resid_norm, n)) || (rc = dalloc(ctx, A, D.cnt, (size_t)n * 4)) ||
+        (rc = dalloc(ctx, A, D.counter, 1))) {
+        delete B;
+        return rc;
+    }
+    // persistent slots
+    int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : wave_blocks_per_cu(M, o->method);
+    if (per_cu < 1) per_cu = 1;
+    int slots = std::min(n, ctx->num_cus * per_cu);
+    if (slots < 1) slots = 1;
+    B->blocks = slots;
+    const int NF = wave_rel_fields(M);
+    const size_t lmax = (size_t)(cap + 63) / 64 + 1;
+    D.tree_base = lmax * NF * 64;
+    D.scratch_per_slot = D.tree_base + (size_t)64 * NF;
+    D.aux_per_slot = (size_t)cap * (2 * M + 1);
+    if ((rc = dalloc(ctx, A, D.scratch, D.scratch_per_slot * slots)) ||
+        (rc = dalloc(ctx, A, D.aux, D.aux_per_slot * slots))) {
+        delete B;
+        return rc;
+    }
+    // options
+    D.adaptive = o->adaptive;
+    D.max_sub = o->max_num_subintervals;
+    D.max_newton = o->max_newton_iters;
+    D.linesearch = o->linesearch_fallback;
+    D.transfer_mode = o->transfer_mode;
+    D.abstol = o->abstol;
+    D.newton_tol = o->newton_abstol > 0 ? o->newton_abstol : o->abstol;
+    D.defect_threshold = o->defect_threshold;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        set_err(ctx, "upload failed: %s", hipGetErrorString(e));
+        delete B;
+        return BVP_EHIP;
+    }
+    *out = B;
+    return 0;
+}
+
+int batch_run(bvp_ctx *ctx, bvp_batch *B) {
+    HIP_TRY(ctx, wave_reset_counter(B->dev.counter, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, wave_launch_solve(B->dev, B->method, B->blocks, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    return 0;
+}
+
+int batch_sync(bvp_ctx *ctx, bvp_batch *B, float *ms) {
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    HIP_TRY(ctx, hipGetLastError());
+    float t = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    B->last_ms = t;
+    if (ms) *ms = t;
+    return 0;
+}
+
+int batch_results(bvp_ctx *ctx, bvp_batch *B, bvp_result *res) {
+    const int n = B->n, M = B->M;
+    const BatchDev &D = B->dev;
+    std::vector<int> ret(n), Nf(n), it(n), ou(n), fb(n);
+    std::vector<double> dn(n), rn(n);
+    HIP_TRY(ctx, hipMemcpy(ret.data(), D.retcode, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(Nf.data(), D.N_final, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(it.data(), D.newton_iters, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(ou.data(), D.outer_iters, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(fb.data(), D.fallback, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(dn.data(), D.defect_norm, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(rn.data(), D.resid_norm, n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<long long> off(n);
+    long long tot = 0;
+    int maxn = 0;
+    for (int i = 0; i < n; ++i) {
+        off[i] = tot;
+        tot += Nf[i] + 1;
+        maxn = std::max(maxn, Nf[i] + 1);
+    }
+    DevArena tmp;
+    long long *doff;
+    double *dm, *dy;
+    int rc;
+    if ((rc = upload(ctx, tmp, doff, off)) || (rc = dalloc(ctx, tmp, dm, (size_t)tot)) ||
+        (rc = dalloc(ctx, tmp, dy, (size_t)tot * M)))
+        return rc;
+    HIP_TRY(ctx, wave_launch_gather(D, doff, dm, dy, maxn, ctx->stream));
+    std::vector<double> hm((size_t)tot), hy((size_t)tot * M);
+    HIP_TRY(ctx, hipMemcpyAsync(hm.data(), dm, hm.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(hy.data(), dy, hy.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    int bad = 0;
+    for (int i = 0; i < n; ++i) {
+        bvp_result &r = res[i];
+        r.retcode = ret[i];
+        r.N_final = Nf[i];
+        r.newton_iters = it[i];
+        r.outer_iters = ou[i];
+        r.fallback_used = fb[i];
+        r.defect_norm = dn[i];
+        r.resid_norm = rn[i];
+        if (r.capacity < Nf[i] + 1) { bad = 1; continue; }
+        if (r.mesh) std::memcpy(r.mesh, &hm[(size_t)off[i]], sizeof(double) * (Nf[i] + 1));
+        if (r.u) std::memcpy(r.u, &hy[(size_t)off[i] * M], sizeof(double) * (size_t)M * (Nf[i] + 1));
+    }
+    if (bad) { set_err(ctx, "result capacity smaller than the final mesh"); return BVP_EINVAL; }
+    return 0;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+void bvp_opts_default(bvp_opts *o) {
+    o->method = BVP_MIRK4;
+    o->abstol = 1e-6;
+    o->newton_abstol = 0;
+    o->adaptive = 1;
+    o->defect_threshold = 0.1;
+    o->max_num_subintervals = 3000;
+    o->max_newton_iters = 1000;
+    o->linesearch_fallback = 1;
+    o->transfer_mode = BVP_TRANSFER_REFERENCE;
+    o->precision = BVP_PREC_FP64;
+    o->refine_iters = 0;
+    o->warm_start = BVP_WARM_NONE;
+}
+
+int bvp_ctx_create(int device, bvp_ctx **out) {
+    if (!out) return BVP_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BVP_EHIP;
+    if (device < 0 || device >= ndev) return BVP_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return BVP_EHIP;
+    auto *c = new bvp_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return BVP_EHIP; }
+    c->num_cus = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return BVP_EHIP; }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return BVP_EHIP; }
+    const char *env = std::getenv("BVP_BLOCKS_PER_CU");
+    c->blocks_per_cu = env ? std::atoi(env) : 0;
+    *out = c;
+    return 0;
+}
+
+void bvp_ctx_destroy(bvp_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *bvp_last_error(const bvp_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int bvp_mirk_solve_batch(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *inst, bvp_result *res) {
+    if (!ctx || !o || !res) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    bvp_batch *B = nullptr;
+    int rc = batch_build(ctx, o, n, inst, 0, &B);
+    if (rc) return rc;
+    if ((rc = batch_run(ctx, B)) == 0 && (rc = batch_sync(ctx, B, nullptr)) == 0) rc = batch_results(ctx, B, res);
+    delete B;
+    return rc;
+}
+
+// ---- plan API (extension of the ABI for repeated timed runs; see bvp_internal.h)
+int bvp_batch_create(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *inst, bvp_batch **out) {
+    if (!ctx || !o || !out) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    return batch_build(ctx, o, n, inst, 0, out);
+}
+int bvp_batch_run(bvp_ctx *ctx, bvp_batch *B) { return (ctx && B) ? batch_run(ctx, B) : BVP_EINVAL; }
+int bvp_batch_sync(bvp_ctx *ctx, bvp_batch *B, float *ms) { return (ctx && B) ? batch_sync(ctx, B, ms) : BVP_EINVAL; }
+int bvp_batch_results(bvp_ctx *ctx, bvp_batch *B, bvp_result *res) {
+    return (ctx && B && res) ? batch_results(ctx, B, res) : BVP_EINVAL;
+}
+int bvp_batch_counters(bvp_ctx *ctx, bvp_batch *B, long long *out4) {
+    if (!ctx || !B || !out4) return BVP_EINVAL;
+    std::vector<long long> c((size_t)B->n * 4);
+    HIP_TRY(ctx, hipMemcpy(c.data(), B->dev.cnt, c.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 4; ++k) out4[k] = 0;
+    for (int i = 0; i < B->n; ++i)
+        for (int k = 0; k < 4; ++k) out4[k] += c[(size_t)i * 4 + k];
+    return 0;
+}
+int bvp_batch_info(bvp_batch *B, int32_t *out4) {
+    if (!B || !out4) return BVP_EINVAL;
+    out4[0] = B->blocks;
+    out4[1] = B->cap;
+    out4[2] = B->M;
+    out4[3] = (int32_t)B->dev.scratch_per_slot;
+    return 0;
+}
+void bvp_batch_destroy(bvp_batch *B) { delete B; }
+
+// ---- component entry points
+static int run_component(bvp_ctx *ctx, int method, int op, int32_t n, const bvp_instance *inst, int cap_req,
+                         const bvp_opts *o_in, bvp_batch **outB) {
+    bvp_opts o;
+    if (o_in) o = *o_in;
+    else bvp_opts_default(&o);
+    o.method = method;
+    int rc = batch_build(ctx, &o, n, inst, cap_req, outB);
+    if (rc) return rc;
+    return 0;
+}
+
+static int launch_op(bvp_ctx *ctx, bvp_batch *B, int op) {
+    HIP_TRY(ctx, wave_reset_counter(B->dev.counter, ctx->stream));
+    HIP_TRY(ctx, wave_launch_component(B->dev, B->method, op, B->blocks, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int bvp_mirk_residual(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, double *resid) {
+    if (!ctx || !resid) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, method, OP_RESIDUAL, n, inst, 0, nullptr, &B);
+    if (rc) return rc;
+    const size_t tot = (size_t)B->total_nodes * B->M;
+    if ((rc = dalloc(ctx, B->arena, B->dev.out0, tot)) == 0 && (rc = launch_op(ctx, B, OP_RESIDUAL)) == 0) {
+        hipError_t e = hipMemcpy(resid, B->dev.out0, tot * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "copy back: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+    }
+    delete B;
+    return rc;
+}
+
+int bvp_mirk_jacobian_blocks(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, double *S, double *R,
+                             int32_t *bc_node) {
+    if (!ctx || !S || !R || !bc_node) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, method, OP_JACOBIAN, n, inst, 0, nullptr, &B);
+    if (rc) return rc;
+    const int M = B->M;
+    const size_t nint = (size_t)(B->total_nodes - n);
+    if ((rc = dalloc(ctx, B->arena, B->dev.out0, nint * M * M)) == 0 &&
+        (rc = dalloc(ctx, B->arena, B->dev.out1, nint * M * M)) == 0 &&
+        (rc = dalloc(ctx, B->arena, B->dev.iout, (size_t)n * M)) == 0 && (rc = launch_op(ctx, B, OP_JACOBIAN)) == 0) {
+        hipError_t e = hipMemcpy(S, B->dev.out0, nint * M * M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(R, B->dev.out1, nint * M * M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(bc_node, B->dev.iout, (size_t)n * M * sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "copy back: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+    }
+    delete B;
+    return rc;
+}
+
+int bvp_mirk_newton_step(bvp_ctx *ctx, int32_t method, int32_t precision, int32_t refine_iters, int32_t n,
+                         const bvp_instance *inst, double *delta) {
+    if (!ctx || !delta) return BVP_EINVAL;
+    (void)refine_iters;
+    if (precision != BVP_PREC_FP64) { set_err(ctx, "newton_step: only FP64 is wired in this build"); return BVP_ENOTSUP; }
+    (void)hipSetDevice(ctx->device);
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, method, OP_NEWTON_STEP, n, inst, 0, nullptr, &B);
+    if (rc) return rc;
+    const size_t tot = (size_t)B->total_nodes * B->M;
+    std::vector<int> ok(n);
+    if ((rc = dalloc(ctx, B->arena, B->dev.out0, tot)) == 0 && (rc = dalloc(ctx, B->arena, B->dev.iout, n)) == 0 &&
+        (rc = launch_op(ctx, B, OP_NEWTON_STEP)) == 0) {
+        hipError_t e = hipMemcpy(delta, B->dev.out0, tot * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(ok.data(), B->dev.iout, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "copy back: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+        for (int i = 0; i < n && rc == 0; ++i)
+            if (ok[i]) { set_err(ctx, "instance %d: singular Newton system", i); rc = 1; }
+    }
+    delete B;
+    return rc;
+}
+
+int bvp_mirk_defect(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, double *errors,
+                    double *defect_norm) {
+    if (!ctx || !errors || !defect_norm) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, method, OP_DEFECT, n, inst, 0, nullptr, &B);
+    if (rc) return rc;
+    const size_t nint = (size_t)(B->total_nodes - n);
+    if ((rc = dalloc(ctx, B->arena, B->dev.out0, nint * B->M)) == 0 && (rc = dalloc(ctx, B->arena, B->dev.out1, n)) == 0 &&
+        (rc = launch_op(ctx, B, OP_DEFECT)) == 0) {
+        hipError_t e = hipMemcpy(errors, B->dev.out0, nint * B->M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(defect_norm, B->dev.out1, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "copy back: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+    }
+    delete B;
+    return rc;
+}
+
+int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const bvp_instance *inst, int32_t capacity,
+                         int32_t *new_N, int32_t *info, double *new_mesh, double *new_y) {
+    if (!ctx || !opts || !new_N || !info || !new_mesh || !new_y) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    bvp_opts o = *opts;
+    if (capacity < o.max_num_subintervals + 1) {
+        set_err(ctx, "capacity must be at least max_num_subintervals + 1");
+        return BVP_EINVAL;
+    }
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, o.method, OP_MESH_SELECT, n, inst, capacity, &o, &B);
+    if (rc) return rc;
+    const int M = B->M, cap = B->cap;
+    if ((rc = dalloc(ctx, B->arena, B->dev.out0, (size_t)n * cap)) == 0 &&
+        (rc = dalloc(ctx, B->arena, B->dev.out1, (size_t)n * cap * M)) == 0 &&
+        (rc = dalloc(ctx, B->arena, B->dev.iout, (size_t)n * 2)) == 0 && (rc = launch_op(ctx, B, OP_MESH_SELECT)) == 0) {
+        std::vector<int> io((size_t)n * 2);
+        std::vector<double> hm((size_t)n * cap), hy((size_t)n * cap * M);
+        hipError_t e = hipMemcpy(io.data(), B->dev.iout, io.size() * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hm.data(), B->dev.out0, hm.size() * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hy.data(), B->dev.out1, hy.size() * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "copy back: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+        for (int i = 0; i < n && rc == 0; ++i) {
+            new_N[i] = io[(size_t)i * 2];
+            info[i] = io[(size_t)i * 2 + 1];
+            if (info[i] == BVP_RET_SUCCESS) {
+                std::memcpy(new_mesh + (size_t)i * capacity, &hm[(size_t)i * cap], sizeof(double) * (new_N[i] + 1));
+                std::memcpy(new_y + (size_t)i * capacity * M, &hy[(size_t)i * cap * M],
+                            sizeof(double) * (size_t)M * (new_N[i] + 1));
+            }
+        }
+    }
+    delete B;
+    return rc;
+}
+
+}  // extern "C"

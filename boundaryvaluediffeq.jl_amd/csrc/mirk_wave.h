@@ -1,0 +1,70 @@
+// mirk_wave.h -- device descriptor of a batch and the launchers of the
+// wave-per-instance kernels (mirk_wave.hip), used by the C ABI (bvp_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+// operations of the component kernel
+enum {
+    OP_RESIDUAL = 0,
+    OP_JACOBIAN = 1,
+    OP_NEWTON_STEP = 2,
+    OP_DEFECT = 3,
+    OP_MESH_SELECT = 4,
+};
+
+// Everything a kernel needs about a batch; all pointers are device pointers.
+struct BatchDev {
+    int n;    // instances
+    int M;    // state dimension (shared by the batch)
+    int cap;  // node capacity per instance
+
+    // problem description, per instance
+    const int *family;
+    const double *t0, *t1;
+    const int *param_off, *param_cnt;
+    const double *params;
+    const int *bc_kind;
+    const double *bc_t;    // [n][M]
+    const int *bc_comp;    // [n][M]
+    const double *bc_val;  // [n][M]
+    const int *N0;
+    const long long *init_off;  // node offset of each instance in the packed inputs
+    const double *init_mesh;    // packed [sum (N0+1)]
+    const double *init_y;       // packed [sum (N0+1)][M]
+    const int *order;           // queue order (largest first)
+
+    // working state, per instance
+    double *mesh;  // [n][2][cap]   (double-buffered)
+    double *y;     // [n][2][cap][M]
+    double *shat;  // [n][cap]
+
+    // per-instance results
+    int *retcode, *N_final, *newton_iters, *outer_iters, *fallback, *cur_buf;
+    double *defect_norm, *resid_norm;
+    long long *cnt;  // [n][4]: interval-iterations, defect intervals, transfer nodes, check passes
+
+    // per-slot (persistent wave) scratch
+    double *scratch;
+    size_t scratch_per_slot;  // doubles
+    size_t tree_base;         // offset of the tree records inside a slot
+    double *aux;              // [slot][cap*(2M+1)]: delta, trial state, transfer indices
+    size_t aux_per_slot;
+    unsigned *counter;        // work-queue head
+
+    // component outputs
+    double *out0, *out1;
+    int *iout;
+
+    // options
+    int adaptive, max_sub, max_newton, linesearch, transfer_mode;
+    double abstol, newton_tol, defect_threshold;
+};
+
+hipError_t wave_reset_counter(unsigned *counter, hipStream_t s);
+hipError_t wave_launch_solve(const BatchDev &B, int method, int blocks, hipStream_t s);
+hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s);
+hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, double *out_mesh, double *out_y,
+                              int max_nodes, hipStream_t s);
+bool wave_supports(int M, int family);
+int wave_rel_fields(int M);

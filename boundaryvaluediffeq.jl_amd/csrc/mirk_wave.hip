@@ -1,0 +1,85 @@
+// mirk_wave.hip -- dispatch of the wave-per-instance kernels over (M, method);
+// the kernels themselves are instantiated in wave_m1..wave_m4.hip.
+#include "mirk_wave_impl.h"
+
+namespace bvpdev {
+#define DECL(m, me)                                                                              \
+    extern template hipError_t launch_solve_t<m, me>(const BatchDev &, int, hipStream_t);        \
+    extern template hipError_t launch_component_t<m, me>(const BatchDev &, int, int, hipStream_t); \
+    extern template int occupancy_t<m, me>();
+DECL(1, 4) DECL(1, 6) DECL(2, 4) DECL(2, 6) DECL(3, 4) DECL(3, 6) DECL(4, 4) DECL(4, 6)
+#undef DECL
+}  // namespace bvpdev
+using namespace bvpdev;
+
+__global__ void k_fill_u32(unsigned *p, unsigned v) { *p = v; }
+
+hipError_t wave_reset_counter(unsigned *counter, hipStream_t s) {
+    k_fill_u32<<<1, 1, 0, s>>>(counter, 0u);
+    return hipGetLastError();
+}
+
+#define DISPATCH_M_METHOD(M_, METHOD_, CALL)                                                  \
+    do {                                                                                       \
+        if ((M_) == 1 && (METHOD_) == 4) { CALL(1, 4); }                                       \
+        else if ((M_) == 1 && (METHOD_) == 6) { CALL(1, 6); }                                  \
+        else if ((M_) == 2 && (METHOD_) == 4) { CALL(2, 4); }                                  \
+        else if ((M_) == 2 && (METHOD_) == 6) { CALL(2, 6); }                                  \
+        else if ((M_) == 3 && (METHOD_) == 4) { CALL(3, 4); }                                  \
+        else if ((M_) == 3 && (METHOD_) == 6) { CALL(3, 6); }                                  \
+        else if ((M_) == 4 && (METHOD_) == 4) { CALL(4, 4); }                                  \
+        else if ((M_) == 4 && (METHOD_) == 6) { CALL(4, 6); }                                  \
+        else return hipErrorInvalidValue;                                                      \
+    } while (0)
+
+hipError_t wave_launch_solve(const BatchDev &B, int method, int blocks, hipStream_t s) {
+#define CALL_SOLVE(m, me) return launch_solve_t<m, me>(B, blocks, s)
+    DISPATCH_M_METHOD(B.M, method, CALL_SOLVE);
+#undef CALL_SOLVE
+}
+
+hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s) {
+#define CALL_COMP(m, me) return launch_component_t<m, me>(B, op, blocks, s)
+    DISPATCH_M_METHOD(B.M, method, CALL_COMP);
+#undef CALL_COMP
+}
+
+hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, double *out_mesh, double *out_y,
+                              int max_nodes, hipStream_t s) {
+    (void)max_nodes;
+    dim3 grid(B.n);
+    switch (B.M) {
+    case 1: k_gather<1><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    case 2: k_gather<2><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    case 3: k_gather<3><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    case 4: k_gather<4><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+bool wave_supports(int M, int family) {
+    switch (M) {
+    case 1: return family_supported<1>(family);
+    case 2: return family_supported<2>(family);
+    case 3: return family_supported<3>(family);
+    case 4: return family_supported<4>(family);
+    }
+    return false;
+}
+
+int wave_rel_fields(int M) { return 2 * M * M + M; }
+
+
+int wave_blocks_per_cu(int M, int method) {
+    const bool six = (method == 6);
+    switch (M) {
+    case 1: return six ? occupancy_t<1, 6>() : occupancy_t<1, 4>();
+    case 2: return six ? occupancy_t<2, 6>() : occupancy_t<2, 4>();
+    case 3: return six ? occupancy_t<3, 6>() : occupancy_t<3, 4>();
+    case 4: return six ? occupancy_t<4, 6>() : occupancy_t<4, 4>();
+    }
+    return 4;
+}
+
+bool blockm_supports(int, int) { return false; }

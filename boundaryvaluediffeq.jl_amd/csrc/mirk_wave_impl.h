@@ -1,0 +1,1365 @@
+// mirk_wave_impl.h -- MIRK collocation on gfx950, one 64-lane wavefront per BVP
+// instance, for small state dimensions (M <= 4).
+//
+// Layout of one instance's work in the wave:
+//   * the N mesh intervals are split into P = min(N, 64) contiguous chunks, one
+//     per lane;
+//   * forward pass: each lane walks its chunk once, computing for interval k the
+//     residual Phi_k and the Jacobian blocks S_k = dPhi_k/dy_k, R_k = dPhi_k/dy_{k+1}
+//     together with forward-mode dual numbers (2M directions), and eliminates
+//     the interior chunk nodes as it goes (stable block elimination with row
+//     pivoting on the 2M x M pivot column block).  What back-substitution needs
+//     is written to per-wave scratch in a lane-interleaved layout
+//     ([step][field][lane]: every store is one 512-byte coalesced line);
+//   * the P chunk relations "E dy_a + F dy_b = g" are merged pairwise up a
+//     binary tree across lanes (same elimination step), leaving one relation
+//     between the two end nodes; with the M boundary rows this is a 2M x 2M
+//     dense system solved with partial pivoting on lane 0;
+//   * corrections flow back down the tree and then backwards through each
+//     chunk, updating y in place.
+// The reference's counterparts (paths relative to the reference root,
+// MIRK/ = lib/BoundaryValueDiffEqMIRK/src/, Core/ = lib/BoundaryValueDiffEqCore/src/):
+//   residual Phi           MIRK/collocation.jl:44-102
+//   Jacobian blocks        MIRK/mirk.jl:697-862, 927-1002 (sparse ForwardDiff)
+//   linear solve           LinearSolve on the almost-banded matrix (Core/internal_problems.jl:107-169)
+//   Newton                 Core/default_internal_solve.jl:31-45 (NewtonRaphson, + backtracking)
+//   defect                 MIRK/adaptivity.jl:370-461, MIRK/interpolation.jl:300-372
+//   mesh selection         MIRK/adaptivity.jl:23-75, 250-304
+//   transfer, outer loop   MIRK/mirk.jl:286-388, MIRK/adaptivity.jl:6-13
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "mirk_device.h"
+#include "mirk_wave.h"
+#include "bvp_internal.h"
+
+namespace bvpdev {
+
+constexpr int WAVE = 64;
+constexpr int MAX_OUTER = 64;
+
+// ------------------------------------------------------------------ helpers
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline bool wave_any(bool p) { return __any(p ? 1 : 0) != 0; }
+
+// a[k] for a runtime k without dynamic register indexing (which would move the
+// array to scratch memory)
+template <int N, typename T>
+__device__ inline T sel(const T (&a)[N], int k) {
+    T r = a[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) r = (k == i) ? a[i] : r;
+    return r;
+}
+
+__device__ inline void chunk_of(int N, int lane, int &P, int &start, int &len) {
+    P = N < WAVE ? N : WAVE;
+    if (lane >= P) { start = N; len = 0; return; }
+    const int q = N / P, r = N % P;
+    len = q + (lane < r ? 1 : 0);
+    start = lane * q + (lane < r ? lane : r);
+}
+
+// interval(mesh, t) = clamp(searchsortedfirst(mesh, t) - 1, 1, N), 0-based (Core/utils.jl:119-121)
+__device__ inline int interval_of(const double *mesh, int N, double t) {
+    int lo = 0, hi = N + 1;  // first index with mesh[i] >= t, in [0, N+1]
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (mesh[mid] < t) lo = mid + 1; else hi = mid;
+    }
+    int iv = lo;  // 1-based interval = lo (searchsortedfirst - 1 with 1-based +1)
+    iv = iv < 1 ? 1 : (iv > N ? N : iv);
+    return iv - 1;
+}
+
+// ------------------------------------------------------------------ stages
+// K_r = f(Y_r, t + c_r h), Y_r = (1-v_r) yi + v_r yj + h sum_{j<r} x_rj K_j (MIRK/collocation.jl:60-66)
+template <int M, int METHOD, typename T>
+__device__ inline void stages(int fam, const Params<M> &P, double ti, double h, const T *yi, const T *yj,
+                              T (&K)[Tab<METHOD>::s][M]) {
+    using TB = Tab<METHOD>;
+#pragma unroll
+    for (int r = 0; r < TB::s; ++r) {
+        T Y[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            T a = (1.0 - TB::v(r)) * yi[c];
+            Y[c] = a + TB::v(r) * yj[c];
+        }
+        if (r > 0) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                T acc = ConstOf<T>::mk(0.0);
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < r; ++j) {
+                    if (TB::x(r, j) != 0.0) {
+                        if (!any) { acc = TB::x(r, j) * K[j][c]; any = true; }
+                        else acc = acc + TB::x(r, j) * K[j][c];
+                    }
+                }
+                if (any) Y[c] = Y[c] + h * acc;
+            }
+        }
+        f_eval<M, T>(fam, P, ti + TB::c(r) * h, Y, K[r]);
+    }
+}
+
+// Phi = (yj - yi) - h sum_r b_r K_r   (MIRK/collocation.jl:69-70)
+template <int M, int METHOD, typename T>
+__device__ inline void phi_of(double h, const T *yi, const T *yj, const T (&K)[Tab<METHOD>::s][M], T *phi) {
+    using TB = Tab<METHOD>;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        T acc = TB::b(0) * K[0][c];
+#pragma unroll
+        for (int r = 1; r < TB::s; ++r) acc = acc + TB::b(r) * K[r][c];
+        phi[c] = (yj[c] - yi[c]) - h * acc;
+    }
+}
+
+// extra interpolation stages k*_r (MIRK/interpolation.jl:337-372), primal
+template <int M, int METHOD>
+__device__ inline void interp_stages(int fam, const Params<M> &P, double ti, double h, const double *yi,
+                                     const double *yj, const double (&K)[Tab<METHOD>::s][M],
+                                     double (&KI)[Tab<METHOD>::sstar - Tab<METHOD>::s][M]) {
+    using TB = Tab<METHOD>;
+    constexpr int NS = TB::sstar - TB::s;
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        double Y[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < TB::s; ++j)
+                if (TB::xs(r, j) != 0.0) a += K[j][c] * TB::xs(r, j);
+#pragma unroll
+            for (int j = 0; j < r; ++j)
+                if (TB::xs(r, TB::s + j) != 0.0) a += KI[j][c] * TB::xs(r, TB::s + j);
+            Y[c] = a * h + (1.0 - TB::vs(r)) * yi[c] + TB::vs(r) * yj[c];
+        }
+        f_eval<M, double>(fam, P, ti + TB::cs(r) * h, Y, KI[r]);
+    }
+}
+
+// z = yi + h (K w + KI w*), z' = K w' + KI w'*   (MIRK/adaptivity.jl:623-666)
+template <int M, int METHOD>
+__device__ inline void sum_stages(double h, const double *yi, const double (&K)[Tab<METHOD>::s][M],
+                                  const double (&KI)[Tab<METHOD>::sstar - Tab<METHOD>::s][M], const double *w,
+                                  const double *wp, double *z, double *zp) {
+    using TB = Tab<METHOD>;
+    constexpr int NS = TB::sstar - TB::s;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        double a = 0.0, ap = 0.0;
+#pragma unroll
+        for (int r = 0; r < TB::s; ++r) { a += K[r][c] * w[r]; if (zp) ap += K[r][c] * wp[r]; }
+#pragma unroll
+        for (int r = 0; r < NS; ++r) { a += KI[r][c] * w[TB::s + r]; if (zp) ap += KI[r][c] * wp[TB::s + r]; }
+        z[c] = a * h + yi[c];
+        if (zp) zp[c] = ap;
+    }
+}
+
+// Residual and Jacobian blocks of one interval with dual numbers.
+template <int M, int METHOD>
+__device__ inline void interval_dual(int fam, const Params<M> &P, double ti, double h, const double *yi,
+                                     const double *yj, double *phi, double (&S)[M][M], double (&R)[M][M],
+                                     double (&Kp)[Tab<METHOD>::s][M]) {
+    using D = Dual<2 * M>;
+    D a[M], b[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        a[c] = dconst<2 * M>(yi[c]);
+        a[c].d[c] = 1.0;
+        b[c] = dconst<2 * M>(yj[c]);
+        b[c].d[M + c] = 1.0;
+    }
+    D K[Tab<METHOD>::s][M];
+    stages<M, METHOD, D>(fam, P, ti, h, a, b, K);
+    D ph[M];
+    phi_of<M, METHOD, D>(h, a, b, K, ph);
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        phi[i] = ph[i].v;
+#pragma unroll
+        for (int j = 0; j < M; ++j) { S[i][j] = ph[i].d[j]; R[i][j] = ph[i].d[M + j]; }
+    }
+#pragma unroll
+    for (int r = 0; r < Tab<METHOD>::s; ++r)
+#pragma unroll
+        for (int c = 0; c < M; ++c) Kp[r][c] = K[r][c].v;
+}
+
+// ------------------------------------------------------------------ elimination
+// Relation E dy_a + F dy_b = g between two nodes.
+template <int M> struct Rel {
+    double E[M][M], F[M][M], g[M];
+};
+// What back-substitution of one eliminated node needs: dy_b = h - X dy_a - Z dy_c.
+template <int M> struct Back {
+    double X[M][M], Z[M][M], h[M];
+};
+template <int M> constexpr int kRelFields = 2 * M * M + M;
+
+// Eliminate the shared node b of  L: E1 dy_a + F1 dy_b = g1  and  S dy_b + R dy_c = r
+// by Gaussian elimination with partial pivoting on the stacked 2M x M column
+// block [F1; S].  Produces the relation for (a, c) and the back-substitution
+// rows of b.
+template <int M>
+__device__ inline void eliminate(const Rel<M> &L, const double (&S)[M][M], const double (&R)[M][M],
+                                 const double *r, Rel<M> &out, Back<M> &bk) {
+    constexpr int M2 = 2 * M;
+    double A[M2][M], B1[M2][M], B2[M2][M], q[M2];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            A[i][j] = L.F[i][j]; B1[i][j] = L.E[i][j]; B2[i][j] = 0.0;
+            A[M + i][j] = S[i][j]; B1[M + i][j] = 0.0; B2[M + i][j] = R[i][j];
+        }
+        q[i] = L.g[i];
+        q[M + i] = r[i];
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        // pivot search over rows j..2M-1 (first maximum, as LAPACK's idamax)
+        int p = j;
+        double best = fabs(A[j][j]);
+#pragma unroll
+        for (int i = j + 1; i < M2; ++i) {
+            const double v = fabs(A[i][j]);
+            if (v > best) { best = v; p = i; }
+        }
+        // swap rows j and p by selects (keeps the arrays in registers)
+#pragma unroll
+        for (int i = j + 1; i < M2; ++i) {
+            const bool sw = (i == p);
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                double t;
+                t = A[j][c]; A[j][c] = sw ? A[i][c] : A[j][c]; A[i][c] = sw ? t : A[i][c];
+                t = B1[j][c]; B1[j][c] = sw ? B1[i][c] : B1[j][c]; B1[i][c] = sw ? t : B1[i][c];
+                t = B2[j][c]; B2[j][c] = sw ? B2[i][c] : B2[j][c]; B2[i][c] = sw ? t : B2[i][c];
+            }
+            double t = q[j];
+            q[j] = sw ? q[i] : q[j];
+            q[i] = sw ? t : q[i];
+        }
+        const double inv = 1.0 / A[j][j];
+#pragma unroll
+        for (int i = j + 1; i < M2; ++i) {
+            const double m = A[i][j] * inv;
+#pragma unroll
+            for (int c = j + 1; c < M; ++c) A[i][c] -= m * A[j][c];
+#pragma unroll
+            for (int c = 0; c < M; ++c) { B1[i][c] -= m * B1[j][c]; B2[i][c] -= m * B2[j][c]; }
+            q[i] -= m * q[j];
+        }
+    }
+    // back-substitution rows: solve U [X Z h] = [B1 B2 q] (top M rows)
+#pragma unroll
+    for (int i = M - 1; i >= 0; --i) {
+        const double inv = 1.0 / A[i][i];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            double x = B1[i][c], z = B2[i][c];
+#pragma unroll
+            for (int k = i + 1; k < M; ++k) { x -= A[i][k] * bk.X[k][c]; z -= A[i][k] * bk.Z[k][c]; }
+            bk.X[i][c] = x * inv;
+            bk.Z[i][c] = z * inv;
+        }
+        double hh = q[i];
+#pragma unroll
+        for (int k = i + 1; k < M; ++k) hh -= A[i][k] * bk.h[k];
+        bk.h[i] = hh * inv;
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) { out.E[i][c] = B1[M + i][c]; out.F[i][c] = B2[M + i][c]; }
+        out.g[i] = q[M + i];
+    }
+}
+
+template <int M>
+__device__ inline void back_apply(const Back<M> &bk, const double *ya, const double *yc, double *yb) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double v = bk.h[i];
+#pragma unroll
+        for (int c = 0; c < M; ++c) v -= bk.X[i][c] * ya[c] + bk.Z[i][c] * yc[c];
+        yb[i] = v;
+    }
+}
+
+// scratch I/O of Back records; lane-interleaved: [slot][field][64]
+// STRIDE = WAVE: chunk records of all lanes interleaved; STRIDE = 1: compact tree records.
+template <int M, int STRIDE = WAVE>
+__device__ inline void store_back(double *base, int slot, int lane, const Back<M> &bk) {
+    constexpr int NF = kRelFields<M>;
+    double *p = base + (size_t)slot * NF * STRIDE + lane;
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) p[(f++) * STRIDE] = bk.X[i][c];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) p[(f++) * STRIDE] = bk.Z[i][c];
+#pragma unroll
+    for (int i = 0; i < M; ++i) p[(f++) * STRIDE] = bk.h[i];
+}
+template <int M, int STRIDE = WAVE>
+__device__ inline void load_back(const double *base, int slot, int lane, Back<M> &bk) {
+    constexpr int NF = kRelFields<M>;
+    const double *p = base + (size_t)slot * NF * STRIDE + lane;
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) bk.X[i][c] = p[(f++) * STRIDE];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) bk.Z[i][c] = p[(f++) * STRIDE];
+#pragma unroll
+    for (int i = 0; i < M; ++i) bk.h[i] = p[(f++) * STRIDE];
+}
+
+// tree records live after the chunk records: index of the merge of lane l at level d
+__device__ inline int tree_index(int d, int l) { return (WAVE - (WAVE >> d)) + (l >> (d + 1)); }
+
+// Dense n x n solve with partial pivoting (n = 2M), single lane.
+template <int NN>
+__device__ inline bool dense_solve(double (&A)[NN][NN], double (&b)[NN]) {
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+        int p = j;
+        double best = fabs(A[j][j]);
+#pragma unroll
+        for (int i = j + 1; i < NN; ++i) if (fabs(A[i][j]) > best) { best = fabs(A[i][j]); p = i; }
+#pragma unroll
+        for (int i = j + 1; i < NN; ++i) {
+            const bool sw = (i == p);
+#pragma unroll
+            for (int c = 0; c < NN; ++c) {
+                const double t = A[j][c];
+                A[j][c] = sw ? A[i][c] : A[j][c];
+                A[i][c] = sw ? t : A[i][c];
+            }
+            const double t = b[j];
+            b[j] = sw ? b[i] : b[j];
+            b[i] = sw ? t : b[i];
+        }
+        if (A[j][j] == 0.0) return false;
+        const double inv = 1.0 / A[j][j];
+#pragma unroll
+        for (int i = j + 1; i < NN; ++i) {
+            const double m = A[i][j] * inv;
+#pragma unroll
+            for (int c = j + 1; c < NN; ++c) A[i][c] -= m * A[j][c];
+            b[i] -= m * b[j];
+        }
+    }
+#pragma unroll
+    for (int i = NN - 1; i >= 0; --i) {
+        double v = b[i];
+#pragma unroll
+        for (int c = i + 1; c < NN; ++c) v -= A[i][c] * b[c];
+        b[i] = v / A[i][i];
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ instance view
+template <int M> struct Inst {
+    int fam;
+    Params<M> P;
+    int bc_kind;
+    double bct[M], bcv[M];
+    int bcc[M];
+    double t0, t1;
+};
+
+template <int M>
+__device__ inline void load_inst(const BatchDev &B, int b, Inst<M> &I) {
+    I.fam = B.family[b];
+    const int po = B.param_off[b], pn = B.param_cnt[b];
+#pragma unroll
+    for (int k = 0; k < Params<M>::MAXP; ++k) I.P.p[k] = k < pn ? B.params[po + k] : 0.0;
+    I.bc_kind = B.bc_kind[b];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        I.bct[k] = B.bc_t[(size_t)b * M + k];
+        I.bcv[k] = B.bc_val[(size_t)b * M + k];
+        I.bcc[k] = B.bc_comp[(size_t)b * M + k];
+    }
+    I.t0 = B.t0[b];
+    I.t1 = B.t1[b];
+}
+
+// BC row resolution on the current mesh (Core/utils.jl:157-209; EvalSol
+// MIRK/interpolation.jl:214-231): node carrying the Jacobian entry, and for an
+// interior STANDARD row the interval and tau of the continuous extension.
+struct BcLoc {
+    int node;
+    int iv;       // -1 when the value is read straight from a node
+    double tau;
+};
+
+template <int M>
+__device__ inline BcLoc bc_locate(const Inst<M> &I, int k, const double *mesh, int N) {
+    BcLoc L;
+    const double t = sel<M>(I.bct, k);
+    L.iv = -1;
+    L.tau = 0.0;
+    if (I.bc_kind == BVP_BC_TWO_POINT) {
+        L.node = (t == I.t0) ? 0 : N;
+        return L;
+    }
+    if (t == mesh[0]) { L.node = 0; return L; }
+    if (t == mesh[N]) { L.node = N; return L; }
+    const int iv = interval_of(mesh, N, t);
+    L.node = iv;
+    L.iv = iv;
+    L.tau = (t - mesh[iv]) / (mesh[iv + 1] - mesh[iv]);
+    return L;
+}
+
+// ------------------------------------------------------------------ LDS layout
+template <int M> struct Shared {
+    double rel[WAVE][kRelFields<M>];  // relation exchange for the tree
+    double lo[WAVE][M], hi[WAVE][M];  // corrections at chunk ends
+    double bcres[M];
+    int bcnode[M];
+    double root[2 * M];
+    int flag;
+    int item;
+};
+
+template <int M>
+__device__ inline void rel_to_lds(Shared<M> &sh, int lane, const Rel<M> &R) {
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) sh.rel[lane][f++] = R.E[i][c];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) sh.rel[lane][f++] = R.F[i][c];
+#pragma unroll
+    for (int i = 0; i < M; ++i) sh.rel[lane][f++] = R.g[i];
+}
+template <int M>
+__device__ inline void rel_from_lds(const Shared<M> &sh, int lane, Rel<M> &R) {
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) R.E[i][c] = sh.rel[lane][f++];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) R.F[i][c] = sh.rel[lane][f++];
+#pragma unroll
+    for (int i = 0; i < M; ++i) R.g[i] = sh.rel[lane][f++];
+}
+
+// ------------------------------------------------------------------ residual pieces
+// BC residual of row k at state y (primal).  For an interior STANDARD row the
+// stages of interval iv are recomputed here.
+template <int M, int METHOD>
+__device__ inline double bc_residual(const Inst<M> &I, int k, const BcLoc &L, const double *mesh, const double *y) {
+    const int comp = sel<M>(I.bcc, k);
+    const double val = sel<M>(I.bcv, k);
+    if (L.iv < 0) return y[(size_t)L.node * M + comp] - val;
+    using TB = Tab<METHOD>;
+    const int i = L.iv;
+    const double ti = mesh[i], h = mesh[i + 1] - mesh[i];
+    double yi[M], yj[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)i * M + c]; yj[c] = y[(size_t)(i + 1) * M + c]; }
+    double K[TB::s][M], KI[TB::sstar - TB::s][M];
+    stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
+    interp_stages<M, METHOD>(I.fam, I.P, ti, h, yi, yj, K, KI);
+    double w[TB::sstar], z[M];
+    interp_weights<METHOD>(L.tau, w, nullptr);
+    sum_stages<M, METHOD>(h, yi, K, KI, w, nullptr, z, nullptr);
+    return sel<M>(z, comp) - val;
+}
+
+// ------------------------------------------------------------------ Newton step
+// One forward pass at y: residual, blocks, chunk elimination.  Returns the
+// lane's max |Phi| (NaN-aware via `bad`), its chunk relation.
+template <int M, int METHOD>
+__device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
+                                    double *scr, Rel<M> &rel, double &mx, bool &bad, double *phi_out,
+                                    double *S_out, double *R_out) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    mx = 0.0;
+    bad = false;
+    if (len == 0) return;
+    double yi[M], yj[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) yi[c] = y[(size_t)start * M + c];
+    double ti = mesh[start];
+    for (int k = start; k < start + len; ++k) {
+        const double tj = mesh[k + 1];
+#pragma unroll
+        for (int c = 0; c < M; ++c) yj[c] = y[(size_t)(k + 1) * M + c];
+        const double h = tj - ti;
+        double phi[M], S[M][M], R[M][M], Kp[Tab<METHOD>::s][M];
+        interval_dual<M, METHOD>(I.fam, I.P, ti, h, yi, yj, phi, S, R, Kp);
+        double r[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            r[c] = -phi[c];
+            const double a = fabs(phi[c]);
+            bad |= !(a <= 1.7976931348623157e308);
+            mx = fmax(mx, a);
+        }
+        if (phi_out) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) phi_out[(size_t)k * M + c] = phi[c];
+        }
+        if (S_out) {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    S_out[(size_t)k * M * M + i * M + j] = S[i][j];
+                    R_out[(size_t)k * M * M + i * M + j] = R[i][j];
+                }
+        }
+        if (scr) {
+            if (k == start) {
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+#pragma unroll
+                    for (int j = 0; j < M; ++j) { rel.E[i][j] = S[i][j]; rel.F[i][j] = R[i][j]; }
+                    rel.g[i] = r[i];
+                }
+            } else {
+                Rel<M> nr;
+                Back<M> bk;
+                eliminate<M>(rel, S, R, r, nr, bk);
+                store_back<M>(scr, k - start - 1, lane, bk);
+                rel = nr;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < M; ++c) yi[c] = yj[c];
+        ti = tj;
+    }
+}
+
+// Tree up-sweep, root solve (lane 0), tree down-sweep and chunk back-substitution.
+// Writes the correction into `delta` (point-major) when non-null and, when
+// `apply` is set, adds it to y.
+template <int M, int METHOD>
+__device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y, int N,
+                                        int lane, double *scr, size_t tree_base, Rel<M> &rel, double *delta,
+                                        bool apply) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    int D = 0;
+    while ((1 << D) < P) ++D;
+    double *tree = scr + tree_base;
+    // ---- up-sweep
+    if (lane < P) rel_to_lds<M>(sh, lane, rel);
+    __syncthreads();
+    for (int d = 0; d < D; ++d) {
+        const int span = 1 << d;
+        if ((lane & ((span << 1) - 1)) == 0 && lane + span < P) {
+            Rel<M> right, nr;
+            rel_from_lds<M>(sh, lane + span, right);
+            Back<M> bk;
+            eliminate<M>(rel, right.E, right.F, right.g, nr, bk);
+            store_back<M, 1>(tree, tree_index(d, lane), 0, bk);
+            rel = nr;
+            rel_to_lds<M>(sh, lane, rel);
+        }
+        __syncthreads();
+    }
+    // ---- root: 2M x 2M system of the boundary rows and the end-to-end relation
+    if (lane == 0) {
+        constexpr int NN = 2 * M;
+        double A[NN][NN], b[NN];
+#pragma unroll
+        for (int i = 0; i < NN; ++i) {
+#pragma unroll
+            for (int c = 0; c < NN; ++c) A[i][c] = 0.0;
+            b[i] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int node = sh.bcnode[k];
+            const int comp = sel<M>(I.bcc, k);
+            b[k] = -sh.bcres[k];
+            if (node == 0 || node == N) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) {
+                    A[k][c] = (node == 0 && c == comp) ? 1.0 : 0.0;
+                    A[k][M + c] = (node == N && c == comp) ? 1.0 : 0.0;
+                }
+            } else {
+                // Interior node: express dy_node affinely in (dy_0, dy_N) by walking the
+                // elimination records from the root to the node (columns: dy_0 | dy_N | 1).
+                constexpr int W = 2 * M + 1;
+                double mlo[M][W], mhi[M][W];
+#pragma unroll
+                for (int i = 0; i < M; ++i)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) {
+                        mlo[i][c] = (c == i) ? 1.0 : 0.0;
+                        mhi[i][c] = (c == M + i) ? 1.0 : 0.0;
+                    }
+                int l = 0;
+                bool found = false;
+                double mt[M][W];
+                for (int d = D - 1; d >= 0 && !found; --d) {
+                    const int span = 1 << d;
+                    if (l + span >= P) continue;
+                    int ps, plen, pP;
+                    chunk_of(N, l + span, pP, ps, plen);
+                    Back<M> bk;
+                    load_back<M, 1>(tree, tree_index(d, l), 0, bk);
+                    double mb[M][W];
+#pragma unroll
+                    for (int i = 0; i < M; ++i)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) {
+                            double v = (c == 2 * M) ? bk.h[i] : 0.0;
+#pragma unroll
+                            for (int q = 0; q < M; ++q) v -= bk.X[i][q] * mlo[q][c] + bk.Z[i][q] * mhi[q][c];
+                            mb[i][c] = v;
+                        }
+                    if (node == ps) {
+#pragma unroll
+                        for (int i = 0; i < M; ++i)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) mt[i][c] = mb[i][c];
+                        found = true;
+                    } else if (node < ps) {
+#pragma unroll
+                        for (int i = 0; i < M; ++i)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) mhi[i][c] = mb[i][c];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < M; ++i)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) mlo[i][c] = mb[i][c];
+                        l += span;
+                    }
+                }
+                if (!found) {
+                    int ls, llen, lP;
+                    chunk_of(N, l, lP, ls, llen);
+                    if (node == ls) {
+#pragma unroll
+                        for (int i = 0; i < M; ++i)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) mt[i][c] = mlo[i][c];
+                    } else {
+                        // backwards through the chunk of lane l, from its end to `node`
+                        double mn[M][W];
+#pragma unroll
+                        for (int i = 0; i < M; ++i)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) mn[i][c] = mhi[i][c];
+                        for (int kk = ls + llen - 1; kk >= node; --kk) {
+                            Back<M> bk;
+                            load_back<M>(scr, kk - ls - 1, l, bk);
+                            double mk[M][W];
+#pragma unroll
+                            for (int i = 0; i < M; ++i)
+#pragma unroll
+                                for (int c = 0; c < W; ++c) {
+                                    double v = (c == 2 * M) ? bk.h[i] : 0.0;
+#pragma unroll
+                                    for (int q = 0; q < M; ++q) v -= bk.X[i][q] * mlo[q][c] + bk.Z[i][q] * mn[q][c];
+                                    mk[i][c] = v;
+                                }
+#pragma unroll
+                            for (int i = 0; i < M; ++i)
+#pragma unroll
+                                for (int c = 0; c < W; ++c) mn[i][c] = mk[i][c];
+                        }
+#pragma unroll
+                        for (int i = 0; i < M; ++i)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) mt[i][c] = mn[i][c];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    if (i == comp) {
+#pragma unroll
+                        for (int c = 0; c < 2 * M; ++c) A[k][c] = mt[i][c];
+                        b[k] -= mt[i][2 * M];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) { A[M + i][c] = rel.E[i][c]; A[M + i][M + c] = rel.F[i][c]; }
+            b[M + i] = rel.g[i];
+        }
+        const bool ok = dense_solve<NN>(A, b);
+        sh.flag = ok ? 1 : 0;
+#pragma unroll
+        for (int c = 0; c < M; ++c) { sh.lo[0][c] = b[c]; sh.hi[0][c] = b[M + c]; }
+    }
+    __syncthreads();
+    if (!sh.flag) return false;
+    // ---- down-sweep
+    for (int d = D - 1; d >= 0; --d) {
+        const int span = 1 << d;
+        if ((lane & ((span << 1) - 1)) == 0 && lane + span < P) {
+            Back<M> bk;
+            load_back<M, 1>(tree, tree_index(d, lane), 0, bk);
+            double ya[M], yc[M], yb[M];
+#pragma unroll
+            for (int c = 0; c < M; ++c) { ya[c] = sh.lo[lane][c]; yc[c] = sh.hi[lane][c]; }
+            back_apply<M>(bk, ya, yc, yb);
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                sh.lo[lane + span][c] = yb[c];
+                sh.hi[lane + span][c] = yc[c];
+                sh.hi[lane][c] = yb[c];
+            }
+        }
+        __syncthreads();
+    }
+    // ---- chunk back-substitution
+    if (len > 0) {
+        double ya[M], yn[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { ya[c] = sh.lo[lane][c]; yn[c] = sh.hi[lane][c]; }
+        if (lane == P - 1) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                if (delta) delta[(size_t)N * M + c] = yn[c];
+                if (apply) y[(size_t)N * M + c] += yn[c];
+            }
+        }
+        for (int k = start + len - 1; k > start; --k) {
+            Back<M> bk;
+            load_back<M>(scr, k - start - 1, lane, bk);
+            double yk[M];
+            back_apply<M>(bk, ya, yn, yk);
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                if (delta) delta[(size_t)k * M + c] = yk[c];
+                if (apply) y[(size_t)k * M + c] += yk[c];
+                yn[c] = yk[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            if (delta) delta[(size_t)start * M + c] = ya[c];
+            if (apply) y[(size_t)start * M + c] += ya[c];
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// Residual evaluation: forward pass + BC rows.  Leaves BC residuals and their
+// nodes in LDS; returns max|F| (NaN if not finite) in all lanes.
+template <int M, int METHOD>
+__device__ inline double residual_and_blocks(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y,
+                                             int N, int lane, double *scr, Rel<M> &rel) {
+    double mx;
+    bool bad;
+    forward_dual<M, METHOD>(I, mesh, y, N, lane, scr, rel, mx, bad, nullptr, nullptr, nullptr);
+    if (lane < M) {
+        const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+        sh.bcnode[lane] = L.node;
+        sh.bcres[lane] = bc_residual<M, METHOD>(I, lane, L, mesh, y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double a = fabs(sh.bcres[k]);
+        bad |= !(a <= 1.7976931348623157e308);
+        mx = fmax(mx, a);
+    }
+    const bool anybad = wave_any(bad);
+    mx = wave_max(mx);
+    __syncthreads();
+    return anybad ? NAN : mx;
+}
+
+// Primal residual at y + alpha*delta: returns sum of squares (NaN if not finite).
+template <int M, int METHOD>
+__device__ inline double residual_sq_trial(const Inst<M> &I, const double *mesh, const double *y,
+                                           const double *delta, double alpha, int N, int lane, double *ytmp) {
+    // trial state into ytmp (same layout as y)
+    for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+        for (int c = 0; c < M; ++c) ytmp[(size_t)k * M + c] = y[(size_t)k * M + c] + alpha * delta[(size_t)k * M + c];
+    __syncthreads();
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double acc = 0.0;
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], yj[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { yi[c] = ytmp[(size_t)k * M + c]; yj[c] = ytmp[(size_t)(k + 1) * M + c]; }
+        double K[Tab<METHOD>::s][M], ph[M];
+        stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
+        phi_of<M, METHOD, double>(h, yi, yj, K, ph);
+#pragma unroll
+        for (int c = 0; c < M; ++c) acc += ph[c] * ph[c];
+    }
+    if (lane < M) {
+        const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+        const double r = bc_residual<M, METHOD>(I, lane, L, mesh, ytmp);
+        acc += r * r;
+    }
+    const bool bad = wave_any(!(acc <= 1.7976931348623157e308));
+    acc = wave_sum(acc);
+    __syncthreads();
+    return bad ? NAN : acc;
+}
+
+struct NewtonOut {
+    int ret, iters, fallback;
+};
+
+// NewtonRaphson, then NewtonRaphson with backtracking from the same start
+// (Core/default_internal_solve.jl:37-43).  Stop when max|F| <= tol, checked
+// before each step.
+template <int M, int METHOD>
+__device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y,
+                            double *ybak, double *delta, double *ytmp, int N, int lane, double *scr,
+                            size_t tree_base) {
+    NewtonOut out{BVP_RET_MAXITERS, 0, 0};
+    const int attempts = B.linesearch ? 2 : 1;
+    if (B.linesearch) {
+        for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+            for (int c = 0; c < M; ++c) ybak[(size_t)k * M + c] = y[(size_t)k * M + c];
+        __syncthreads();
+    }
+    for (int attempt = 0; attempt < attempts; ++attempt) {
+        if (attempt == 1) {
+            for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+                for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = ybak[(size_t)k * M + c];
+            __syncthreads();
+        }
+        int ret = BVP_RET_MAXITERS;
+        for (int it = 0;; ++it) {
+            Rel<M> rel;
+            const double fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+            if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
+            if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
+            if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
+            const bool plain = (attempt == 0);
+            if (!solve_correction<M, METHOD>(I, sh, mesh, y, N, lane, scr, tree_base, rel, plain ? nullptr : delta,
+                                             plain)) {
+                ret = BVP_RET_FAILURE;
+                break;
+            }
+            out.iters++;
+            if (!plain) {
+                // backtracking: halve alpha until 0.5||F||^2 decreases by the Armijo factor
+                double phi0 = 0.0;
+                {
+                    // recompute ||F(y)||_2^2 with the same primal code as the trials
+                    phi0 = residual_sq_trial<M, METHOD>(I, mesh, y, delta, 0.0, N, lane, ytmp);
+                }
+                double alpha = 1.0;
+                for (int ls = 0; ls < 12; ++ls) {
+                    const double ph = residual_sq_trial<M, METHOD>(I, mesh, y, delta, alpha, N, lane, ytmp);
+                    if (ph <= (1.0 - 2e-4 * alpha) * phi0) break;
+                    alpha *= 0.5;
+                }
+                for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) y[(size_t)k * M + c] += alpha * delta[(size_t)k * M + c];
+                __syncthreads();
+            }
+        }
+        out.ret = ret;
+        out.fallback = attempt;
+        if (ret == BVP_RET_SUCCESS) break;
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------ defect
+// error_estimate!(::DefectControl): per interval the defect vector with the
+// larger max-abs of the two sample points; returns defect norm (NaN-aware) and
+// fills shat (= (max|d_i|/abstol)^(1/(p+1)), MIRK/adaptivity.jl:39-40).
+template <int M, int METHOD>
+__device__ inline double defect_pass(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
+                                     double abstol, double *shat, double *errors, double &r1, double &r2) {
+    using TB = Tab<METHOD>;
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double w1[TB::sstar], w1p[TB::sstar], w2[TB::sstar], w2p[TB::sstar];
+    interp_weights<METHOD>(TB::tau(), w1, w1p);
+    interp_weights<METHOD>(1.0 - TB::tau(), w2, w2p);
+    double dmax = 0.0, smax = 0.0, ssum = 0.0;
+    bool bad = false;
+    const double ex = 1.0 / (TB::order + 1);
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], yj[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
+        double K[TB::s][M], KI[TB::sstar - TB::s][M];
+        stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
+        interp_stages<M, METHOD>(I.fam, I.P, ti, h, yi, yj, K, KI);
+        double z[M], zp[M], f1[M], f2[M], d1[M], d2[M];
+        sum_stages<M, METHOD>(h, yi, K, KI, w1, w1p, z, zp);
+        f_eval<M, double>(I.fam, I.P, ti + TB::tau() * h, z, f1);
+        double e1 = 0.0, e2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < M; ++c) { d1[c] = (zp[c] - f1[c]) / (fabs(f1[c]) + 1.0); e1 = fmax(e1, fabs(d1[c])); }
+        sum_stages<M, METHOD>(h, yi, K, KI, w2, w2p, z, zp);
+        f_eval<M, double>(I.fam, I.P, ti + (1.0 - TB::tau()) * h, z, f2);
+#pragma unroll
+        for (int c = 0; c < M; ++c) { d2[c] = (zp[c] - f2[c]) / (fabs(f2[c]) + 1.0); e2 = fmax(e2, fabs(d2[c])); }
+        const bool first = e1 > e2;
+        double ei = 0.0;
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const double d = first ? d1[c] : d2[c];
+            if (errors) errors[(size_t)k * M + c] = d;
+            bad |= !(fabs(d) <= 1.7976931348623157e308);
+            ei = fmax(ei, fabs(d));
+        }
+        dmax = fmax(dmax, ei);
+        const double sh = pow(ei / abstol, ex);
+        if (shat) shat[k] = sh;
+        smax = fmax(smax, sh);
+        ssum += sh;
+    }
+    const bool anybad = wave_any(bad);
+    dmax = wave_max(dmax);
+    r1 = wave_max(smax);
+    r2 = wave_sum(ssum);
+    return anybad ? NAN : dmax;
+}
+
+// ------------------------------------------------------------------ mesh selection
+// Julia round(Int, x): nearest, ties to even.
+__device__ inline long long round_even(double x) { return (long long)rint(x); }
+
+// half_mesh! (MIRK/adaptivity.jl:287-304) into `out`
+__device__ inline void half_mesh_into(const double *mesh, int N, double *out, int lane) {
+    for (int i = lane; i <= N; i += WAVE) out[2 * i] = mesh[i];
+    for (int i = lane; i < N; i += WAVE) out[2 * i + 1] = (mesh[i + 1] + mesh[i]) / 2.0;
+}
+
+// mesh_selector!(::DefectControl) + redistribute! (MIRK/adaptivity.jl:23-75, 250-278).
+// shat holds (max|d|/abstol)^(1/(p+1)) per interval on entry (it is divided by h
+// in place).  Writes the new mesh into new_mesh and returns the new N, or -1 on
+// failure (cap exceeded).
+template <int M>
+__device__ inline int mesh_select(const double *mesh, int N, double *shat, double r1, double r2, int max_sub,
+                                  double *new_mesh, int lane, Shared<M> &sh) {
+    const int n = N;
+    const double r3 = r2 / (double)n;
+    long long npred = round_even(1.3 * r2 + 1.0);
+    const double n_ = 0.1 * n;
+    if (fabs((double)(npred - n)) < n_) npred = round_even((double)n + n_);
+    if (r1 <= 1.0 * r3) {
+        const long long Ns = 2LL * n;
+        if (Ns > max_sub) return -1;
+        half_mesh_into(mesh, N, new_mesh, lane);
+        __syncthreads();
+        return (int)Ns;
+    }
+    const long long lb = (n + 1) / 2, ub = 4LL * n;
+    const long long Ns = npred < lb ? lb : (npred > ub ? ub : npred);
+    if (Ns > max_sub) return -1;
+    // s_hat ./= h, then zeta = sum(s_hat .* h) / Ns
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double acc = 0.0;
+    for (int k = start; k < start + len; ++k) {
+        const double h = mesh[k + 1] - mesh[k];
+        const double s = shat[k] / h;
+        shat[k] = s;
+        acc += s * h;
+    }
+    const double zeta = wave_sum(acc) / (double)Ns;
+    __syncthreads();
+    if (lane == 0) {
+        // sequential equidistribution walk (MIRK/adaptivity.jl:254-273)
+        new_mesh[0] = mesh[0];
+        double t = mesh[0], integral = 0.0;
+        int k = 0;
+        long long i = 0;
+        int ok = 1;
+        while (k < n) {
+            const double next_piece = shat[k] * (mesh[k + 1] - t);
+            const double int_next = integral + next_piece;
+            if (int_next > zeta) {
+                const double tn = (zeta - integral) / shat[k] + t;
+                if (i + 1 > Ns) { ok = 0; break; }
+                new_mesh[i + 1] = tn;
+                t = tn;
+                ++i;
+                integral = 0.0;
+            } else {
+                integral = int_next;
+                t = mesh[k + 1];
+                ++k;
+            }
+        }
+        new_mesh[Ns] = mesh[n];
+        sh.flag = ok;
+    }
+    __syncthreads();
+    return sh.flag ? (int)Ns : -1;
+}
+
+// Solution transfer from the old continuous extension to the new nodes
+// (MIRK/mirk.jl:366-370; MIRK/adaptivity.jl:6-13, 606-621).  REFERENCE mode
+// reproduces the reference's in-place update: the base value of node j is the
+// already-written new value when the old interval index ii < j.
+template <int M, int METHOD>
+__device__ inline void transfer(const Inst<M> &I, const double *mesh, const double *y, int N, const double *nm,
+                                double *ny, int Nn, int mode, int lane, double *iiscr) {
+    using TB = Tab<METHOD>;
+    for (int j = lane; j <= Nn; j += WAVE) {
+        const double m = nm[j];
+        const int ii = interval_of(mesh, N, m);
+        const double ti = mesh[ii], dt = mesh[ii + 1] - ti;
+        const double tau = (m - ti) / dt;
+        double yi[M], yj[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)ii * M + c]; yj[c] = y[(size_t)(ii + 1) * M + c]; }
+        double K[TB::s][M], KI[TB::sstar - TB::s][M];
+        stages<M, METHOD, double>(I.fam, I.P, ti, dt, yi, yj, K);
+        interp_stages<M, METHOD>(I.fam, I.P, ti, dt, yi, yj, K, KI);
+        double w[TB::sstar];
+        interp_weights<METHOD>(tau, w, nullptr);
+        if (mode == BVP_TRANSFER_CLEAN || ii >= j) {
+            double z[M];
+            sum_stages<M, METHOD>(dt, yi, K, KI, w, nullptr, z, nullptr);
+#pragma unroll
+            for (int c = 0; c < M; ++c) ny[(size_t)j * M + c] = z[c];
+            iiscr[j] = -1.0;
+        } else {
+            double zero[M], z[M];
+#pragma unroll
+            for (int c = 0; c < M; ++c) zero[c] = 0.0;
+            sum_stages<M, METHOD>(dt, zero, K, KI, w, nullptr, z, nullptr);
+#pragma unroll
+            for (int c = 0; c < M; ++c) ny[(size_t)j * M + c] = z[c];  // increment only
+            iiscr[j] = (double)ii;
+        }
+    }
+    __syncthreads();
+    if (mode == BVP_TRANSFER_REFERENCE && lane == 0) {
+        for (int j = 0; j <= Nn; ++j) {
+            const double iid = iiscr[j];
+            if (iid >= 0.0) {
+                const int ii = (int)iid;
+#pragma unroll
+                for (int c = 0; c < M; ++c) ny[(size_t)j * M + c] += ny[(size_t)ii * M + c];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ instance buffers
+template <int M> struct Bufs {
+    double *mesh[2];
+    double *y[2];
+};
+
+template <int M>
+__device__ inline Bufs<M> bufs_of(const BatchDev &B, int b) {
+    Bufs<M> r;
+    const size_t cap = (size_t)B.cap;
+    r.mesh[0] = B.mesh + (size_t)b * 2 * cap;
+    r.mesh[1] = r.mesh[0] + cap;
+    r.y[0] = B.y + (size_t)b * 2 * cap * M;
+    r.y[1] = r.y[0] + cap * M;
+    return r;
+}
+
+// ------------------------------------------------------------------ full solve of one instance
+template <int M, int METHOD>
+__device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<M> &sh, int lane, double *scr, size_t tree_base,
+                               double *aux) {
+    Inst<M> I;
+    load_inst<M>(B, b, I);
+    Bufs<M> bf = bufs_of<M>(B, b);
+    int cur = 0;
+    int N = B.N0[b];
+    // initial mesh and guess from the packed inputs
+    {
+        const long long off = B.init_off[b];
+        for (int k = lane; k <= N; k += WAVE) {
+            bf.mesh[0][k] = B.init_mesh[off + k];
+#pragma unroll
+            for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = B.init_y[(off + k) * M + c];
+        }
+        __syncthreads();
+    }
+    double *shat = B.shat + (size_t)b * B.cap;
+    double *delta = aux;                               // [cap*M]
+    double *ytmp = aux + (size_t)B.cap * M;            // [cap*M]
+    double *iiscr = aux + (size_t)2 * B.cap * M;       // [cap]
+    int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
+    int outer = 0, iters = 0, fb = 0;
+    double last_defect = NAN;
+    long long cnt_it = 0, cnt_def = 0, cnt_tr = 0, cnt_chk = 0;
+    for (;;) {
+        ++outer;
+        double *mesh = bf.mesh[cur], *y = bf.y[cur];
+        double *mesh_o = bf.mesh[1 - cur], *y_o = bf.y[1 - cur];
+        NewtonOut nl = newton<M, METHOD>(B, I, sh, mesh, y, y_o, delta, ytmp, N, lane, scr, tree_base);
+        iters += nl.iters;
+        cnt_it += (long long)N * nl.iters;
+        cnt_chk += N;
+        if (nl.fallback) fb |= 1;
+        nl_ret = nl.ret;
+        double error_norm = 2.0 * B.abstol;
+        if (!B.adaptive) { info = nl.ret; break; }
+        info = nl.ret;
+        double r1 = 0.0, r2 = 0.0;
+        if (info == BVP_RET_SUCCESS) {
+            error_norm = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2);
+            cnt_def += N;
+            last_defect = error_norm;
+            info = (error_norm > B.defect_threshold || isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
+        }
+        __syncthreads();
+        if (info == BVP_RET_SUCCESS) {
+            if (error_norm > B.abstol) {
+                const int Nn = mesh_select<M>(mesh, N, shat, r1, r2, B.max_sub, mesh_o, lane, sh);
+                if (Nn < 0) {
+                    info = BVP_RET_FAILURE;
+                } else {
+                    transfer<M, METHOD>(I, mesh, y, N, mesh_o, y_o, Nn, B.transfer_mode, lane, iiscr);
+                    cnt_tr += N + Nn;
+                    N = Nn;
+                    cur = 1 - cur;
+                }
+            }
+        } else {
+            if (2 * N > B.max_sub) {
+                info = BVP_RET_FAILURE;
+            } else {
+                half_mesh_into(mesh, N, mesh_o, lane);
+                for (int k = lane; k <= 2 * N; k += WAVE)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) y_o[(size_t)k * M + c] = 0.0;
+                __syncthreads();
+                N = 2 * N;
+                cur = 1 - cur;
+                info = BVP_RET_SUCCESS;
+            }
+        }
+        if (!(info == BVP_RET_SUCCESS && error_norm > B.abstol)) break;
+        if (outer >= MAX_OUTER) { info = BVP_RET_FAILURE; break; }
+    }
+    // final residual norm
+    Rel<M> rel;
+    const double rn = residual_and_blocks<M, METHOD>(I, sh, bf.mesh[cur], bf.y[cur], N, lane, nullptr, rel);
+    if (lane == 0) {
+        B.retcode[b] = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
+        B.N_final[b] = N;
+        B.newton_iters[b] = iters;
+        B.outer_iters[b] = outer;
+        B.fallback[b] = fb;
+        B.cur_buf[b] = cur;
+        B.defect_norm[b] = last_defect;
+        B.resid_norm[b] = rn;
+        B.cnt[(size_t)b * 4 + 0] = cnt_it;
+        B.cnt[(size_t)b * 4 + 1] = cnt_def;
+        B.cnt[(size_t)b * 4 + 2] = cnt_tr;
+        B.cnt[(size_t)b * 4 + 3] = cnt_chk;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ kernels
+template <int M>
+__device__ inline int next_item(unsigned *counter, Shared<M> &sh, int lane) {
+    if (lane == 0) sh.item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = sh.item;
+    __syncthreads();
+    return it;
+}
+
+// Persistent fused solve: each wave pulls instances from the queue (ordered by
+// decreasing size) and runs the whole residual -> Jacobian -> LU -> update ->
+// defect -> mesh loop for it.
+template <int M, int METHOD>
+__global__ __launch_bounds__(64) void k_solve(BatchDev B) {
+    __shared__ Shared<M> sh;
+    const int lane = threadIdx.x;
+    double *scr = B.scratch + (size_t)blockIdx.x * B.scratch_per_slot;
+    double *aux = B.aux + (size_t)blockIdx.x * B.aux_per_slot;
+    for (;;) {
+        const int item = next_item<M>(B.counter, sh, lane);
+        if (item >= B.n) break;
+        solve_instance<M, METHOD>(B, B.order[item], sh, lane, scr, B.tree_base, aux);
+    }
+}
+
+// Component kernels: one instance per queue item, state read from the packed
+// inputs (mesh and guess), outputs packed by the offsets in the descriptor.
+template <int M, int METHOD>
+__global__ __launch_bounds__(64) void k_component(BatchDev B, int op) {
+    __shared__ Shared<M> sh;
+    const int lane = threadIdx.x;
+    double *scr = B.scratch + (size_t)blockIdx.x * B.scratch_per_slot;
+    double *aux = B.aux + (size_t)blockIdx.x * B.aux_per_slot;
+    for (;;) {
+        const int b = next_item<M>(B.counter, sh, lane);
+        if (b >= B.n) break;
+        Inst<M> I;
+        load_inst<M>(B, b, I);
+        const int N = B.N0[b];
+        const long long off = B.init_off[b];
+        // private working copy of the state (the solve may update y)
+        Bufs<M> bf = bufs_of<M>(B, b);
+        for (int k = lane; k <= N; k += WAVE) {
+            bf.mesh[0][k] = B.init_mesh[off + k];
+#pragma unroll
+            for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = B.init_y[(off + k) * M + c];
+        }
+        __syncthreads();
+        const double *mesh = bf.mesh[0];
+        double *y = bf.y[0];
+        if (op == OP_RESIDUAL) {
+            // reference layout: TWO_POINT [bca; Phi; bcb], STANDARD [bc; Phi]
+            double *out = B.out0 + off * M;
+            int La = 0;
+            if (I.bc_kind == BVP_BC_TWO_POINT) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) La += (I.bct[k] == I.t0) ? 1 : 0;
+            } else {
+                La = M;
+            }
+            Rel<M> rel;
+            double mx;
+            bool bad;
+            forward_dual<M, METHOD>(I, mesh, y, N, lane, nullptr, rel, mx, bad, out + La, nullptr, nullptr);
+            if (lane == 0) {
+                int ia = 0, ib = 0;
+                for (int k = 0; k < M; ++k) {
+                    const BcLoc L = bc_locate<M>(I, k, mesh, N);
+                    const double r = bc_residual<M, METHOD>(I, k, L, mesh, y);
+                    if (I.bc_kind == BVP_BC_TWO_POINT && sel<M>(I.bct, k) != I.t0) out[La + (size_t)N * M + (ib++)] = r;
+                    else out[ia++] = r;
+                }
+            }
+        } else if (op == OP_JACOBIAN) {
+            Rel<M> rel;
+            double mx;
+            bool bad;
+            const long long ioff = off - b;  // interval offset = node offset - instance index
+            forward_dual<M, METHOD>(I, mesh, y, N, lane, nullptr, rel, mx, bad, nullptr, B.out0 + ioff * M * M,
+                                    B.out1 + ioff * M * M);
+            if (lane < M) B.iout[(size_t)b * M + lane] = bc_locate<M>(I, lane, mesh, N).node;
+        } else if (op == OP_NEWTON_STEP) {
+            Rel<M> rel;
+            const double fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+            (void)fn;
+            bool ok = solve_correction<M, METHOD>(I, sh, mesh, y, N, lane, scr, B.tree_base, rel,
+                                                  B.out0 + off * M, false);
+            if (lane == 0) B.iout[b] = ok ? 0 : 1;
+        } else if (op == OP_DEFECT) {
+            double r1, r2;
+            const long long ioff = off - b;
+            const double dn = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, nullptr, B.out0 + ioff * M, r1, r2);
+            if (lane == 0) B.out1[b] = dn;
+        } else if (op == OP_MESH_SELECT) {
+            double r1, r2;
+            double *shat = B.shat + (size_t)b * B.cap;
+            const double dn = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2);
+            (void)dn;
+            __syncthreads();
+            const int Nn = mesh_select<M>(mesh, N, shat, r1, r2, B.max_sub, bf.mesh[1], lane, sh);
+            if (Nn < 0) {
+                if (lane == 0) { B.iout[(size_t)b * 2] = N; B.iout[(size_t)b * 2 + 1] = BVP_RET_FAILURE; }
+            } else {
+                double *iiscr = aux + (size_t)2 * B.cap * M;
+                transfer<M, METHOD>(I, mesh, y, N, bf.mesh[1], bf.y[1], Nn, B.transfer_mode, lane, iiscr);
+                double *om = B.out0 + (size_t)b * B.cap, *oy = B.out1 + (size_t)b * B.cap * M;
+                for (int k = lane; k <= Nn; k += WAVE) {
+                    om[k] = bf.mesh[1][k];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) oy[(size_t)k * M + c] = bf.y[1][(size_t)k * M + c];
+                }
+                if (lane == 0) { B.iout[(size_t)b * 2] = Nn; B.iout[(size_t)b * 2 + 1] = BVP_RET_SUCCESS; }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Gather the final state of every instance into packed outputs (offsets given
+// per instance in out_off, in nodes); one block per instance.
+template <int M>
+__global__ __launch_bounds__(256) void k_gather(BatchDev B, const long long *out_off, double *out_mesh,
+                                                double *out_y) {
+    const int b = blockIdx.x;
+    if (b >= B.n) return;
+    const int N = B.N_final[b];
+    const int cur = B.cur_buf[b];
+    Bufs<M> bf = bufs_of<M>(B, b);
+    const long long o = out_off[b];
+    for (int k = threadIdx.x; k <= N; k += blockDim.x) {
+        out_mesh[o + k] = bf.mesh[cur][k];
+#pragma unroll
+        for (int c = 0; c < M; ++c) out_y[(o + k) * M + c] = bf.y[cur][(size_t)k * M + c];
+    }
+}
+
+// ------------------------------------------------------------------ host launchers
+template <int M, int METHOD>
+hipError_t launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
+    k_solve<M, METHOD><<<blocks, 64, 0, s>>>(B);
+    return hipGetLastError();
+}
+template <int M, int METHOD>
+hipError_t launch_component_t(const BatchDev &B, int op, int blocks, hipStream_t s) {
+    k_component<M, METHOD><<<blocks, 64, 0, s>>>(B, op);
+    return hipGetLastError();
+}
+
+template <int M, int METHOD>
+int occupancy_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<M, METHOD>, 64, 0) != hipSuccess) return 4;
+    return nb;
+}
+
+}  // namespace bvpdev

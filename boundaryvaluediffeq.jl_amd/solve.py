@@ -1,0 +1,114 @@
+"""``solve`` of the host-side mirror: the reference's entry point
+``solve(prob, MIRK4()/MIRK6(); dt, abstol, adaptive, controller, nlsolve_kwargs)``
+(Core/BoundaryValueDiffEqCore.jl:48-54 -> MIRK/mirk.jl:49-265 __init,
+MIRK/mirk.jl:286-332 solve!), served by the HIP library; and the batched
+``EnsembleProblem`` form (MIRKt/Core/ensemble_tests.jl:7-41) as one launch.
+"""
+import numpy as np
+
+from . import _abi, _lib
+from .problems import BVProblem, extract
+
+
+class DefectControl:
+    """Core/calc_errors.jl:54-60."""
+
+    def __init__(self, defect_threshold=0.1):
+        self.defect_threshold = float(defect_threshold)
+
+
+class _MIRK:
+    order = 0
+
+    def __init__(self, nlsolve=None, jac_alg=None, defect_threshold=0.1, max_num_subintervals=3000,
+                 precision=_abi.PREC_FP64, refine_iters=0, transfer_mode=_abi.TRANSFER_REFERENCE,
+                 warm_start=None):
+        # fields of MIRK/algorithms.jl:55-61; `defect_threshold` is stored but, as in the
+        # reference, the controller's threshold is the one used.
+        self.nlsolve = nlsolve
+        self.jac_alg = jac_alg
+        self.defect_threshold = defect_threshold
+        self.max_num_subintervals = int(max_num_subintervals)
+        self.precision = precision
+        self.refine_iters = refine_iters
+        self.transfer_mode = transfer_mode
+        self.warm_start = warm_start
+
+    def __repr__(self):
+        return f"MIRK{self.order}()"
+
+
+class MIRK4(_MIRK):
+    order = 4
+
+
+class MIRK6(_MIRK):
+    order = 6
+
+
+class BVPSolution:
+    def __init__(self, raw, prob, alg):
+        self.prob = prob
+        self.alg = alg
+        self.t = raw.t
+        self.u = [raw.u[i].copy() for i in range(raw.u.shape[0])]
+        self.retcode = _abi.RETCODE_NAMES.get(raw.retcode, str(raw.retcode))
+        self.resid = raw.resid_norm
+        self.stats = {"newton_iters": raw.newton_iters, "outer_iters": raw.outer_iters,
+                      "defect_norm": raw.defect_norm, "N": raw.N, "fallback_used": raw.fallback_used}
+
+    def __array__(self, dtype=None):
+        return np.asarray(self.u, dtype=dtype)
+
+
+def successful_retcode(sol):
+    return sol.retcode == "Success"
+
+
+class EnsembleProblem:
+    """SciMLBase.EnsembleProblem: trajectory i solves prob_func(prob, i, repeat)."""
+
+    def __init__(self, prob, prob_func=None, output_func=None):
+        self.prob = prob
+        self.prob_func = prob_func or (lambda p, i, repeat: p)
+        self.output_func = output_func
+
+
+class EnsembleSolution(list):
+    @property
+    def converged(self):
+        return all(successful_retcode(s) for s in self)
+
+
+def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=None):
+    o = _abi.default_opts(alg.order)
+    o.abstol = float(abstol)
+    nk = dict(nlsolve_kwargs or {})
+    o.newton_abstol = float(nk.get("abstol", abstol))   # MIRK/mirk.jl:51
+    o.max_newton_iters = int(nk.get("maxiters", 1000))
+    o.adaptive = 1 if adaptive else 0
+    o.defect_threshold = (controller or DefectControl()).defect_threshold
+    o.max_num_subintervals = alg.max_num_subintervals
+    o.precision = alg.precision
+    o.refine_iters = alg.refine_iters
+    o.transfer_mode = alg.transfer_mode
+    return o
+
+
+def solve(prob, alg, *, dt=0.0, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=None,
+          trajectories=None, device=0):
+    """Solve a BVProblem / TwoPointBVProblem (or an EnsembleProblem) with MIRK4/MIRK6 on the GPU."""
+    opts = make_opts(alg, abstol, adaptive, controller, nlsolve_kwargs)
+    ctx = _lib.context(device)
+    if isinstance(prob, EnsembleProblem):
+        if trajectories is None:
+            raise ValueError("trajectories is required for an EnsembleProblem")
+        probs = [prob.prob_func(prob.prob, i + 1, 1) for i in range(trajectories)]
+        specs = [extract(p, dt) for p in probs]
+        raws = _lib.solve_batch(ctx, opts, specs)
+        return EnsembleSolution(BVPSolution(r, p, alg) for r, p in zip(raws, probs))
+    if not isinstance(prob, BVProblem):
+        raise TypeError("prob must be a BVProblem, TwoPointBVProblem or EnsembleProblem")
+    spec = extract(prob, dt)
+    raw = _lib.solve_batch(ctx, opts, [spec])[0]
+    return BVPSolution(raw, prob, alg)

@@ -206,6 +206,28 @@ int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
                          int32_t *new_N, int32_t *info, double *new_mesh,
                          double *new_y);
 
+/* ---- batch plans: upload once, solve many times (no counterpart in the
+ * reference, whose batches are EnsembleProblem solves run one per thread,
+ * MIRKt/Core/ensemble_tests.jl:7-41).  For callers that time or repeat the
+ * device solve with the inputs already resident in device memory.
+ *   create : validates, packs and uploads the instances (synchronous)
+ *   run    : enqueues the persistent solve on the context's stream
+ *   sync   : waits for it; *ms = device time of the solve kernel
+ *   results: copies the per-instance results out (as bvp_mirk_solve_batch)
+ *   counters: work done by the last run, summed over instances:
+ *            [0] interval-Newton-iterations, [1] defect intervals,
+ *            [2] transfer nodes (old + new), [3] convergence-check intervals
+ *   info   : [0] persistent waves, [1] node capacity, [2] M, [3] scratch doubles per wave */
+typedef struct bvp_batch bvp_batch;
+int  bvp_batch_create(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
+                      const bvp_instance *inst, bvp_batch **out);
+int  bvp_batch_run(bvp_ctx *ctx, bvp_batch *batch);
+int  bvp_batch_sync(bvp_ctx *ctx, bvp_batch *batch, float *ms);
+int  bvp_batch_results(bvp_ctx *ctx, bvp_batch *batch, bvp_result *res);
+int  bvp_batch_counters(bvp_ctx *ctx, bvp_batch *batch, long long *out4);
+int  bvp_batch_info(bvp_batch *batch, int32_t *out4);
+void bvp_batch_destroy(bvp_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

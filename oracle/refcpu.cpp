@@ -732,7 +732,8 @@ int mesh_select(int order, double abstol, int max_sub, const std::vector<double>
         const double int_next = integral + next_piece;
         if (int_next > zeta) {
             const double tn = (zeta - integral) / shat[k] + t;
-            if (i + 1 <= Nsub) new_mesh[(size_t)(i + 1)] = tn;
+            if (i + 1 > Nsub) return BVP_RET_FAILURE;  // more nodes than planned: the reference would index out of range
+            new_mesh[(size_t)(i + 1)] = tn;
             t = tn;
             ++i;
             integral = 0;

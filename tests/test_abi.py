@@ -1,0 +1,115 @@
+"""CPU checks of the C ABI: the library loads and exports every symbol the
+header declares, and the Python mirror's constants and struct sizes match."""
+import ctypes as C
+import pathlib
+import re
+
+import pytest
+
+import boundaryvaluediffeq_amd as bvp
+from boundaryvaluediffeq_amd import _abi, _lib
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADERS = sorted((ROOT / "include").glob("*.h"))
+
+
+def _declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*?[\s\*]+(bvp_[a-z0-9_]+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def _defines():
+    out = {}
+    for h in HEADERS:
+        for m in re.finditer(r"#define\s+(BVP_[A-Z0-9_]+)\s+(-?\d+)", h.read_text()):
+            out[m.group(1)] = int(m.group(2))
+    return out
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(str(_lib.LIB_PATH))
+    declared = _declared_functions()
+    assert len(declared) >= 15
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+    assert declared <= set(_lib.EXPORTED) | {"bvp_opts_default"}
+
+
+def test_constants_match_header():
+    d = _defines()
+    pairs = {
+        "BVP_MIRK4": _abi.MIRK4, "BVP_MIRK6": _abi.MIRK6,
+        "BVP_FAMILY_LINEAR_TANH": _abi.FAMILY_LINEAR_TANH, "BVP_FAMILY_PENDULUM": _abi.FAMILY_PENDULUM,
+        "BVP_FAMILY_BRATU": _abi.FAMILY_BRATU, "BVP_FAMILY_SINGPERT": _abi.FAMILY_SINGPERT,
+        "BVP_FAMILY_SCALAR_TU": _abi.FAMILY_SCALAR_TU, "BVP_FAMILY_TURNING": _abi.FAMILY_TURNING,
+        "BVP_BC_TWO_POINT": _abi.BC_TWO_POINT, "BVP_BC_STANDARD": _abi.BC_STANDARD,
+        "BVP_RET_SUCCESS": _abi.RET_SUCCESS, "BVP_RET_FAILURE": _abi.RET_FAILURE,
+        "BVP_RET_MAXITERS": _abi.RET_MAXITERS, "BVP_RET_UNSTABLE": _abi.RET_UNSTABLE,
+        "BVP_PREC_FP64": _abi.PREC_FP64, "BVP_PREC_FP32_REFINE64": _abi.PREC_FP32_REFINE64,
+        "BVP_TRANSFER_REFERENCE": _abi.TRANSFER_REFERENCE, "BVP_TRANSFER_CLEAN": _abi.TRANSFER_CLEAN,
+        "BVP_EINVAL": _abi.EINVAL, "BVP_EHIP": _abi.EHIP, "BVP_ENOTSUP": _abi.ENOTSUP,
+    }
+    for k, v in pairs.items():
+        assert d[k] == v, k
+
+
+def test_struct_layouts(tmp_path):
+    """ctypes mirrors (product and oracle) have the C compiler's sizes and offsets."""
+    import subprocess
+    import pyoracle
+    structs = {"bvp_instance": (_abi.Instance, pyoracle._Instance), "bvp_opts": (_abi.Opts, pyoracle.Opts),
+               "bvp_result": (_abi.Result, pyoracle._Result)}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADERS[0]}"', "int main(void) {"]
+    for cname, (py, _) in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _t in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            a, b, c = line.split()
+            got[(a, b)] = int(c)
+    for cname, (py, orc) in structs.items():
+        assert C.sizeof(py) == got[(cname, "size")] == C.sizeof(orc), cname
+        for f, _t in py._fields_:
+            assert getattr(py, f).offset == got[(cname, f)], (cname, f)
+            assert getattr(orc, f).offset == got[(cname, f)], (cname, f)
+
+
+def test_opts_default_matches_reference_defaults():
+    o = _abi.Opts()
+    _lib.lib().bvp_opts_default(C.byref(o))
+    ref = _abi.default_opts()
+    for name, _ in _abi.Opts._fields_:
+        assert getattr(o, name) == getattr(ref, name), name
+    assert o.abstol == 1e-6 and o.defect_threshold == 0.1 and o.max_num_subintervals == 3000
+
+
+def test_dt_to_mesh_like_reference():
+    """N = cld(t1 - t0, dt); nodes of range(t0, t1, N+1) (Core/utils.jl:354-362, 694-700)."""
+    import math
+    assert bvp.cld(math.pi / 2, 0.05) == 32
+    assert bvp.cld(1.0, 0.1) == 10
+    m = bvp.uniform_mesh(0.0, math.pi / 2, 32)
+    assert m[16] == math.pi / 4
+    import pyoracle
+    assert (pyoracle.uniform_mesh(-1.0, 1.0, 37) == bvp.uniform_mesh(-1.0, 1.0, 37)).all()
+
+
+def test_context_fails_loudly_without_gpu():
+    """No CPU fallback: on a machine without a GPU, creating a context raises."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(_lib.BVPError):
+        _lib.Context(0)
