@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel on BYTES (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); instances shard across "
@@ -69,6 +71,12 @@ def main():
 
     bvp = load_package()
     from boundaryvaluediffeq_amd import _abi, _lib, synthetic
+
+    if args.copy_probe:
+        ms = _lib.copy_probe(_lib.context(0), args.copy_probe, repeats=args.steps)
+        gbs = 2 * args.copy_probe / (ms * 1e-3) / 1e9
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "ms": ms, "GB_per_s_read_plus_write": gbs}))
+        return
 
     n = args.instances
     abstol = 1e-8

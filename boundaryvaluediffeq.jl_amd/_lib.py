@@ -53,6 +53,7 @@ def lib():
         "bvp_batch_counters": (C.c_int, [vp, vp, _lp]),
         "bvp_batch_info": (C.c_int, [vp, _ip]),
         "bvp_batch_destroy": (None, [vp]),
+        "bvp_copy_probe": (C.c_int, [vp, C.c_longlong, C.c_int, C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -66,7 +67,7 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_mirk_solve_batch", "bvp_mirk_residual", "bvp_mirk_jacobian_blocks",
             "bvp_mirk_newton_step", "bvp_mirk_defect", "bvp_mirk_mesh_select", "bvp_batch_create",
             "bvp_batch_run", "bvp_batch_sync", "bvp_batch_results", "bvp_batch_counters",
-            "bvp_batch_info", "bvp_batch_destroy"]
+            "bvp_batch_info", "bvp_batch_destroy", "bvp_copy_probe"]
 
 
 class Context:
@@ -192,6 +193,12 @@ def mesh_select(ctx, opts, specs, capacity=None):
         out.append((int(info[i]), k, nm[i * cap:i * cap + k + 1].copy(),
                     ny[i * cap * M:i * cap * M + M * (k + 1)].copy()))
     return out
+
+
+def copy_probe(ctx, nbytes, repeats=3):
+    ms = C.c_float()
+    ctx.check(lib().bvp_copy_probe(ctx.handle, int(nbytes), int(repeats), C.byref(ms)), "copy_probe")
+    return float(ms.value)
 
 
 class RawSolution:

@@ -548,4 +548,30 @@ int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const bv
     return rc;
 }
 
+// Streams `nbytes` (rounded down to doubles) from one device buffer to another
+// with a plain copy kernel and reports its device time; the reference point for
+// converting HBM byte counters of a profiler into bytes.
+int bvp_copy_probe(bvp_ctx *ctx, long long nbytes, int repeats, float *ms) {
+    if (!ctx || nbytes < 8 || repeats < 1) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    const size_t n = (size_t)nbytes / 8;
+    DevArena A;
+    double *src, *dst;
+    int rc;
+    if ((rc = dalloc(ctx, A, src, n)) || (rc = dalloc(ctx, A, dst, n))) return rc;
+    HIP_TRY(ctx, hipMemsetAsync(src, 0, n * 8, ctx->stream));
+    float best = 1e30f;
+    for (int r = 0; r < repeats; ++r) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(ctx, launch_copy_probe(src, dst, n, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        float t = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        best = std::min(best, t);
+    }
+    if (ms) *ms = best;
+    return 0;
+}
+
 }  // extern "C"

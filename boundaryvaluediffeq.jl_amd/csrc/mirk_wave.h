@@ -68,3 +68,5 @@ hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, doubl
                               int max_nodes, hipStream_t s);
 bool wave_supports(int M, int family);
 int wave_rel_fields(int M);
+
+hipError_t launch_copy_probe(const double *src, double *dst, size_t n, hipStream_t s);

@@ -14,6 +14,19 @@ using namespace bvpdev;
 
 __global__ void k_fill_u32(unsigned *p, unsigned v) { *p = v; }
 
+// Plain streaming copy of n doubles: the calibration reference for HBM byte
+// counters (it reads 8n and writes 8n bytes).
+__global__ __launch_bounds__(256) void k_copy_probe(const double *__restrict__ src, double *__restrict__ dst,
+                                                    size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const double *src, double *dst, size_t n, hipStream_t s) {
+    k_copy_probe<<<4096, 256, 0, s>>>(src, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t wave_reset_counter(unsigned *counter, hipStream_t s) {
     k_fill_u32<<<1, 1, 0, s>>>(counter, 0u);
     return hipGetLastError();

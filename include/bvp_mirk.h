@@ -228,6 +228,11 @@ int  bvp_batch_counters(bvp_ctx *ctx, bvp_batch *batch, long long *out4);
 int  bvp_batch_info(bvp_batch *batch, int32_t *out4);
 void bvp_batch_destroy(bvp_batch *batch);
 
+/* ---- measurement helper (no counterpart in the reference): a plain device
+ * copy of nbytes, run `repeats` times; *ms = fastest device time.  It is the
+ * known-size reference for scaling profiler HBM byte counters. */
+int  bvp_copy_probe(bvp_ctx *ctx, long long nbytes, int repeats, float *ms);
+
 #ifdef __cplusplus
 }
 #endif
