@@ -30,6 +30,7 @@ struct DevArena {
 
 struct bvp_batch {
     int n = 0, M = 0, method = 4, cap = 0, blocks = 0;
+    bool block = false;  // served by the block kernels (M = 8, 16, 32)
     bvp_opts opts{};
     std::vector<int> N0;
     std::vector<long long> init_off;

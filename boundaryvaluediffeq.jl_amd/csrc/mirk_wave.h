@@ -56,6 +56,9 @@ struct BatchDev {
     double *out0, *out1;
     int *iout;
 
+    int rec_cap;              // block kernels: intervals whose factors fit a wave's storage
+    int refine_iters;         // FP64 refinement steps per linear solve with FP32 factors
+
     // options
     int adaptive, max_sub, max_newton, linesearch, transfer_mode;
     double abstol, newton_tol, defect_threshold;
@@ -70,3 +73,10 @@ bool wave_supports(int M, int family);
 int wave_rel_fields(int M);
 
 hipError_t launch_copy_probe(const double *src, double *dst, size_t n, hipStream_t s);
+
+// block kernels (M = 8, 16, 32; LINEAR_TANH): precision 0 = FP64 factors, 1 = FP32 factors + FP64 refinement
+bool block_supports(int M, int family);
+hipError_t block_launch_solve(const BatchDev &B, int method, int precision, int blocks, hipStream_t s);
+hipError_t block_launch_component(const BatchDev &B, int method, int precision, int op, int blocks, hipStream_t s);
+int block_blocks_per_cu(int M, int method, int precision);
+size_t block_record_values(int M);  // values (of the factor type) per interval record

@@ -66,6 +66,9 @@ hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, doubl
     case 2: k_gather<2><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     case 3: k_gather<3><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     case 4: k_gather<4><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    case 8: k_gather<8><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    case 16: k_gather<16><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    case 32: k_gather<32><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -95,4 +98,4 @@ int wave_blocks_per_cu(int M, int method) {
     return 4;
 }
 
-bool blockm_supports(int, int) { return false; }
+bool blockm_supports(int M, int family) { return block_supports(M, family); }

@@ -981,9 +981,9 @@ __device__ inline void half_mesh_into(const double *mesh, int N, double *out, in
 // shat holds (max|d|/abstol)^(1/(p+1)) per interval on entry (it is divided by h
 // in place).  Writes the new mesh into new_mesh and returns the new N, or -1 on
 // failure (cap exceeded).
-template <int M>
+template <typename SH>
 __device__ inline int mesh_select(const double *mesh, int N, double *shat, double r1, double r2, int max_sub,
-                                  double *new_mesh, int lane, Shared<M> &sh) {
+                                  double *new_mesh, int lane, SH &sh) {
     const int n = N;
     const double r3 = r2 / (double)n;
     long long npred = round_even(1.3 * r2 + 1.0);
@@ -1159,7 +1159,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         __syncthreads();
         if (info == BVP_RET_SUCCESS) {
             if (error_norm > B.abstol) {
-                const int Nn = mesh_select<M>(mesh, N, shat, r1, r2, B.max_sub, mesh_o, lane, sh);
+                const int Nn = mesh_select(mesh, N, shat, r1, r2, B.max_sub, mesh_o, lane, sh);
                 if (Nn < 0) {
                     info = BVP_RET_FAILURE;
                 } else {
@@ -1306,7 +1306,7 @@ __global__ __launch_bounds__(64) void k_component(BatchDev B, int op) {
             const double dn = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2);
             (void)dn;
             __syncthreads();
-            const int Nn = mesh_select<M>(mesh, N, shat, r1, r2, B.max_sub, bf.mesh[1], lane, sh);
+            const int Nn = mesh_select(mesh, N, shat, r1, r2, B.max_sub, bf.mesh[1], lane, sh);
             if (Nn < 0) {
                 if (lane == 0) { B.iout[(size_t)b * 2] = N; B.iout[(size_t)b * 2 + 1] = BVP_RET_FAILURE; }
             } else {
