@@ -51,6 +51,57 @@ def read_traffic(workload):
     return None
 
 
+def lintanh_flops(M, counters, refine, s=5):
+    """FLOPs of the block path per launch: per interval-iteration the tangent GEMMs of
+    stages 2..s-1 (4 M^3 each), the pivoted elimination of the 2M x (3M+1) stacked
+    rows and the FP64 residual; per refinement step a J*v product and two sweeps."""
+    it, dfct, tr, chk = counters
+    elim = sum(2 * (2 * M - j - 1) * (3 * M + 1 - j) for j in range(M))
+    per_it = 4 * M ** 3 * (s - 2) + elim + 2 * s * 2 * M * M
+    per_ref = 2 * s * 2 * M * M * 2 + 2 * 2 * M * 2 * M + 2 * 3 * M * M
+    per_chk = 2 * s * 2 * M * M
+    per_def = 2 * 9 * 2 * M * M
+    return it * (per_it + refine * per_ref) + chk * per_chk + dfct * per_def
+
+
+def bench_lintanh(args, _abi, _lib, synthetic):
+    n = args.instances if args.instances != 16384 else 2048
+    M = 32
+    specs, Ns = synthetic.lintanh_batch(n, M=M)
+    opts = _abi.default_opts(_abi.MIRK6)
+    opts.abstol = 1e-6
+    opts.max_num_subintervals = 8192
+    opts.precision = _abi.PREC_FP32_REFINE64
+    opts.refine_iters = 2
+    ctx = _lib.context(0)
+    log(f"[bench] lintanh: uploading {n} instances, M={M} (sum N = {int(Ns.sum())})")
+    batch = _lib.Batch(ctx, opts, specs)
+    log(f"[bench] persistent waves {batch.info()[0]}")
+    for w in range(args.warmup):
+        batch.run()
+        log(f"[bench] warmup {w}: kernel {batch.sync():.1f} ms")
+    kms = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.run()
+        kms.append(batch.sync())
+        log(f"[bench] step {k}: kernel {kms[-1]:.1f} ms")
+    t1 = time.perf_counter()
+    res = batch.results()
+    counters = batch.counters()
+    good = sum(1 for r in res if r.retcode == 0 and r.defect_norm <= 1e-6)
+    wall = (t1 - t0) / args.steps
+    kmean = sum(kms) / len(kms)
+    flops = lintanh_flops(M, counters, opts.refine_iters)
+    tfs = flops / (kmean * 1e-3) / 1e12
+    log(f"[bench] {good}/{n} solved; {good / wall:.1f} solves/s; {tfs:.2f} TFLOP/s; counters {counters}; "
+        f"newton iters mean {sum(r.newton_iters for r in res) / n:.2f}")
+    print(json.dumps({"workload": f"lintanh{n}: {n} BVPs u'=Au+0.1 tanh(u), n=32, MIRK6, FP32 factors + FP64 "
+                                  "refinement, abstol 1e-6", "value": good / wall, "unit": "solves/s",
+                      "ms_per_step": wall * 1e3, "kernel_ms": kmean, "solved": good, "tflops": tfs,
+                      "peak_fp32_mfma_tflops": 157.3, "counters": counters}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +111,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh"],
+                    help="bratu: 16,384 Bratu BVPs (MIRK4, FP64); lintanh: 2,048 n=32 linear+tanh BVPs "
+                         "(MIRK6, FP32 factors + FP64 refinement)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -78,6 +132,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "ms": ms, "GB_per_s_read_plus_write": gbs}))
         return
 
+    if args.workload == "lintanh":
+        return bench_lintanh(args, _abi, _lib, synthetic)
     n = args.instances
     abstol = 1e-8
     specs, lams, Ns = synthetic.bratu_batch(n)

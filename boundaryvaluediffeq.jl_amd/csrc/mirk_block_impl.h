@@ -28,22 +28,34 @@ namespace bvpdev {
 template <int M, int METHOD, typename LT> struct BShared {
     static constexpr int S = Tab<METHOD>::s;
     static constexpr int NS = Tab<METHOD>::sstar - Tab<METHOD>::s;
-    static constexpr int W = 3 * M + 1;  // stacked row: pivot block | y_0 block | y_{k+1} block | rhs
-    double At[M][M];                     // A transposed (At[q][i] = A[i][q]): conflict-free lane reads
-    LT Al[M][M];                         // A in the factorisation precision, row-major (broadcast reads)
+    static constexpr int W = 3 * M + 1;   // stacked row: pivot block | y_0 block | y_{k+1} block | rhs
+    static constexpr int SW = W | 1;      // odd row stride (spreads the rows over the LDS banks)
+    static constexpr int RW = 2 * M + 1;  // root row: d_0 block | d_N block | rhs
+    double At[M][M];                      // A transposed (At[q][i] = A[i][q]): conflict-free lane reads
+    LT Al[sizeof(LT) == 4 ? M : 1][M];    // A in FP32 for the FP32 tangents (the FP64 ones read At)
     double c;
-    double va[M], vb[M], vd[M];          // node vectors (state or corrections)
-    double Y[S][M], K[S][M], KI[NS][M];  // stage arguments and values
-    double Yd[S][M], Kd[S][M];           // their tangents (J v products)
+    double va[M], vb[M], vd[M];           // node vectors (state or corrections)
+    double Y[S][M], K[S][M], KI[NS][M];   // stage arguments and values
+    double Yd[S][M], Kd[S][M];            // their tangents (J v products)
     LT sech2[S][M];
-    LT G[M][2 * M + 1];                  // rows [S | R | r] of the new block row
-    LT rel[M][2 * M + 1];                // relation rows [E | F | g] for the next interval
-    LT prow[W];                          // pivot row broadcast
-    double drow[2 * M + 1];              // root pivot row broadcast
+    union {
+        LT stk[2 * M][SW];                // stacked rows of one elimination step (rows 0..M-1: relation)
+        double root[2 * M][RW];           // the end-node system
+    } w;
+    double xr[2 * M];
     double bcres[M];
     int piv[2 * M];
+    int src[M];
+    int bc_end[M], bcc[M];
+    double bcv[M];
     int flag, item;
 };
+
+template <int M, int METHOD, typename LT>
+__device__ inline LT aval(const BShared<M, METHOD, LT> &sh, int i, int q) {
+    if constexpr (sizeof(LT) == 4) return sh.Al[i][q];
+    else return sh.At[q][i];
+}
 
 // ------------------------------------------------------------------ FP64 stage work (lanes = components)
 // K_r = f(Y_r), Y_r = (1-v_r) a + v_r b + h sum_{j<r} x_rj K_j ; lanes c < M.
@@ -158,80 +170,77 @@ __device__ inline double bf_of_vd(BShared<M, METHOD, LT> &sh, int lane) {
 }
 
 // ------------------------------------------------------------------ tangent columns: [S | R]
-// Lane l < 2M computes column l of G = sum_r b_r dK_r with dK_r = J_r dY_r, then
-// writes S = -I - h G[:, :M], R = I - h G[:, M:] into sh.G (row-major rows).
+// Lane l < 2M computes column l of G = sum_r b_r dK_r, dK_r = J_r dY_r (forward-mode
+// derivative of the stage recursion along the unit direction l), and writes
+// S = -I - h G[:, :M] / R = I - h G[:, M:] straight into the stacked rows in LDS:
+//   first == false: rows M..2M-1 get [S | 0 | R | .]   (a new block row)
+//   first == true : rows 0..M-1  get [R | S | 0 | .]   (interval 0 is the first relation)
 template <int M, int METHOD, typename LT>
-__device__ inline void btangent_blocks(BShared<M, METHOD, LT> &sh, int lane, double hd) {
+__device__ inline void btangent_blocks(BShared<M, METHOD, LT> &sh, int lane, double hd, bool first) {
     using TB = Tab<METHOD>;
+    using SHT = BShared<M, METHOD, LT>;
     const LT h = (LT)hd;
     const bool left = lane < M;
-    const int col = left ? lane : lane - M;  // column of the identity seed inside its half
-    LT g[M];   // column l of G
-    LT q4[M];  // accumulated x_{s-1,j} dK_j (j >= 2) for the last stage of MIRK6
-#pragma unroll
-    for (int i = 0; i < M; ++i) { g[i] = (LT)0; q4[i] = (LT)0; }
+    const int col = left ? lane : lane - M;
+    const int rb = first ? 0 : M;
+    const int tc = first ? (left ? M + col : col) : (left ? col : 2 * M + col);  // target column
+    const int zc = first ? 2 * M + col : M + col;                               // column to clear (left lanes)
     if (lane < 2 * M) {
-        // stages 0 and 1: dK_0 = [J_0 | 0], dK_1 = [0 | J_1]; column l of J_r is A[:, col] + c sech2_r[col] e_col
+        const int r0 = left ? 0 : 1;
+        const LT sj = (LT)sh.c * sh.sech2[r0][col];
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            const int r0 = left ? 0 : 1;
-            LT jv = sh.Al[i][col] + ((i == col) ? (LT)sh.c * sh.sech2[r0][col] : (LT)0);
-            g[i] = (LT)TB::b(r0) * jv;
+            const LT jv = aval<M, METHOD, LT>(sh, i, col) + ((i == col) ? sj : (LT)0);
+            const LT e = (i == col) ? (left ? (LT)-1 : (LT)1) : (LT)0;
+            sh.w.stk[rb + i][tc] = e - h * (LT)TB::b(r0) * jv;
+            if (left) sh.w.stk[rb + i][zc] = (LT)0;
         }
     }
+    if (TB::s > 2 && lane < 2 * M) {
+        LT q4[M];  // sum_{j>=2} x_{s-1,j} dK_j for the last stage (MIRK6)
 #pragma unroll
-    for (int r = 2; r < TB::s; ++r) {
-        if (lane < 2 * M) {
-            // dY_r column: seed (1-v_r) or v_r on e_col, plus h (x_r0 dK_0 + x_r1 dK_1) + h q (r == s-1)
+        for (int i = 0; i < M; ++i) q4[i] = (LT)0;
+        const int r0 = left ? 0 : 1;
+        const LT sj0 = (LT)sh.c * sh.sech2[r0][col];
+#pragma unroll
+        for (int r = 2; r < TB::s; ++r) {
             LT dy[M];
             const LT seed = left ? (LT)(1.0 - TB::v(r)) : (LT)TB::v(r);
             const LT xr = left ? (LT)TB::x(r, 0) : (LT)TB::x(r, 1);
-            const int r0 = left ? 0 : 1;
 #pragma unroll
             for (int i = 0; i < M; ++i) {
-                LT jv = sh.Al[i][col] + ((i == col) ? (LT)sh.c * sh.sech2[r0][col] : (LT)0);
+                const LT jv = aval<M, METHOD, LT>(sh, i, col) + ((i == col) ? sj0 : (LT)0);
                 dy[i] = ((i == col) ? seed : (LT)0) + h * xr * jv;
                 if (r == TB::s - 1 && TB::s > 3) dy[i] += h * q4[i];
             }
-            // dK_r = J_r dy
-            LT dk[M];
+            const LT hb = h * (LT)TB::b(r);
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 LT acc = (LT)sh.c * sh.sech2[r][i] * dy[i];
 #pragma unroll
-                for (int qq = 0; qq < M; ++qq) acc += sh.Al[i][qq] * dy[qq];
-                dk[i] = acc;
+                for (int qq = 0; qq < M; ++qq) acc += aval<M, METHOD, LT>(sh, i, qq) * dy[qq];
+                sh.w.stk[rb + i][tc] -= hb * acc;
+                if (TB::s > 3 && r < TB::s - 1) q4[i] += (LT)TB::x(TB::s - 1, r) * acc;
             }
-#pragma unroll
-            for (int i = 0; i < M; ++i) {
-                g[i] += (LT)TB::b(r) * dk[i];
-                if (TB::s > 3 && r < TB::s - 1) q4[i] += (LT)TB::x(TB::s - 1, r) * dk[i];
-            }
-        }
-    }
-    if (lane < 2 * M) {
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const LT e = (i == col) ? (left ? (LT)-1 : (LT)1) : (LT)0;
-            sh.G[i][lane] = e - h * g[i];
         }
     }
     __syncthreads();
+    (void)sizeof(SHT);
 }
 
-// ------------------------------------------------------------------ row elimination
-// Gaussian elimination with partial pivoting on NC pivot columns of a 64-row
-// system held one row per lane (rows of lanes >= nrows are absent).  The pivot
-// row of column j is lane piv[j]; it is broadcast through LDS and not swapped.
-// On return `done` marks pivot lanes.  When `mult` is non-null, the multiplier
-// used on this lane for column j is stored at mult[j * 64].
-template <int NC, int NW, typename T>
-__device__ inline void eliminate_rows(T (&row)[NW], bool &done, int lane, int nrows, T *prow, int *piv, T *mult) {
-    done = lane >= nrows;
-#pragma unroll
+// ------------------------------------------------------------------ elimination in LDS
+// Gaussian elimination with partial pivoting on the first NC columns of an
+// NR-row system stored row-major in LDS (row stride ST); lane l owns row l and
+// updates it in place; the pivot row of column j (lane piv[j]) is read by all
+// lanes and not swapped.  When `mult` is non-null the multiplier used on this
+// lane for column j is stored at mult[j * 64].  Returns whether this lane's row
+// became a pivot row.
+template <int NC, int NR, int NWID, int ST, typename T>
+__device__ inline bool eliminate_lds(T *mat, int lane, int *piv, T *mult) {
+    bool done = lane >= NR;
+    T *rowl = mat + (size_t)(lane < NR ? lane : 0) * ST;
     for (int j = 0; j < NC; ++j) {
-        // argmax |row[j]| over not-done lanes (ties: lowest lane)
-        double v = done ? -1.0 : (double)fabs((double)row[j]);
+        double v = done ? -1.0 : (double)fabs((double)rowl[j]);
         int who = lane;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
@@ -240,29 +249,26 @@ __device__ inline void eliminate_rows(T (&row)[NW], bool &done, int lane, int nr
             if (ov > v || (ov == v && ow < who)) { v = ov; who = ow; }
         }
         const int p = who;
-        // (loops run over the full row with compile-time conditions so that they
-        // unroll completely and the row stays in registers)
+        if (lane == 0) piv[j] = p;
+        const T *rowp = mat + (size_t)p * ST;
+        const T pv = rowp[j];
         if (lane == p) {
-#pragma unroll
-            for (int c = 0; c < NW; ++c)
-                if (c >= j) prow[c] = row[c];
-            piv[j] = p;
             done = true;
-        }
-        __syncthreads();
-        const T pv = prow[j];
-        if (!done) {
-            const T m = row[j] / pv;
-            row[j] = (T)0;
+            if (mult) mult[(size_t)j * WAVE] = (T)0;
+        } else if (!done) {
+            const T m = rowl[j] / pv;
+            rowl[j] = (T)0;
+            // fixed trip count, predicated: unrolls completely so the LDS reads pipeline
 #pragma unroll
-            for (int c = 0; c < NW; ++c)
-                if (c > j) row[c] -= m * prow[c];
+            for (int c = 1; c < NWID; ++c)
+                if (c > j) rowl[c] -= m * rowp[c];
             if (mult) mult[(size_t)j * WAVE] = m;
         } else if (mult) {
             mult[(size_t)j * WAVE] = (T)0;
         }
         __syncthreads();
     }
+    return done && lane < NR;
 }
 
 // ------------------------------------------------------------------ records
@@ -278,10 +284,10 @@ template <int M> struct BRec {
 };
 
 // ------------------------------------------------------------------ instance
-template <int M> struct BInst {
-    int bc_node_is_end[M];  // 0: row at node 0, 1: row at node N
-    int bcc[M];
-    double bcv[M];
+template <int M> struct BInst {  // boundary rows; the arrays live in LDS (BShared)
+    const int *bc_node_is_end;   // 0: row at node 0, 1: row at node N
+    const int *bcc;
+    const double *bcv;
 };
 
 template <int M, int METHOD, typename LT>
@@ -291,16 +297,18 @@ __device__ inline void bload_inst(const BatchDev &B, int b, BShared<M, METHOD, L
     for (int e = lane; e < M * M; e += WAVE) {
         const int i = e / M, q = e % M;
         sh.At[q][i] = A[e];
-        sh.Al[i][q] = (LT)A[e];
+        if constexpr (sizeof(LT) == 4) sh.Al[i][q] = (LT)A[e];
     }
     if (lane == 0) sh.c = B.params[po];
     const double t0 = B.t0[b];
-#pragma unroll
-    for (int k = 0; k < M; ++k) {
-        I.bc_node_is_end[k] = (B.bc_t[(size_t)b * M + k] == t0) ? 0 : 1;
-        I.bcc[k] = B.bc_comp[(size_t)b * M + k];
-        I.bcv[k] = B.bc_val[(size_t)b * M + k];
+    if (lane < M) {
+        sh.bc_end[lane] = (B.bc_t[(size_t)b * M + lane] == t0) ? 0 : 1;
+        sh.bcc[lane] = B.bc_comp[(size_t)b * M + lane];
+        sh.bcv[lane] = B.bc_val[(size_t)b * M + lane];
     }
+    I.bc_node_is_end = sh.bc_end;
+    I.bcc = sh.bcc;
+    I.bcv = sh.bcv;
     __syncthreads();
 }
 
@@ -333,11 +341,11 @@ __device__ double bresidual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const
         __syncthreads();
     }
     if (lane < M) {
-        const int node = sel<M>(I.bc_node_is_end, lane) ? N : 0;
-        const int comp = sel<M>(I.bcc, lane);
+        const int node = I.bc_node_is_end[lane] ? N : 0;
+        const int comp = I.bcc[lane];
         double v = y[(size_t)node * M + comp];
         if (delta) v += alpha * delta[(size_t)node * M + comp];
-        const double r = v - sel<M>(I.bcv, lane);
+        const double r = v - I.bcv[lane];
         sh.bcres[lane] = r;
         bad |= !(fabs(r) <= 1.7976931348623157e308);
         mx = fmax(mx, fabs(r));
@@ -352,21 +360,21 @@ __device__ double bresidual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const
 }
 
 // Factorise J(y) interval by interval (single chunk from node 0) and solve
-// J d = rhs where rhs = -F(y) (FP64, computed on the fly) when first == true,
+// J d = rhs where rhs = -F(y) (FP64, computed on the fly) when factor == true,
 // or rhs taken from `rin` (point-major residual layout: Phi_k at k*M, BC rows at
-// N*M + k) when first == false (a refinement solve through the stored factors).
+// N*M + k) when factor == false (a refinement solve through the stored factors).
 // The correction is written to dout (point-major, M*(N+1)).
 template <int M, int METHOD, typename LT>
 __device__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh, const double *y, int N,
                        int lane, LT *rec, bool factor, const double *rin, double *dout) {
-    constexpr int W = 3 * M + 1;
-    constexpr bool KEEP = true;  // multipliers are stored so that refinement can reuse the factors
+    using SHT = BShared<M, METHOD, LT>;
+    constexpr int W = SHT::W, SW = SHT::SW, RW = SHT::RW;
+    LT *stk = &sh.w.stk[0][0];
     // ---- forward sweep
-    LT row[W];
-    bool done;
     for (int k = 0; k < N; ++k) {
         const double h = mesh[k + 1] - mesh[k];
         LT *R = rec + (size_t)k * BRec<M>::SIZE;
+        const int rb = (k == 0) ? 0 : M;  // rows receiving this interval's block row
         if (factor) {
             if (lane < M) {
                 sh.va[lane] = y[(size_t)k * M + lane];
@@ -376,50 +384,16 @@ __device__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const doub
             bstages<M, METHOD, LT>(sh, lane, h);
             double rk = 0.0;
             if (lane < M) rk = -bphi<M, METHOD, LT>(sh, lane, h);
-            btangent_blocks<M, METHOD, LT>(sh, lane, h);
-            if (lane < M) sh.G[lane][2 * M] = (LT)rk;
+            btangent_blocks<M, METHOD, LT>(sh, lane, h, k == 0);
+            if (lane < M) sh.w.stk[rb + lane][3 * M] = (LT)rk;
             __syncthreads();
-        } else {
-            if (lane < M) sh.G[lane][2 * M] = (LT)rin[(size_t)k * M + lane];
-            __syncthreads();
-        }
-        if (k == 0) {
-            // the first block row is the relation itself: E = S_0 (on y_0), F = R_0 (on y_1)
-            if (lane < M) {
-                if (factor) {
-#pragma unroll
-                    for (int c = 0; c < M; ++c) {
-                        sh.rel[lane][c] = sh.G[lane][c];
-                        sh.rel[lane][M + c] = sh.G[lane][M + c];
-                    }
-                }
-                sh.rel[lane][2 * M] = sh.G[lane][2 * M];
-            }
-            __syncthreads();
-            continue;
-        }
-        if (factor) {
-            // stacked rows: lanes < M hold the relation [F | E | 0 | g], lanes M..2M-1 hold [S | 0 | R | r]
-#pragma unroll
-            for (int c = 0; c < W; ++c) row[c] = (LT)0;
-            if (lane < M) {
-#pragma unroll
-                for (int c = 0; c < M; ++c) { row[c] = sh.rel[lane][M + c]; row[M + c] = sh.rel[lane][c]; }
-                row[3 * M] = sh.rel[lane][2 * M];
-            } else if (lane < 2 * M) {
-                const int i = lane - M;
-#pragma unroll
-                for (int c = 0; c < M; ++c) { row[c] = sh.G[i][c]; row[2 * M + c] = sh.G[i][M + c]; }
-                row[3 * M] = sh.G[i][2 * M];
-            }
-            __syncthreads();
-            eliminate_rows<M, W, LT>(row, done, lane, 2 * M, sh.prow, sh.piv, KEEP ? R + BRec<M>::MULT + lane : nullptr);
-            // records of the pivot rows, in pivot order
-            if (done && lane < 2 * M) {
+            if (k == 0) continue;
+            const bool piv_row = eliminate_lds<M, 2 * M, W, SW, LT>(stk, lane, sh.piv, R + BRec<M>::MULT + lane);
+            // records of the pivot rows, in pivot order: U | X | Z | h
+            if (piv_row) {
                 int j = 0;
-#pragma unroll
                 for (int q = 0; q < M; ++q) j = (sh.piv[q] == lane) ? q : j;
-#pragma unroll
+                const LT *row = stk + (size_t)lane * SW;
                 for (int c = 0; c < M; ++c) {
                     R[BRec<M>::U + (size_t)c * M + j] = row[c];
                     R[BRec<M>::X + (size_t)c * M + j] = row[M + c];
@@ -428,80 +402,109 @@ __device__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const doub
                 R[BRec<M>::H + j] = row[3 * M];
             }
             if (lane < M) R[BRec<M>::PIV + lane] = (LT)sh.piv[lane];
-            // compact the remaining rows into the next relation [E' | F' | g']
-            const unsigned long long rest = __ballot(!done && lane < 2 * M);
-            if (!done && lane < 2 * M) {
-                const int slot = __popcll(rest & ((1ull << lane) - 1ull));
+            // the remaining rows become the next relation [F' | E' | 0 | g'] in rows 0..M-1
+            // (each remaining row is read whole into registers before any row is written)
+            const unsigned long long rest = __ballot(!piv_row && lane < 2 * M);
+            const bool keep = !piv_row && lane < 2 * M;
+            LT cp[2 * M + 1];
+            if (keep) {
+                const LT *from = stk + (size_t)lane * SW;
 #pragma unroll
-                for (int c = 0; c < M; ++c) { sh.rel[slot][c] = row[M + c]; sh.rel[slot][M + c] = row[2 * M + c]; }
-                sh.rel[slot][2 * M] = row[3 * M];
+                for (int c = 0; c < M; ++c) { cp[c] = from[2 * M + c]; cp[M + c] = from[M + c]; }
+                cp[2 * M] = from[3 * M];
+            }
+            __syncthreads();
+            if (keep) {
+                LT *to = stk + (size_t)__popcll(rest & ((1ull << lane) - 1ull)) * SW;
+#pragma unroll
+                for (int c = 0; c < 2 * M; ++c) to[c] = cp[c];
+#pragma unroll
+                for (int c = 0; c < M; ++c) to[2 * M + c] = (LT)0;
+                to[3 * M] = cp[2 * M];
             }
             __syncthreads();
         } else {
             // push the new right-hand side through the stored elimination
+            if (k == 0) {
+                if (lane < M) sh.w.stk[lane][3 * M] = (LT)rin[lane];
+                __syncthreads();
+                continue;
+            }
             LT q = (LT)0;
-            if (lane < M) q = sh.rel[lane][2 * M];
-            else if (lane < 2 * M) q = sh.G[lane - M][2 * M];
+            if (lane < M) q = sh.w.stk[lane][3 * M];
+            else if (lane < 2 * M) q = (LT)rin[(size_t)k * M + lane - M];
+            bool dn = lane >= 2 * M;
+            LT mj[M];
             int pv[M];
 #pragma unroll
-            for (int j = 0; j < M; ++j) pv[j] = (int)R[BRec<M>::PIV + j];
-            bool dn = lane >= 2 * M;
+            for (int j = 0; j < M; ++j) {
+                mj[j] = R[BRec<M>::MULT + (size_t)j * WAVE + lane];
+                pv[j] = (int)R[BRec<M>::PIV + j];
+            }
 #pragma unroll
             for (int j = 0; j < M; ++j) {
-                const LT qp = __shfl(q, pv[j]);
-                if (lane == pv[j]) { dn = true; R[BRec<M>::H + j] = q; }
-                else if (!dn) q -= R[BRec<M>::MULT + (size_t)j * WAVE + lane] * qp;
+                const int pj = pv[j];
+                const LT qp = __shfl(q, pj);
+                if (lane == pj) { dn = true; R[BRec<M>::H + j] = q; }
+                else if (!dn) q -= mj[j] * qp;
             }
             const unsigned long long rest = __ballot(!dn && lane < 2 * M);
             __syncthreads();
-            if (!dn && lane < 2 * M) {
-                const int slot = __popcll(rest & ((1ull << lane) - 1ull));
-                sh.rel[slot][2 * M] = q;
-            }
+            if (!dn && lane < 2 * M) sh.w.stk[__popcll(rest & ((1ull << lane) - 1ull))][3 * M] = q;
             __syncthreads();
         }
     }
-    // ---- root: BC rows and the relation E d_0 + F d_N = g (2M x 2M, FP64, row per lane)
-    double rr[2 * M + 1];
-#pragma unroll
-    for (int c = 0; c < 2 * M + 1; ++c) rr[c] = 0.0;
-    if (lane < M) {
-        const bool end = sel<M>(I.bc_node_is_end, lane) != 0;
-        const int comp = sel<M>(I.bcc, lane);
-#pragma unroll
-        for (int c = 0; c < M; ++c) {
-            rr[c] = (!end && c == comp) ? 1.0 : 0.0;
-            rr[M + c] = (end && c == comp) ? 1.0 : 0.0;
-        }
-        rr[2 * M] = factor ? -sh.bcres[lane] : rin[(size_t)N * M + lane];
-    } else if (lane < 2 * M) {
+    // ---- root: BC rows and the relation E d_0 + F d_N = g (2M x 2M, FP64, in LDS).
+    // The factor pass keeps [E | F] of the final relation in record 0 (interval 0
+    // has no elimination record) for the refinement passes, since the root
+    // system overwrites the stacked rows in LDS.
+    LT relrow[2 * M + 1];
+    if (lane >= M && lane < 2 * M) {
         const int i = lane - M;
+        const LT *row = stk + (size_t)i * SW;  // [F | E | 0 | g]
+        if (factor) {
 #pragma unroll
-        for (int c = 0; c < 2 * M; ++c) rr[c] = (double)sh.rel[i][c];
-        rr[2 * M] = (double)sh.rel[i][2 * M];
+            for (int c = 0; c < M; ++c) { relrow[c] = row[M + c]; relrow[M + c] = row[c]; }
+#pragma unroll
+            for (int c = 0; c < 2 * M; ++c) rec[(size_t)c * M + i] = relrow[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 2 * M; ++c) relrow[c] = rec[(size_t)c * M + i];
+        }
+        relrow[2 * M] = row[3 * M];
     }
     __syncthreads();
-    bool rdone;
-    eliminate_rows<2 * M, 2 * M + 1, double>(rr, rdone, lane, 2 * M, sh.drow, sh.piv, (double *)nullptr);
-    // back substitution over the pivot order; the solution goes to LDS (drow)
+    if (lane < M) {
+        const bool end = I.bc_node_is_end[lane] != 0;
+        const int comp = I.bcc[lane];
+        for (int c = 0; c < M; ++c) {
+            sh.w.root[lane][c] = (!end && c == comp) ? 1.0 : 0.0;
+            sh.w.root[lane][M + c] = (end && c == comp) ? 1.0 : 0.0;
+        }
+        sh.w.root[lane][2 * M] = factor ? -sh.bcres[lane] : rin[(size_t)N * M + lane];
+    } else if (lane < 2 * M) {
 #pragma unroll
+        for (int c = 0; c < 2 * M + 1; ++c) sh.w.root[lane][c] = (double)relrow[c];
+    }
+    __syncthreads();
+    const bool rpiv = eliminate_lds<2 * M, 2 * M, RW, RW, double>(&sh.w.root[0][0], lane, sh.piv, (double *)nullptr);
+    // back substitution over the pivot order
     for (int j = 2 * M - 1; j >= 0; --j) {
         const int p = sh.piv[j];
-        double xj = 0.0;
-        if (lane == p) xj = rr[2 * M] / rr[j];
-        xj = __shfl(xj, p);
-        if (lane == 0) sh.drow[j] = xj;
-        if (rdone && lane != p) rr[2 * M] -= rr[j] * xj;
+        if (lane == p) sh.xr[j] = sh.w.root[p][2 * M] / sh.w.root[p][j];
+        __syncthreads();
+        const double xj = sh.xr[j];
+        if (rpiv && lane != p) sh.w.root[lane][2 * M] -= sh.w.root[lane][j] * xj;
+        __syncthreads();
     }
-    __syncthreads();
     bool ok = true;
-    if (lane < 2 * M) ok = (sh.drow[lane] <= 1.7976931348623157e308 && sh.drow[lane] >= -1.7976931348623157e308);
+    if (lane < 2 * M) ok = (sh.xr[lane] <= 1.7976931348623157e308 && sh.xr[lane] >= -1.7976931348623157e308);
     if (lane < M) {
-        dout[lane] = sh.drow[lane];
-        dout[(size_t)N * M + lane] = sh.drow[M + lane];
+        dout[lane] = sh.xr[lane];
+        dout[(size_t)N * M + lane] = sh.xr[M + lane];
     }
     // ---- backward sweep: d_k = U^-1 (h - X d_0 - Z d_{k+1})
-    if (lane < M) { sh.va[lane] = sh.drow[lane]; sh.vb[lane] = sh.drow[M + lane]; }
+    if (lane < M) { sh.va[lane] = sh.xr[lane]; sh.vb[lane] = sh.xr[M + lane]; }
     __syncthreads();
     for (int k = N - 1; k >= 1; --k) {
         const LT *R = rec + (size_t)k * BRec<M>::SIZE;
@@ -513,15 +516,18 @@ __device__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const doub
             for (int c = 0; c < M; ++c)
                 v -= (double)R[BRec<M>::X + (size_t)c * M + j] * sh.va[c] + (double)R[BRec<M>::Z + (size_t)c * M + j] * sh.vb[c];
         }
-        // triangular solve U d = v, U_j in pivot order: U[j][c] = R[U + c*M + j]
+        // triangular solve U d = v, U_j in pivot order: U[j][c] = R[U + c*M + j];
+        // the U block is staged in LDS first (independent loads, one latency)
+        LT *Us = stk;
+        for (int e = lane; e < M * M; e += WAVE) Us[e] = R[BRec<M>::U + e];
+        __syncthreads();
         double dk = 0.0;
-#pragma unroll
         for (int j = M - 1; j >= 0; --j) {
             double dj = 0.0;
-            if (lane == j) dj = v / (double)R[BRec<M>::U + (size_t)j * M + j];
+            if (lane == j) dj = v / (double)Us[(size_t)j * M + j];
             dj = __shfl(dj, j);
             if (lane == j) dk = dj;
-            if (lane < j) v -= (double)R[BRec<M>::U + (size_t)j * M + lane] * dj;
+            if (lane < j) v -= (double)Us[(size_t)j * M + lane] * dj;
         }
         __syncthreads();
         if (lane < M) {
@@ -557,9 +563,9 @@ __device__ void brefine_residual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, 
         __syncthreads();
     }
     if (lane < M) {
-        const int node = sel<M>(I.bc_node_is_end, lane) ? N : 0;
-        const int comp = sel<M>(I.bcc, lane);
-        const double v = y[(size_t)node * M + comp] - sel<M>(I.bcv, lane);
+        const int node = I.bc_node_is_end[lane] ? N : 0;
+        const int comp = I.bcc[lane];
+        const double v = y[(size_t)node * M + comp] - I.bcv[lane];
         rout[(size_t)N * M + lane] = -v - d[(size_t)node * M + comp];
     }
     __syncthreads();
@@ -880,16 +886,16 @@ __global__ __launch_bounds__(64) void k_bcomponent(BatchDev B, int op) {
                 if (lane < M) { sh.va[lane] = y[(size_t)k * M + lane]; sh.vb[lane] = y[(size_t)(k + 1) * M + lane]; }
                 __syncthreads();
                 bstages<M, METHOD, LT>(sh, lane, h);
-                btangent_blocks<M, METHOD, LT>(sh, lane, h);
+                btangent_blocks<M, METHOD, LT>(sh, lane, h, false);
                 double *So = B.out0 + (ioff + k) * M * M, *Ro = B.out1 + (ioff + k) * M * M;
                 for (int e = lane; e < M * M; e += WAVE) {
                     const int i = e / M, c = e % M;
-                    So[e] = (double)sh.G[i][c];
-                    Ro[e] = (double)sh.G[i][M + c];
+                    So[e] = (double)sh.w.stk[M + i][c];
+                    Ro[e] = (double)sh.w.stk[M + i][2 * M + c];
                 }
                 __syncthreads();
             }
-            if (lane < M) B.iout[(size_t)b * M + lane] = sel<M>(I.bc_node_is_end, lane) ? N : 0;
+            if (lane < M) B.iout[(size_t)b * M + lane] = I.bc_node_is_end[lane] ? N : 0;
         } else if (op == OP_NEWTON_STEP) {
             double ss;
             bresidual<M, METHOD, LT>(I, sh, mesh, y, nullptr, 0.0, N, lane, &ss);

@@ -78,3 +78,31 @@ def bratu_exact(lam, t):
     for _ in range(400):
         th = math.sqrt(2 * lam) * math.cosh(th / 4)
     return -2 * np.log(np.cosh((np.asarray(t) - 0.5) * th / 2) / math.cosh(th / 4))
+
+
+def lintanh_batch(n=2048, seed=SEED_LINEAR_TANH, M=32, Nrange=(512, 4096), spread=(0.5, 50.0), c=0.1):
+    """u' = A u + c tanh(u) on [0,1] with M components: A = Q diag(mu) Q^T, Q from the
+    QR factorisation of a seeded Gaussian matrix (signs fixed), mu = -/+ magnitudes
+    log-spaced in [0.5, 50] (half each); separated BCs, the first M/2 components fixed
+    at t = 0 and the last M/2 at t = 1 to N(0,1) values; N ~ U{512..4096}; zero guess.
+    Gaussian draws use numpy's generator seeded with (seed, i); N uses splitmix64."""
+    u = _u01(seed, n, 1)
+    Ns = _uint(u[:, 0], Nrange[0], Nrange[1])
+    half = M // 2
+    mags = np.logspace(math.log10(spread[0]), math.log10(spread[1]), half)
+    mu = np.concatenate([-mags, mags])
+    zero = np.zeros(M)
+    bc_t = np.concatenate([np.zeros(half), np.ones(M - half)])
+    bc_c = np.arange(M, dtype=np.int32)
+    specs = []
+    for i in range(n):
+        rng = np.random.default_rng([seed, i])
+        G = rng.standard_normal((M, M))
+        Q, R = np.linalg.qr(G)
+        Q = Q * np.sign(np.diag(R))
+        A = (Q * mu) @ Q.T
+        bcv = rng.standard_normal(M)
+        p = np.concatenate([[c], A.reshape(-1)])
+        specs.append(InstanceSpec(_abi.FAMILY_LINEAR_TANH, M, 0.0, 1.0, p, _abi.BC_TWO_POINT, bc_t, bc_c, bcv,
+                                  int(Ns[i]), None, None, zero))
+    return specs, Ns
