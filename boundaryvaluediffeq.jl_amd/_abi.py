@@ -32,6 +32,10 @@ TRANSFER_CLEAN = 1
 WARM_NONE = 0
 WARM_MLP = 1
 
+MLP_NPARAM = 133378
+FEAT_PARAM = 0
+FEAT_LOG10_PARAM = 1
+
 OK = 0
 EINVAL = -1
 EHIP = -2
@@ -91,6 +95,19 @@ class Result(C.Structure):
         ("resid_norm", C.c_double),
         ("mesh", _dp),
         ("u", _dp),
+    ]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_int32),
+        ("batch", C.c_int32),
+        ("lr", C.c_float),
+        ("lr_min", C.c_float),
+        ("weight_decay", C.c_float),
+        ("warmup", C.c_int32),
+        ("seed", C.c_uint64),
+        ("use_graph", C.c_int32),
     ]
 
 

@@ -14,6 +14,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "_build" / "libbvp_mirk.so"
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lp = C.POINTER(C.c_longlong)
+_fp = C.POINTER(C.c_float)
 
 
 class BVPError(RuntimeError):
@@ -54,6 +55,17 @@ def lib():
         "bvp_batch_info": (C.c_int, [vp, _ip]),
         "bvp_batch_destroy": (None, [vp]),
         "bvp_copy_probe": (C.c_int, [vp, C.c_longlong, C.c_int, C.POINTER(C.c_float)]),
+        "bvp_batch_set_graph": (C.c_int, [vp, C.c_int32]),
+        "bvp_mlp_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+        "bvp_mlp_destroy": (None, [vp]),
+        "bvp_mlp_set_normalization": (C.c_int, [vp, _fp, _fp, _fp, _fp]),
+        "bvp_mlp_get_params": (C.c_int, [vp, _fp]),
+        "bvp_mlp_set_params": (C.c_int, [vp, _fp]),
+        "bvp_mlp_gradient": (C.c_int, [vp, C.c_int32, _fp, _fp, _fp, _fp]),
+        "bvp_mlp_predict": (C.c_int, [vp, C.c_int32, _fp, _fp]),
+        "bvp_mlp_train": (C.c_int, [vp, C.c_int64, _fp, _fp, C.POINTER(_abi.TrainOpts), _fp, _ip,
+                                    C.POINTER(C.c_float)]),
+        "bvp_batch_set_warm_start": (C.c_int, [vp, vp, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -67,7 +79,10 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_mirk_solve_batch", "bvp_mirk_residual", "bvp_mirk_jacobian_blocks",
             "bvp_mirk_newton_step", "bvp_mirk_defect", "bvp_mirk_mesh_select", "bvp_batch_create",
             "bvp_batch_run", "bvp_batch_sync", "bvp_batch_results", "bvp_batch_counters",
-            "bvp_batch_info", "bvp_batch_destroy", "bvp_copy_probe"]
+            "bvp_batch_info", "bvp_batch_destroy", "bvp_copy_probe", "bvp_batch_set_graph",
+            "bvp_mlp_create", "bvp_mlp_destroy", "bvp_mlp_set_normalization", "bvp_mlp_get_params",
+            "bvp_mlp_set_params", "bvp_mlp_gradient", "bvp_mlp_predict", "bvp_mlp_train",
+            "bvp_batch_set_warm_start"]
 
 
 class Context:
@@ -263,6 +278,21 @@ class Batch:
         self.ctx.check(lib().bvp_batch_results(self.ctx.handle, self._h, res), "batch_results")
         return [RawSolution(res[i], self.specs[i].M, *bufs[i]) for i in range(len(self.specs))]
 
+    def set_graph(self, enable):
+        """Capture the run's launches in a hipGraph and replay it (default on)."""
+        rc = lib().bvp_batch_set_graph(self._h, 1 if enable else 0)
+        if rc != 0:
+            raise BVPError(f"batch_set_graph failed ({rc})")
+
+    def set_warm_start(self, net, feature_kind=_abi.FEAT_PARAM):
+        """Attach an MLP (or detach with None): each run writes the learned guess
+        of every node on the device before the solve."""
+        self._net = net
+        h = net.handle if net is not None else C.c_void_p()
+        rc = lib().bvp_batch_set_warm_start(self._h, h, feature_kind)
+        if rc != 0:
+            raise BVPError(f"batch_set_warm_start failed ({rc}): {lib().bvp_last_error(self.ctx.handle).decode()}")
+
     def counters(self):
         c = (C.c_longlong * 4)()
         self.ctx.check(lib().bvp_batch_counters(self.ctx.handle, self._h, c), "batch_counters")
@@ -276,6 +306,88 @@ class Batch:
     def close(self):
         if self._h:
             lib().bvp_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class MLP:
+    """The warm-start network (bvp_mlp_* entry points): 4 -> 256 -> 256 -> 256 -> 2,
+    tanh, BF16 weights, FP32 accumulation, trained and evaluated on the GPU."""
+
+    NPARAM = _abi.MLP_NPARAM
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        ctx.check(lib().bvp_mlp_create(ctx.handle, C.c_uint64(seed), C.byref(self._h)), "mlp_create")
+        self.norm = (np.zeros(4, np.float32), np.ones(4, np.float32), np.zeros(2, np.float32),
+                     np.ones(2, np.float32))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_normalization(self, in_shift, in_scale, out_shift, out_scale):
+        v = [_f32(in_shift, 4), _f32(in_scale, 4), _f32(out_shift, 2), _f32(out_scale, 2)]
+        self.ctx.check(lib().bvp_mlp_set_normalization(self._h, *[a.ctypes.data_as(_fp) for a in v]),
+                       "mlp_set_normalization")
+        self.norm = tuple(v)
+
+    def params(self):
+        w = np.zeros(self.NPARAM, np.float32)
+        self.ctx.check(lib().bvp_mlp_get_params(self._h, w.ctypes.data_as(_fp)), "mlp_get_params")
+        return w
+
+    def set_params(self, w):
+        w = _f32(w, self.NPARAM)
+        self.ctx.check(lib().bvp_mlp_set_params(self._h, w.ctypes.data_as(_fp)), "mlp_set_params")
+
+    def gradient(self, x, y):
+        x = _f32(x).reshape(-1, 4)
+        y = _f32(y).reshape(-1, 2)
+        g = np.zeros(self.NPARAM, np.float32)
+        loss = C.c_float()
+        self.ctx.check(lib().bvp_mlp_gradient(self._h, len(x), x.ctypes.data_as(_fp), y.ctypes.data_as(_fp),
+                                              g.ctypes.data_as(_fp), C.byref(loss)), "mlp_gradient")
+        return float(loss.value), g
+
+    def predict(self, x):
+        x = _f32(x).reshape(-1, 4)
+        y = np.zeros((len(x), 2), np.float32)
+        self.ctx.check(lib().bvp_mlp_predict(self._h, len(x), x.ctypes.data_as(_fp), y.ctypes.data_as(_fp)),
+                       "mlp_predict")
+        return y
+
+    def train(self, x, y, epochs=10, batch=8192, lr=2e-3, lr_min=2e-5, weight_decay=0.0, warmup=100, seed=7,
+              use_graph=True):
+        """Adam on (x, y); returns (loss per step, device ms)."""
+        x = _f32(x).reshape(-1, 4)
+        y = _f32(y).reshape(-1, 2)
+        o = _abi.TrainOpts(epochs, batch, lr, lr_min, weight_decay, warmup, seed, 1 if use_graph else 0)
+        nsteps = (len(x) // batch) * epochs
+        loss = np.zeros(max(nsteps, 1), np.float32)
+        steps = C.c_int32()
+        ms = C.c_float()
+        self.ctx.check(lib().bvp_mlp_train(self._h, len(x), x.ctypes.data_as(_fp), y.ctypes.data_as(_fp),
+                                           C.byref(o), loss.ctypes.data_as(_fp), C.byref(steps), C.byref(ms)),
+                       "mlp_train")
+        return loss[:steps.value], float(ms.value)
+
+    def close(self):
+        if self._h:
+            lib().bvp_mlp_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

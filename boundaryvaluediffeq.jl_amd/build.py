@@ -9,7 +9,7 @@ CSRC = HERE / "csrc"
 OUT = HERE / "_build"
 LIB = OUT / "libbvp_mirk.so"
 SOURCES = ["bvp_api.hip", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m3.hip", "wave_m4.hip",
-           "block_dispatch.hip", "block_m8.hip", "block_m16.hip", "block_m32.hip"]
+           "block_dispatch.hip", "block_m8.hip", "block_m16.hip", "block_m32.hip", "mlp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BVP_OFFLOAD_ARCH", "gfx950")

@@ -151,6 +151,17 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         return BVP_ENOTSUP;
     }
     if (o->method != BVP_MIRK4 && o->method != BVP_MIRK6) { set_err(ctx, "method must be 4 or 6"); return BVP_EINVAL; }
+    bool has_u0 = true;
+    for (int i = 0; i < n; ++i) has_u0 &= inst[i].u0 != nullptr;
+    if (o->warm_start == BVP_WARM_MLP) {
+        if (blk || M != 2) { set_err(ctx, "the learned warm start serves M = 2 problems"); return BVP_ENOTSUP; }
+        if (!has_u0) { set_err(ctx, "warm start: every instance needs u0 (the fallback guess)"); return BVP_EINVAL; }
+        for (int i = 0; i < n; ++i)
+            if (!inst[i].guess) { set_err(ctx, "warm start: instance %d has no guess", i); return BVP_EINVAL; }
+    } else if (o->warm_start != BVP_WARM_NONE) {
+        set_err(ctx, "unknown warm_start");
+        return BVP_EINVAL;
+    }
     if (!(o->abstol > 0)) { set_err(ctx, "abstol must be positive"); return BVP_EINVAL; }
     int maxN0 = 0;
     for (int i = 0; i < n; ++i) maxN0 = std::max(maxN0, inst[i].N);
@@ -162,6 +173,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->cap = cap;
     B->opts = *o;
     B->block = blk;
+    B->has_u0 = has_u0;
     // ---- pack the problem description
     std::vector<int> fam(n), poff(n), pcnt(n), bck(n), N0(n), bcc((size_t)n * M), order(n);
     std::vector<double> t0(n), t1(n), params, bct((size_t)n * M), bcv((size_t)n * M);
@@ -188,7 +200,10 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->N0 = N0;
     B->init_off = ioff;
     B->total_nodes = nodes;
-    std::vector<double> imesh((size_t)nodes), iy((size_t)nodes * M);
+    std::vector<double> imesh((size_t)nodes), iy((size_t)nodes * M), u0c((size_t)n * M, 0.0);
+    for (int i = 0; i < n; ++i)
+        if (inst[i].u0)
+            for (int c = 0; c < M; ++c) u0c[(size_t)i * M + c] = inst[i].u0[c];
     for (int i = 0; i < n; ++i) {
         const bvp_instance &P = inst[i];
         double *m = &imesh[(size_t)ioff[i]];
@@ -211,19 +226,22 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     D.cap = cap;
     DevArena &A = B->arena;
     int *dfam, *dpoff, *dpcnt, *dbck, *dbcc, *dN0, *dorder;
-    double *dt0, *dt1, *dparams, *dbct, *dbcv, *dim, *diy;
+    double *dt0, *dt1, *dparams, *dbct, *dbcv, *dim, *diy, *du0c;
     long long *dioff;
     if ((rc = upload(ctx, A, dfam, fam)) || (rc = upload(ctx, A, dpoff, poff)) || (rc = upload(ctx, A, dpcnt, pcnt)) ||
         (rc = upload(ctx, A, dbck, bck)) || (rc = upload(ctx, A, dbcc, bcc)) || (rc = upload(ctx, A, dN0, N0)) ||
         (rc = upload(ctx, A, dorder, order)) || (rc = upload(ctx, A, dt0, t0)) || (rc = upload(ctx, A, dt1, t1)) ||
         (rc = upload(ctx, A, dparams, params)) || (rc = upload(ctx, A, dbct, bct)) || (rc = upload(ctx, A, dbcv, bcv)) ||
-        (rc = upload(ctx, A, dim, imesh)) || (rc = upload(ctx, A, diy, iy)) || (rc = upload(ctx, A, dioff, ioff))) {
+        (rc = upload(ctx, A, dim, imesh)) || (rc = upload(ctx, A, diy, iy)) || (rc = upload(ctx, A, dioff, ioff)) ||
+        (rc = upload(ctx, A, du0c, u0c))) {
         delete B;
         return rc;
     }
     D.family = dfam; D.param_off = dpoff; D.param_cnt = dpcnt; D.bc_kind = dbck; D.bc_comp = dbcc;
     D.N0 = dN0; D.order = dorder; D.t0 = dt0; D.t1 = dt1; D.params = dparams; D.bc_t = dbct; D.bc_val = dbcv;
     D.init_mesh = dim; D.init_y = diy; D.init_off = dioff;
+    D.u0c = du0c;
+    D.warm = (o->warm_start == BVP_WARM_MLP) ? 1 : 0;
     // working state and results
     if ((rc = dalloc(ctx, A, D.mesh, (size_t)n * 2 * cap)) || (rc = dalloc(ctx, A, D.y, (size_t)n * 2 * cap * M)) ||
         (rc = dalloc(ctx, A, D.shat, (size_t)n * cap)) || (rc = dalloc(ctx, A, D.retcode, n)) ||
@@ -303,14 +321,51 @@ static bool fp64_block32_blocked(bvp_ctx *ctx, const bvp_batch *B) {
     return false;
 }
 
+// The launches of one run: queue reset (a fill kernel, so the sequence can be
+// captured), the learned guess of every node when a network is attached, the
+// persistent solve.
+static int enqueue_solve(bvp_ctx *ctx, bvp_batch *B) {
+    BatchDev D = B->dev;
+    HIP_TRY(ctx, wave_reset_counter(D.counter, ctx->stream));
+    if (B->mlp) {
+        HIP_TRY(ctx, mlp::launch_batch_guess(B->mlp->net(), D, B->warm_y, B->total_nodes, B->feature_kind,
+                                             ctx->stream));
+        D.init_y = B->warm_y;
+        D.warm = 1;
+    }
+    if (B->block)
+        HIP_TRY(ctx, block_launch_solve(D, B->method, B->opts.precision, B->blocks, ctx->stream));
+    else
+        HIP_TRY(ctx, wave_launch_solve(D, B->method, B->blocks, ctx->stream));
+    return 0;
+}
+
 int batch_run(bvp_ctx *ctx, bvp_batch *B) {
     if (fp64_block32_blocked(ctx, B)) return BVP_ENOTSUP;
-    HIP_TRY(ctx, wave_reset_counter(B->dev.counter, ctx->stream));
+    if (!B->use_graph) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        int rc = enqueue_solve(ctx, B);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        return 0;
+    }
+    if (B->graph_exec && B->mlp && B->mlp_gen != B->mlp->gen) B->drop_graph();  // normalisation changed
+    if (!B->graph_exec) {
+        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_solve(ctx, B);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIP_TRY(ctx, e);
+        B->graph = g;
+        HIP_TRY(ctx, hipGraphInstantiate(&B->graph_exec, g, nullptr, nullptr, 0));
+        if (B->mlp) B->mlp_gen = B->mlp->gen;
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (B->block)
-        HIP_TRY(ctx, block_launch_solve(B->dev, B->method, B->opts.precision, B->blocks, ctx->stream));
-    else
-        HIP_TRY(ctx, wave_launch_solve(B->dev, B->method, B->blocks, ctx->stream));
+    HIP_TRY(ctx, hipGraphLaunch(B->graph_exec, ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     return 0;
 }
@@ -628,6 +683,265 @@ int bvp_copy_probe(bvp_ctx *ctx, long long nbytes, int repeats, float *ms) {
         best = std::min(best, t);
     }
     if (ms) *ms = best;
+    return 0;
+}
+
+int bvp_batch_set_graph(bvp_batch *B, int32_t enable) {
+    if (!B) return BVP_EINVAL;
+    B->drop_graph();
+    B->use_graph = enable != 0;
+    return 0;
+}
+
+// ---- learned warm start
+static uint64_t mix64(uint64_t &s) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double unit01(uint64_t &s) { return (double)(mix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+void bvp_mlp_destroy(bvp_mlp *m) {
+    if (!m) return;
+    if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+}
+
+static int mlp_clear_state(bvp_mlp *m) {
+    bvp_ctx *ctx = m->ctx;
+    HIP_TRY(ctx, hipMemsetAsync(m->nb.m, 0, sizeof(float) * mlp::NPARAM, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(m->nb.v, 0, sizeof(float) * mlp::NPARAM, ctx->stream));
+    HIP_TRY(ctx, mlp::launch_quantize(m->nb, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int bvp_mlp_set_params(bvp_mlp *m, const float *w) {
+    if (!m || !w) return BVP_EINVAL;
+    bvp_ctx *ctx = m->ctx;
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipMemcpyAsync(m->nb.w, w, sizeof(float) * mlp::NPARAM, hipMemcpyHostToDevice, ctx->stream));
+    return mlp_clear_state(m);
+}
+
+int bvp_mlp_create(bvp_ctx *ctx, uint64_t seed, bvp_mlp **out) {
+    if (!ctx || !out) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    *out = nullptr;
+    auto *m = new bvp_mlp();
+    m->ctx = ctx;
+    int rc;
+    mlp::NetBufs &nb = m->nb;
+    if ((rc = dalloc(ctx, m->arena, nb.w, mlp::NPARAM)) || (rc = dalloc(ctx, m->arena, nb.m, mlp::NPARAM)) ||
+        (rc = dalloc(ctx, m->arena, nb.v, mlp::NPARAM)) || (rc = dalloc(ctx, m->arena, nb.q, mlp::NPARAM)) ||
+        (rc = dalloc(ctx, m->arena, nb.w2f, (size_t)mlp::H * mlp::H)) ||
+        (rc = dalloc(ctx, m->arena, nb.w3f, (size_t)mlp::H * mlp::H)) ||
+        (rc = dalloc(ctx, m->arena, nb.w2b, (size_t)mlp::H * mlp::H)) ||
+        (rc = dalloc(ctx, m->arena, nb.w3b, (size_t)mlp::H * mlp::H))) {
+        delete m;
+        return rc;
+    }
+    // Glorot-uniform matrices, zero biases
+    std::vector<float> w(mlp::NPARAM, 0.f);
+    uint64_t s = seed;
+    auto fill = [&](int off, int fin, int fout) {
+        const double r = std::sqrt(6.0 / (fin + fout));
+        for (int e = 0; e < fin * fout; ++e) w[off + e] = (float)((2.0 * unit01(s) - 1.0) * r);
+    };
+    fill(mlp::OW1, mlp::NIN, mlp::H);
+    fill(mlp::OW2, mlp::H, mlp::H);
+    fill(mlp::OW3, mlp::H, mlp::H);
+    fill(mlp::OW4, mlp::H, mlp::NOUT);
+    if ((rc = bvp_mlp_set_params(m, w.data()))) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+int bvp_mlp_set_normalization(bvp_mlp *m, const float *in_shift, const float *in_scale, const float *out_shift,
+                              const float *out_scale) {
+    if (!m || !in_shift || !in_scale || !out_shift || !out_scale) return BVP_EINVAL;
+    for (int i = 0; i < mlp::NIN; ++i) {
+        if (!std::isfinite(in_shift[i]) || !std::isfinite(in_scale[i]) || in_scale[i] == 0.f) {
+            set_err(m->ctx, "bad input normalisation");
+            return BVP_EINVAL;
+        }
+        m->in_shift[i] = in_shift[i];
+        m->in_scale[i] = in_scale[i];
+    }
+    for (int o = 0; o < mlp::NOUT; ++o) {
+        if (!std::isfinite(out_shift[o]) || !std::isfinite(out_scale[o]) || out_scale[o] == 0.f) {
+            set_err(m->ctx, "bad output normalisation");
+            return BVP_EINVAL;
+        }
+        m->out_shift[o] = out_shift[o];
+        m->out_scale[o] = out_scale[o];
+    }
+    m->gen++;
+    return 0;
+}
+
+int bvp_mlp_get_params(bvp_mlp *m, float *w) {
+    if (!m || !w) return BVP_EINVAL;
+    bvp_ctx *ctx = m->ctx;
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(w, m->nb.w, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int bvp_mlp_gradient(bvp_mlp *m, int32_t n, const float *x, const float *y, float *grad, float *loss) {
+    if (!m || n < 1 || !x || !y || !grad) return BVP_EINVAL;
+    bvp_ctx *ctx = m->ctx;
+    (void)hipSetDevice(ctx->device);
+    DevArena A;
+    float *dx, *dy, *dpart, *dg;
+    const int tiles = (n + mlp::T - 1) / mlp::T;
+    int rc;
+    if ((rc = dalloc(ctx, A, dx, (size_t)n * mlp::NIN)) || (rc = dalloc(ctx, A, dy, (size_t)n * mlp::NOUT)) ||
+        (rc = dalloc(ctx, A, dpart, (size_t)tiles * mlp::NSLOT)) || (rc = dalloc(ctx, A, dg, mlp::NSLOT)))
+        return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(dx, x, sizeof(float) * n * mlp::NIN, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dy, y, sizeof(float) * n * mlp::NOUT, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, mlp::launch_grad(m->net(), dx, dy, n, dpart, dg, ctx->stream));
+    std::vector<float> h(mlp::NSLOT);
+    HIP_TRY(ctx, hipMemcpyAsync(h.data(), dg, sizeof(float) * mlp::NSLOT, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    std::memcpy(grad, h.data(), sizeof(float) * mlp::NPARAM);
+    if (loss) *loss = h[mlp::NPARAM];
+    return 0;
+}
+
+int bvp_mlp_predict(bvp_mlp *m, int32_t n, const float *x, float *y) {
+    if (!m || n < 1 || !x || !y) return BVP_EINVAL;
+    bvp_ctx *ctx = m->ctx;
+    (void)hipSetDevice(ctx->device);
+    DevArena A;
+    float *dx, *dy;
+    int rc;
+    if ((rc = dalloc(ctx, A, dx, (size_t)n * mlp::NIN)) || (rc = dalloc(ctx, A, dy, (size_t)n * mlp::NOUT))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(dx, x, sizeof(float) * n * mlp::NIN, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, mlp::launch_predict(m->net(), dx, n, dy, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(y, dy, sizeof(float) * n * mlp::NOUT, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+static long long gcd_ll(long long a, long long b) {
+    while (b) { const long long t = a % b; a = b; b = t; }
+    return a;
+}
+
+int bvp_mlp_train(bvp_mlp *m, int64_t n, const float *x, const float *y, const bvp_train_opts *o,
+                  float *loss_per_step, int32_t *steps, float *ms) {
+    if (!m || !x || !y || !o) return BVP_EINVAL;
+    bvp_ctx *ctx = m->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (o->batch < mlp::T || o->batch % mlp::T) { set_err(ctx, "batch must be a positive multiple of 64"); return BVP_EINVAL; }
+    if (n < o->batch || n >= (1ll << 31)) { set_err(ctx, "need batch <= n < 2^31 samples"); return BVP_EINVAL; }
+    if (o->epochs < 1 || !(o->lr > 0)) { set_err(ctx, "need epochs >= 1 and lr > 0"); return BVP_EINVAL; }
+    const int spe = (int)(n / o->batch);
+    const long long total = (long long)spe * o->epochs;
+    if (total >= (1ll << 31)) { set_err(ctx, "too many steps"); return BVP_EINVAL; }
+    // per-epoch sample order j -> (a j + c) mod n with gcd(a, n) = 1
+    std::vector<unsigned long long> perm(2 * (size_t)o->epochs);
+    uint64_t s = o->seed ^ 0x6A09E667F3BCC909ull;
+    for (int e = 0; e < o->epochs; ++e) {
+        long long a = 1;
+        if (n > 2)
+            do { a = 1 + (long long)(unit01(s) * (double)(n - 1)); } while (a >= n || gcd_ll(a, n) != 1);
+        perm[2 * e] = (unsigned long long)a;
+        perm[2 * e + 1] = (unsigned long long)(unit01(s) * (double)n) % (unsigned long long)n;
+    }
+    DevArena A;
+    float *dx, *dy, *dpart, *dloss;
+    unsigned long long *dperm;
+    int *dstep;
+    const int tiles = o->batch / mlp::T;
+    int rc;
+    if ((rc = dalloc(ctx, A, dx, (size_t)n * mlp::NIN)) || (rc = dalloc(ctx, A, dy, (size_t)n * mlp::NOUT)) ||
+        (rc = dalloc(ctx, A, dpart, (size_t)tiles * mlp::NSLOT)) || (rc = dalloc(ctx, A, dloss, (size_t)total)) ||
+        (rc = upload(ctx, A, dperm, perm)) || (rc = dalloc(ctx, A, dstep, 1)))
+        return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(dx, x, sizeof(float) * n * mlp::NIN, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dy, y, sizeof(float) * n * mlp::NOUT, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(dstep, 0, sizeof(int), ctx->stream));
+    mlp::TrainDev td{};
+    td.x = dx;
+    td.y = dy;
+    td.n = n;
+    td.batch_tiles = tiles;
+    td.steps_per_epoch = spe;
+    td.total_steps = (int)total;
+    td.perm = dperm;
+    td.step = dstep;
+    td.partial = dpart;
+    td.loss = dloss;
+    td.lr = o->lr;
+    td.lr_min = o->lr_min;
+    td.beta1 = 0.9f;
+    td.beta2 = 0.999f;
+    td.eps = 1e-8f;
+    td.weight_decay = o->weight_decay;
+    td.warmup = o->warmup;
+    const mlp::Net net = m->net();
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (o->use_graph) {
+        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e1 = mlp::launch_train_step(m->nb, net, td, ctx->stream);
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            set_err(ctx, "capture of the training step failed: %s",
+                    hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+            return BVP_EHIP;
+        }
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            set_err(ctx, "graph instantiation failed");
+            return BVP_EHIP;
+        }
+    }
+    hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+    for (long long k = 0; k < total && e == hipSuccess; ++k)
+        e = o->use_graph ? hipGraphLaunch(ge, ctx->stream) : mlp::launch_train_step(m->nb, net, td, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    HIP_TRY(ctx, e);
+    if (loss_per_step)
+        HIP_TRY(ctx, hipMemcpy(loss_per_step, dloss, sizeof(float) * total, hipMemcpyDeviceToHost));
+    if (steps) *steps = (int32_t)total;
+    if (ms) *ms = t;
+    return 0;
+}
+
+int bvp_batch_set_warm_start(bvp_batch *B, const bvp_mlp *m, int32_t kind) {
+    if (!B) return BVP_EINVAL;
+    if (!m) {
+        B->mlp = nullptr;
+        B->drop_graph();
+        return 0;
+    }
+    bvp_ctx *ctx = m->ctx;
+    if (B->block || B->M != 2) { set_err(ctx, "the learned warm start serves M = 2 plans"); return BVP_ENOTSUP; }
+    if (!B->has_u0) { set_err(ctx, "warm start: every instance needs u0 (the fallback guess)"); return BVP_EINVAL; }
+    if (kind != BVP_FEAT_PARAM && kind != BVP_FEAT_LOG10_PARAM) { set_err(ctx, "bad feature kind"); return BVP_EINVAL; }
+    if (!B->warm_y) {
+        const int rc = dalloc(ctx, B->arena, B->warm_y, (size_t)B->total_nodes * B->M);
+        if (rc) return rc;
+    }
+    B->mlp = m;
+    B->feature_kind = kind;
+    B->drop_graph();
     return 0;
 }
 

@@ -8,6 +8,7 @@
 
 #include "../../include/bvp_mirk.h"
 #include "mirk_wave.h"
+#include "mlp.h"
 
 struct bvp_ctx {
     int device = 0;
@@ -28,9 +29,25 @@ struct DevArena {
     ~DevArena();
 };
 
+struct bvp_mlp {
+    bvp_ctx *ctx = nullptr;
+    mlp::NetBufs nb{};
+    float in_shift[mlp::NIN] = {0, 0, 0, 0}, in_scale[mlp::NIN] = {1, 1, 1, 1};
+    float out_shift[mlp::NOUT] = {0, 0}, out_scale[mlp::NOUT] = {1, 1};
+    int gen = 0;  // bumped when the normalisation changes (plans re-capture)
+    DevArena arena;
+    mlp::Net net() const {
+        mlp::Net n = nb.view();
+        for (int i = 0; i < mlp::NIN; ++i) { n.in_shift[i] = in_shift[i]; n.in_scale[i] = in_scale[i]; }
+        for (int o = 0; o < mlp::NOUT; ++o) { n.out_shift[o] = out_shift[o]; n.out_scale[o] = out_scale[o]; }
+        return n;
+    }
+};
+
 struct bvp_batch {
     int n = 0, M = 0, method = 4, cap = 0, blocks = 0;
     bool block = false;  // served by the block kernels (M = 8, 16, 32)
+    bool has_u0 = false; // every instance carries u0 (needed by a warm start)
     bvp_opts opts{};
     std::vector<int> N0;
     std::vector<long long> init_off;
@@ -38,6 +55,22 @@ struct bvp_batch {
     float last_ms = 0.f;
     BatchDev dev{};
     DevArena arena;
+    // learned warm start attached to the plan
+    const bvp_mlp *mlp = nullptr;
+    int feature_kind = 0;
+    double *warm_y = nullptr;  // [total_nodes][M]
+    int mlp_gen = 0;
+    // graph replay of run()
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+    ~bvp_batch() { drop_graph(); }
 };
 
 int wave_blocks_per_cu(int M, int method);

@@ -56,6 +56,11 @@ struct BatchDev {
     double *out0, *out1;
     int *iout;
 
+    // learned warm start: init_y holds the learned guess; the first Newton step on
+    // the first mesh must contract max|F|, else the solve restarts from u0c
+    int warm;
+    const double *u0c;        // [n][M] constant guess of each instance
+
     int rec_cap;              // block kernels: intervals whose factors fit a wave's storage
     int refine_iters;         // FP64 refinement steps per linear solve with FP32 factors
 

@@ -846,7 +846,7 @@ __device__ inline double residual_sq_trial(const Inst<M> &I, const double *mesh,
 }
 
 struct NewtonOut {
-    int ret, iters, fallback;
+    int ret, iters, fallback, warm_fallback;
 };
 
 // NewtonRaphson, then NewtonRaphson with backtracking from the same start
@@ -855,8 +855,13 @@ struct NewtonOut {
 template <int M, int METHOD>
 __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y,
                             double *ybak, double *delta, double *ytmp, int N, int lane, double *scr,
-                            size_t tree_base) {
-    NewtonOut out{BVP_RET_MAXITERS, 0, 0};
+                            size_t tree_base, const double *u0c) {
+    NewtonOut out{BVP_RET_MAXITERS, 0, 0, 0};
+    // u0c != null: y holds a learned warm start; when the first Newton step from it
+    // does not contract max|F| (or the guess is unusable), restart from the constant
+    // guess u0c, as the plain method would have started.
+    bool warm = u0c != nullptr;
+    double fn_warm = 0.0;
     const int attempts = B.linesearch ? 2 : 1;
     if (B.linesearch) {
         for (int k = lane; k <= N; k += WAVE)
@@ -874,13 +879,49 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
         int ret = BVP_RET_MAXITERS;
         for (int it = 0;; ++it) {
             Rel<M> rel;
-            const double fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+            double fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+            if (warm) {
+                bool fall;
+                if (it == 0) {
+                    fn_warm = fn;
+                    fall = !(fn <= 1.7976931348623157e308);
+                } else {
+                    fall = !(fn < fn_warm);
+                    warm = false;
+                }
+                if (fall) {
+                    warm = false;
+                    out.warm_fallback = 1;
+                    for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            y[(size_t)k * M + c] = u0c[c];
+                            if (B.linesearch) ybak[(size_t)k * M + c] = u0c[c];
+                        }
+                    __syncthreads();
+                    fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+                }
+            }
             if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
             if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
             if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
             const bool plain = (attempt == 0);
             if (!solve_correction<M, METHOD>(I, sh, mesh, y, N, lane, scr, tree_base, rel, plain ? nullptr : delta,
                                              plain)) {
+                if (warm) {  // singular at the warm start: restart from the constant guess
+                    warm = false;
+                    out.warm_fallback = 1;
+                    out.iters++;
+                    for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            y[(size_t)k * M + c] = u0c[c];
+                            if (B.linesearch) ybak[(size_t)k * M + c] = u0c[c];
+                        }
+                    __syncthreads();
+                    it = -1;
+                    continue;
+                }
                 ret = BVP_RET_FAILURE;
                 break;
             }
@@ -1140,11 +1181,13 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         ++outer;
         double *mesh = bf.mesh[cur], *y = bf.y[cur];
         double *mesh_o = bf.mesh[1 - cur], *y_o = bf.y[1 - cur];
-        NewtonOut nl = newton<M, METHOD>(B, I, sh, mesh, y, y_o, delta, ytmp, N, lane, scr, tree_base);
+        const double *u0c = (B.warm && outer == 1) ? B.u0c + (size_t)b * M : nullptr;
+        NewtonOut nl = newton<M, METHOD>(B, I, sh, mesh, y, y_o, delta, ytmp, N, lane, scr, tree_base, u0c);
         iters += nl.iters;
         cnt_it += (long long)N * nl.iters;
-        cnt_chk += N;
+        cnt_chk += N + (nl.warm_fallback ? N : 0);
         if (nl.fallback) fb |= 1;
+        if (nl.warm_fallback) fb |= 2;
         nl_ret = nl.ret;
         double error_norm = 2.0 * B.abstol;
         if (!B.adaptive) { info = nl.ret; break; }

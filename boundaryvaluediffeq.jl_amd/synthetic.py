@@ -15,6 +15,8 @@ GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 SEED_BRATU16K = 0xB0A70002
 SEED_LINEAR_TANH = 0xB0A70003
 SEED_SINGPERT = 0xB0A70005
+SEED_BRATU_AB_TRAIN = 0xB0A70004
+SEED_BRATU_AB_EVAL = 0xB0A70044
 
 
 def _u01(seed, n, ndraw):
@@ -50,6 +52,21 @@ def bratu_batch(n=16384, seed=SEED_BRATU16K, lam=(0.1, 3.4), Nrange=(64, 1024)):
         specs.append(InstanceSpec(_abi.FAMILY_BRATU, 2, 0.0, 1.0, [lams[k]], _abi.BC_TWO_POINT,
                                   [0.0, 1.0], [0, 0], [0.0, 0.0], int(Ns[k]), None, None, zero))
     return specs, lams, Ns
+
+
+def bratu_ab_batch(n, seed=SEED_BRATU_AB_EVAL, lam=(0.1, 3.4), ab=(-1.0, 1.0), Nrange=(64, 1024)):
+    """Bratu with u(0) = a, u(1) = b: lambda ~ U(0.1, 3.4), a, b ~ U(-1, 1),
+    N ~ U{64..1024}, uniform meshes, constant zero guess (u0 = 0).  Part of this
+    box has no solution (the solve then ends in Failure or MaxIters)."""
+    u = _u01(seed, n, 4)
+    lams = lam[0] + (lam[1] - lam[0]) * u[:, 0]
+    a = ab[0] + (ab[1] - ab[0]) * u[:, 1]
+    b = ab[0] + (ab[1] - ab[0]) * u[:, 2]
+    Ns = _uint(u[:, 3], Nrange[0], Nrange[1])
+    zero = np.zeros(2)
+    specs = [InstanceSpec(_abi.FAMILY_BRATU, 2, 0.0, 1.0, [lams[k]], _abi.BC_TWO_POINT, [0.0, 1.0], [0, 0],
+                          [a[k], b[k]], int(Ns[k]), None, None, zero) for k in range(n)]
+    return specs, np.stack([lams, a, b], axis=1), Ns
 
 
 def singpert_batch(n=8192, seed=SEED_SINGPERT, N0=32):

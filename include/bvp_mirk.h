@@ -228,6 +228,73 @@ int  bvp_batch_counters(bvp_ctx *ctx, bvp_batch *batch, long long *out4);
 int  bvp_batch_info(bvp_batch *batch, int32_t *out4);
 void bvp_batch_destroy(bvp_batch *batch);
 
+/* ---- hipGraph replay of a plan (no counterpart in the reference).  With
+ * enable != 0 (the default) the first run captures the plan's launches (queue
+ * reset, learned guess if attached, persistent solve) into a graph and later
+ * runs replay it; attaching or detaching a network re-captures. */
+int  bvp_batch_set_graph(bvp_batch *batch, int32_t enable);
+
+/* ---- learned warm start (no counterpart in the reference; it plugs in where
+ * the reference takes the initial guess from the caller, as a constant vector,
+ * a vector of arrays, a function of (p, t) or an earlier solution:
+ * __initial_guess_on_mesh, Core/utils.jl:750-777, and the u0 handling of
+ * MIRK/mirk.jl:83-95).
+ *
+ * The network is a 4 -> 256 -> 256 -> 256 -> 2 MLP with tanh hidden units,
+ * BF16 weights and FP32 accumulation, that maps
+ *     (p, a, b, t) -> (y1, y2)(t)
+ * for M = 2 families, p being params[0] (or log10 params[0], feature kind
+ * BVP_FEAT_LOG10_PARAM), a and b the values of boundary rows 0 and 1.  Inputs
+ * are normalised as (x - in_shift) * in_scale and outputs mapped back as
+ * out * out_scale + out_shift.  Parameter vector (FP32, BVP_MLP_NPARAM values,
+ * matrices input-major W[i][o]): W1[4][256] b1[256] W2[256][256] b2[256]
+ * W3[256][256] b3[256] W4[256][2] b4[2].
+ *
+ * The solve uses it as follows (opts.warm_start == BVP_WARM_MLP, or a plan with
+ * a network attached): Newton starts from the learned guess on the first mesh;
+ * when its first step does not reduce max|F| (or the guess gives a non-finite
+ * residual or a singular Newton matrix), the solve restarts from the constant
+ * guess u0 exactly as the plain method would, and sets bit1 of fallback_used. */
+#define BVP_MLP_NPARAM 133378
+#define BVP_FEAT_PARAM       0
+#define BVP_FEAT_LOG10_PARAM 1
+
+typedef struct bvp_mlp bvp_mlp;
+
+typedef struct {
+    int32_t  epochs;        /* passes over the samples */
+    int32_t  batch;         /* points per step, a multiple of 64 */
+    float    lr, lr_min;    /* peak and final learning rate (cosine decay) */
+    float    weight_decay;  /* decoupled, on weight matrices */
+    int32_t  warmup;        /* linear learning-rate warm-up steps */
+    uint64_t seed;          /* sample order */
+    int32_t  use_graph;     /* replay one captured training step */
+} bvp_train_opts;
+
+/* Glorot-uniform weights from `seed`, zero biases, identity normalisation. */
+int  bvp_mlp_create(bvp_ctx *ctx, uint64_t seed, bvp_mlp **out);
+void bvp_mlp_destroy(bvp_mlp *mlp);
+int  bvp_mlp_set_normalization(bvp_mlp *mlp, const float *in_shift, const float *in_scale,
+                               const float *out_shift, const float *out_scale);
+int  bvp_mlp_get_params(bvp_mlp *mlp, float *w);        /* FP32 master parameters */
+int  bvp_mlp_set_params(bvp_mlp *mlp, const float *w);  /* also clears the optimiser state */
+/* Loss L = sum_points 0.5 |out - y_normalised|^2 / n and its FP32 gradient with
+ * respect to the parameter vector, over n points x[n][4] (raw features) with
+ * targets y[n][2] (raw), in one forward/backward pass of the training kernels. */
+int  bvp_mlp_gradient(bvp_mlp *mlp, int32_t n, const float *x, const float *y, float *grad, float *loss);
+/* y[n][2] = network(x[n][4]) in raw units. */
+int  bvp_mlp_predict(bvp_mlp *mlp, int32_t n, const float *x, float *y);
+/* Adam training on n samples (n < 2^31); loss_per_step (optional) receives the
+ * loss of every step, *steps the number of steps, *ms the device time. */
+int  bvp_mlp_train(bvp_mlp *mlp, int64_t n, const float *x, const float *y, const bvp_train_opts *opts,
+                   float *loss_per_step, int32_t *steps, float *ms);
+/* Attach (or with mlp == NULL detach) a network to a plan: each run first writes
+ * the learned guess of every node on the device, then solves with the
+ * contraction-checked fallback described above.  M = 2 plans only; every
+ * instance must carry u0.  The plan keeps the pointer: detach or destroy the
+ * plan before the network. */
+int  bvp_batch_set_warm_start(bvp_batch *batch, const bvp_mlp *mlp, int32_t feature_kind);
+
 /* ---- measurement helper (no counterpart in the reference): a plain device
  * copy of nbytes, run `repeats` times; *ms = fastest device time.  It is the
  * known-size reference for scaling profiler HBM byte counters. */
