@@ -1,11 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark: batched MIRK4 solves of 16,384 seeded Bratu problems on one MI355X.
 
-A step is one full batched solve (Newton + defect control + mesh refinement,
-persistent kernel) of all instances, inputs resident in device memory.  Prints
-one JSON line with the throughput in BVP solves/s (successful solves with
-defect <= 1e-6), the roofline of the solve kernel, the oracle timed on the
-host cores, and the plain one-instance-at-a-time GPU rate.
+A step is one full batched solve (learned guess of every node, then Newton +
+defect control + mesh refinement in one persistent kernel, replayed as a
+hipGraph) of all instances, inputs resident in device memory.  The warm-start
+network is trained first (untimed) on solves of the Bratu family with boundary
+values u(0)=a, u(1)=b that the plain method itself generates.  Prints one JSON
+line with the throughput in BVP solves/s (successful solves with defect <=
+1e-6), the same batch without the network, the plain one-instance-at-a-time
+FP64 GPU rate, the roofline of the solve kernel and the oracle timed on the
+host cores.  Other workloads (--workload) print their own lines.
 """
 import argparse
 import json
@@ -102,6 +106,111 @@ def bench_lintanh(args, _abi, _lib, synthetic):
                       "peak_fp32_mfma_tflops": 157.3, "counters": counters}), flush=True)
 
 
+def timed_runs(batch, steps, warmup, tag):
+    for w in range(warmup):
+        batch.run()
+        log(f"[bench] {tag} warmup {w}: {batch.sync():.2f} ms")
+    kms, gms, sms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+        kms.append(batch.sync())
+        g, s_ = batch.timing()
+        gms.append(g)
+        sms.append(s_)
+    wall = (time.perf_counter() - t0) / steps
+    log(f"[bench] {tag}: run ms {['%.3f' % x for x in kms]} (guess {sum(gms) / steps:.3f}, "
+        f"solve {sum(sms) / steps:.3f}); wall {wall * 1e3:.3f} ms/step")
+    return dict(run_ms=sum(kms) / steps, guess_ms=sum(gms) / steps, solve_ms=sum(sms) / steps, wall=wall)
+
+
+def n_solved(res, abstol):
+    return sum(1 for r in res if r.retcode == 0 and r.defect_norm <= min(1e-6, abstol))
+
+
+def train_warm_start(ctx, args, opts, _abi, synthetic, learned):
+    """The warm-start network, trained on plain solves of the Bratu family with
+    u(0)=a, u(1)=b (lambda ~ U(0.1, 3.4), a, b ~ U(-1, 1)); untimed."""
+    specs, _, _ = synthetic.bratu_ab_batch(args.train_solves, seed=synthetic.SEED_BRATU_AB_TRAIN)
+    hopts = _abi.Opts.from_buffer_copy(opts)
+    hopts.max_newton_iters = 50  # the box has instances without a solution; they are left out anyway
+    t0 = time.perf_counter()
+    x, y, sols = learned.harvest(ctx, specs, hopts, per_solve=args.train_nodes)
+    th = time.perf_counter() - t0
+    solved = sum(1 for r in sols if r.retcode == 0)
+    log(f"[bench] training data: {len(specs)} plain solves ({solved} solved) -> {len(x)} samples in {th:.1f} s")
+    t0 = time.perf_counter()
+    net, loss, ms = learned.train(ctx, x, y, epochs=args.train_epochs, batch=8192, lr=2e-3, lr_min=2e-5,
+                                  warmup=200)
+    log(f"[bench] training: {len(loss)} Adam steps, loss {loss[0]:.3e} -> {loss[-1]:.3e}, device {ms:.0f} ms, "
+        f"wall {time.perf_counter() - t0:.1f} s")
+    return net, {"train_solves": len(specs), "train_solved": solved, "samples": int(len(x)),
+                 "nodes_per_solve": args.train_nodes, "adam_steps": int(len(loss)), "epochs": args.train_epochs,
+                 "final_loss": float(loss[-1]), "train_device_ms": ms, "harvest_s": th}
+
+
+def plain_single(ctx, opts, specs, sample, abstol, _lib):
+    """The plain form: one instance per launch, FP64 factors, constant guess."""
+    n = len(specs)
+    stride = max(1, n // sample)
+    idx = list(range(0, n, stride))[:sample]
+    tot, ok = 0.0, 0
+    for i in idx:
+        b1 = _lib.Batch(ctx, opts, [specs[i]])
+        b1.set_graph(False)
+        b1.run()
+        b1.sync()
+        ta = time.perf_counter()
+        b1.run()
+        b1.sync()
+        tot += time.perf_counter() - ta
+        ok += n_solved(b1.results(), abstol)
+        b1.close()
+    log(f"[bench] plain single-instance GPU: {ok / tot:.1f} solves/s")
+    return {"value": ok / tot, "unit": "solves/s",
+            "sample": f"{len(idx)} instances (every {stride}th), each solved alone in its own launch"}
+
+
+def cpu_baseline(specs, abstol, threads, method=4):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    oo = O.opts(method, abstol=abstol)
+    log(f"[bench] oracle on {threads} threads, {len(specs)} instances")
+    tc = time.perf_counter()
+    rets = O.solve_batch(oo, specs, nthreads=threads, keep_solutions=False)
+    tcpu = time.perf_counter() - tc
+    okc = sum(1 for r in rets if r == 0)
+    log(f"[bench] oracle: {okc / tcpu:.1f} solves/s")
+    return {"value": okc / tcpu, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{len(specs)} of the same instances, one per OpenMP thread (dynamic schedule), "
+                      f"{tcpu:.1f} s wall"}
+
+
+def bench_learned(args, _abi, _lib, synthetic, learned):
+    """Learned warm start on 16,384 unseen Bratu (a, b) instances: solves/s,
+    Newton iterations and fallback rate with and without the network."""
+    ctx = _lib.context(0)
+    opts = _abi.default_opts(_abi.MIRK4)
+    opts.abstol = 1e-8
+    net, tinfo = train_warm_start(ctx, args, opts, _abi, synthetic, learned)
+    specs, _, _ = synthetic.bratu_ab_batch(args.instances, seed=synthetic.SEED_BRATU_AB_EVAL)
+    batch = _lib.Batch(ctx, opts, specs)
+    tp = timed_runs(batch, args.steps, args.warmup, "plain guess")
+    rp = batch.results()
+    batch.set_warm_start(net, _abi.FEAT_PARAM, args.anchors)
+    tw = timed_runs(batch, args.steps, args.warmup, "learned guess")
+    rw = batch.results()
+    sp, sw = learned.summary(rp), learned.summary(rw)
+    log(f"[bench] plain {sp}\n[bench] learned {sw}")
+    print(json.dumps({"workload": f"bratu_ab{args.instances}: {args.instances} unseen Bratu BVPs with u(0)=a, "
+                                  "u(1)=b, lambda~U(0.1,3.4), a,b~U(-1,1), MIRK4, abstol 1e-8",
+                      "plain_guess": {"solves_per_s": n_solved(rp, 1e-8) / tp["wall"], **tp, **sp},
+                      "learned_guess": {"solves_per_s": n_solved(rw, 1e-8) / tw["wall"], **tw, **sw},
+                      "newton_iters_saved_per_solved_instance":
+                          sp["mean_newton_iters_solved"] - sw["mean_newton_iters_solved"],
+                      "training": tinfo}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,29 +220,42 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=32)
-    ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh"],
-                    help="bratu: 16,384 Bratu BVPs (MIRK4, FP64); lintanh: 2,048 n=32 linear+tanh BVPs "
-                         "(MIRK6, FP32 factors + FP64 refinement)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the warm-start network (constant guess only)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (variant 0 at full residency)")
+    ap.add_argument("--train-solves", type=int, default=200000)
+    ap.add_argument("--train-nodes", type=int, default=64, help="training samples per solve")
+    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--anchors", type=int, default=4,
+                    help="network evaluations per instance (Hermite-interpolated to the nodes); 0: every node")
+    ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh", "learned"],
+                    help="bratu: 16,384 Bratu BVPs (MIRK4, FP64, learned warm start); lintanh: 2,048 n=32 "
+                         "linear+tanh BVPs (MIRK6, FP32 factors + FP64 refinement); learned: warm start on "
+                         "16,384 unseen Bratu (a, b) BVPs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); instances shard across "
-              "GPUs with no exchange, so the whole-job rate is the per-GPU rate times the GPU count",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); instances shard across GPUs with no "
+              "exchange, so the whole-job rate is the per-GPU rate times the GPU count", file=sys.stderr)
         sys.exit(2)
 
-    bvp = load_package()
-    from boundaryvaluediffeq_amd import _abi, _lib, synthetic
+    pkg = load_package()
+    from importlib import import_module
+    _abi = import_module(pkg.__name__ + "._abi")
+    _lib = import_module(pkg.__name__ + "._lib")
+    synthetic = import_module(pkg.__name__ + ".synthetic")
+    learned = import_module(pkg.__name__ + ".learned")
 
     if args.copy_probe:
         ms = _lib.copy_probe(_lib.context(0), args.copy_probe, repeats=args.steps)
         gbs = 2 * args.copy_probe / (ms * 1e-3) / 1e9
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "ms": ms, "GB_per_s_read_plus_write": gbs}))
         return
-
     if args.workload == "lintanh":
         return bench_lintanh(args, _abi, _lib, synthetic)
+    if args.workload == "learned":
+        return bench_learned(args, _abi, _lib, synthetic, learned)
+
     n = args.instances
     abstol = 1e-8
     specs, lams, Ns = synthetic.bratu_batch(n)
@@ -145,68 +267,46 @@ def main():
     batch = _lib.Batch(ctx, opts, specs)
     info = batch.info()
     log(f"[bench] persistent waves {info[0]}, node capacity {info[1]}")
-    for w in range(args.warmup):
-        batch.run()
-        ms = batch.sync()
-        log(f"[bench] warmup {w}: kernel {ms:.2f} ms")
-    kms = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        batch.run()
-        kms.append(batch.sync())
-    t1 = time.perf_counter()
-    log(f"[bench] timed {args.steps} steps: kernel ms {['%.2f' % x for x in kms]}")
-    res = batch.results()
-    counters = batch.counters()
-    good = sum(1 for r in res if r.retcode == 0 and r.defect_norm <= min(1e-6, abstol))
-    wall = (t1 - t0) / args.steps
-    value = good / wall
-    kmean = sum(kms) / len(kms)
     n0_nodes = int(sum(s.N + 1 for s in specs))
-    nf_nodes = int(sum(r.N + 1 for r in res))
-    abytes = algorithmic_bytes(2, counters, n0_nodes, nf_nodes)
-    achieved = abytes / (kmean * 1e-3) / 1e9
-    log(f"[bench] {good}/{n} solved; {value:.1f} solves/s; kernel {kmean:.2f} ms; "
-        f"algorithmic {abytes / 1e9:.3f} GB -> {achieved:.1f} GB/s; counters {counters}")
+    tune = None
+    if not args.no_tune:  # untimed: pick the launch configuration by measurement
+        cand_ms, (variant, blocks) = batch.autotune(repeats=3)
+        tune = {"candidates_ms": cand_ms, "variant": variant, "persistent_waves": blocks,
+                "variants": "solve kernel register allocation held to 1, 2, 3 waves per SIMD"}
+        log(f"[bench] autotune: candidates {['%.3f' % x for x in cand_ms]} ms -> variant {variant}, {blocks} waves")
 
-    # plain form: one instance per launch, FP64, constant guess
-    plain = None
-    if args.plain_sample > 0:
-        stride = max(1, n // args.plain_sample)
-        idx = list(range(0, n, stride))[:args.plain_sample]
-        tot = 0.0
-        ok = 0
-        for i in idx:
-            b1 = _lib.Batch(ctx, opts, [specs[i]])
-            b1.run()
-            b1.sync()
-            ta = time.perf_counter()
-            b1.run()
-            b1.sync()
-            tot += time.perf_counter() - ta
-            r = b1.results()[0]
-            ok += int(r.retcode == 0 and r.defect_norm <= min(1e-6, abstol))
-            b1.close()
-        plain = {"value": ok / tot, "unit": "solves/s", "sample": f"{len(idx)} instances (every {stride}th), "
-                 "each solved alone in its own launch", "speedup_batched_over_plain": value / (ok / tot)}
-        log(f"[bench] plain single-instance GPU: {ok / tot:.1f} solves/s")
+    # batched, constant guess
+    ta = timed_runs(batch, args.steps, args.warmup, "batched fp64, constant guess")
+    res_a = batch.results()
+    cnt_a = batch.counters()
+    good_a = n_solved(res_a, abstol)
+    # batched, learned guess (the headline form)
+    tinfo = None
+    tb, res_b, cnt_b, good_b = ta, res_a, cnt_a, good_a
+    if not args.no_learned:
+        net, tinfo = train_warm_start(ctx, args, opts, _abi, synthetic, learned)
+        batch.set_warm_start(net, _abi.FEAT_PARAM, args.anchors)
+        tb = timed_runs(batch, args.steps, args.warmup, "batched fp64, learned guess")
+        res_b = batch.results()
+        cnt_b = batch.counters()
+        good_b = n_solved(res_b, abstol)
+    value = good_b / tb["wall"]
+    nf_nodes = int(sum(r.N + 1 for r in res_b))
+    abytes = algorithmic_bytes(2, cnt_b, n0_nodes, nf_nodes)
+    achieved = abytes / (tb["solve_ms"] * 1e-3) / 1e9
+    sa, sb = learned.summary(res_a), learned.summary(res_b)
+    log(f"[bench] {good_b}/{n} solved; {value:.1f} solves/s; solve kernel {tb['solve_ms']:.3f} ms; algorithmic "
+        f"{abytes / 1e9:.3f} GB -> {achieved:.1f} GB/s; counters {cnt_b}; newton iters/instance "
+        f"{sa['mean_newton_iters']:.2f} -> {sb['mean_newton_iters']:.2f}")
 
+    plain = plain_single(ctx, opts, specs, args.plain_sample, abstol, _lib) if args.plain_sample > 0 else None
+    if plain:
+        plain["speedup_of_value"] = value / plain["value"]
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import pyoracle as O
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-        oo = O.opts(4, abstol=abstol)
-        log(f"[bench] oracle on {threads} threads, {n} instances")
-        tc = time.perf_counter()
-        rets = O.solve_batch(oo, specs, nthreads=threads, keep_solutions=False)
-        tcpu = time.perf_counter() - tc
-        okc = sum(1 for r in rets if r == 0)
-        cpu = {"value": okc / tcpu, "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": f"the same {n} instances, one per OpenMP thread (dynamic schedule), {tcpu:.1f} s wall"}
-        log(f"[bench] oracle: {okc / tcpu:.1f} solves/s")
+        cpu = cpu_baseline(specs, abstol, threads)
 
-    traffic = read_traffic("k_solve")
     line = {
         "metric": METRIC,
         "value": value,
@@ -214,19 +314,28 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3,
+        "ms_per_step": tb["wall"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{workload}: {n} Bratu BVPs u''+lambda e^u=0, lambda~U(0.1,3.4), "
-                               "N~U{64..1024}, zero guess, MIRK4, abstol 1e-8, adaptive",
-                   "instances": n, "method": "MIRK4", "precision": "fp64", "variant": "batched persistent",
-                   "solved": good},
+                               "N~U{64..1024}, MIRK4, abstol 1e-8, adaptive; learned warm start "
+                               f"(BF16 MLP at {args.anchors} anchors per instance + Hermite, trained untimed on plain "
+                               "solves of the Bratu (a, b) family) with fallback to the zero guess",
+                   "instances": n, "method": "MIRK4", "precision": "fp64 factors (M = 2)",
+                   "variant": "batched persistent, hipGraph replay" + ("" if args.no_learned else
+                                                                       " + learned warm start"),
+                   "solved": good_b,
+                   "batched_constant_guess": {"value": good_a / ta["wall"], "ms_per_step": ta["wall"] * 1e3,
+                                              "solved": good_a, **sa},
+                   "batched_learned_guess": None if args.no_learned else
+                   {"value": value, "guess_ms": tb["guess_ms"], "solve_ms": tb["solve_ms"], **sb},
+                   "training": tinfo, "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_solve<2,4>", "kernel_ms": kmean, "algorithmic_bytes": abytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("k_solve"),
+                     "kernel": "k_solve<2,4>", "kernel_ms": tb["solve_ms"], "algorithmic_bytes": abytes},
         "cpu_baseline": cpu,
         "plain_gpu": plain,
     }

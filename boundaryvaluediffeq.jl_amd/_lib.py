@@ -56,6 +56,11 @@ def lib():
         "bvp_batch_destroy": (None, [vp]),
         "bvp_copy_probe": (C.c_int, [vp, C.c_longlong, C.c_int, C.POINTER(C.c_float)]),
         "bvp_batch_set_graph": (C.c_int, [vp, C.c_int32]),
+        "bvp_batch_timing": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "bvp_batch_learned_guess": (C.c_int, [vp, vp, _dp]),
+        "bvp_batch_autotune": (C.c_int, [vp, vp, C.c_int32, _fp, _ip]),
+        "bvp_batch_config": (C.c_int, [vp, _ip, _ip]),
+        "bvp_batch_set_config": (C.c_int, [vp, vp, C.c_int32, C.c_int32]),
         "bvp_mlp_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
         "bvp_mlp_destroy": (None, [vp]),
         "bvp_mlp_set_normalization": (C.c_int, [vp, _fp, _fp, _fp, _fp]),
@@ -65,7 +70,7 @@ def lib():
         "bvp_mlp_predict": (C.c_int, [vp, C.c_int32, _fp, _fp]),
         "bvp_mlp_train": (C.c_int, [vp, C.c_int64, _fp, _fp, C.POINTER(_abi.TrainOpts), _fp, _ip,
                                     C.POINTER(C.c_float)]),
-        "bvp_batch_set_warm_start": (C.c_int, [vp, vp, C.c_int32]),
+        "bvp_batch_set_warm_start": (C.c_int, [vp, vp, C.c_int32, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -82,7 +87,8 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_batch_info", "bvp_batch_destroy", "bvp_copy_probe", "bvp_batch_set_graph",
             "bvp_mlp_create", "bvp_mlp_destroy", "bvp_mlp_set_normalization", "bvp_mlp_get_params",
             "bvp_mlp_set_params", "bvp_mlp_gradient", "bvp_mlp_predict", "bvp_mlp_train",
-            "bvp_batch_set_warm_start"]
+            "bvp_batch_set_warm_start", "bvp_batch_timing", "bvp_batch_learned_guess", "bvp_batch_autotune",
+            "bvp_batch_config", "bvp_batch_set_config"]
 
 
 class Context:
@@ -278,18 +284,55 @@ class Batch:
         self.ctx.check(lib().bvp_batch_results(self.ctx.handle, self._h, res), "batch_results")
         return [RawSolution(res[i], self.specs[i].M, *bufs[i]) for i in range(len(self.specs))]
 
+    def learned_guess(self):
+        """The attached network's guess at every node, one (N+1, M) array per instance."""
+        M = self.specs[0].M
+        tot = sum(s.N + 1 for s in self.specs)
+        y = np.zeros(tot * M)
+        self.ctx.check(lib().bvp_batch_learned_guess(self.ctx.handle, self._h, y.ctypes.data_as(_dp)),
+                       "batch_learned_guess")
+        out, o = [], 0
+        for s in self.specs:
+            out.append(y[o * M:(o + s.N + 1) * M].reshape(s.N + 1, M).copy())
+            o += s.N + 1
+        return out
+
+    def autotune(self, repeats=3):
+        """Pick the fastest launch configuration by measurement; returns the median
+        device ms of every candidate and the chosen (variant, persistent waves)."""
+        ms = np.zeros(16, np.float32)
+        n = C.c_int32()
+        self.ctx.check(lib().bvp_batch_autotune(self.ctx.handle, self._h, repeats, ms.ctypes.data_as(_fp),
+                                                C.byref(n)), "batch_autotune")
+        return [float(x) for x in ms[:n.value]], self.config()
+
+    def set_config(self, variant, blocks):
+        self.ctx.check(lib().bvp_batch_set_config(self.ctx.handle, self._h, variant, blocks), "batch_set_config")
+
+    def config(self):
+        v, b = C.c_int32(), C.c_int32()
+        lib().bvp_batch_config(self._h, C.byref(v), C.byref(b))
+        return int(v.value), int(b.value)
+
+    def timing(self):
+        """(guess ms, solve ms) of the last synced run."""
+        g, s = C.c_float(), C.c_float()
+        lib().bvp_batch_timing(self._h, C.byref(g), C.byref(s))
+        return float(g.value), float(s.value)
+
     def set_graph(self, enable):
         """Capture the run's launches in a hipGraph and replay it (default on)."""
         rc = lib().bvp_batch_set_graph(self._h, 1 if enable else 0)
         if rc != 0:
             raise BVPError(f"batch_set_graph failed ({rc})")
 
-    def set_warm_start(self, net, feature_kind=_abi.FEAT_PARAM):
+    def set_warm_start(self, net, feature_kind=_abi.FEAT_PARAM, anchors=16):
         """Attach an MLP (or detach with None): each run writes the learned guess
-        of every node on the device before the solve."""
+        of every node on the device before the solve (the network at `anchors`
+        points per instance and Hermite interpolation, or at every node for 0)."""
         self._net = net
         h = net.handle if net is not None else C.c_void_p()
-        rc = lib().bvp_batch_set_warm_start(self._h, h, feature_kind)
+        rc = lib().bvp_batch_set_warm_start(self._h, h, feature_kind, anchors)
         if rc != 0:
             raise BVPError(f"batch_set_warm_start failed ({rc}): {lib().bvp_last_error(self.ctx.handle).decode()}")
 

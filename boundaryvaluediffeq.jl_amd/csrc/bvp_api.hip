@@ -253,12 +253,22 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         delete B;
         return rc;
     }
-    // persistent slots
-    int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu
-                 : (blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method));
+    // persistent slots: the kernel variant and waves per CU from an earlier tuning
+    // of this kind of plan, else variant 0 at its occupancy; scratch is sized for
+    // the most waves any variant can keep resident, so that the autotuner can try them
+    const auto key = std::make_tuple(M, o->method, o->precision, blk ? 1 : 0);
+    int variant = 0, per_cu = 0;
+    if (ctx->tuned.count(key)) std::tie(variant, per_cu) = ctx->tuned[key];
+    if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
+    if (per_cu < 1)
+        per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant);
     if (per_cu < 1) per_cu = 1;
     int slots = std::min(n, ctx->num_cus * per_cu);
+    if (!blk)
+        for (int v = 0; v < wave_variants(M); ++v)
+            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v))));
     if (slots < 1) slots = 1;
+    B->variant = variant;
     size_t scratch_doubles;
     if (!blk) {
         const int NF = wave_rel_fields(M);
@@ -284,7 +294,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         D.aux_per_slot = (size_t)cap * (3 * M + 1);
         scratch_doubles = (per_slot * esz * slots + 7) / 8;
     }
-    B->blocks = slots;
+    B->max_slots = slots;
+    B->blocks = std::min(slots, std::max(1, std::min(n, ctx->num_cus * per_cu)));
     D.refine_iters = o->refine_iters;
     if ((rc = dalloc(ctx, A, D.scratch, scratch_doubles)) ||
         (rc = dalloc(ctx, A, D.aux, D.aux_per_slot * slots))) {
@@ -321,51 +332,73 @@ static bool fp64_block32_blocked(bvp_ctx *ctx, const bvp_batch *B) {
     return false;
 }
 
-// The launches of one run: queue reset (a fill kernel, so the sequence can be
-// captured), the learned guess of every node when a network is attached, the
-// persistent solve.
+// The launches of one run: the learned guess of every node when a network is
+// attached, then the queue reset (a fill kernel, so that it can be captured) and
+// the persistent solve.
+static int enqueue_guess(bvp_ctx *ctx, bvp_batch *B) {
+    HIP_TRY(ctx, mlp::launch_batch_guess(B->mlp->net(), B->dev, B->warm_y, B->total_nodes, B->feature_kind,
+                                         B->anchors, B->anc, ctx->num_cus, ctx->stream));
+    return 0;
+}
+
 static int enqueue_solve(bvp_ctx *ctx, bvp_batch *B) {
     BatchDev D = B->dev;
     HIP_TRY(ctx, wave_reset_counter(D.counter, ctx->stream));
     if (B->mlp) {
-        HIP_TRY(ctx, mlp::launch_batch_guess(B->mlp->net(), D, B->warm_y, B->total_nodes, B->feature_kind,
-                                             ctx->stream));
-        D.init_y = B->warm_y;
+        if (B->anchors > 0) {
+            D.anc = B->anc;
+            D.anc_K = B->anchors;
+        } else {
+            D.init_y = B->warm_y;
+        }
         D.warm = 1;
     }
     if (B->block)
         HIP_TRY(ctx, block_launch_solve(D, B->method, B->opts.precision, B->blocks, ctx->stream));
     else
-        HIP_TRY(ctx, wave_launch_solve(D, B->method, B->blocks, ctx->stream));
+        HIP_TRY(ctx, wave_launch_solve(D, B->method, B->variant, B->blocks, ctx->stream));
+    return 0;
+}
+
+static int capture(bvp_ctx *ctx, bvp_batch *B, int (*fn)(bvp_ctx *, bvp_batch *), hipGraph_t &g,
+                   hipGraphExec_t &ge) {
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = fn(ctx, B);
+    hipGraph_t gg = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &gg);
+    if (rc) {
+        if (gg) (void)hipGraphDestroy(gg);
+        return rc;
+    }
+    HIP_TRY(ctx, e);
+    g = gg;
+    HIP_TRY(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     return 0;
 }
 
 int batch_run(bvp_ctx *ctx, bvp_batch *B) {
     if (fp64_block32_blocked(ctx, B)) return BVP_ENOTSUP;
-    if (!B->use_graph) {
-        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        int rc = enqueue_solve(ctx, B);
-        if (rc) return rc;
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        return 0;
-    }
-    if (B->graph_exec && B->mlp && B->mlp_gen != B->mlp->gen) B->drop_graph();  // normalisation changed
-    if (!B->graph_exec) {
-        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_solve(ctx, B);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
+    if (B->use_graph) {
+        if (B->graph_exec && B->mlp && B->mlp_gen != B->mlp->gen) B->drop_graph();  // normalisation changed
+        if (!B->graph_exec) {
+            int rc = capture(ctx, B, enqueue_solve, B->graph, B->graph_exec);
+            if (!rc && B->mlp) rc = capture(ctx, B, enqueue_guess, B->guess_graph, B->guess_exec);
+            if (rc) {
+                B->drop_graph();
+                return rc;
+            }
+            if (B->mlp) B->mlp_gen = B->mlp->gen;
         }
-        HIP_TRY(ctx, e);
-        B->graph = g;
-        HIP_TRY(ctx, hipGraphInstantiate(&B->graph_exec, g, nullptr, nullptr, 0));
-        if (B->mlp) B->mlp_gen = B->mlp->gen;
     }
+    B->timed_split = B->mlp != nullptr;
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, hipGraphLaunch(B->graph_exec, ctx->stream));
+    if (B->mlp) {
+        if (B->use_graph) HIP_TRY(ctx, hipGraphLaunch(B->guess_exec, ctx->stream));
+        else if (int rc = enqueue_guess(ctx, B)) return rc;
+        HIP_TRY(ctx, hipEventRecord(ctx->evm, ctx->stream));
+    }
+    if (B->use_graph) HIP_TRY(ctx, hipGraphLaunch(B->graph_exec, ctx->stream));
+    else if (int rc = enqueue_solve(ctx, B)) return rc;
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     return 0;
 }
@@ -376,6 +409,15 @@ int batch_sync(bvp_ctx *ctx, bvp_batch *B, float *ms) {
     float t = 0;
     HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
     B->last_ms = t;
+    B->last_guess_ms = 0.f;
+    B->last_solve_ms = t;
+    if (B->timed_split) {
+        float g = 0, s = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&g, ctx->ev0, ctx->evm));
+        HIP_TRY(ctx, hipEventElapsedTime(&s, ctx->evm, ctx->ev1));
+        B->last_guess_ms = g;
+        B->last_solve_ms = s;
+    }
     if (ms) *ms = t;
     return 0;
 }
@@ -462,7 +504,11 @@ int bvp_ctx_create(int device, bvp_ctx **out) {
     c->num_cus = prop.multiProcessorCount;
     c->arch = prop.gcnArchName;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return BVP_EHIP; }
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return BVP_EHIP; }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->evm) != hipSuccess) {
+        delete c;
+        return BVP_EHIP;
+    }
     const char *env = std::getenv("BVP_BLOCKS_PER_CU");
     c->blocks_per_cu = env ? std::atoi(env) : 0;
     *out = c;
@@ -475,6 +521,7 @@ void bvp_ctx_destroy(bvp_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evm) (void)hipEventDestroy(c->evm);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -686,6 +733,95 @@ int bvp_copy_probe(bvp_ctx *ctx, long long nbytes, int repeats, float *ms) {
     return 0;
 }
 
+int bvp_batch_learned_guess(bvp_ctx *ctx, bvp_batch *B, double *y) {
+    if (!ctx || !B || !y) return BVP_EINVAL;
+    if (!B->mlp) { set_err(ctx, "no network attached to the plan"); return BVP_EINVAL; }
+    (void)hipSetDevice(ctx->device);
+    if (int rc = enqueue_guess(ctx, B)) return rc;
+    if (B->anchors > 0) HIP_TRY(ctx, mlp::launch_anchor_fill(B->dev, B->anc, B->anchors, B->warm_y, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(y, B->warm_y, sizeof(double) * (size_t)B->total_nodes * B->M,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_ms, int32_t *n_cand) {
+    if (!ctx || !B || repeats < 1) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    struct Cand { int variant, blocks; };
+    std::vector<Cand> cands;
+    if (B->block) {
+        const int occ = std::max(1, block_blocks_per_cu(B->M, B->method, B->opts.precision));
+        for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc)});
+        if (cands.back().blocks != std::min(B->max_slots, ctx->num_cus * occ))
+            cands.push_back({0, std::min(B->max_slots, ctx->num_cus * occ)});
+    } else {
+        for (int v = 0; v < wave_variants(B->M); ++v)
+            cands.push_back({v, std::min(B->max_slots, ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v)))});
+    }
+    const int keep_v = B->variant, keep_b = B->blocks;
+    int best = -1;
+    float best_ms = 1e30f;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        B->variant = cands[c].variant;
+        B->blocks = std::max(1, cands[c].blocks);
+        B->drop_graph();
+        std::vector<float> t;
+        for (int r = 0; r <= repeats; ++r) {  // first run untimed
+            int rc = batch_run(ctx, B);
+            if (!rc) rc = batch_sync(ctx, B, nullptr);
+            if (rc) {
+                B->variant = keep_v;
+                B->blocks = keep_b;
+                B->drop_graph();
+                return rc;
+            }
+            if (r > 0) t.push_back(B->last_solve_ms);
+        }
+        std::sort(t.begin(), t.end());
+        const float med = t[t.size() / 2];
+        if (cand_ms && (int)c < 16) cand_ms[c] = med;
+        if (med < best_ms) {
+            best_ms = med;
+            best = (int)c;
+        }
+    }
+    if (n_cand) *n_cand = (int32_t)cands.size();
+    B->variant = cands[best].variant;
+    B->blocks = std::max(1, cands[best].blocks);
+    B->drop_graph();
+    const int per_cu = (B->blocks + ctx->num_cus - 1) / ctx->num_cus;
+    ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision, B->block ? 1 : 0)] = {B->variant, per_cu};
+    return 0;
+}
+
+int bvp_batch_set_config(bvp_ctx *ctx, bvp_batch *B, int32_t variant, int32_t blocks) {
+    if (!ctx || !B) return BVP_EINVAL;
+    const int nv = B->block ? 1 : wave_variants(B->M);
+    if (variant < 0 || variant >= nv || blocks < 1 || blocks > B->max_slots) {
+        set_err(ctx, "variant must be in [0, %d) and blocks in [1, %d]", nv, B->max_slots);
+        return BVP_EINVAL;
+    }
+    B->variant = variant;
+    B->blocks = blocks;
+    B->drop_graph();
+    return 0;
+}
+
+int bvp_batch_config(bvp_batch *B, int32_t *variant, int32_t *blocks) {
+    if (!B || !variant || !blocks) return BVP_EINVAL;
+    *variant = B->variant;
+    *blocks = B->blocks;
+    return 0;
+}
+
+int bvp_batch_timing(bvp_batch *B, float *guess_ms, float *solve_ms) {
+    if (!B || !guess_ms || !solve_ms) return BVP_EINVAL;
+    *guess_ms = B->last_guess_ms;
+    *solve_ms = B->last_solve_ms;
+    return 0;
+}
+
 int bvp_batch_set_graph(bvp_batch *B, int32_t enable) {
     if (!B) return BVP_EINVAL;
     B->drop_graph();
@@ -825,7 +961,7 @@ int bvp_mlp_predict(bvp_mlp *m, int32_t n, const float *x, float *y) {
     int rc;
     if ((rc = dalloc(ctx, A, dx, (size_t)n * mlp::NIN)) || (rc = dalloc(ctx, A, dy, (size_t)n * mlp::NOUT))) return rc;
     HIP_TRY(ctx, hipMemcpyAsync(dx, x, sizeof(float) * n * mlp::NIN, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, mlp::launch_predict(m->net(), dx, n, dy, ctx->stream));
+    HIP_TRY(ctx, mlp::launch_predict(m->net(), dx, n, dy, ctx->num_cus, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(y, dy, sizeof(float) * n * mlp::NOUT, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
@@ -924,7 +1060,7 @@ int bvp_mlp_train(bvp_mlp *m, int64_t n, const float *x, const float *y, const b
     return 0;
 }
 
-int bvp_batch_set_warm_start(bvp_batch *B, const bvp_mlp *m, int32_t kind) {
+int bvp_batch_set_warm_start(bvp_batch *B, const bvp_mlp *m, int32_t kind, int32_t anchors) {
     if (!B) return BVP_EINVAL;
     if (!m) {
         B->mlp = nullptr;
@@ -935,10 +1071,20 @@ int bvp_batch_set_warm_start(bvp_batch *B, const bvp_mlp *m, int32_t kind) {
     if (B->block || B->M != 2) { set_err(ctx, "the learned warm start serves M = 2 plans"); return BVP_ENOTSUP; }
     if (!B->has_u0) { set_err(ctx, "warm start: every instance needs u0 (the fallback guess)"); return BVP_EINVAL; }
     if (kind != BVP_FEAT_PARAM && kind != BVP_FEAT_LOG10_PARAM) { set_err(ctx, "bad feature kind"); return BVP_EINVAL; }
+    if (anchors == 1 || anchors < 0 || anchors > 4096) {
+        set_err(ctx, "anchors must be 0 (every node) or 2..4096");
+        return BVP_EINVAL;
+    }
     if (!B->warm_y) {
         const int rc = dalloc(ctx, B->arena, B->warm_y, (size_t)B->total_nodes * B->M);
         if (rc) return rc;
     }
+    if (anchors > B->anc_cap) {  // grows only; the arena frees with the plan
+        const int rc = dalloc(ctx, B->arena, B->anc, (size_t)B->n * anchors * 4);
+        if (rc) return rc;
+        B->anc_cap = anchors;
+    }
+    B->anchors = anchors;
     B->mlp = m;
     B->feature_kind = kind;
     B->drop_graph();

@@ -3,7 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../../include/bvp_mirk.h"
@@ -16,8 +19,12 @@ struct bvp_ctx {
     std::string arch;
     hipStream_t stream = nullptr;  // default priority
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evm = nullptr;      // between the learned guess and the solve of a plan run
     int blocks_per_cu = 0;         // 0: from the occupancy calculator
     std::string err;
+    // autotuner choices, (M, method, precision, block kernel) -> (variant, waves per CU);
+    // later plans of the same kind start from them
+    std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tuned;
 };
 
 // device allocations owned by one batch; freed together
@@ -46,6 +53,8 @@ struct bvp_mlp {
 
 struct bvp_batch {
     int n = 0, M = 0, method = 4, cap = 0, blocks = 0;
+    int variant = 0;     // solve-kernel variant (wave_variants)
+    int max_slots = 0;   // persistent waves the per-wave scratch was sized for
     bool block = false;  // served by the block kernels (M = 8, 16, 32)
     bool has_u0 = false; // every instance carries u0 (needed by a warm start)
     bvp_opts opts{};
@@ -53,25 +62,34 @@ struct bvp_batch {
     std::vector<long long> init_off;
     long long total_nodes = 0;
     float last_ms = 0.f;
+    float last_guess_ms = 0.f, last_solve_ms = 0.f;
+    bool timed_split = false;     // the last run recorded ctx->evm
     BatchDev dev{};
     DevArena arena;
     // learned warm start attached to the plan
     const bvp_mlp *mlp = nullptr;
     int feature_kind = 0;
     double *warm_y = nullptr;  // [total_nodes][M]
+    int anchors = 0;           // 0: network at every node; else anchors per instance + Hermite
+    double *anc = nullptr;     // [n][anchors][4]: (y1, y2, f1, f2) at the anchors
+    int anc_cap = 0;
     int mlp_gen = 0;
-    // graph replay of run()
+    // graph replay of run(): the solve (queue reset + persistent kernel) and the
+    // learned guess are captured as two graphs so that an event between them
+    // splits the device time
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr, guess_graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr, guess_exec = nullptr;
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
+        if (guess_exec) (void)hipGraphExecDestroy(guess_exec);
+        if (guess_graph) (void)hipGraphDestroy(guess_graph);
+        graph_exec = guess_exec = nullptr;
+        graph = guess_graph = nullptr;
     }
     ~bvp_batch() { drop_graph(); }
 };
 
-int wave_blocks_per_cu(int M, int method);
+int wave_blocks_per_cu(int M, int method, int variant);
 bool blockm_supports(int M, int family);

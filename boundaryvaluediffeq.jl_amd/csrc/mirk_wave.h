@@ -60,6 +60,8 @@ struct BatchDev {
     // the first mesh must contract max|F|, else the solve restarts from u0c
     int warm;
     const double *u0c;        // [n][M] constant guess of each instance
+    const double *anc;        // learned anchors [n][anc_K][4] (M = 2): when set, the initial
+    int anc_K;                //   guess is their Hermite interpolant instead of init_y
 
     int rec_cap;              // block kernels: intervals whose factors fit a wave's storage
     int refine_iters;         // FP64 refinement steps per linear solve with FP32 factors
@@ -70,7 +72,10 @@ struct BatchDev {
 };
 
 hipError_t wave_reset_counter(unsigned *counter, hipStream_t s);
-hipError_t wave_launch_solve(const BatchDev &B, int method, int blocks, hipStream_t s);
+// variant v of the persistent solve kernel (register allocation held to v + 1
+// waves per SIMD for M = 2; one variant otherwise)
+int wave_variants(int M);
+hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, hipStream_t s);
 hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s);
 hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, double *out_mesh, double *out_y,
                               int max_nodes, hipStream_t s);

@@ -4,11 +4,15 @@
 
 namespace bvpdev {
 #define DECL(m, me)                                                                              \
-    extern template hipError_t launch_solve_t<m, me>(const BatchDev &, int, hipStream_t);        \
-    extern template hipError_t launch_component_t<m, me>(const BatchDev &, int, int, hipStream_t); \
-    extern template int occupancy_t<m, me>();
+    extern template hipError_t launch_component_t<m, me>(const BatchDev &, int, int, hipStream_t);
+#define DECLV(m, me, w)                                                                          \
+    extern template hipError_t launch_solve_t<m, me, w>(const BatchDev &, int, hipStream_t);     \
+    extern template int occupancy_t<m, me, w>();
 DECL(1, 4) DECL(1, 6) DECL(2, 4) DECL(2, 6) DECL(3, 4) DECL(3, 6) DECL(4, 4) DECL(4, 6)
+DECLV(1, 4, 1) DECLV(1, 6, 1) DECLV(3, 4, 1) DECLV(3, 6, 1) DECLV(4, 4, 1) DECLV(4, 6, 1)
+DECLV(2, 4, 1) DECLV(2, 6, 1) DECLV(2, 4, 2) DECLV(2, 6, 2) DECLV(2, 4, 3) DECLV(2, 6, 3)
 #undef DECL
+#undef DECLV
 }  // namespace bvpdev
 using namespace bvpdev;
 
@@ -45,8 +49,12 @@ hipError_t wave_reset_counter(unsigned *counter, hipStream_t s) {
         else return hipErrorInvalidValue;                                                      \
     } while (0)
 
-hipError_t wave_launch_solve(const BatchDev &B, int method, int blocks, hipStream_t s) {
-#define CALL_SOLVE(m, me) return launch_solve_t<m, me>(B, blocks, s)
+int wave_variants(int M) { return M == 2 ? 3 : 1; }
+
+hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, hipStream_t s) {
+    if (B.M == 2 && variant == 1) return method == 6 ? launch_solve_t<2, 6, 2>(B, blocks, s) : launch_solve_t<2, 4, 2>(B, blocks, s);
+    if (B.M == 2 && variant == 2) return method == 6 ? launch_solve_t<2, 6, 3>(B, blocks, s) : launch_solve_t<2, 4, 3>(B, blocks, s);
+#define CALL_SOLVE(m, me) return launch_solve_t<m, me, 1>(B, blocks, s)
     DISPATCH_M_METHOD(B.M, method, CALL_SOLVE);
 #undef CALL_SOLVE
 }
@@ -87,13 +95,15 @@ bool wave_supports(int M, int family) {
 int wave_rel_fields(int M) { return 2 * M * M + M; }
 
 
-int wave_blocks_per_cu(int M, int method) {
+int wave_blocks_per_cu(int M, int method, int variant) {
     const bool six = (method == 6);
+    if (M == 2 && variant == 1) return six ? occupancy_t<2, 6, 2>() : occupancy_t<2, 4, 2>();
+    if (M == 2 && variant == 2) return six ? occupancy_t<2, 6, 3>() : occupancy_t<2, 4, 3>();
     switch (M) {
-    case 1: return six ? occupancy_t<1, 6>() : occupancy_t<1, 4>();
-    case 2: return six ? occupancy_t<2, 6>() : occupancy_t<2, 4>();
-    case 3: return six ? occupancy_t<3, 6>() : occupancy_t<3, 4>();
-    case 4: return six ? occupancy_t<4, 6>() : occupancy_t<4, 4>();
+    case 1: return six ? occupancy_t<1, 6, 1>() : occupancy_t<1, 4, 1>();
+    case 2: return six ? occupancy_t<2, 6, 1>() : occupancy_t<2, 4, 1>();
+    case 3: return six ? occupancy_t<3, 6, 1>() : occupancy_t<3, 4, 1>();
+    case 4: return six ? occupancy_t<4, 6, 1>() : occupancy_t<4, 4, 1>();
     }
     return 4;
 }

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "anchors.h"
 #include "mirk_device.h"
 #include "mirk_wave.h"
 #include "bvp_internal.h"
@@ -1159,15 +1160,28 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     Bufs<M> bf = bufs_of<M>(B, b);
     int cur = 0;
     int N = B.N0[b];
-    // initial mesh and guess from the packed inputs
+    // initial mesh and guess from the packed inputs (or from learned anchors)
     {
         const long long off = B.init_off[b];
-        for (int k = lane; k <= N; k += WAVE) {
-            bf.mesh[0][k] = B.init_mesh[off + k];
-#pragma unroll
-            for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = B.init_y[(off + k) * M + c];
+        if constexpr (M == 2) {
+            if (B.anc) {
+                const double t0 = B.t0[b], t1 = B.t1[b];
+                for (int k = lane; k <= N; k += WAVE) {
+                    const double t = B.init_mesh[off + k];
+                    bf.mesh[0][k] = t;
+                    anchor_hermite(B.anc, B.anc_K, b, t0, t1, t, bf.y[0] + (size_t)k * M);
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
+        if (!(M == 2 && B.anc)) {
+            for (int k = lane; k <= N; k += WAVE) {
+                bf.mesh[0][k] = B.init_mesh[off + k];
+#pragma unroll
+                for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = B.init_y[(off + k) * M + c];
+            }
+            __syncthreads();
+        }
     }
     double *shat = B.shat + (size_t)b * B.cap;
     double *delta = aux;                               // [cap*M]
@@ -1261,9 +1275,11 @@ __device__ inline int next_item(unsigned *counter, Shared<M> &sh, int lane) {
 
 // Persistent fused solve: each wave pulls instances from the queue (ordered by
 // decreasing size) and runs the whole residual -> Jacobian -> LU -> update ->
-// defect -> mesh loop for it.
-template <int M, int METHOD>
-__global__ __launch_bounds__(64) void k_solve(BatchDev B) {
+// defect -> mesh loop for it.  WPE = waves per SIMD the register allocation is
+// held to (1: no spills; more: more latency hiding, some spills) -- a variant
+// the autotuner picks by measurement.
+template <int M, int METHOD, int WPE>
+__global__ __launch_bounds__(64, WPE) void k_solve(BatchDev B) {
     __shared__ Shared<M> sh;
     const int lane = threadIdx.x;
     double *scr = B.scratch + (size_t)blockIdx.x * B.scratch_per_slot;
@@ -1387,9 +1403,9 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev B, const long long *out
 }
 
 // ------------------------------------------------------------------ host launchers
-template <int M, int METHOD>
+template <int M, int METHOD, int WPE>
 hipError_t launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
-    k_solve<M, METHOD><<<blocks, 64, 0, s>>>(B);
+    k_solve<M, METHOD, WPE><<<blocks, 64, 0, s>>>(B);
     return hipGetLastError();
 }
 template <int M, int METHOD>
@@ -1398,10 +1414,10 @@ hipError_t launch_component_t(const BatchDev &B, int op, int blocks, hipStream_t
     return hipGetLastError();
 }
 
-template <int M, int METHOD>
+template <int M, int METHOD, int WPE>
 int occupancy_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<M, METHOD>, 64, 0) != hipSuccess) return 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<M, METHOD, WPE>, 64, 0) != hipSuccess) return 4 * WPE;
     return nb;
 }
 

@@ -74,9 +74,15 @@ hipError_t launch_train_step(const NetBufs &nb, const Net &net, const TrainDev &
 // one forward/backward over n points in order, FP32 gradient (sum over tiles) -> grad[NPARAM], loss -> grad[NPARAM]
 hipError_t launch_grad(const Net &net, const float *x, const float *y, int n, float *partial, float *grad,
                        hipStream_t s);
-hipError_t launch_predict(const Net &net, const float *x, int n, float *out, hipStream_t s);
-// writes the predicted guess into init_y (double, point-major) for every node of every instance
+hipError_t launch_predict(const Net &net, const float *x, int n, float *out, int num_cus, hipStream_t s);
+// The learned guess of a batch (M = 2): the network at every node of the initial
+// meshes into init_y[node][2] (anchors == 0), or at `anchors` equally spaced
+// points per instance into anc[n][anchors][4] = (y1, y2, f1, f2), f = f(t, y)
+// being the problem's slope there; a solve then interpolates the nodes from the
+// anchors (anchors.h).  Persistent launch over num_cus workgroups.
 hipError_t launch_batch_guess(const Net &net, const BatchDev &B, double *init_y, long long total_nodes,
-                              int feature_kind, hipStream_t s);
+                              int feature_kind, int anchors, double *anc, int num_cus, hipStream_t s);
+// materialises the anchor interpolation at every node into init_y[node][2]
+hipError_t launch_anchor_fill(const BatchDev &B, const double *anc, int anchors, double *init_y, hipStream_t s);
 
 }  // namespace mlp

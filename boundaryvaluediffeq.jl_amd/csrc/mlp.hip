@@ -20,18 +20,21 @@
 
 #include <math.h>
 
+#include "anchors.h"
+#include "mirk_device.h"
+
 namespace mlp {
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-__device__ inline uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (gfx950: one v_cvt_pk_bf16_f32)
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// tanh z = 1 - 2 / (1 + e^{2z}) with the hardware exponential (absolute error
+// ~1e-7, far below the BF16 rounding the activations then get)
+__device__ inline float tanh_f(float z) { return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * z)); }
 
 struct __align__(16) Tile {
     uint16_t h[3][T][HP];   // activations of the hidden layers (BF16)
@@ -131,11 +134,12 @@ __device__ void forward_tile(Tile &S, const Net &net, int tid) {
 #pragma unroll
         for (int i = 0; i < NIN; ++i) w[i] = net.q[OW1 + i * H + n];
         const float b = net.q[OB1 + n];
+#pragma unroll 8
         for (int p = 0; p < T; ++p) {
             float z = b;
 #pragma unroll
             for (int i = 0; i < NIN; ++i) z += S.x[p][i] * w[i];
-            S.h[0][p][n] = f2bf(tanhf(z));
+            S.h[0][p][n] = f2bf(tanh_f(z));
         }
     }
     __syncthreads();
@@ -152,13 +156,14 @@ __device__ void forward_tile(Tile &S, const Net &net, int tid) {
                 const int n = 64 * wave + 16 * ct + i;
                 const float bn = bias[n];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) S.h[l + 1][16 * rt + 4 * g + j][n] = f2bf(tanhf(acc[rt][ct][j] + bn));
+                for (int j = 0; j < 4; ++j) S.h[l + 1][16 * rt + 4 * g + j][n] = f2bf(tanh_f(acc[rt][ct][j] + bn));
             }
         __syncthreads();
     }
     {  // output layer (256 -> 2): four threads per point, 64 inputs each
         const int p = tid >> 2, part = tid & 3;
         float o0 = 0.f, o1 = 0.f;
+#pragma unroll 16
         for (int k = 64 * part; k < 64 * part + 64; ++k) {
             const float hk = bf2f(S.h[2][p][k]);
             o0 += hk * net.q[OW4 + 2 * k];
@@ -364,57 +369,199 @@ __global__ void k_advance(int *step) {
     if (threadIdx.x == 0) step[0] += 1;
 }
 
-__global__ __launch_bounds__(256) void k_predict(Net net, const float *x, int n, float *out) {
-    __shared__ Tile S;
-    const int tid = threadIdx.x;
-    if (tid < T) {
-        const long long j = (long long)blockIdx.x * T + tid;
-        const bool v = j < n;
-        load_point(S, net, tid, v, x + (v ? j : 0) * NIN, nullptr);
+// last instance whose first node is <= gidx
+__device__ inline int instance_of_node(const BatchDev &B, long long gidx) {
+    int lo = 0, hi = B.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.init_off[mid] <= gidx) lo = mid;
+        else hi = mid - 1;
     }
-    __syncthreads();
-    forward_tile(S, net, tid);
-    if (tid < T) {
-        const long long j = (long long)blockIdx.x * T + tid;
-        if (j < n)
+    return lo;
+}
+
+// ------------------------------------------------------------------ inference
+// Persistent inference: each workgroup keeps its waves' column slices of both
+// hidden weight matrices in registers (2 x 4 x 8 operands of 8 BF16 per lane)
+// and streams 64-point tiles through two LDS activation buffers, so that the
+// weights are read from memory once per workgroup instead of once per tile.
+struct InferTile {
+    uint16_t h[2][T][HP];
+    float w4[H][NOUT];
+    float x[T][NIN];
+    float out[T][NOUT];
+    int valid[T];
+};
+
+__device__ inline void hidden_reg(const uint16_t *A, uint16_t *Hout, const bf8 (&w)[4][8], const float (&bias)[4],
+                                  int wave, int lane) {
+    const int i = lane & 15, g = lane >> 4;
+    f4 acc[4][4];
 #pragma unroll
-            for (int o = 0; o < NOUT; ++o) out[j * NOUT + o] = S.out[tid][o] * net.out_scale[o] + net.out_shift[o];
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+        bf8 a[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) a[rt] = *(const bf8 *)(A + (size_t)(16 * rt + i) * HP + 32 * ks + 8 * g);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], w[ct][ks], acc[rt][ct], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                Hout[(size_t)(16 * rt + 4 * g + j) * HP + 64 * wave + 16 * ct + i] =
+                    f2bf(tanh_f(acc[rt][ct][j] + bias[ct]));
+}
+
+template <class Src>
+__global__ __launch_bounds__(256) void k_infer(Net net, Src src, long long npts) {
+    __shared__ InferTile S;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, i = lane & 15, g = lane >> 4;
+    bf8 w2[4][8], w3[4][8];
+    float b2[4], b3[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int n = 64 * wave + 16 * ct + i;
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            w2[ct][ks] = *(const bf8 *)(net.w2f + (size_t)n * H + 32 * ks + 8 * g);
+            w3[ct][ks] = *(const bf8 *)(net.w3f + (size_t)n * H + 32 * ks + 8 * g);
+        }
+        b2[ct] = net.q[OB2 + n];
+        b3[ct] = net.q[OB3 + n];
+    }
+    float w1[NIN];
+#pragma unroll
+    for (int q = 0; q < NIN; ++q) w1[q] = net.q[OW1 + q * H + tid];
+    const float b1 = net.q[OB1 + tid];
+    S.w4[tid][0] = net.q[OW4 + 2 * tid];
+    S.w4[tid][1] = net.q[OW4 + 2 * tid + 1];
+    const float b40 = net.q[OB4], b41 = net.q[OB4 + 1];
+    const long long ntiles = (npts + T - 1) / T;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        if (tid < T) {
+            const long long gi = tile * T + tid;
+            const bool v = gi < npts;
+            float xr[NIN] = {0.f, 0.f, 0.f, 0.f};
+            if (v) src.features(gi, xr);
+#pragma unroll
+            for (int q = 0; q < NIN; ++q) S.x[tid][q] = v ? (xr[q] - net.in_shift[q]) * net.in_scale[q] : 0.f;
+            S.valid[tid] = v;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int p = 0; p < T; ++p) {
+            float z = b1;
+#pragma unroll
+            for (int q = 0; q < NIN; ++q) z += S.x[p][q] * w1[q];
+            S.h[0][p][tid] = f2bf(tanh_f(z));
+        }
+        __syncthreads();
+        hidden_reg(&S.h[0][0][0], &S.h[1][0][0], w2, b2, wave, lane);
+        __syncthreads();
+        hidden_reg(&S.h[1][0][0], &S.h[0][0][0], w3, b3, wave, lane);
+        __syncthreads();
+        {
+            const int p = tid >> 2, part = tid & 3;
+            float o0 = 0.f, o1 = 0.f;
+#pragma unroll 16
+            for (int k = 64 * part; k < 64 * part + 64; ++k) {
+                const float hk = bf2f(S.h[0][p][k]);
+                o0 += hk * S.w4[k][0];
+                o1 += hk * S.w4[k][1];
+            }
+            o0 += __shfl_xor(o0, 1);
+            o1 += __shfl_xor(o1, 1);
+            o0 += __shfl_xor(o0, 2);
+            o1 += __shfl_xor(o1, 2);
+            if (part == 0) {
+                S.out[p][0] = (o0 + b40) * net.out_scale[0] + net.out_shift[0];
+                S.out[p][1] = (o1 + b41) * net.out_scale[1] + net.out_shift[1];
+            }
+        }
+        __syncthreads();
+        if (tid < T && S.valid[tid]) src.store(tile * T + tid, S.out[tid]);
     }
 }
 
-// Warm-start guess for every node of a batch (M = 2): features (p0 or log10 p0,
-// boundary value of row 0, boundary value of row 1, t).
-__global__ __launch_bounds__(256) void k_batch_guess(Net net, BatchDev B, double *init_y, long long total, int kind) {
-    __shared__ Tile S;
-    const int tid = threadIdx.x;
-    if (tid < T) {
-        const long long gidx = (long long)blockIdx.x * T + tid;
-        const bool v = gidx < total;
-        float xr[NIN] = {0.f, 0.f, 0.f, 0.f};
-        if (v) {
-            int lo = 0, hi = B.n - 1;  // last instance whose first node is <= gidx
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (B.init_off[mid] <= gidx) lo = mid;
-                else hi = mid - 1;
-            }
-            const double p0 = B.params[B.param_off[lo]];
-            xr[0] = (float)(kind == FEAT_LOG10_PARAM ? log10(p0) : p0);
-            xr[1] = (float)B.bc_val[(size_t)lo * B.M];
-            xr[2] = (float)B.bc_val[(size_t)lo * B.M + 1];
-            xr[3] = (float)B.init_mesh[gidx];
-        }
-        load_point(S, net, tid, v, xr, nullptr);
-    }
-    __syncthreads();
-    forward_tile(S, net, tid);
-    if (tid < T) {
-        const long long gidx = (long long)blockIdx.x * T + tid;
-        if (gidx < total)
+struct PredictSrc {
+    const float *x;
+    float *y;
+    __device__ void features(long long gi, float *xr) const {
 #pragma unroll
-            for (int o = 0; o < NOUT; ++o)
-                init_y[gidx * NOUT + o] = (double)(S.out[tid][o] * net.out_scale[o] + net.out_shift[o]);
+        for (int q = 0; q < NIN; ++q) xr[q] = x[gi * NIN + q];
     }
+    __device__ void store(long long gi, const float *o) const {
+        y[gi * NOUT] = o[0];
+        y[gi * NOUT + 1] = o[1];
+    }
+};
+
+__device__ inline void batch_features(const BatchDev &B, int b, double t, int kind, float *xr) {
+    const double p0 = B.params[B.param_off[b]];
+    xr[0] = (float)(kind == FEAT_LOG10_PARAM ? log10(p0) : p0);
+    xr[1] = (float)B.bc_val[(size_t)b * B.M];
+    xr[2] = (float)B.bc_val[(size_t)b * B.M + 1];
+    xr[3] = (float)t;
+}
+
+// every node of the initial meshes of a batch -> out[node][2]
+struct NodeSrc {
+    BatchDev B;
+    double *out;
+    int kind;
+    __device__ void features(long long gi, float *xr) const {
+        batch_features(B, instance_of_node(B, gi), B.init_mesh[gi], kind, xr);
+    }
+    __device__ void store(long long gi, const float *o) const {
+        out[gi * 2] = (double)o[0];
+        out[gi * 2 + 1] = (double)o[1];
+    }
+};
+
+// K anchors per instance -> anc[b][j] = (y1, y2, f1, f2) with f = f(t_j, y)
+struct AnchorSrc {
+    BatchDev B;
+    double *anc;
+    int K;
+    int kind;
+    __device__ void features(long long gi, float *xr) const {
+        const int b = (int)(gi / K);
+        batch_features(B, b, anchor_t(B, b, (int)(gi % K), K), kind, xr);
+    }
+    __device__ void store(long long gi, const float *o) const {
+        const int b = (int)(gi / K);
+        bvpdev::Params<2> P;
+        const int po = B.param_off[b], pn = B.param_cnt[b];
+#pragma unroll
+        for (int k = 0; k < bvpdev::Params<2>::MAXP; ++k) P.p[k] = k < pn ? B.params[po + k] : 0.0;
+        const double y[2] = {(double)o[0], (double)o[1]};
+        double f[2];
+        bvpdev::f_eval<2, double>(B.family[b], P, anchor_t(B, b, (int)(gi % K), K), y, f);
+        anc[gi * 4] = y[0];
+        anc[gi * 4 + 1] = y[1];
+        anc[gi * 4 + 2] = f[0];
+        anc[gi * 4 + 3] = f[1];
+    }
+};
+
+// the guess at every node from the anchors (materialised for inspection; a solve
+// interpolates on the fly)
+__global__ void k_anchor_fill(BatchDev B, const double *anc, int K, double *init_y) {
+    const int b = blockIdx.x;
+    const long long first = B.init_off[b];
+    const int nodes = B.N0[b] + 1;
+    for (int k = threadIdx.x; k < nodes; k += blockDim.x)
+        anchor_hermite(anc, K, b, B.t0[b], B.t1[b], B.init_mesh[first + k], init_y + (first + k) * 2);
 }
 
 // ------------------------------------------------------------------ host side
@@ -448,15 +595,30 @@ hipError_t launch_grad(const Net &net, const float *x, const float *y, int n, fl
     return hipGetLastError();
 }
 
-hipError_t launch_predict(const Net &net, const float *x, int n, float *out, hipStream_t s) {
-    k_predict<<<(n + T - 1) / T, 256, 0, s>>>(net, x, n, out);
+static unsigned infer_grid(long long npts, int num_cus) {
+    const long long tiles = (npts + T - 1) / T;
+    return (unsigned)(tiles < num_cus ? (tiles > 0 ? tiles : 1) : num_cus);
+}
+
+hipError_t launch_predict(const Net &net, const float *x, int n, float *out, int num_cus, hipStream_t s) {
+    k_infer<PredictSrc><<<infer_grid(n, num_cus), 256, 0, s>>>(net, PredictSrc{x, out}, n);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_guess(const Net &net, const BatchDev &B, double *init_y, long long total_nodes,
-                              int feature_kind, hipStream_t s) {
-    const long long tiles = (total_nodes + T - 1) / T;
-    k_batch_guess<<<(unsigned)tiles, 256, 0, s>>>(net, B, init_y, total_nodes, feature_kind);
+                              int feature_kind, int anchors, double *anc, int num_cus, hipStream_t s) {
+    if (anchors == 0) {
+        k_infer<NodeSrc><<<infer_grid(total_nodes, num_cus), 256, 0, s>>>(net, NodeSrc{B, init_y, feature_kind},
+                                                                          total_nodes);
+        return hipGetLastError();
+    }
+    const long long pts = (long long)B.n * anchors;
+    k_infer<AnchorSrc><<<infer_grid(pts, num_cus), 256, 0, s>>>(net, AnchorSrc{B, anc, anchors, feature_kind}, pts);
+    return hipGetLastError();
+}
+
+hipError_t launch_anchor_fill(const BatchDev &B, const double *anc, int anchors, double *init_y, hipStream_t s) {
+    k_anchor_fill<<<B.n, 256, 0, s>>>(B, anc, anchors, init_y);
     return hipGetLastError();
 }
 

@@ -129,7 +129,9 @@ typedef struct {
     int32_t transfer_mode;        /* BVP_TRANSFER_*                              */
     int32_t precision;            /* BVP_PREC_*                                  */
     int32_t refine_iters;         /* FP64 refinement steps per linear solve (FP32 LU) */
-    int32_t warm_start;           /* BVP_WARM_* (needs a loaded MLP)             */
+    int32_t warm_start;           /* BVP_WARM_MLP: guess holds a learned guess;
+                                     the solve falls back to u0 when its first
+                                     Newton step does not contract (see below) */
 } bvp_opts;
 
 /* Per-instance result.  mesh/u are caller-allocated with room for
@@ -233,6 +235,24 @@ void bvp_batch_destroy(bvp_batch *batch);
  * reset, learned guess if attached, persistent solve) into a graph and later
  * runs replay it; attaching or detaching a network re-captures. */
 int  bvp_batch_set_graph(bvp_batch *batch, int32_t enable);
+/* ---- autotuner (no counterpart in the reference).  Times the plan's solve
+ * with each launch configuration it can take -- the persistent solve kernel's
+ * variants (for M = 2: register allocation held to 1, 2 or 3 waves per SIMD,
+ * trading spills for latency hiding) at their full residency, or, for the block
+ * kernels, 1, 2, 4 ... waves per CU -- `repeats` timed runs each (median), keeps
+ * the fastest for the plan and remembers it in the context for later plans of
+ * the same kind (M, method, precision).  cand_ms (optional, room for 16)
+ * receives the median of every candidate, *n_cand their number.  The runs
+ * overwrite the plan's results. */
+int  bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *batch, int32_t repeats, float *cand_ms, int32_t *n_cand);
+/* The plan's current launch configuration: kernel variant and persistent waves;
+ * and setting it by hand (blocks at most the waves the plan was sized for). */
+int  bvp_batch_config(bvp_batch *batch, int32_t *variant, int32_t *blocks);
+int  bvp_batch_set_config(bvp_ctx *ctx, bvp_batch *batch, int32_t variant, int32_t blocks);
+
+/* Device time of the last synced run split into the learned-guess part (queue
+ * reset + guess kernel; 0 without a network) and the persistent solve. */
+int  bvp_batch_timing(bvp_batch *batch, float *guess_ms, float *solve_ms);
 
 /* ---- learned warm start (no counterpart in the reference; it plugs in where
  * the reference takes the initial guess from the caller, as a constant vector,
@@ -290,10 +310,18 @@ int  bvp_mlp_train(bvp_mlp *mlp, int64_t n, const float *x, const float *y, cons
                    float *loss_per_step, int32_t *steps, float *ms);
 /* Attach (or with mlp == NULL detach) a network to a plan: each run first writes
  * the learned guess of every node on the device, then solves with the
- * contraction-checked fallback described above.  M = 2 plans only; every
- * instance must carry u0.  The plan keeps the pointer: detach or destroy the
- * plan before the network. */
-int  bvp_batch_set_warm_start(bvp_batch *batch, const bvp_mlp *mlp, int32_t feature_kind);
+ * contraction-checked fallback described above.  anchors == 0 evaluates the
+ * network at every node of the initial mesh; anchors = K >= 2 evaluates it at K
+ * equally spaced points of [t0, t1] per instance and fills the nodes by cubic
+ * Hermite interpolation with the slopes y' = f(t, y) of the problem at those
+ * points (K network evaluations per instance instead of N + 1).  M = 2 plans
+ * only; every instance must carry u0.  The plan keeps the pointer: detach or
+ * destroy the plan before the network. */
+int  bvp_batch_set_warm_start(bvp_batch *batch, const bvp_mlp *mlp, int32_t feature_kind, int32_t anchors);
+/* The learned guess of the attached network for every node of the initial
+ * meshes, point-major and packed in instance order (sum (N+1) * M doubles): the
+ * initial guess a warm-started run starts Newton from. */
+int  bvp_batch_learned_guess(bvp_ctx *ctx, bvp_batch *batch, double *y);
 
 /* ---- measurement helper (no counterpart in the reference): a plain device
  * copy of nbytes, run `repeats` times; *ms = fastest device time.  It is the

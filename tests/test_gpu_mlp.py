@@ -124,26 +124,74 @@ def test_bad_warm_start_falls_back_to_plain_bitwise(ctx):
         assert np.array_equal(w.t, p.t) and np.array_equal(w.u, p.u)
 
 
-def test_learned_warm_start_saves_newton_iterations(ctx):
+@pytest.fixture(scope="module")
+def bratu_net(ctx):
+    """A network trained on 4,096 plain solves of the Bratu (a, b) family."""
     _say("warm start training")
     opts = _plain_opts()
+    opts.max_newton_iters = 50
     train_specs, _, _ = synthetic.bratu_ab_batch(4096, seed=synthetic.SEED_BRATU_AB_TRAIN)
     x, y, sols = learned.harvest(ctx, train_specs, opts, per_solve=64)
     _say("  samples", len(x), "solved", sum(r.retcode == 0 for r in sols))
     net, loss, ms = learned.train(ctx, x, y, epochs=40, batch=4096, lr=2e-3, lr_min=2e-5, warmup=50)
     _say("  loss", loss[0], "->", loss[-1], "ms", ms)
     assert loss[-1] < 0.02 * loss[0]
+    return net
+
+
+def _bratu_f(lam, y):
+    return np.stack([y[:, 1], -lam * np.exp(y[:, 0])], 1)
+
+
+@pytest.mark.parametrize("anchors", [0, 2, 9])
+def test_learned_guess_is_network_then_hermite(ctx, bratu_net, anchors):
+    """The device guess equals the network at every node (anchors = 0), or the
+    network at the anchors followed by cubic Hermite interpolation with the slopes
+    f(t, y) of the problem (checked against numpy, 1e-12 relative)."""
+    specs, _, _ = synthetic.bratu_ab_batch(40, seed=77, Nrange=(3, 90))
+    b = _lib.Batch(ctx, _plain_opts(), specs)
+    b.set_warm_start(bratu_net, _abi.FEAT_PARAM, anchors)
+    got = b.learned_guess()
+    for s, g in zip(specs, got):
+        lam, a, bb = float(s.params[0]), float(s.bc_val[0]), float(s.bc_val[1])
+        t = s.full_mesh()
+        if anchors == 0:
+            x = np.stack([np.full_like(t, lam), np.full_like(t, a), np.full_like(t, bb), t], 1)
+            ref = bratu_net.predict(x).astype(np.float64)
+            assert np.array_equal(g, ref)
+            continue
+        K = anchors
+        ta = np.array([s.t0 + (s.t1 - s.t0) * (j / (K - 1)) for j in range(K)])
+        ta[-1] = s.t1
+        xa = np.stack([np.full(K, lam), np.full(K, a), np.full(K, bb), ta], 1)
+        ya = bratu_net.predict(xa).astype(np.float64)
+        ma = _bratu_f(lam, ya)
+        sc = (t - s.t0) / (s.t1 - s.t0) * (K - 1)
+        j = np.clip(np.floor(sc).astype(int), 0, K - 2)
+        th = (sc - j)[:, None]
+        h = (s.t1 - s.t0) / (K - 1)
+        ref = ((2 * th ** 3 - 3 * th ** 2 + 1) * ya[j] + (th ** 3 - 2 * th ** 2 + th) * h * ma[j]
+               + (-2 * th ** 3 + 3 * th ** 2) * ya[j + 1] + (th ** 3 - th ** 2) * h * ma[j + 1])
+        assert np.all(np.abs(g - ref) <= 1e-12 * (1 + np.abs(ref)))
+
+
+@pytest.mark.parametrize("anchors", [0, 16])
+def test_learned_warm_start_saves_newton_iterations(ctx, bratu_net, anchors):
+    # Newton tolerance 1e-12: the (a, b) family has Jacobians with norm(J^-1) ~ 1e2,
+    # so that the solutions sit within 1e-9 of the discrete solution
+    opts = _plain_opts()
+    opts.newton_abstol = 1e-12
     eval_specs, _, _ = synthetic.bratu_ab_batch(512, seed=synthetic.SEED_BRATU_AB_EVAL)
     b = _lib.Batch(ctx, opts, eval_specs)
     b.run()
     b.sync()
     plain = b.results()
-    b.set_warm_start(net, _abi.FEAT_PARAM)
+    b.set_warm_start(bratu_net, _abi.FEAT_PARAM, anchors)
     b.run()
     b.sync()
     warm = b.results()
     sp, sw = learned.summary(plain), learned.summary(warm)
-    _say("  plain", sp, "\n  warm ", sw)
+    _say("  anchors", anchors, "plain", sp, "\n  warm ", sw)
     # instances without a solution (part of the (lambda, a, b) box) run to maxiters
     # from either guess; the saving shows on the solved ones
     assert sw["mean_newton_iters_solved"] < 0.85 * sp["mean_newton_iters_solved"]

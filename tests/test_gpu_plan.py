@@ -1,0 +1,98 @@
+"""GPU checks of the plan API around the persistent solve: hipGraph replay, the
+autotuner's launch variants, and the timing split.
+
+Every configuration runs the same arithmetic, so the solutions must agree to the
+bit across graph/no-graph runs and across the solve-kernel variants the
+autotuner chooses from; the oracle parity of the solutions themselves is covered
+by test_gpu_parity.py.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from boundaryvaluediffeq_amd import _abi, _lib, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return _lib.context(0)
+
+
+def _same(ra, rb):
+    for a, b in zip(ra, rb):
+        assert a.retcode == b.retcode and a.N == b.N and a.newton_iters == b.newton_iters
+        assert np.array_equal(a.t, b.t) and np.array_equal(a.u, b.u)
+
+
+def _opts():
+    o = _abi.default_opts(_abi.MIRK4)
+    o.abstol = 1e-8
+    return o
+
+
+def test_graph_replay_matches_direct_launches(ctx):
+    _say("graph replay")
+    specs, _, _ = synthetic.bratu_batch(600)
+    b = _lib.Batch(ctx, _opts(), specs)
+    b.set_graph(False)
+    b.run()
+    b.sync()
+    direct = b.results()
+    b.set_graph(True)
+    for _ in range(3):  # replays with eager work and synchronisation in between
+        b.run()
+        b.sync()
+        _same(direct, b.results())
+        g, s = b.timing()
+        assert g == 0.0 and s > 0.0
+
+
+def test_autotuner_variants_agree_and_choice_is_fastest(ctx):
+    _say("autotune")
+    specs, _, _ = synthetic.bratu_batch(2048)
+    b = _lib.Batch(ctx, _opts(), specs)
+    b.run()
+    b.sync()
+    ref = b.results()
+    ms, (variant, blocks) = b.autotune(repeats=2)
+    _say("  candidates ms", ms, "chosen", variant, blocks)
+    assert len(ms) == 3  # M = 2: 1, 2 or 3 waves per SIMD
+    assert ms[variant] == min(ms)
+    assert blocks >= 1
+    b.run()
+    b.sync()
+    _same(ref, b.results())
+    # every variant computes the same bits, at any number of persistent waves
+    for v in range(len(ms)):
+        for nb in (blocks, 7):
+            b.set_config(v, nb)
+            b.run()
+            b.sync()
+            _same(ref, b.results())
+
+
+def test_block_plan_autotune(ctx):
+    """The block kernels (M = 8) take 1, 2, 4 ... waves per CU."""
+    _say("autotune block")
+    specs, _ = synthetic.lintanh_batch(64, M=8, Nrange=(40, 80))
+    o = _abi.default_opts(_abi.MIRK6)
+    o.precision = _abi.PREC_FP32_REFINE64
+    o.refine_iters = 2
+    b = _lib.Batch(ctx, o, specs)
+    b.run()
+    b.sync()
+    ref = b.results()
+    ms, (variant, blocks) = b.autotune(repeats=1)
+    _say("  candidates ms", ms, "chosen", variant, blocks)
+    assert len(ms) >= 1 and variant == 0
+    b.run()
+    b.sync()
+    _same(ref, b.results())
