@@ -141,7 +141,7 @@ def train_warm_start(ctx, args, opts, _abi, synthetic, learned):
     log(f"[bench] training data: {len(specs)} plain solves ({solved} solved) -> {len(x)} samples in {th:.1f} s")
     t0 = time.perf_counter()
     net, loss, ms = learned.train(ctx, x, y, epochs=args.train_epochs, batch=8192, lr=2e-3, lr_min=2e-5,
-                                  warmup=200)
+                                  warmup=200, use_graph=not args.no_graph)
     log(f"[bench] training: {len(loss)} Adam steps, loss {loss[0]:.3e} -> {loss[-1]:.3e}, device {ms:.0f} ms, "
         f"wall {time.perf_counter() - t0:.1f} s")
     return net, {"train_solves": len(specs), "train_solved": solved, "samples": int(len(x)),
@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--no-learned", action="store_true", help="skip the warm-start network (constant guess only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (variant 0 at full residency)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernels directly instead of replaying captured graphs (for counter profiling)")
     ap.add_argument("--train-solves", type=int, default=200000)
     ap.add_argument("--train-nodes", type=int, default=64, help="training samples per solve")
     ap.add_argument("--train-epochs", type=int, default=6)
@@ -265,6 +267,7 @@ def main():
     ctx = _lib.context(0)
     log(f"[bench] uploading {n} instances (sum N = {int(Ns.sum())})")
     batch = _lib.Batch(ctx, opts, specs)
+    batch.set_graph(not args.no_graph)
     info = batch.info()
     log(f"[bench] persistent waves {info[0]}, node capacity {info[1]}")
     n0_nodes = int(sum(s.N + 1 for s in specs))
