@@ -211,6 +211,52 @@ def bench_learned(args, _abi, _lib, synthetic, learned):
                       "training": tinfo}), flush=True)
 
 
+def bench_singpert(args, _abi, _lib, synthetic, learned):
+    """Adaptive batch: 8,192 problems eps u'' + t u' - u = 0 on [-1, 1], u(-1)=1, u(1)=2,
+    eps ~ LogUniform(1e-4, 1e-1), initial mesh 32 intervals, refinement to <= 16,384
+    intervals at abstol 1e-6, one persistent launch; with and without a warm-start
+    network trained on solves of the same family (features log10 eps, t)."""
+    n = args.instances if args.instances != 16384 else 8192
+    ctx = _lib.context(0)
+    opts = _abi.default_opts(_abi.MIRK4)
+    opts.abstol = 1e-6
+    opts.max_num_subintervals = 16384
+    specs, eps = synthetic.singpert_batch(n)
+    batch = _lib.Batch(ctx, opts, specs)
+    batch.set_graph(not args.no_graph)
+    tune = None
+    if not args.no_tune:
+        cand_ms, (variant, blocks) = batch.autotune(repeats=1)
+        tune = {"candidates_ms": cand_ms, "variant": variant, "persistent_waves": blocks}
+        log(f"[bench] autotune: {['%.1f' % x for x in cand_ms]} ms -> variant {variant}, {blocks} waves")
+    tp = timed_runs(batch, args.steps, args.warmup, "constant guess")
+    rp = batch.results()
+    cp = batch.counters()
+    out = {"workload": f"singpert{n}: {n} BVPs eps u''+t u'-u=0, eps~LogUniform(1e-4,1e-1), N0=32, adaptive to "
+                       "<=16384 intervals, MIRK4, abstol 1e-6, FP64",
+           "constant_guess": {"solves_per_s": n_solved(rp, 1e-6) / tp["wall"], **tp, **learned.summary(rp),
+                              "final_N_mean": float(sum(r.N for r in rp) / n),
+                              "final_N_max": int(max(r.N for r in rp)), "counters": cp},
+           "autotune": tune}
+    if not args.no_learned:
+        tspecs, _ = synthetic.singpert_batch(args.train_solves // 10, seed=synthetic.SEED_SINGPERT_TRAIN)
+        t0 = time.perf_counter()
+        x, y, sols = learned.harvest(ctx, tspecs, opts, per_solve=args.train_nodes, kind=_abi.FEAT_LOG10_PARAM,
+                                     chunk=4096)
+        log(f"[bench] training data: {len(tspecs)} solves -> {len(x)} samples in {time.perf_counter() - t0:.1f} s")
+        net, loss, ms = learned.train(ctx, x, y, epochs=args.train_epochs * 4, batch=8192, lr=2e-3, lr_min=2e-5,
+                                      warmup=200, use_graph=not args.no_graph)
+        log(f"[bench] training: {len(loss)} steps, loss {loss[0]:.3e} -> {loss[-1]:.3e}, device {ms:.0f} ms")
+        batch.set_warm_start(net, _abi.FEAT_LOG10_PARAM, 0)  # 33 nodes per instance: the network at each
+        tw = timed_runs(batch, args.steps, args.warmup, "learned guess")
+        rw = batch.results()
+        out["learned_guess"] = {"solves_per_s": n_solved(rw, 1e-6) / tw["wall"], **tw, **learned.summary(rw),
+                                "final_N_mean": float(sum(r.N for r in rw) / n), "counters": batch.counters()}
+        out["training"] = {"solves": len(tspecs), "samples": int(len(x)), "adam_steps": int(len(loss)),
+                           "final_loss": float(loss[-1]), "device_ms": ms}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,7 +275,7 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--anchors", type=int, default=4,
                     help="network evaluations per instance (Hermite-interpolated to the nodes); 0: every node")
-    ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh", "learned"],
+    ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh", "learned", "singpert"],
                     help="bratu: 16,384 Bratu BVPs (MIRK4, FP64, learned warm start); lintanh: 2,048 n=32 "
                          "linear+tanh BVPs (MIRK6, FP32 factors + FP64 refinement); learned: warm start on "
                          "16,384 unseen Bratu (a, b) BVPs")
@@ -257,6 +303,8 @@ def main():
         return bench_lintanh(args, _abi, _lib, synthetic)
     if args.workload == "learned":
         return bench_learned(args, _abi, _lib, synthetic, learned)
+    if args.workload == "singpert":
+        return bench_singpert(args, _abi, _lib, synthetic, learned)
 
     n = args.instances
     abstol = 1e-8

@@ -517,14 +517,24 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
     mx = 0.0;
     bad = false;
     if (len == 0) return;
-    double yi[M], yj[M];
+    double yi[M], yj[M], yn[M];
 #pragma unroll
-    for (int c = 0; c < M; ++c) yi[c] = y[(size_t)start * M + c];
-    double ti = mesh[start];
+    for (int c = 0; c < M; ++c) {
+        yi[c] = y[(size_t)start * M + c];
+        yn[c] = y[(size_t)(start + 1) * M + c];
+    }
+    double ti = mesh[start], tn = mesh[start + 1];
     for (int k = start; k < start + len; ++k) {
-        const double tj = mesh[k + 1];
+        // the next node's values are loaded one interval ahead, so that their
+        // memory latency overlaps this interval's arithmetic
+        const double tj = tn;
 #pragma unroll
-        for (int c = 0; c < M; ++c) yj[c] = y[(size_t)(k + 1) * M + c];
+        for (int c = 0; c < M; ++c) yj[c] = yn[c];
+        if (k + 2 <= start + len) {
+            tn = mesh[k + 2];
+#pragma unroll
+            for (int c = 0; c < M; ++c) yn[c] = y[(size_t)(k + 2) * M + c];
+        }
         const double h = tj - ti;
         double phi[M], S[M][M], R[M][M], Kp[Tab<METHOD>::s][M];
         interval_dual<M, METHOD>(I.fam, I.P, ti, h, yi, yj, phi, S, R, Kp);
@@ -764,9 +774,11 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
                 if (apply) y[(size_t)N * M + c] += yn[c];
             }
         }
+        Back<M> bnext;
+        if (len > 1) load_back<M>(scr, len - 2, lane, bnext);
         for (int k = start + len - 1; k > start; --k) {
-            Back<M> bk;
-            load_back<M>(scr, k - start - 1, lane, bk);
+            const Back<M> bk = bnext;  // record of node k; the next one is loaded ahead
+            if (k - 1 > start) load_back<M>(scr, k - start - 2, lane, bnext);
             double yk[M];
             back_apply<M>(bk, ya, yn, yk);
 #pragma unroll

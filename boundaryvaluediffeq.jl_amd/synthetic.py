@@ -17,6 +17,7 @@ SEED_LINEAR_TANH = 0xB0A70003
 SEED_SINGPERT = 0xB0A70005
 SEED_BRATU_AB_TRAIN = 0xB0A70004
 SEED_BRATU_AB_EVAL = 0xB0A70044
+SEED_SINGPERT_TRAIN = 0xB0A70055
 
 
 def _u01(seed, n, ndraw):
