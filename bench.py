@@ -80,6 +80,7 @@ def bench_lintanh(args, _abi, _lib, synthetic):
     ctx = _lib.context(0)
     log(f"[bench] lintanh: uploading {n} instances, M={M} (sum N = {int(Ns.sum())})")
     batch = _lib.Batch(ctx, opts, specs)
+    batch.set_graph(not args.no_graph)
     log(f"[bench] persistent waves {batch.info()[0]}")
     for w in range(args.warmup):
         batch.run()

@@ -43,6 +43,7 @@ template <int M, int METHOD, typename LT> struct BShared {
         double root[2 * M][RW];           // the end-node system
     } w;
     double xr[2 * M];
+    LT prow[W + 3];                       // pivot row broadcast of the register elimination
     double bcres[M];
     int piv[2 * M];
     int src[M];
@@ -271,6 +272,73 @@ __device__ inline bool eliminate_lds(T *mat, int lane, int *piv, T *mult) {
     return done && lane < NR;
 }
 
+// ------------------------------------------------------------------ elimination in registers
+// Lane of the first maximum of v over the wave: an inclusive max-scan inside
+// each row of 16 lanes and across rows with DPP moves (no LDS round trips), the
+// maximum read from lane 63, the first lane holding it found by a ballot.
+__device__ inline int wave_argmax_first(float v) {
+    float m = v;
+#define BVP_DPP_MAX(ctrl, rmask) \
+    m = fmaxf(m, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(m), __float_as_int(m), ctrl, rmask, 0xf, false)))
+    BVP_DPP_MAX(0x111, 0xf);  // row_shr:1
+    BVP_DPP_MAX(0x112, 0xf);  // row_shr:2
+    BVP_DPP_MAX(0x114, 0xf);  // row_shr:4
+    BVP_DPP_MAX(0x118, 0xf);  // row_shr:8
+    BVP_DPP_MAX(0x142, 0xa);  // row_bcast:15 into rows 1 and 3
+    BVP_DPP_MAX(0x143, 0xc);  // row_bcast:31 into rows 2 and 3
+#undef BVP_DPP_MAX
+    const float mx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), 63));
+    const unsigned long long b = __ballot(v == mx);
+    return b ? __ffsll((long long)b) - 1 : 0;
+}
+
+template <typename T> __device__ inline T rdlane(T v, int l);
+template <> __device__ inline float rdlane<float>(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <> __device__ inline double rdlane<double>(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The same elimination as eliminate_lds with row l of the NR x NW system held
+// by lane l in registers.  The step loop over j is not unrolled, so every
+// access to row[] uses a compile-time column index (selects and predicated
+// updates instead of row[j]), which keeps the row in registers.  At step j the
+// pivot lane (piv[j], the first maximum of |row[j]| over the rows not yet used,
+// found with DPP moves) is read column by column with v_readlane (a scalar
+// broadcast with no memory round trip).  Pivot rows are not swapped.  Returns the step at which this
+// lane's row became a pivot row, or -1.
+template <int NC, int NR, int NW, typename T>
+__device__ inline int eliminate_reg(T (&row)[NW], int lane, int (&piv)[NR], T *mult, T *prow) {
+    bool done = lane >= NR;
+    int step = -1;
+#pragma unroll 1
+    for (int j = 0; j < NC; ++j) {
+        T rj = (T)0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) rj = (c == j) ? row[c] : rj;
+        // pivot: first maximum of |row[j]| over the rows not yet used (compared in FP32)
+        const int p = __builtin_amdgcn_readfirstlane(wave_argmax_first(done ? -1.f : (float)fabs(rj)));
+        if (lane == 0) piv[j] = p;
+        const T pv = rdlane(rj, p);
+        const bool act = !done && lane != p;
+        const T m = act ? rj / pv : (T)0;
+        mult[(size_t)j * WAVE] = m;
+        if (lane == p) { done = true; step = j; }
+#pragma unroll
+        for (int c = 0; c < NW; ++c) {
+            const T mc = (c > j) ? m : (T)0;
+            T r = row[c] - mc * rdlane(row[c], p);
+            if (c < NC) r = (c == j && act) ? (T)0 : r;
+            row[c] = r;
+        }
+    }
+    return step;
+}
+
 // ------------------------------------------------------------------ records
 // Per interval k (interior node k of the single chunk): for the pivot rows in
 // pivot order j: U_j (M), X_j (M), Z_j (M), h_j; with the FP32 factorisation also
@@ -312,11 +380,15 @@ __device__ inline void bload_inst(const BatchDev &B, int b, BShared<M, METHOD, L
     __syncthreads();
 }
 
+// The phases below (residual, factor/solve, refinement residual, defect,
+// transfer) are kept out of line: each gets its own register allocation instead
+// of one allocation for their union, which is what spilled when they were inlined.
+//
 // FP64 residual of all intervals at y (+ alpha*delta if delta): returns max|F|
 // (NaN if not finite) and, when ss != null, the sum of squares.  BC residuals
 // are left in sh.bcres.
 template <int M, int METHOD, typename LT>
-__device__ double bresidual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh, const double *y,
+__device__ __noinline__ double bresidual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh, const double *y,
                             const double *delta, double alpha, int N, int lane, double *ss,
                             double *phi_out = nullptr) {
     double mx = 0.0, acc = 0.0;
@@ -365,7 +437,7 @@ __device__ double bresidual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const
 // N*M + k) when factor == false (a refinement solve through the stored factors).
 // The correction is written to dout (point-major, M*(N+1)).
 template <int M, int METHOD, typename LT>
-__device__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh, const double *y, int N,
+__device__ __noinline__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh, const double *y, int N,
                        int lane, LT *rec, bool factor, const double *rin, double *dout) {
     using SHT = BShared<M, METHOD, LT>;
     constexpr int W = SHT::W, SW = SHT::SW, RW = SHT::RW;
@@ -388,41 +460,77 @@ __device__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const doub
             if (lane < M) sh.w.stk[rb + lane][3 * M] = (LT)rk;
             __syncthreads();
             if (k == 0) continue;
-            const bool piv_row = eliminate_lds<M, 2 * M, W, SW, LT>(stk, lane, sh.piv, R + BRec<M>::MULT + lane);
-            // records of the pivot rows, in pivot order: U | X | Z | h
-            if (piv_row) {
-                int j = 0;
-                for (int q = 0; q < M; ++q) j = (sh.piv[q] == lane) ? q : j;
-                const LT *row = stk + (size_t)lane * SW;
-                for (int c = 0; c < M; ++c) {
-                    R[BRec<M>::U + (size_t)c * M + j] = row[c];
-                    R[BRec<M>::X + (size_t)c * M + j] = row[M + c];
-                    R[BRec<M>::Z + (size_t)c * M + j] = row[2 * M + c];
+            if constexpr (M >= 16) {
+                // the stacked rows go to registers (lane l: row l) and are eliminated there
+                LT row[W];
+    #pragma unroll
+                for (int c = 0; c < W; ++c) row[c] = (lane < 2 * M) ? sh.w.stk[lane][c] : (LT)0;
+                __syncthreads();
+                const int pstep = eliminate_reg<M, 2 * M, W, LT>(row, lane, sh.piv, R + BRec<M>::MULT + lane, sh.prow);
+                (void)stk;
+                // records of the pivot rows, in pivot order: U | X | Z | h
+                if (pstep >= 0) {
+    #pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        R[BRec<M>::U + (size_t)c * M + pstep] = row[c];
+                        R[BRec<M>::X + (size_t)c * M + pstep] = row[M + c];
+                        R[BRec<M>::Z + (size_t)c * M + pstep] = row[2 * M + c];
+                    }
+                    R[BRec<M>::H + pstep] = row[3 * M];
                 }
-                R[BRec<M>::H + j] = row[3 * M];
+                // the remaining rows become the next relation [F' | E' | 0 | g'] in rows 0..M-1
+                const bool keep = pstep < 0 && lane < 2 * M;
+                const unsigned long long rest = __ballot(keep);
+                __syncthreads();
+                if (lane < M) R[BRec<M>::PIV + lane] = (LT)sh.piv[lane];
+                if (keep) {
+                    const int to = __popcll(rest & ((1ull << lane) - 1ull));
+    #pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        sh.w.stk[to][c] = row[2 * M + c];
+                        sh.w.stk[to][M + c] = row[M + c];
+                        sh.w.stk[to][2 * M + c] = (LT)0;
+                    }
+                    sh.w.stk[to][3 * M] = row[3 * M];
+                }
+                __syncthreads();
+            } else {
+                const bool piv_row = eliminate_lds<M, 2 * M, W, SW, LT>(stk, lane, sh.piv, R + BRec<M>::MULT + lane);
+                // records of the pivot rows, in pivot order: U | X | Z | h
+                if (piv_row) {
+                    int j = 0;
+                    for (int q = 0; q < M; ++q) j = (sh.piv[q] == lane) ? q : j;
+                    const LT *row = stk + (size_t)lane * SW;
+                    for (int c = 0; c < M; ++c) {
+                        R[BRec<M>::U + (size_t)c * M + j] = row[c];
+                        R[BRec<M>::X + (size_t)c * M + j] = row[M + c];
+                        R[BRec<M>::Z + (size_t)c * M + j] = row[2 * M + c];
+                    }
+                    R[BRec<M>::H + j] = row[3 * M];
+                }
+                if (lane < M) R[BRec<M>::PIV + lane] = (LT)sh.piv[lane];
+                // the remaining rows become the next relation [F' | E' | 0 | g'] in rows 0..M-1
+                // (each remaining row is read whole into registers before any row is written)
+                const unsigned long long rest = __ballot(!piv_row && lane < 2 * M);
+                const bool keep = !piv_row && lane < 2 * M;
+                LT cp[2 * M + 1];
+                if (keep) {
+                    const LT *from = stk + (size_t)lane * SW;
+    #pragma unroll
+                    for (int c = 0; c < M; ++c) { cp[c] = from[2 * M + c]; cp[M + c] = from[M + c]; }
+                    cp[2 * M] = from[3 * M];
+                }
+                __syncthreads();
+                if (keep) {
+                    LT *to = stk + (size_t)__popcll(rest & ((1ull << lane) - 1ull)) * SW;
+    #pragma unroll
+                    for (int c = 0; c < 2 * M; ++c) to[c] = cp[c];
+    #pragma unroll
+                    for (int c = 0; c < M; ++c) to[2 * M + c] = (LT)0;
+                    to[3 * M] = cp[2 * M];
+                }
+                __syncthreads();
             }
-            if (lane < M) R[BRec<M>::PIV + lane] = (LT)sh.piv[lane];
-            // the remaining rows become the next relation [F' | E' | 0 | g'] in rows 0..M-1
-            // (each remaining row is read whole into registers before any row is written)
-            const unsigned long long rest = __ballot(!piv_row && lane < 2 * M);
-            const bool keep = !piv_row && lane < 2 * M;
-            LT cp[2 * M + 1];
-            if (keep) {
-                const LT *from = stk + (size_t)lane * SW;
-#pragma unroll
-                for (int c = 0; c < M; ++c) { cp[c] = from[2 * M + c]; cp[M + c] = from[M + c]; }
-                cp[2 * M] = from[3 * M];
-            }
-            __syncthreads();
-            if (keep) {
-                LT *to = stk + (size_t)__popcll(rest & ((1ull << lane) - 1ull)) * SW;
-#pragma unroll
-                for (int c = 0; c < 2 * M; ++c) to[c] = cp[c];
-#pragma unroll
-                for (int c = 0; c < M; ++c) to[2 * M + c] = (LT)0;
-                to[3 * M] = cp[2 * M];
-            }
-            __syncthreads();
         } else {
             // push the new right-hand side through the stored elimination
             if (k == 0) {
@@ -542,7 +650,7 @@ __device__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const doub
 
 // r = -F(y) - J(y) d  (FP64), point-major residual layout as used by bsolve
 template <int M, int METHOD, typename LT>
-__device__ void brefine_residual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh, const double *y,
+__device__ __noinline__ void brefine_residual(const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh, const double *y,
                                  const double *d, int N, int lane, double *rout) {
     for (int k = 0; k < N; ++k) {
         const double h = mesh[k + 1] - mesh[k];
@@ -627,7 +735,7 @@ __device__ NewtonOut bnewton(const BatchDev &B, const BInst<M> &I, BShared<M, ME
 
 // ------------------------------------------------------------------ defect (block)
 template <int M, int METHOD, typename LT>
-__device__ double bdefect(BShared<M, METHOD, LT> &sh, const double *mesh, const double *y, int N, int lane,
+__device__ __noinline__ double bdefect(BShared<M, METHOD, LT> &sh, const double *mesh, const double *y, int N, int lane,
                           double abstol, double *shat, double *errors, double &r1, double &r2) {
     using TB = Tab<METHOD>;
     double w1[TB::sstar], w1p[TB::sstar], w2[TB::sstar], w2p[TB::sstar];
@@ -694,7 +802,7 @@ __device__ double bdefect(BShared<M, METHOD, LT> &sh, const double *mesh, const 
 // (MIRK/mirk.jl:366-370); REFERENCE mode reads the base value y0.u[ii] from the
 // array being written, as the reference's in-place update does.
 template <int M, int METHOD, typename LT>
-__device__ void btransfer(BShared<M, METHOD, LT> &sh, const double *mesh, const double *y, int N, const double *nm,
+__device__ __noinline__ void btransfer(BShared<M, METHOD, LT> &sh, const double *mesh, const double *y, int N, const double *nm,
                           double *ny, int Nn, int mode, int lane) {
     using TB = Tab<METHOD>;
     for (int j = 0; j <= Nn; ++j) {
