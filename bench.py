@@ -82,6 +82,11 @@ def bench_lintanh(args, _abi, _lib, synthetic):
     batch = _lib.Batch(ctx, opts, specs)
     batch.set_graph(not args.no_graph)
     log(f"[bench] persistent waves {batch.info()[0]}")
+    tune = None
+    if not args.no_tune:
+        cand_ms, (variant, blocks) = batch.autotune(repeats=1)
+        tune = {"candidates_ms": cand_ms, "persistent_waves": blocks}
+        log(f"[bench] autotune: {['%.0f' % x for x in cand_ms]} ms -> {blocks} waves")
     for w in range(args.warmup):
         batch.run()
         log(f"[bench] warmup {w}: kernel {batch.sync():.1f} ms")
@@ -101,10 +106,17 @@ def bench_lintanh(args, _abi, _lib, synthetic):
     tfs = flops / (kmean * 1e-3) / 1e12
     log(f"[bench] {good}/{n} solved; {good / wall:.1f} solves/s; {tfs:.2f} TFLOP/s; counters {counters}; "
         f"newton iters mean {sum(r.newton_iters for r in res) / n:.2f}")
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        k = min(n, args.cpu_sample)
+        sample = specs[:: max(1, n // k)][:k]
+        cpu = cpu_baseline(sample, opts.abstol, threads, method=6, max_num_subintervals=opts.max_num_subintervals)
     print(json.dumps({"workload": f"lintanh{n}: {n} BVPs u'=Au+0.1 tanh(u), n=32, MIRK6, FP32 factors + FP64 "
                                   "refinement, abstol 1e-6", "value": good / wall, "unit": "solves/s",
                       "ms_per_step": wall * 1e3, "kernel_ms": kmean, "solved": good, "tflops": tfs,
-                      "peak_fp32_mfma_tflops": 157.3, "counters": counters}), flush=True)
+                      "peak_fp32_mfma_tflops": 157.3, "counters": counters, "autotune": tune,
+                      "cpu_baseline": cpu}), flush=True)
 
 
 def timed_runs(batch, steps, warmup, tag):
@@ -172,10 +184,10 @@ def plain_single(ctx, opts, specs, sample, abstol, _lib):
             "sample": f"{len(idx)} instances (every {stride}th), each solved alone in its own launch"}
 
 
-def cpu_baseline(specs, abstol, threads, method=4):
+def cpu_baseline(specs, abstol, threads, method=4, **kw):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
-    oo = O.opts(method, abstol=abstol)
+    oo = O.opts(method, abstol=abstol, **kw)
     log(f"[bench] oracle on {threads} threads, {len(specs)} instances")
     tc = time.perf_counter()
     rets = O.solve_batch(oo, specs, nthreads=threads, keep_solutions=False)
@@ -267,6 +279,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=128, help="instances of the oracle run (lintanh workload)")
     ap.add_argument("--no-learned", action="store_true", help="skip the warm-start network (constant guess only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (variant 0 at full residency)")
     ap.add_argument("--no-graph", action="store_true",
