@@ -283,7 +283,11 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         // Room for meshes up to twice the largest initial mesh when memory allows.
         const size_t esz = (o->precision == BVP_PREC_FP32_REFINE64) ? 4 : 8;
         const size_t rv = block_record_values(M);
-        const size_t budget = (size_t)120 << 30;
+        // budget: 80% of the device memory free at this point (the problem data is
+        // already allocated), so that meshes can grow to twice the largest initial one
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) free_b = (size_t)120 << 30;
+        const size_t budget = free_b / 10 * 8;
         int rec_cap = std::min(o->max_num_subintervals > maxN0 ? o->max_num_subintervals : maxN0, 2 * maxN0);
         if ((size_t)rec_cap * rv * esz * slots > budget) rec_cap = maxN0;
         const size_t per_slot = (size_t)rec_cap * rv;  // in factor-type elements

@@ -40,7 +40,7 @@ template <int M, int METHOD, typename LT> struct BShared {
     LT sech2[S][M];
     union {
         LT stk[2 * M][SW];                // stacked rows of one elimination step (rows 0..M-1: relation)
-        double root[2 * M][RW];           // the end-node system
+        double root[M >= 16 ? 1 : 2 * M][RW];  // the end-node system (M < 16; larger M: in registers)
     } w;
     double xr[2 * M];
     LT prow[W + 3];                       // pivot row broadcast of the register elimination
@@ -311,7 +311,7 @@ template <> __device__ inline double rdlane<double>(double v, int l) {
 // found with DPP moves) is read column by column with v_readlane (a scalar
 // broadcast with no memory round trip).  Pivot rows are not swapped.  Returns the step at which this
 // lane's row became a pivot row, or -1.
-template <int NC, int NR, int NW, typename T>
+template <int NC, int NR, int NW, typename T, bool STORE_MULT = true>
 __device__ inline int eliminate_reg(T (&row)[NW], int lane, int (&piv)[NR], T *mult, T *prow) {
     bool done = lane >= NR;
     int step = -1;
@@ -326,7 +326,7 @@ __device__ inline int eliminate_reg(T (&row)[NW], int lane, int (&piv)[NR], T *m
         const T pv = rdlane(rj, p);
         const bool act = !done && lane != p;
         const T m = act ? rj / pv : (T)0;
-        mult[(size_t)j * WAVE] = m;
+        if constexpr (STORE_MULT) mult[(size_t)j * WAVE] = m;
         if (lane == p) { done = true; step = j; }
 #pragma unroll
         for (int c = 0; c < NW; ++c) {
@@ -582,28 +582,61 @@ __device__ __noinline__ bool bsolve(const BInst<M> &I, BShared<M, METHOD, LT> &s
         relrow[2 * M] = row[3 * M];
     }
     __syncthreads();
-    if (lane < M) {
-        const bool end = I.bc_node_is_end[lane] != 0;
-        const int comp = I.bcc[lane];
-        for (int c = 0; c < M; ++c) {
-            sh.w.root[lane][c] = (!end && c == comp) ? 1.0 : 0.0;
-            sh.w.root[lane][M + c] = (end && c == comp) ? 1.0 : 0.0;
-        }
-        sh.w.root[lane][2 * M] = factor ? -sh.bcres[lane] : rin[(size_t)N * M + lane];
-    } else if (lane < 2 * M) {
+    if constexpr (M >= 16) {
+        // the 2M x (2M + 1) end-node system with row l in lane l's registers (FP64)
+        double rrow[2 * M + 1];
+        if (lane < M) {
+            const bool end = I.bc_node_is_end[lane] != 0;
+            const int comp = I.bcc[lane];
 #pragma unroll
-        for (int c = 0; c < 2 * M + 1; ++c) sh.w.root[lane][c] = (double)relrow[c];
-    }
-    __syncthreads();
-    const bool rpiv = eliminate_lds<2 * M, 2 * M, RW, RW, double>(&sh.w.root[0][0], lane, sh.piv, (double *)nullptr);
-    // back substitution over the pivot order
-    for (int j = 2 * M - 1; j >= 0; --j) {
-        const int p = sh.piv[j];
-        if (lane == p) sh.xr[j] = sh.w.root[p][2 * M] / sh.w.root[p][j];
+            for (int c = 0; c < M; ++c) {
+                rrow[c] = (!end && c == comp) ? 1.0 : 0.0;
+                rrow[M + c] = (end && c == comp) ? 1.0 : 0.0;
+            }
+            rrow[2 * M] = factor ? -sh.bcres[lane] : rin[(size_t)N * M + lane];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 2 * M + 1; ++c) rrow[c] = (lane < 2 * M) ? (double)relrow[c] : 0.0;
+        }
+        eliminate_reg<2 * M, 2 * M, 2 * M + 1, double, false>(rrow, lane, sh.piv, (double *)nullptr, (double *)nullptr);
         __syncthreads();
-        const double xj = sh.xr[j];
-        if (rpiv && lane != p) sh.w.root[lane][2 * M] -= sh.w.root[lane][j] * xj;
+        // back substitution over the pivot order; x_j goes to LDS and into every row's right-hand side
+#pragma unroll 1
+        for (int j = 2 * M - 1; j >= 0; --j) {
+            const int p = __builtin_amdgcn_readfirstlane(sh.piv[j]);
+            double rj = 0.0;
+#pragma unroll
+            for (int c = 0; c < 2 * M; ++c) rj = (c == j) ? rrow[c] : rj;
+            const double xj = rdlane(rrow[2 * M], p) / rdlane(rj, p);
+            if (lane == 0) sh.xr[j] = xj;
+            rrow[2 * M] -= rj * xj;
+        }
         __syncthreads();
+    } else {
+        if (lane < M) {
+            const bool end = I.bc_node_is_end[lane] != 0;
+            const int comp = I.bcc[lane];
+            for (int c = 0; c < M; ++c) {
+                sh.w.root[lane][c] = (!end && c == comp) ? 1.0 : 0.0;
+                sh.w.root[lane][M + c] = (end && c == comp) ? 1.0 : 0.0;
+            }
+            sh.w.root[lane][2 * M] = factor ? -sh.bcres[lane] : rin[(size_t)N * M + lane];
+        } else if (lane < 2 * M) {
+#pragma unroll
+            for (int c = 0; c < 2 * M + 1; ++c) sh.w.root[lane][c] = (double)relrow[c];
+        }
+        __syncthreads();
+        const bool rpiv =
+            eliminate_lds<2 * M, 2 * M, RW, RW, double>(&sh.w.root[0][0], lane, sh.piv, (double *)nullptr);
+        // back substitution over the pivot order
+        for (int j = 2 * M - 1; j >= 0; --j) {
+            const int p = sh.piv[j];
+            if (lane == p) sh.xr[j] = sh.w.root[p][2 * M] / sh.w.root[p][j];
+            __syncthreads();
+            const double xj = sh.xr[j];
+            if (rpiv && lane != p) sh.w.root[lane][2 * M] -= sh.w.root[lane][j] * xj;
+            __syncthreads();
+        }
     }
     bool ok = true;
     if (lane < 2 * M) ok = (sh.xr[lane] <= 1.7976931348623157e308 && sh.xr[lane] >= -1.7976931348623157e308);
