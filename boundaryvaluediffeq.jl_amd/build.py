@@ -8,7 +8,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "_build"
 LIB = OUT / "libbvp_mirk.so"
-SOURCES = ["bvp_api.hip", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m2v.hip", "wave_m3.hip", "wave_m4.hip",
+SOURCES = ["bvp_api.hip", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m2v.hip", "wave_m2_bratu.hip", "wave_m2_singpert.hip", "wave_m3.hip", "wave_m4.hip",
            "block_dispatch.hip", "block_m8.hip", "block_m16.hip", "block_m32.hip", "mlp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -256,17 +256,21 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     // persistent slots: the kernel variant and waves per CU from an earlier tuning
     // of this kind of plan, else variant 0 at its occupancy; scratch is sized for
     // the most waves any variant can keep resident, so that the autotuner can try them
-    const auto key = std::make_tuple(M, o->method, o->precision, blk ? 1 : 0);
+    int ufam = inst[0].family;  // the batch's single family, or -1
+    for (int i = 1; i < n; ++i)
+        if (inst[i].family != ufam) ufam = -1;
+    B->family = ufam;
+    const auto key = std::make_tuple(M, o->method, o->precision, blk ? 1 : 0, ufam);
     int variant = 0, per_cu = 0;
     if (ctx->tuned.count(key)) std::tie(variant, per_cu) = ctx->tuned[key];
     if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
     if (per_cu < 1)
-        per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant);
+        per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant, ufam);
     if (per_cu < 1) per_cu = 1;
     int slots = std::min(n, ctx->num_cus * per_cu);
     if (!blk)
         for (int v = 0; v < wave_variants(M); ++v)
-            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v))));
+            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v, ufam))));
     if (slots < 1) slots = 1;
     B->variant = variant;
     size_t scratch_doubles;
@@ -360,7 +364,7 @@ static int enqueue_solve(bvp_ctx *ctx, bvp_batch *B) {
     if (B->block)
         HIP_TRY(ctx, block_launch_solve(D, B->method, B->opts.precision, B->blocks, ctx->stream));
     else
-        HIP_TRY(ctx, wave_launch_solve(D, B->method, B->variant, B->blocks, ctx->stream));
+        HIP_TRY(ctx, wave_launch_solve(D, B->method, B->variant, B->blocks, B->family, ctx->stream));
     return 0;
 }
 
@@ -761,7 +765,8 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
             cands.push_back({0, std::min(B->max_slots, ctx->num_cus * occ)});
     } else {
         for (int v = 0; v < wave_variants(B->M); ++v)
-            cands.push_back({v, std::min(B->max_slots, ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v)))});
+            cands.push_back({v, std::min(B->max_slots,
+                                         ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v, B->family)))});
     }
     const int keep_v = B->variant, keep_b = B->blocks;
     int best = -1;
@@ -795,7 +800,7 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     B->blocks = std::max(1, cands[best].blocks);
     B->drop_graph();
     const int per_cu = (B->blocks + ctx->num_cus - 1) / ctx->num_cus;
-    ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision, B->block ? 1 : 0)] = {B->variant, per_cu};
+    ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision, B->block ? 1 : 0, B->family)] = {B->variant, per_cu};
     return 0;
 }
 

@@ -22,9 +22,9 @@ struct bvp_ctx {
     hipEvent_t evm = nullptr;      // between the learned guess and the solve of a plan run
     int blocks_per_cu = 0;         // 0: from the occupancy calculator
     std::string err;
-    // autotuner choices, (M, method, precision, block kernel) -> (variant, waves per CU);
-    // later plans of the same kind start from them
-    std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tuned;
+    // autotuner choices, (M, method, precision, block kernel, family or -1) -> (variant,
+    // waves per CU); later plans of the same kind start from them
+    std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> tuned;
 };
 
 // device allocations owned by one batch; freed together
@@ -54,6 +54,7 @@ struct bvp_mlp {
 struct bvp_batch {
     int n = 0, M = 0, method = 4, cap = 0, blocks = 0;
     int variant = 0;     // solve-kernel variant (wave_variants)
+    int family = -1;     // the batch's single family, or -1 when mixed
     int max_slots = 0;   // persistent waves the per-wave scratch was sized for
     bool block = false;  // served by the block kernels (M = 8, 16, 32)
     bool has_u0 = false; // every instance carries u0 (needed by a warm start)
@@ -91,5 +92,5 @@ struct bvp_batch {
     ~bvp_batch() { drop_graph(); }
 };
 
-int wave_blocks_per_cu(int M, int method, int variant);
+int wave_blocks_per_cu(int M, int method, int variant, int fam);
 bool blockm_supports(int M, int family);

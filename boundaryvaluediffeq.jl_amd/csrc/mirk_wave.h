@@ -75,7 +75,9 @@ hipError_t wave_reset_counter(unsigned *counter, hipStream_t s);
 // variant v of the persistent solve kernel (register allocation held to v + 1
 // waves per SIMD for M = 2; one variant otherwise)
 int wave_variants(int M);
-hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, hipStream_t s);
+// fam: the batch's single family, or -1 for a mixed batch (a kernel specialised
+// to one family exists for some (M, method, family))
+hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, int fam, hipStream_t s);
 hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s);
 hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, double *out_mesh, double *out_y,
                               int max_nodes, hipStream_t s);

@@ -11,8 +11,14 @@ namespace bvpdev {
 DECL(1, 4) DECL(1, 6) DECL(2, 4) DECL(2, 6) DECL(3, 4) DECL(3, 6) DECL(4, 4) DECL(4, 6)
 DECLV(1, 4, 1) DECLV(1, 6, 1) DECLV(3, 4, 1) DECLV(3, 6, 1) DECLV(4, 4, 1) DECLV(4, 6, 1)
 DECLV(2, 4, 1) DECLV(2, 6, 1) DECLV(2, 4, 2) DECLV(2, 6, 2) DECLV(2, 4, 3) DECLV(2, 6, 3)
+#define DECLF(w, f)                                                                              \
+    extern template hipError_t launch_solve_t<2, 4, w, f>(const BatchDev &, int, hipStream_t);   \
+    extern template int occupancy_t<2, 4, w, f>();
+DECLF(1, BVP_FAMILY_BRATU) DECLF(2, BVP_FAMILY_BRATU) DECLF(3, BVP_FAMILY_BRATU)
+DECLF(1, BVP_FAMILY_SINGPERT) DECLF(2, BVP_FAMILY_SINGPERT) DECLF(3, BVP_FAMILY_SINGPERT)
 #undef DECL
 #undef DECLV
+#undef DECLF
 }  // namespace bvpdev
 using namespace bvpdev;
 
@@ -51,7 +57,21 @@ hipError_t wave_reset_counter(unsigned *counter, hipStream_t s) {
 
 int wave_variants(int M) { return M == 2 ? 3 : 1; }
 
-hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, hipStream_t s) {
+static bool specialised(int M, int method, int fam) {
+    return M == 2 && method == 4 && (fam == BVP_FAMILY_BRATU || fam == BVP_FAMILY_SINGPERT);
+}
+
+hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, int fam, hipStream_t s) {
+    if (specialised(B.M, method, fam)) {
+        const int w = variant + 1;
+        if (fam == BVP_FAMILY_BRATU)
+            return w == 1 ? launch_solve_t<2, 4, 1, BVP_FAMILY_BRATU>(B, blocks, s)
+                 : w == 2 ? launch_solve_t<2, 4, 2, BVP_FAMILY_BRATU>(B, blocks, s)
+                          : launch_solve_t<2, 4, 3, BVP_FAMILY_BRATU>(B, blocks, s);
+        return w == 1 ? launch_solve_t<2, 4, 1, BVP_FAMILY_SINGPERT>(B, blocks, s)
+             : w == 2 ? launch_solve_t<2, 4, 2, BVP_FAMILY_SINGPERT>(B, blocks, s)
+                      : launch_solve_t<2, 4, 3, BVP_FAMILY_SINGPERT>(B, blocks, s);
+    }
     if (B.M == 2 && variant == 1) return method == 6 ? launch_solve_t<2, 6, 2>(B, blocks, s) : launch_solve_t<2, 4, 2>(B, blocks, s);
     if (B.M == 2 && variant == 2) return method == 6 ? launch_solve_t<2, 6, 3>(B, blocks, s) : launch_solve_t<2, 4, 3>(B, blocks, s);
 #define CALL_SOLVE(m, me) return launch_solve_t<m, me, 1>(B, blocks, s)
@@ -95,7 +115,17 @@ bool wave_supports(int M, int family) {
 int wave_rel_fields(int M) { return 2 * M * M + M; }
 
 
-int wave_blocks_per_cu(int M, int method, int variant) {
+int wave_blocks_per_cu(int M, int method, int variant, int fam) {
+    if (specialised(M, method, fam)) {
+        const int w = variant + 1;
+        if (fam == BVP_FAMILY_BRATU)
+            return w == 1 ? occupancy_t<2, 4, 1, BVP_FAMILY_BRATU>()
+                 : w == 2 ? occupancy_t<2, 4, 2, BVP_FAMILY_BRATU>()
+                          : occupancy_t<2, 4, 3, BVP_FAMILY_BRATU>();
+        return w == 1 ? occupancy_t<2, 4, 1, BVP_FAMILY_SINGPERT>()
+             : w == 2 ? occupancy_t<2, 4, 2, BVP_FAMILY_SINGPERT>()
+                      : occupancy_t<2, 4, 3, BVP_FAMILY_SINGPERT>();
+    }
     const bool six = (method == 6);
     if (M == 2 && variant == 1) return six ? occupancy_t<2, 6, 2>() : occupancy_t<2, 4, 2>();
     if (M == 2 && variant == 2) return six ? occupancy_t<2, 6, 3>() : occupancy_t<2, 4, 3>();

@@ -1164,11 +1164,15 @@ __device__ inline Bufs<M> bufs_of(const BatchDev &B, int b) {
 }
 
 // ------------------------------------------------------------------ full solve of one instance
-template <int M, int METHOD>
+// FAM >= 0: the batch holds one family only, known at compile time; every
+// f evaluation then folds to that family's code, which lowers the register
+// peak the other families' functions (sin/cos reduction and the like) impose.
+template <int M, int METHOD, int FAM = -1>
 __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<M> &sh, int lane, double *scr, size_t tree_base,
                                double *aux) {
     Inst<M> I;
     load_inst<M>(B, b, I);
+    if constexpr (FAM >= 0) I.fam = FAM;
     Bufs<M> bf = bufs_of<M>(B, b);
     int cur = 0;
     int N = B.N0[b];
@@ -1290,7 +1294,7 @@ __device__ inline int next_item(unsigned *counter, Shared<M> &sh, int lane) {
 // defect -> mesh loop for it.  WPE = waves per SIMD the register allocation is
 // held to (1: no spills; more: more latency hiding, some spills) -- a variant
 // the autotuner picks by measurement.
-template <int M, int METHOD, int WPE>
+template <int M, int METHOD, int WPE, int FAM = -1>
 __global__ __launch_bounds__(64, WPE) void k_solve(BatchDev B) {
     __shared__ Shared<M> sh;
     const int lane = threadIdx.x;
@@ -1299,7 +1303,7 @@ __global__ __launch_bounds__(64, WPE) void k_solve(BatchDev B) {
     for (;;) {
         const int item = next_item<M>(B.counter, sh, lane);
         if (item >= B.n) break;
-        solve_instance<M, METHOD>(B, B.order[item], sh, lane, scr, B.tree_base, aux);
+        solve_instance<M, METHOD, FAM>(B, B.order[item], sh, lane, scr, B.tree_base, aux);
     }
 }
 
@@ -1415,9 +1419,9 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev B, const long long *out
 }
 
 // ------------------------------------------------------------------ host launchers
-template <int M, int METHOD, int WPE>
+template <int M, int METHOD, int WPE, int FAM = -1>
 hipError_t launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
-    k_solve<M, METHOD, WPE><<<blocks, 64, 0, s>>>(B);
+    k_solve<M, METHOD, WPE, FAM><<<blocks, 64, 0, s>>>(B);
     return hipGetLastError();
 }
 template <int M, int METHOD>
@@ -1426,10 +1430,11 @@ hipError_t launch_component_t(const BatchDev &B, int op, int blocks, hipStream_t
     return hipGetLastError();
 }
 
-template <int M, int METHOD, int WPE>
+template <int M, int METHOD, int WPE, int FAM = -1>
 int occupancy_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<M, METHOD, WPE>, 64, 0) != hipSuccess) return 4 * WPE;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<M, METHOD, WPE, FAM>, 64, 0) != hipSuccess)
+        return 4 * WPE;
     return nb;
 }
 

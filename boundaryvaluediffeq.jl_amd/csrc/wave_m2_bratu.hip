@@ -1,0 +1,12 @@
+// wave_m2_bratu.hip -- the M = 2 MIRK4 persistent solve kernel specialised to
+// batches of the bratu family only (its three register-allocation variants).
+#include "mirk_wave_impl.h"
+
+namespace bvpdev {
+template hipError_t launch_solve_t<2, 4, 1, BVP_FAMILY_BRATU>(const BatchDev &, int, hipStream_t);
+template int occupancy_t<2, 4, 1, BVP_FAMILY_BRATU>();
+template hipError_t launch_solve_t<2, 4, 2, BVP_FAMILY_BRATU>(const BatchDev &, int, hipStream_t);
+template int occupancy_t<2, 4, 2, BVP_FAMILY_BRATU>();
+template hipError_t launch_solve_t<2, 4, 3, BVP_FAMILY_BRATU>(const BatchDev &, int, hipStream_t);
+template int occupancy_t<2, 4, 3, BVP_FAMILY_BRATU>();
+}  // namespace bvpdev
