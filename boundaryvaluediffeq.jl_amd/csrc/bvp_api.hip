@@ -256,9 +256,9 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     // persistent slots: the kernel variant and waves per CU from an earlier tuning
     // of this kind of plan, else variant 0 at its occupancy; scratch is sized for
     // the most waves any variant can keep resident, so that the autotuner can try them
-    int ufam = inst[0].family;  // the batch's single family, or -1
-    for (int i = 1; i < n; ++i)
-        if (inst[i].family != ufam) ufam = -1;
+    int ufam = inst[0].family;  // the batch's single family (two-point BCs only), or -1
+    for (int i = 0; i < n; ++i)
+        if (inst[i].family != ufam || inst[i].bc_kind != BVP_BC_TWO_POINT) ufam = -1;
     B->family = ufam;
     const auto key = std::make_tuple(M, o->method, o->precision, blk ? 1 : 0, ufam);
     int variant = 0, per_cu = 0;

@@ -54,7 +54,7 @@ struct bvp_mlp {
 struct bvp_batch {
     int n = 0, M = 0, method = 4, cap = 0, blocks = 0;
     int variant = 0;     // solve-kernel variant (wave_variants)
-    int family = -1;     // the batch's single family, or -1 when mixed
+    int family = -1;     // the batch's single family when all its BCs are two-point, else -1
     int max_slots = 0;   // persistent waves the per-wave scratch was sized for
     bool block = false;  // served by the block kernels (M = 8, 16, 32)
     bool has_u0 = false; // every instance carries u0 (needed by a warm start)

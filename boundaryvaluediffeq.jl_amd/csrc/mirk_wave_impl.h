@@ -624,7 +624,7 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
             const int node = sh.bcnode[k];
             const int comp = sel<M>(I.bcc, k);
             b[k] = -sh.bcres[k];
-            if (node == 0 || node == N) {
+            if (I.bc_kind == BVP_BC_TWO_POINT || node == 0 || node == N) {
 #pragma unroll
                 for (int c = 0; c < M; ++c) {
                     A[k][c] = (node == 0 && c == comp) ? 1.0 : 0.0;
@@ -1164,15 +1164,19 @@ __device__ inline Bufs<M> bufs_of(const BatchDev &B, int b) {
 }
 
 // ------------------------------------------------------------------ full solve of one instance
-// FAM >= 0: the batch holds one family only, known at compile time; every
-// f evaluation then folds to that family's code, which lowers the register
-// peak the other families' functions (sin/cos reduction and the like) impose.
+// FAM >= 0: the batch holds one family only, known at compile time, and only
+// two-point boundary conditions; every f evaluation then folds to that family's
+// code and the interior-node boundary walk drops out, which lowers the register
+// peak the other paths (sin/cos reduction and the like) impose.
 template <int M, int METHOD, int FAM = -1>
 __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<M> &sh, int lane, double *scr, size_t tree_base,
                                double *aux) {
     Inst<M> I;
     load_inst<M>(B, b, I);
-    if constexpr (FAM >= 0) I.fam = FAM;
+    if constexpr (FAM >= 0) {
+        I.fam = FAM;
+        I.bc_kind = BVP_BC_TWO_POINT;  // the specialised kernels serve two-point batches only
+    }
     Bufs<M> bf = bufs_of<M>(B, b);
     int cur = 0;
     int N = B.N0[b];
