@@ -775,16 +775,31 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
             }
         }
         Back<M> bnext;
-        if (len > 1) load_back<M>(scr, len - 2, lane, bnext);
+        double ycur[M], ynext[M];  // y at node k and k - 1, read ahead for the update
+        if (len > 1) {
+            load_back<M>(scr, len - 2, lane, bnext);
+            if (apply) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) ynext[c] = y[(size_t)(start + len - 1) * M + c];
+            }
+        }
         for (int k = start + len - 1; k > start; --k) {
             const Back<M> bk = bnext;  // record of node k; the next one is loaded ahead
             if (k - 1 > start) load_back<M>(scr, k - start - 2, lane, bnext);
+            if (apply) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) ycur[c] = ynext[c];
+                if (k - 1 > start) {
+#pragma unroll
+                    for (int c = 0; c < M; ++c) ynext[c] = y[(size_t)(k - 1) * M + c];
+                }
+            }
             double yk[M];
             back_apply<M>(bk, ya, yn, yk);
 #pragma unroll
             for (int c = 0; c < M; ++c) {
                 if (delta) delta[(size_t)k * M + c] = yk[c];
-                if (apply) y[(size_t)k * M + c] += yk[c];
+                if (apply) y[(size_t)k * M + c] = ycur[c] + yk[c];
                 yn[c] = yk[c];
             }
         }
@@ -981,11 +996,28 @@ __device__ inline double defect_pass(const Inst<M> &I, const double *mesh, const
     double dmax = 0.0, smax = 0.0, ssum = 0.0;
     bool bad = false;
     const double ex = 1.0 / (TB::order + 1);
-    for (int k = start; k < start + len; ++k) {
-        const double ti = mesh[k], h = mesh[k + 1] - ti;
-        double yi[M], yj[M];
+    // node k + 1 is loaded one interval ahead (its latency overlaps interval k's work)
+    double yi[M], yj[M], yn[M];
+    double ti = 0.0, tj = 0.0, tn = 0.0;
+    if (len > 0) {
+        ti = mesh[start];
+        tn = mesh[start + 1];
 #pragma unroll
-        for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
+        for (int c = 0; c < M; ++c) {
+            yi[c] = y[(size_t)start * M + c];
+            yn[c] = y[(size_t)(start + 1) * M + c];
+        }
+    }
+    for (int k = start; k < start + len; ++k) {
+        tj = tn;
+#pragma unroll
+        for (int c = 0; c < M; ++c) yj[c] = yn[c];
+        if (k + 2 <= start + len) {
+            tn = mesh[k + 2];
+#pragma unroll
+            for (int c = 0; c < M; ++c) yn[c] = y[(size_t)(k + 2) * M + c];
+        }
+        const double h = tj - ti;
         double K[TB::s][M], KI[TB::sstar - TB::s][M];
         stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
         interp_stages<M, METHOD>(I.fam, I.P, ti, h, yi, yj, K, KI);
@@ -1013,6 +1045,9 @@ __device__ inline double defect_pass(const Inst<M> &I, const double *mesh, const
         if (shat) shat[k] = sh;
         smax = fmax(smax, sh);
         ssum += sh;
+        ti = tj;
+#pragma unroll
+        for (int c = 0; c < M; ++c) yi[c] = yj[c];
     }
     const bool anybad = wave_any(bad);
     dmax = wave_max(dmax);
