@@ -445,6 +445,7 @@ __device__ inline BcLoc bc_locate(const Inst<M> &I, int k, const double *mesh, i
 // ------------------------------------------------------------------ LDS layout
 template <int M> struct Shared {
     double rel[WAVE][kRelFields<M>];  // relation exchange for the tree
+    double tree[(WAVE - 1) * kRelFields<M>];  // back-substitution records of the tree merges
     double lo[WAVE][M], hi[WAVE][M];  // corrections at chunk ends
     double bcres[M];
     int bcnode[M];
@@ -592,7 +593,8 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
     chunk_of(N, lane, P, start, len);
     int D = 0;
     while ((1 << D) < P) ++D;
-    double *tree = scr + tree_base;
+    double *tree = sh.tree;  // tree records stay on chip (read back level by level in the down-sweep)
+    (void)tree_base;
     // ---- up-sweep
     if (lane < P) rel_to_lds<M>(sh, lane, rel);
     __syncthreads();
