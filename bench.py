@@ -153,8 +153,9 @@ def train_warm_start(ctx, args, opts, _abi, synthetic, learned):
     solved = sum(1 for r in sols if r.retcode == 0)
     log(f"[bench] training data: {len(specs)} plain solves ({solved} solved) -> {len(x)} samples in {th:.1f} s")
     t0 = time.perf_counter()
-    net, loss, ms = learned.train(ctx, x, y, epochs=args.train_epochs, batch=8192, lr=2e-3, lr_min=2e-5,
-                                  warmup=200, use_graph=not args.no_graph)
+    net, loss, ms = learned.train(ctx, x, y, epochs=args.train_epochs, batch=8192, lr=args.train_lr, lr_min=2e-5,
+                                  warmup=200, use_graph=not args.no_graph, chunk_epochs=5,
+                                  progress=lambda e, l: log(f"[bench] training: epoch {e}, loss {l:.3e}"))
     log(f"[bench] training: {len(loss)} Adam steps, loss {loss[0]:.3e} -> {loss[-1]:.3e}, device {ms:.0f} ms, "
         f"wall {time.perf_counter() - t0:.1f} s")
     return net, {"train_solves": len(specs), "train_solved": solved, "samples": int(len(x)),
@@ -257,8 +258,9 @@ def bench_singpert(args, _abi, _lib, synthetic, learned):
         x, y, sols = learned.harvest(ctx, tspecs, opts, per_solve=args.train_nodes, kind=_abi.FEAT_LOG10_PARAM,
                                      chunk=4096)
         log(f"[bench] training data: {len(tspecs)} solves -> {len(x)} samples in {time.perf_counter() - t0:.1f} s")
-        net, loss, ms = learned.train(ctx, x, y, epochs=args.train_epochs * 4, batch=8192, lr=2e-3, lr_min=2e-5,
-                                      warmup=200, use_graph=not args.no_graph)
+        net, loss, ms = learned.train(ctx, x, y, epochs=args.train_epochs, batch=8192, lr=args.train_lr,
+                                      lr_min=2e-5, warmup=200, use_graph=not args.no_graph, chunk_epochs=5,
+                                      progress=lambda e, l: log(f"[bench] training: epoch {e}, loss {l:.3e}"))
         log(f"[bench] training: {len(loss)} steps, loss {loss[0]:.3e} -> {loss[-1]:.3e}, device {ms:.0f} ms")
         batch.set_warm_start(net, _abi.FEAT_LOG10_PARAM, 0)  # 33 nodes per instance: the network at each
         tw = timed_runs(batch, args.steps, args.warmup, "learned guess")
@@ -286,7 +288,10 @@ def main():
                     help="launch kernels directly instead of replaying captured graphs (for counter profiling)")
     ap.add_argument("--train-solves", type=int, default=200000)
     ap.add_argument("--train-nodes", type=int, default=64, help="training samples per solve")
-    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--train-lr", type=float, default=1e-3)
+    ap.add_argument("--save-net", default="", help="write the trained network to this .npz")
+    ap.add_argument("--load-net", default="", help="use the network in this .npz instead of training one")
     ap.add_argument("--anchors", type=int, default=4,
                     help="network evaluations per instance (Hermite-interpolated to the nodes); 0: every node")
     ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh", "learned", "singpert"],
@@ -349,7 +354,19 @@ def main():
     tinfo = None
     tb, res_b, cnt_b, good_b = ta, res_a, cnt_a, good_a
     if not args.no_learned:
-        net, tinfo = train_warm_start(ctx, args, opts, _abi, synthetic, learned)
+        if args.load_net:  # a network saved by an earlier run (profiling runs skip the training)
+            import numpy as np
+            z = np.load(args.load_net)
+            net = _lib.MLP(ctx)
+            net.set_params(z["params"])
+            net.set_normalization(z["in_shift"], z["in_scale"], z["out_shift"], z["out_scale"])
+            tinfo = {"loaded": args.load_net}
+        else:
+            net, tinfo = train_warm_start(ctx, args, opts, _abi, synthetic, learned)
+        if args.save_net:
+            import numpy as np
+            np.savez(args.save_net, params=net.params(), in_shift=net.norm[0], in_scale=net.norm[1],
+                     out_shift=net.norm[2], out_scale=net.norm[3])
         batch.set_warm_start(net, _abi.FEAT_PARAM, args.anchors)
         tb = timed_runs(batch, args.steps, args.warmup, "batched fp64, learned guess")
         res_b = batch.results()

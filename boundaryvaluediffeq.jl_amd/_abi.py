@@ -108,6 +108,8 @@ class TrainOpts(C.Structure):
         ("warmup", C.c_int32),
         ("seed", C.c_uint64),
         ("use_graph", C.c_int32),
+        ("first_epoch", C.c_int32),
+        ("total_epochs", C.c_int32),
     ]
 
 

@@ -414,11 +414,13 @@ class MLP:
         return y
 
     def train(self, x, y, epochs=10, batch=8192, lr=2e-3, lr_min=2e-5, weight_decay=0.0, warmup=100, seed=7,
-              use_graph=True):
-        """Adam on (x, y); returns (loss per step, device ms)."""
+              use_graph=True, first_epoch=0, total_epochs=0):
+        """Adam on (x, y) for epochs [first_epoch, first_epoch + epochs) of a schedule of
+        total_epochs (0: epochs); returns (loss per step of this call, device ms)."""
         x = _f32(x).reshape(-1, 4)
         y = _f32(y).reshape(-1, 2)
-        o = _abi.TrainOpts(epochs, batch, lr, lr_min, weight_decay, warmup, seed, 1 if use_graph else 0)
+        o = _abi.TrainOpts(epochs, batch, lr, lr_min, weight_decay, warmup, seed, 1 if use_graph else 0,
+                           first_epoch, total_epochs)
         nsteps = (len(x) // batch) * epochs
         loss = np.zeros(max(nsteps, 1), np.float32)
         steps = C.c_int32()

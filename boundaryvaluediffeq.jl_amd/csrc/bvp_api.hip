@@ -989,13 +989,21 @@ int bvp_mlp_train(bvp_mlp *m, int64_t n, const float *x, const float *y, const b
     if (o->batch < mlp::T || o->batch % mlp::T) { set_err(ctx, "batch must be a positive multiple of 64"); return BVP_EINVAL; }
     if (n < o->batch || n >= (1ll << 31)) { set_err(ctx, "need batch <= n < 2^31 samples"); return BVP_EINVAL; }
     if (o->epochs < 1 || !(o->lr > 0)) { set_err(ctx, "need epochs >= 1 and lr > 0"); return BVP_EINVAL; }
+    const int all_epochs = o->total_epochs > 0 ? o->total_epochs : o->epochs;
+    if (o->first_epoch < 0 || o->first_epoch + o->epochs > all_epochs) {
+        set_err(ctx, "need 0 <= first_epoch and first_epoch + epochs <= total_epochs");
+        return BVP_EINVAL;
+    }
     const int spe = (int)(n / o->batch);
-    const long long total = (long long)spe * o->epochs;
-    if (total >= (1ll << 31)) { set_err(ctx, "too many steps"); return BVP_EINVAL; }
-    // per-epoch sample order j -> (a j + c) mod n with gcd(a, n) = 1
-    std::vector<unsigned long long> perm(2 * (size_t)o->epochs);
-    uint64_t s = o->seed ^ 0x6A09E667F3BCC909ull;
-    for (int e = 0; e < o->epochs; ++e) {
+    const long long total = (long long)spe * o->epochs;          // steps of this call
+    const long long sched = (long long)spe * all_epochs;         // steps of the whole schedule
+    const long long step0 = (long long)spe * o->first_epoch;
+    if (sched >= (1ll << 31)) { set_err(ctx, "too many steps"); return BVP_EINVAL; }
+    // per-epoch sample order j -> (a j + c) mod n with gcd(a, n) = 1, drawn from a
+    // stream of its own per epoch (the same for any split of the schedule)
+    std::vector<unsigned long long> perm(2 * (size_t)all_epochs, 1ull);
+    for (int e = o->first_epoch; e < o->first_epoch + o->epochs; ++e) {
+        uint64_t s = o->seed ^ 0x6A09E667F3BCC909ull ^ ((uint64_t)(e + 1) * 0xD1B54A32D192ED03ull);
         long long a = 1;
         if (n > 2)
             do { a = 1 + (long long)(unit01(s) * (double)(n - 1)); } while (a >= n || gcd_ll(a, n) != 1);
@@ -1014,14 +1022,19 @@ int bvp_mlp_train(bvp_mlp *m, int64_t n, const float *x, const float *y, const b
         return rc;
     HIP_TRY(ctx, hipMemcpyAsync(dx, x, sizeof(float) * n * mlp::NIN, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(dy, y, sizeof(float) * n * mlp::NOUT, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipMemsetAsync(dstep, 0, sizeof(int), ctx->stream));
+    {
+        const int st0 = (int)step0;
+        HIP_TRY(ctx, hipMemcpyAsync(dstep, &st0, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
     mlp::TrainDev td{};
     td.x = dx;
     td.y = dy;
     td.n = n;
     td.batch_tiles = tiles;
     td.steps_per_epoch = spe;
-    td.total_steps = (int)total;
+    td.total_steps = (int)sched;
+    td.step0 = (int)step0;
     td.perm = dperm;
     td.step = dstep;
     td.partial = dpart;

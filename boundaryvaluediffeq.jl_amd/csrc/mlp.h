@@ -61,7 +61,8 @@ struct TrainDev {
     const unsigned long long *perm; // per epoch: (a, c) of the index permutation j -> (a j + c) mod n
     int *step;                      // device step counter
     float *partial;                 // [batch_tiles][NSLOT]
-    float *loss;                    // [total_steps]
+    float *loss;                    // [steps of this call], indexed by step - step0
+    int step0;                      // first step of this call
     float lr, lr_min, beta1, beta2, eps, weight_decay;
     int warmup;
 };

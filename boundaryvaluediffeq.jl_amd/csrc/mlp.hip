@@ -343,7 +343,7 @@ __global__ void k_adam(NetBufs nb, TrainDev td) {
     float g = 0.f;
     for (int t = 0; t < td.batch_tiles; ++t) g += td.partial[(size_t)t * NSLOT + j];
     if (j == NPARAM) {
-        td.loss[step] = g;
+        td.loss[step - td.step0] = g;
         return;
     }
     const int pj = slot_param(j);

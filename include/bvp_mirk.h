@@ -289,6 +289,9 @@ typedef struct {
     int32_t  warmup;        /* linear learning-rate warm-up steps */
     uint64_t seed;          /* sample order */
     int32_t  use_graph;     /* replay one captured training step */
+    int32_t  first_epoch;   /* this call runs epochs [first_epoch, first_epoch + epochs) of a */
+    int32_t  total_epochs;  /* schedule of total_epochs (<= 0: epochs); calls that split a
+                               schedule this way give the same result as one call */
 } bvp_train_opts;
 
 /* Glorot-uniform weights from `seed`, zero biases, identity normalisation. */
@@ -305,7 +308,7 @@ int  bvp_mlp_gradient(bvp_mlp *mlp, int32_t n, const float *x, const float *y, f
 /* y[n][2] = network(x[n][4]) in raw units. */
 int  bvp_mlp_predict(bvp_mlp *mlp, int32_t n, const float *x, float *y);
 /* Adam training on n samples (n < 2^31); loss_per_step (optional) receives the
- * loss of every step, *steps the number of steps, *ms the device time. */
+ * loss of every step of this call, *steps their number, *ms the device time. */
 int  bvp_mlp_train(bvp_mlp *mlp, int64_t n, const float *x, const float *y, const bvp_train_opts *opts,
                    float *loss_per_step, int32_t *steps, float *ms);
 /* Attach (or with mlp == NULL detach) a network to a plan: each run first writes

@@ -103,6 +103,22 @@ def test_training_lowers_loss_and_graph_replay_matches(ctx):
     assert np.array_equal(n1.params(), n2.params())
 
 
+def test_chunked_training_equals_one_call(ctx):
+    """A schedule split over several calls (first_epoch / total_epochs) gives the
+    same bits as one call."""
+    x, y = _toy(8192, seed=4)
+    norm = learned.fit_normalization(x, y)
+    a = _lib.MLP(ctx, seed=2)
+    a.set_normalization(*norm)
+    la, _ = a.train(x, y, epochs=6, batch=1024, lr=2e-3, warmup=5, seed=5)
+    b = _lib.MLP(ctx, seed=2)
+    b.set_normalization(*norm)
+    parts = [b.train(x, y, epochs=k, batch=1024, lr=2e-3, warmup=5, seed=5, first_epoch=e0, total_epochs=6)[0]
+             for e0, k in ((0, 2), (2, 3), (5, 1))]
+    assert np.array_equal(la, np.concatenate(parts))
+    assert np.array_equal(a.params(), b.params())
+
+
 def _plain_opts(abstol=1e-8):
     o = _abi.default_opts(_abi.MIRK4)
     o.abstol = abstol
