@@ -841,6 +841,60 @@ __device__ inline double residual_and_blocks(const Inst<M> &I, Shared<M> &sh, co
     return anybad ? NAN : mx;
 }
 
+// Residual only (primal stages, no Jacobian blocks, no elimination): max|F| (NaN
+// if not finite) in all lanes.  Newton's convergence check after a step uses it,
+// so an iterate that has converged does not pay for the dual-number pass.
+template <int M, int METHOD>
+__device__ inline double residual_primal(const Inst<M> &I, const double *mesh, const double *y, int N, int lane) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double mx = 0.0;
+    bool bad = false;
+    if (len > 0) {
+        double yi[M], yj[M], yn[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            yi[c] = y[(size_t)start * M + c];
+            yn[c] = y[(size_t)(start + 1) * M + c];
+        }
+        double ti = mesh[start], tn = mesh[start + 1];
+        for (int k = start; k < start + len; ++k) {
+            // node k + 2 is loaded while interval k is evaluated
+            const double tj = tn;
+#pragma unroll
+            for (int c = 0; c < M; ++c) yj[c] = yn[c];
+            if (k + 2 <= start + len) {
+                tn = mesh[k + 2];
+#pragma unroll
+                for (int c = 0; c < M; ++c) yn[c] = y[(size_t)(k + 2) * M + c];
+            }
+            const double h = tj - ti;
+            double K[Tab<METHOD>::s][M], ph[M];
+            stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
+            phi_of<M, METHOD, double>(h, yi, yj, K, ph);
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                const double a = fabs(ph[c]);
+                bad |= !(a <= 1.7976931348623157e308);
+                mx = fmax(mx, a);
+            }
+#pragma unroll
+            for (int c = 0; c < M; ++c) yi[c] = yj[c];
+            ti = tj;
+        }
+    }
+    if (lane < M) {
+        const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+        const double a = fabs(bc_residual<M, METHOD>(I, lane, L, mesh, y));
+        bad |= !(a <= 1.7976931348623157e308);
+        mx = fmax(mx, a);
+    }
+    const bool anybad = wave_any(bad);
+    mx = wave_max(mx);
+    __syncthreads();
+    return anybad ? NAN : mx;
+}
+
 // Primal residual at y + alpha*delta: returns sum of squares (NaN if not finite).
 template <int M, int METHOD>
 __device__ inline double residual_sq_trial(const Inst<M> &I, const double *mesh, const double *y,
@@ -909,7 +963,11 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
         int ret = BVP_RET_MAXITERS;
         for (int it = 0;; ++it) {
             Rel<M> rel;
-            double fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+            // After a step, max|F| comes from the primal pass; the dual pass that
+            // builds the blocks runs only when another step is taken.
+            bool blocks = (it == 0);
+            double fn = blocks ? residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel)
+                               : residual_primal<M, METHOD>(I, mesh, y, N, lane);
             if (warm) {
                 bool fall;
                 if (it == 0) {
@@ -930,11 +988,13 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                         }
                     __syncthreads();
                     fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+                    blocks = true;
                 }
             }
             if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
             if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
             if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
+            if (!blocks) residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
             const bool plain = (attempt == 0);
             if (!solve_correction<M, METHOD>(I, sh, mesh, y, N, lane, scr, tree_base, rel, plain ? nullptr : delta,
                                              plain)) {
