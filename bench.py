@@ -284,6 +284,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=128, help="instances of the oracle run (lintanh workload)")
     ap.add_argument("--no-learned", action="store_true", help="skip the warm-start network (constant guess only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (variant 0 at full residency)")
+    ap.add_argument("--no-mixed", action="store_true", help="headline run with FP64 factor storage")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying captured graphs (for counter profiling)")
     ap.add_argument("--train-solves", type=int, default=200000)
@@ -372,9 +373,35 @@ def main():
         res_b = batch.results()
         cnt_b = batch.counters()
         good_b = n_solved(res_b, abstol)
+    fp64_learned = None if args.no_learned else {"value": good_b / tb["wall"], "ms_per_step": tb["wall"] * 1e3,
+                                                 "guess_ms": tb["guess_ms"], "solve_ms": tb["solve_ms"],
+                                                 "solved": good_b}
+    # batched + learned + mixed (the headline form): the same plan with the chunk
+    # factors stored in FP32; residuals and the convergence test stay FP64
+    p_j = 8
+    tune_mx = None
+    if not args.no_mixed:
+        omx = _abi.default_opts(_abi.MIRK4)
+        omx.abstol = abstol
+        omx.precision = _abi.PREC_FP32_REFINE64
+        batch.close()
+        batch = _lib.Batch(ctx, omx, specs)
+        batch.set_graph(not args.no_graph)
+        if not args.no_tune:
+            cand_ms, (variant, blocks) = batch.autotune(repeats=3)
+            tune_mx = {"candidates_ms": cand_ms, "variant": variant, "persistent_waves": blocks}
+            log(f"[bench] autotune (mixed): candidates {['%.3f' % x for x in cand_ms]} ms -> variant {variant}")
+        if not args.no_learned:
+            batch.set_warm_start(net, _abi.FEAT_PARAM, args.anchors)
+        tag = "batched mixed, " + ("constant" if args.no_learned else "learned") + " guess"
+        tb = timed_runs(batch, args.steps, args.warmup, tag)
+        res_b = batch.results()
+        cnt_b = batch.counters()
+        good_b = n_solved(res_b, abstol)
+        p_j = 4
     value = good_b / tb["wall"]
     nf_nodes = int(sum(r.N + 1 for r in res_b))
-    abytes = algorithmic_bytes(2, cnt_b, n0_nodes, nf_nodes)
+    abytes = algorithmic_bytes(2, cnt_b, n0_nodes, nf_nodes, p_j=p_j)
     achieved = abytes / (tb["solve_ms"] * 1e-3) / 1e9
     sa, sb = learned.summary(res_a), learned.summary(res_b)
     log(f"[bench] {good_b}/{n} solved; {value:.1f} solves/s; solve kernel {tb['solve_ms']:.3f} ms; algorithmic "
@@ -406,15 +433,18 @@ def main():
                                "N~U{64..1024}, MIRK4, abstol 1e-8, adaptive; learned warm start "
                                f"(BF16 MLP at {args.anchors} anchors per instance + Hermite, trained untimed on plain "
                                "solves of the Bratu (a, b) family) with fallback to the zero guess",
-                   "instances": n, "method": "MIRK4", "precision": "fp64 factors (M = 2)",
+                   "instances": n, "method": "MIRK4",
+                   "precision": "fp64 factors" if args.no_mixed else
+                   "mixed: chunk factors stored in FP32, FP64 elimination, residual and Newton test",
                    "variant": "batched persistent, hipGraph replay" + ("" if args.no_learned else
-                                                                       " + learned warm start"),
+                                                                       " + learned warm start") +
+                   ("" if args.no_mixed else " + mixed precision"),
                    "solved": good_b,
                    "batched_constant_guess": {"value": good_a / ta["wall"], "ms_per_step": ta["wall"] * 1e3,
                                               "solved": good_a, **sa},
-                   "batched_learned_guess": None if args.no_learned else
-                   {"value": value, "guess_ms": tb["guess_ms"], "solve_ms": tb["solve_ms"], **sb},
-                   "training": tinfo, "autotune": tune},
+                   "batched_learned_guess_fp64": fp64_learned,
+                   "headline_run": {"value": value, "guess_ms": tb["guess_ms"], "solve_ms": tb["solve_ms"], **sb},
+                   "training": tinfo, "autotune": tune, "autotune_mixed": tune_mx},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("k_solve"),
                      "kernel": "k_solve<2,4>", "kernel_ms": tb["solve_ms"], "algorithmic_bytes": abytes},

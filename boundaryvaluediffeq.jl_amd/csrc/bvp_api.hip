@@ -146,9 +146,9 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
                     return BVP_ENOTSUP;
                 }
         }
-    } else if (o->precision != BVP_PREC_FP64) {
-        set_err(ctx, "FP32 factorisation is served by the block kernels (M = 8, 16, 32)");
-        return BVP_ENOTSUP;
+    } else if (o->precision != BVP_PREC_FP64 && o->precision != BVP_PREC_FP32_REFINE64) {
+        set_err(ctx, "precision must be BVP_PREC_FP64 or BVP_PREC_FP32_REFINE64");
+        return BVP_EINVAL;
     }
     if (o->method != BVP_MIRK4 && o->method != BVP_MIRK6) { set_err(ctx, "method must be 4 or 6"); return BVP_EINVAL; }
     bool has_u0 = true;
@@ -256,6 +256,9 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     // persistent slots: the kernel variant and waves per CU from an earlier tuning
     // of this kind of plan, else variant 0 at its occupancy; scratch is sized for
     // the most waves any variant can keep resident, so that the autotuner can try them
+    // wave kernels: FP32 storage of the chunk elimination factors (the records are
+    // smaller than the FP64 slots they are laid out in)
+    const int f32 = (!blk && o->precision == BVP_PREC_FP32_REFINE64) ? 1 : 0;
     int ufam = inst[0].family;  // the batch's single family (two-point BCs only), or -1
     for (int i = 0; i < n; ++i)
         if (inst[i].family != ufam || inst[i].bc_kind != BVP_BC_TWO_POINT) ufam = -1;
@@ -265,12 +268,12 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     if (ctx->tuned.count(key)) std::tie(variant, per_cu) = ctx->tuned[key];
     if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
     if (per_cu < 1)
-        per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant, ufam);
+        per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant, ufam, f32);
     if (per_cu < 1) per_cu = 1;
     int slots = std::min(n, ctx->num_cus * per_cu);
     if (!blk)
         for (int v = 0; v < wave_variants(M); ++v)
-            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v, ufam))));
+            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v, ufam, f32))));
     if (slots < 1) slots = 1;
     B->variant = variant;
     size_t scratch_doubles;
@@ -305,6 +308,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->max_slots = slots;
     B->blocks = std::min(slots, std::max(1, std::min(n, ctx->num_cus * per_cu)));
     D.refine_iters = o->refine_iters;
+    D.f32rec = f32;
     if ((rc = dalloc(ctx, A, D.scratch, scratch_doubles)) ||
         (rc = dalloc(ctx, A, D.aux, D.aux_per_slot * slots))) {
         delete B;
@@ -766,7 +770,7 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     } else {
         for (int v = 0; v < wave_variants(B->M); ++v)
             cands.push_back({v, std::min(B->max_slots,
-                                         ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v, B->family)))});
+                                         ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v, B->family, B->dev.f32rec)))});
     }
     const int keep_v = B->variant, keep_b = B->blocks;
     int best = -1;

@@ -92,5 +92,5 @@ struct bvp_batch {
     ~bvp_batch() { drop_graph(); }
 };
 
-int wave_blocks_per_cu(int M, int method, int variant, int fam);
+int wave_blocks_per_cu(int M, int method, int variant, int fam, int f32);
 bool blockm_supports(int M, int family);

@@ -65,6 +65,7 @@ struct BatchDev {
 
     int rec_cap;              // block kernels: intervals whose factors fit a wave's storage
     int refine_iters;         // FP64 refinement steps per linear solve with FP32 factors
+    int f32rec;               // wave kernels: elimination factors stored in FP32 (mixed precision)
 
     // options
     int adaptive, max_sub, max_newton, linesearch, transfer_mode;

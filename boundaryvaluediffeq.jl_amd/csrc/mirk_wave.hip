@@ -5,20 +5,27 @@
 namespace bvpdev {
 #define DECL(m, me)                                                                              \
     extern template hipError_t launch_component_t<m, me>(const BatchDev &, int, int, hipStream_t);
-#define DECLV(m, me, w)                                                                          \
-    extern template hipError_t launch_solve_t<m, me, w>(const BatchDev &, int, hipStream_t);     \
-    extern template int occupancy_t<m, me, w>();
+#define DECLV(m, me, w, f32)                                                                     \
+    extern template hipError_t launch_solve_t<m, me, w, -1, f32>(const BatchDev &, int, hipStream_t); \
+    extern template int occupancy_t<m, me, w, -1, f32>();
+#define DECLP(f32)                                                                               \
+    DECLV(1, 4, 1, f32) DECLV(1, 6, 1, f32) DECLV(3, 4, 1, f32) DECLV(3, 6, 1, f32)             \
+    DECLV(4, 4, 1, f32) DECLV(4, 6, 1, f32) DECLV(2, 4, 1, f32) DECLV(2, 6, 1, f32)             \
+    DECLV(2, 4, 2, f32) DECLV(2, 6, 2, f32) DECLV(2, 4, 3, f32) DECLV(2, 6, 3, f32)
 DECL(1, 4) DECL(1, 6) DECL(2, 4) DECL(2, 6) DECL(3, 4) DECL(3, 6) DECL(4, 4) DECL(4, 6)
-DECLV(1, 4, 1) DECLV(1, 6, 1) DECLV(3, 4, 1) DECLV(3, 6, 1) DECLV(4, 4, 1) DECLV(4, 6, 1)
-DECLV(2, 4, 1) DECLV(2, 6, 1) DECLV(2, 4, 2) DECLV(2, 6, 2) DECLV(2, 4, 3) DECLV(2, 6, 3)
-#define DECLF(w, f)                                                                              \
-    extern template hipError_t launch_solve_t<2, 4, w, f>(const BatchDev &, int, hipStream_t);   \
-    extern template int occupancy_t<2, 4, w, f>();
-DECLF(1, BVP_FAMILY_BRATU) DECLF(2, BVP_FAMILY_BRATU) DECLF(3, BVP_FAMILY_BRATU)
-DECLF(1, BVP_FAMILY_SINGPERT) DECLF(2, BVP_FAMILY_SINGPERT) DECLF(3, BVP_FAMILY_SINGPERT)
+DECLP(false) DECLP(true)
+#define DECLF(w, f, f32)                                                                         \
+    extern template hipError_t launch_solve_t<2, 4, w, f, f32>(const BatchDev &, int, hipStream_t); \
+    extern template int occupancy_t<2, 4, w, f, f32>();
+#define DECLFP(f32)                                                                              \
+    DECLF(1, BVP_FAMILY_BRATU, f32) DECLF(2, BVP_FAMILY_BRATU, f32) DECLF(3, BVP_FAMILY_BRATU, f32) \
+    DECLF(1, BVP_FAMILY_SINGPERT, f32) DECLF(2, BVP_FAMILY_SINGPERT, f32) DECLF(3, BVP_FAMILY_SINGPERT, f32)
+DECLFP(false) DECLFP(true)
 #undef DECL
 #undef DECLV
+#undef DECLP
 #undef DECLF
+#undef DECLFP
 }  // namespace bvpdev
 using namespace bvpdev;
 
@@ -61,22 +68,32 @@ static bool specialised(int M, int method, int fam) {
     return M == 2 && method == 4 && (fam == BVP_FAMILY_BRATU || fam == BVP_FAMILY_SINGPERT);
 }
 
-hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, int fam, hipStream_t s) {
+// F32: the chunk factors are stored in FP32 (BatchDev::f32rec); each storage
+// precision is its own kernel, with its own register allocation.
+template <bool F32>
+static hipError_t launch_solve_p(const BatchDev &B, int method, int variant, int blocks, int fam, hipStream_t s) {
     if (specialised(B.M, method, fam)) {
         const int w = variant + 1;
         if (fam == BVP_FAMILY_BRATU)
-            return w == 1 ? launch_solve_t<2, 4, 1, BVP_FAMILY_BRATU>(B, blocks, s)
-                 : w == 2 ? launch_solve_t<2, 4, 2, BVP_FAMILY_BRATU>(B, blocks, s)
-                          : launch_solve_t<2, 4, 3, BVP_FAMILY_BRATU>(B, blocks, s);
-        return w == 1 ? launch_solve_t<2, 4, 1, BVP_FAMILY_SINGPERT>(B, blocks, s)
-             : w == 2 ? launch_solve_t<2, 4, 2, BVP_FAMILY_SINGPERT>(B, blocks, s)
-                      : launch_solve_t<2, 4, 3, BVP_FAMILY_SINGPERT>(B, blocks, s);
+            return w == 1 ? launch_solve_t<2, 4, 1, BVP_FAMILY_BRATU, F32>(B, blocks, s)
+                 : w == 2 ? launch_solve_t<2, 4, 2, BVP_FAMILY_BRATU, F32>(B, blocks, s)
+                          : launch_solve_t<2, 4, 3, BVP_FAMILY_BRATU, F32>(B, blocks, s);
+        return w == 1 ? launch_solve_t<2, 4, 1, BVP_FAMILY_SINGPERT, F32>(B, blocks, s)
+             : w == 2 ? launch_solve_t<2, 4, 2, BVP_FAMILY_SINGPERT, F32>(B, blocks, s)
+                      : launch_solve_t<2, 4, 3, BVP_FAMILY_SINGPERT, F32>(B, blocks, s);
     }
-    if (B.M == 2 && variant == 1) return method == 6 ? launch_solve_t<2, 6, 2>(B, blocks, s) : launch_solve_t<2, 4, 2>(B, blocks, s);
-    if (B.M == 2 && variant == 2) return method == 6 ? launch_solve_t<2, 6, 3>(B, blocks, s) : launch_solve_t<2, 4, 3>(B, blocks, s);
-#define CALL_SOLVE(m, me) return launch_solve_t<m, me, 1>(B, blocks, s)
+    if (B.M == 2 && variant == 1)
+        return method == 6 ? launch_solve_t<2, 6, 2, -1, F32>(B, blocks, s) : launch_solve_t<2, 4, 2, -1, F32>(B, blocks, s);
+    if (B.M == 2 && variant == 2)
+        return method == 6 ? launch_solve_t<2, 6, 3, -1, F32>(B, blocks, s) : launch_solve_t<2, 4, 3, -1, F32>(B, blocks, s);
+#define CALL_SOLVE(m, me) return launch_solve_t<m, me, 1, -1, F32>(B, blocks, s)
     DISPATCH_M_METHOD(B.M, method, CALL_SOLVE);
 #undef CALL_SOLVE
+}
+
+hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, int fam, hipStream_t s) {
+    return B.f32rec ? launch_solve_p<true>(B, method, variant, blocks, fam, s)
+                    : launch_solve_p<false>(B, method, variant, blocks, fam, s);
 }
 
 hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s) {
@@ -115,27 +132,32 @@ bool wave_supports(int M, int family) {
 int wave_rel_fields(int M) { return 2 * M * M + M; }
 
 
-int wave_blocks_per_cu(int M, int method, int variant, int fam) {
+template <bool F32>
+static int blocks_per_cu_p(int M, int method, int variant, int fam) {
     if (specialised(M, method, fam)) {
         const int w = variant + 1;
         if (fam == BVP_FAMILY_BRATU)
-            return w == 1 ? occupancy_t<2, 4, 1, BVP_FAMILY_BRATU>()
-                 : w == 2 ? occupancy_t<2, 4, 2, BVP_FAMILY_BRATU>()
-                          : occupancy_t<2, 4, 3, BVP_FAMILY_BRATU>();
-        return w == 1 ? occupancy_t<2, 4, 1, BVP_FAMILY_SINGPERT>()
-             : w == 2 ? occupancy_t<2, 4, 2, BVP_FAMILY_SINGPERT>()
-                      : occupancy_t<2, 4, 3, BVP_FAMILY_SINGPERT>();
+            return w == 1 ? occupancy_t<2, 4, 1, BVP_FAMILY_BRATU, F32>()
+                 : w == 2 ? occupancy_t<2, 4, 2, BVP_FAMILY_BRATU, F32>()
+                          : occupancy_t<2, 4, 3, BVP_FAMILY_BRATU, F32>();
+        return w == 1 ? occupancy_t<2, 4, 1, BVP_FAMILY_SINGPERT, F32>()
+             : w == 2 ? occupancy_t<2, 4, 2, BVP_FAMILY_SINGPERT, F32>()
+                      : occupancy_t<2, 4, 3, BVP_FAMILY_SINGPERT, F32>();
     }
     const bool six = (method == 6);
-    if (M == 2 && variant == 1) return six ? occupancy_t<2, 6, 2>() : occupancy_t<2, 4, 2>();
-    if (M == 2 && variant == 2) return six ? occupancy_t<2, 6, 3>() : occupancy_t<2, 4, 3>();
+    if (M == 2 && variant == 1) return six ? occupancy_t<2, 6, 2, -1, F32>() : occupancy_t<2, 4, 2, -1, F32>();
+    if (M == 2 && variant == 2) return six ? occupancy_t<2, 6, 3, -1, F32>() : occupancy_t<2, 4, 3, -1, F32>();
     switch (M) {
-    case 1: return six ? occupancy_t<1, 6, 1>() : occupancy_t<1, 4, 1>();
-    case 2: return six ? occupancy_t<2, 6, 1>() : occupancy_t<2, 4, 1>();
-    case 3: return six ? occupancy_t<3, 6, 1>() : occupancy_t<3, 4, 1>();
-    case 4: return six ? occupancy_t<4, 6, 1>() : occupancy_t<4, 4, 1>();
+    case 1: return six ? occupancy_t<1, 6, 1, -1, F32>() : occupancy_t<1, 4, 1, -1, F32>();
+    case 2: return six ? occupancy_t<2, 6, 1, -1, F32>() : occupancy_t<2, 4, 1, -1, F32>();
+    case 3: return six ? occupancy_t<3, 6, 1, -1, F32>() : occupancy_t<3, 4, 1, -1, F32>();
+    case 4: return six ? occupancy_t<4, 6, 1, -1, F32>() : occupancy_t<4, 4, 1, -1, F32>();
     }
     return 4;
+}
+
+int wave_blocks_per_cu(int M, int method, int variant, int fam, int f32) {
+    return f32 ? blocks_per_cu_p<true>(M, method, variant, fam) : blocks_per_cu_p<false>(M, method, variant, fam);
 }
 
 bool blockm_supports(int M, int family) { return block_supports(M, family); }

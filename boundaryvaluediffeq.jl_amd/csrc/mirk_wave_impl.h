@@ -55,6 +55,16 @@ __device__ inline double wave_sum(double v) {
 }
 __device__ inline bool wave_any(bool p) { return __any(p ? 1 : 0) != 0; }
 
+// p as a pointer into global memory.  Pointers that reach the compiler as
+// generic (flat) pointers -- read back from a runtime-indexed array, or derived
+// in ways it cannot follow -- compile to flat_* accesses, and the use of any
+// flat load's result waits for every outstanding memory operation
+// (vmcnt(0) lgkmcnt(0)), which serialises loads that were issued ahead.
+template <typename T>
+__device__ __forceinline__ T *as_global(T *p) {
+    return (T *)(__attribute__((address_space(1))) T *)p;
+}
+
 // a[k] for a runtime k without dynamic register indexing (which would move the
 // array to scratch memory)
 template <int N, typename T>
@@ -72,6 +82,57 @@ __device__ inline void chunk_of(int N, int lane, int &P, int &start, int &len) {
     len = q + (lane < r ? 1 : 0);
     start = lane * q + (lane < r ? lane : r);
 }
+
+// Walks intervals start .. start+len-1 of a lane's chunk in order and calls
+// body(k, t_k, t_{k+1}, y_k, y_{k+1}) for each.  The nodes are read PF intervals
+// ahead into a ring of registers: the body is unrolled PF times so that every
+// ring slot has a fixed register, and the load of node k+1+PF is issued before
+// interval k is evaluated, so that PF intervals of arithmetic cover its latency.
+template <int M, int PF, typename Body>
+__device__ __forceinline__ void walk_chunk(const double *mesh, const double *y, int start, int len, Body &&body) {
+    if (len <= 0) return;
+    double ti = mesh[start], yi[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) yi[c] = y[(size_t)start * M + c];
+    double rt[PF], ry[PF][M];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        if (j < len) {
+            rt[j] = mesh[start + 1 + j];
+#pragma unroll
+            for (int c = 0; c < M; ++c) ry[j][c] = y[(size_t)(start + 1 + j) * M + c];
+        }
+    }
+    for (int i0 = 0; i0 < len; i0 += PF) {
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            const int i = i0 + j;
+            if (i < len) {
+                const double tj = rt[j];
+                double yj[M];
+#pragma unroll
+                for (int c = 0; c < M; ++c) yj[c] = ry[j][c];
+                if (i + PF < len) {
+                    const int nd = start + 1 + i + PF;
+                    rt[j] = mesh[nd];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) ry[j][c] = y[(size_t)nd * M + c];
+                }
+                body(start + i, ti, tj, yi, yj);
+                ti = tj;
+#pragma unroll
+                for (int c = 0; c < M; ++c) yi[c] = yj[c];
+            }
+        }
+    }
+}
+
+// Nodes in flight per chunk walk: deeper for the light passes, shallow where the
+// dual-number pass sets the kernel's register peak.
+template <int M> constexpr int kPfPrimal = M <= 2 ? 3 : 2;
+template <int M> constexpr int kPfDefect = M <= 2 ? 4 : 2;
+template <int M> constexpr int kPfDual = M <= 2 ? 2 : 1;
+template <int M> constexpr int kPfBack = M <= 2 ? 2 : 1;
 
 // interval(mesh, t) = clamp(searchsortedfirst(mesh, t) - 1, 1, N), 0-based (Core/utils.jl:119-121)
 __device__ inline int interval_of(const double *mesh, int N, double t) {
@@ -342,6 +403,57 @@ __device__ inline void load_back(const double *base, int slot, int lane, Back<M>
     for (int i = 0; i < M; ++i) bk.h[i] = p[(f++) * STRIDE];
 }
 
+// Chunk records with the factors X and Z stored in FP32 (h stays FP64): a slot
+// holds M doubles and 2 M^2 floats per lane, [h fields][X, Z float fields] x 64
+// lanes, every field a contiguous run of 64 values.  The elimination itself runs
+// in FP64; Newton's next residual, evaluated in FP64, corrects the rounding of
+// the factors (the correction is an inexact Newton step).
+template <int M> constexpr int kRecDoublesF32 = M + M * M;
+template <int M>
+__device__ inline void store_back_f32(double *base, int slot, int lane, const Back<M> &bk) {
+    double *p = base + (size_t)slot * kRecDoublesF32<M> * WAVE;
+#pragma unroll
+    for (int i = 0; i < M; ++i) p[i * WAVE + lane] = bk.h[i];
+    float *q = reinterpret_cast<float *>(p + M * WAVE) + lane;
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) q[(f++) * WAVE] = (float)bk.X[i][c];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) q[(f++) * WAVE] = (float)bk.Z[i][c];
+}
+template <int M>
+__device__ inline void load_back_f32(const double *base, int slot, int lane, Back<M> &bk) {
+    const double *p = base + (size_t)slot * kRecDoublesF32<M> * WAVE;
+#pragma unroll
+    for (int i = 0; i < M; ++i) bk.h[i] = p[i * WAVE + lane];
+    const float *q = reinterpret_cast<const float *>(p + M * WAVE) + lane;
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) bk.X[i][c] = (double)q[(f++) * WAVE];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int c = 0; c < M; ++c) bk.Z[i][c] = (double)q[(f++) * WAVE];
+}
+// chunk record of slot `slot` in the storage precision F32 (a template parameter of
+// the solve path, so that each precision has its own register allocation)
+template <int M, bool F32>
+__device__ inline void put_rec(double *base, int slot, int lane, const Back<M> &bk) {
+    if constexpr (F32) store_back_f32<M>(base, slot, lane, bk);
+    else store_back<M>(base, slot, lane, bk);
+}
+template <int M, bool F32>
+__device__ inline void get_rec(const double *base, int slot, int lane, Back<M> &bk) {
+    if constexpr (F32) load_back_f32<M>(base, slot, lane, bk);
+    else load_back<M>(base, slot, lane, bk);
+}
+
 // tree records live after the chunk records: index of the merge of lane l at level d
 __device__ inline int tree_index(int d, int l) { return (WAVE - (WAVE >> d)) + (l >> (d + 1)); }
 
@@ -509,7 +621,7 @@ __device__ inline double bc_residual(const Inst<M> &I, int k, const BcLoc &L, co
 // ------------------------------------------------------------------ Newton step
 // One forward pass at y: residual, blocks, chunk elimination.  Returns the
 // lane's max |Phi| (NaN-aware via `bad`), its chunk relation.
-template <int M, int METHOD>
+template <int M, int METHOD, bool F32 = false>
 __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
                                     double *scr, Rel<M> &rel, double &mx, bool &bad, double *phi_out,
                                     double *S_out, double *R_out) {
@@ -517,25 +629,7 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
     chunk_of(N, lane, P, start, len);
     mx = 0.0;
     bad = false;
-    if (len == 0) return;
-    double yi[M], yj[M], yn[M];
-#pragma unroll
-    for (int c = 0; c < M; ++c) {
-        yi[c] = y[(size_t)start * M + c];
-        yn[c] = y[(size_t)(start + 1) * M + c];
-    }
-    double ti = mesh[start], tn = mesh[start + 1];
-    for (int k = start; k < start + len; ++k) {
-        // the next node's values are loaded one interval ahead, so that their
-        // memory latency overlaps this interval's arithmetic
-        const double tj = tn;
-#pragma unroll
-        for (int c = 0; c < M; ++c) yj[c] = yn[c];
-        if (k + 2 <= start + len) {
-            tn = mesh[k + 2];
-#pragma unroll
-            for (int c = 0; c < M; ++c) yn[c] = y[(size_t)(k + 2) * M + c];
-        }
+    walk_chunk<M, kPfDual<M>>(mesh, y, start, len, [&](int k, double ti, double tj, const double *yi, const double *yj) {
         const double h = tj - ti;
         double phi[M], S[M][M], R[M][M], Kp[Tab<METHOD>::s][M];
         interval_dual<M, METHOD>(I.fam, I.P, ti, h, yi, yj, phi, S, R, Kp);
@@ -572,20 +666,17 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
                 Rel<M> nr;
                 Back<M> bk;
                 eliminate<M>(rel, S, R, r, nr, bk);
-                store_back<M>(scr, k - start - 1, lane, bk);
+                put_rec<M, F32>(scr, k - start - 1, lane, bk);
                 rel = nr;
             }
         }
-#pragma unroll
-        for (int c = 0; c < M; ++c) yi[c] = yj[c];
-        ti = tj;
-    }
+    });
 }
 
 // Tree up-sweep, root solve (lane 0), tree down-sweep and chunk back-substitution.
 // Writes the correction into `delta` (point-major) when non-null and, when
 // `apply` is set, adds it to y.
-template <int M, int METHOD>
+template <int M, int METHOD, bool F32 = false>
 __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y, int N,
                                         int lane, double *scr, size_t tree_base, Rel<M> &rel, double *delta,
                                         bool apply) {
@@ -700,7 +791,7 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
                             for (int c = 0; c < W; ++c) mn[i][c] = mhi[i][c];
                         for (int kk = ls + llen - 1; kk >= node; --kk) {
                             Back<M> bk;
-                            load_back<M>(scr, kk - ls - 1, l, bk);
+                            get_rec<M, F32>(scr, kk - ls - 1, l, bk);
                             double mk[M][W];
 #pragma unroll
                             for (int i = 0; i < M; ++i)
@@ -776,33 +867,48 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
                 if (apply) y[(size_t)N * M + c] += yn[c];
             }
         }
-        Back<M> bnext;
-        double ycur[M], ynext[M];  // y at node k and k - 1, read ahead for the update
-        if (len > 1) {
-            load_back<M>(scr, len - 2, lane, bnext);
-            if (apply) {
+        // nodes k = start+len-1 down to start+1; step i is node start+len-1-i, whose record
+        // is slot len-2-i.  Records (and y) are read PB steps ahead into a register ring.
+        constexpr int PB = kPfBack<M>;
+        const int ns = len - 1;
+        Back<M> rb[PB];
+        double ry[PB][M];
 #pragma unroll
-                for (int c = 0; c < M; ++c) ynext[c] = y[(size_t)(start + len - 1) * M + c];
-            }
-        }
-        for (int k = start + len - 1; k > start; --k) {
-            const Back<M> bk = bnext;  // record of node k; the next one is loaded ahead
-            if (k - 1 > start) load_back<M>(scr, k - start - 2, lane, bnext);
-            if (apply) {
+        for (int j = 0; j < PB; ++j) {
+            if (j < ns) {
+                get_rec<M, F32>(scr, len - 2 - j, lane, rb[j]);
+                if (apply) {
 #pragma unroll
-                for (int c = 0; c < M; ++c) ycur[c] = ynext[c];
-                if (k - 1 > start) {
-#pragma unroll
-                    for (int c = 0; c < M; ++c) ynext[c] = y[(size_t)(k - 1) * M + c];
+                    for (int c = 0; c < M; ++c) ry[j][c] = y[(size_t)(start + len - 1 - j) * M + c];
                 }
             }
-            double yk[M];
-            back_apply<M>(bk, ya, yn, yk);
+        }
+        for (int i0 = 0; i0 < ns; i0 += PB) {
 #pragma unroll
-            for (int c = 0; c < M; ++c) {
-                if (delta) delta[(size_t)k * M + c] = yk[c];
-                if (apply) y[(size_t)k * M + c] = ycur[c] + yk[c];
-                yn[c] = yk[c];
+            for (int j = 0; j < PB; ++j) {
+                const int i = i0 + j;
+                if (i < ns) {
+                    const int k = start + len - 1 - i;
+                    const Back<M> bk = rb[j];
+                    double ycur[M];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) ycur[c] = ry[j][c];
+                    if (i + PB < ns) {
+                        get_rec<M, F32>(scr, len - 2 - (i + PB), lane, rb[j]);
+                        if (apply) {
+#pragma unroll
+                            for (int c = 0; c < M; ++c) ry[j][c] = y[(size_t)(k - PB) * M + c];
+                        }
+                    }
+                    double yk[M];
+                    back_apply<M>(bk, ya, yn, yk);
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        if (delta) delta[(size_t)k * M + c] = yk[c];
+                        if (apply) y[(size_t)k * M + c] = ycur[c] + yk[c];
+                        yn[c] = yk[c];
+                    }
+                }
             }
         }
 #pragma unroll
@@ -817,12 +923,12 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
 
 // Residual evaluation: forward pass + BC rows.  Leaves BC residuals and their
 // nodes in LDS; returns max|F| (NaN if not finite) in all lanes.
-template <int M, int METHOD>
+template <int M, int METHOD, bool F32 = false>
 __device__ inline double residual_and_blocks(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y,
                                              int N, int lane, double *scr, Rel<M> &rel) {
     double mx;
     bool bad;
-    forward_dual<M, METHOD>(I, mesh, y, N, lane, scr, rel, mx, bad, nullptr, nullptr, nullptr);
+    forward_dual<M, METHOD, F32>(I, mesh, y, N, lane, scr, rel, mx, bad, nullptr, nullptr, nullptr);
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
         sh.bcnode[lane] = L.node;
@@ -850,39 +956,18 @@ __device__ inline double residual_primal(const Inst<M> &I, const double *mesh, c
     chunk_of(N, lane, P, start, len);
     double mx = 0.0;
     bool bad = false;
-    if (len > 0) {
-        double yi[M], yj[M], yn[M];
+    walk_chunk<M, kPfPrimal<M>>(mesh, y, start, len, [&](int, double ti, double tj, const double *yi, const double *yj) {
+        const double h = tj - ti;
+        double K[Tab<METHOD>::s][M], ph[M];
+        stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
+        phi_of<M, METHOD, double>(h, yi, yj, K, ph);
 #pragma unroll
         for (int c = 0; c < M; ++c) {
-            yi[c] = y[(size_t)start * M + c];
-            yn[c] = y[(size_t)(start + 1) * M + c];
+            const double a = fabs(ph[c]);
+            bad |= !(a <= 1.7976931348623157e308);
+            mx = fmax(mx, a);
         }
-        double ti = mesh[start], tn = mesh[start + 1];
-        for (int k = start; k < start + len; ++k) {
-            // node k + 2 is loaded while interval k is evaluated
-            const double tj = tn;
-#pragma unroll
-            for (int c = 0; c < M; ++c) yj[c] = yn[c];
-            if (k + 2 <= start + len) {
-                tn = mesh[k + 2];
-#pragma unroll
-                for (int c = 0; c < M; ++c) yn[c] = y[(size_t)(k + 2) * M + c];
-            }
-            const double h = tj - ti;
-            double K[Tab<METHOD>::s][M], ph[M];
-            stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
-            phi_of<M, METHOD, double>(h, yi, yj, K, ph);
-#pragma unroll
-            for (int c = 0; c < M; ++c) {
-                const double a = fabs(ph[c]);
-                bad |= !(a <= 1.7976931348623157e308);
-                mx = fmax(mx, a);
-            }
-#pragma unroll
-            for (int c = 0; c < M; ++c) yi[c] = yj[c];
-            ti = tj;
-        }
-    }
+    });
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
         const double a = fabs(bc_residual<M, METHOD>(I, lane, L, mesh, y));
@@ -931,16 +1016,17 @@ __device__ inline double residual_sq_trial(const Inst<M> &I, const double *mesh,
 
 struct NewtonOut {
     int ret, iters, fallback, warm_fallback;
+    double fn;  // max|F| at the state Newton returns
 };
 
 // NewtonRaphson, then NewtonRaphson with backtracking from the same start
 // (Core/default_internal_solve.jl:37-43).  Stop when max|F| <= tol, checked
 // before each step.
-template <int M, int METHOD>
+template <int M, int METHOD, bool F32 = false>
 __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y,
                             double *ybak, double *delta, double *ytmp, int N, int lane, double *scr,
                             size_t tree_base, const double *u0c) {
-    NewtonOut out{BVP_RET_MAXITERS, 0, 0, 0};
+    NewtonOut out{BVP_RET_MAXITERS, 0, 0, 0, NAN};
     // u0c != null: y holds a learned warm start; when the first Newton step from it
     // does not contract max|F| (or the guess is unusable), restart from the constant
     // guess u0c, as the plain method would have started.
@@ -966,7 +1052,7 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
             // After a step, max|F| comes from the primal pass; the dual pass that
             // builds the blocks runs only when another step is taken.
             bool blocks = (it == 0);
-            double fn = blocks ? residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel)
+            double fn = blocks ? residual_and_blocks<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, rel)
                                : residual_primal<M, METHOD>(I, mesh, y, N, lane);
             if (warm) {
                 bool fall;
@@ -987,16 +1073,17 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                             if (B.linesearch) ybak[(size_t)k * M + c] = u0c[c];
                         }
                     __syncthreads();
-                    fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+                    fn = residual_and_blocks<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, rel);
                     blocks = true;
                 }
             }
+            out.fn = fn;  // every exit below leaves y at the state fn was computed for
             if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
             if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
             if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
-            if (!blocks) residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+            if (!blocks) residual_and_blocks<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, rel);
             const bool plain = (attempt == 0);
-            if (!solve_correction<M, METHOD>(I, sh, mesh, y, N, lane, scr, tree_base, rel, plain ? nullptr : delta,
+            if (!solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, plain ? nullptr : delta,
                                              plain)) {
                 if (warm) {  // singular at the warm start: restart from the constant guess
                     warm = false;
@@ -1046,8 +1133,9 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
 // error_estimate!(::DefectControl): per interval the defect vector with the
 // larger max-abs of the two sample points; returns defect norm (NaN-aware) and
 // fills shat (= (max|d_i|/abstol)^(1/(p+1)), MIRK/adaptivity.jl:39-40).
+// forced inline: the specialised kernels fold the family into it
 template <int M, int METHOD>
-__device__ inline double defect_pass(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
+__device__ __forceinline__ double defect_pass(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
                                      double abstol, double *shat, double *errors, double &r1, double &r2) {
     using TB = Tab<METHOD>;
     int P, start, len;
@@ -1058,27 +1146,7 @@ __device__ inline double defect_pass(const Inst<M> &I, const double *mesh, const
     double dmax = 0.0, smax = 0.0, ssum = 0.0;
     bool bad = false;
     const double ex = 1.0 / (TB::order + 1);
-    // node k + 1 is loaded one interval ahead (its latency overlaps interval k's work)
-    double yi[M], yj[M], yn[M];
-    double ti = 0.0, tj = 0.0, tn = 0.0;
-    if (len > 0) {
-        ti = mesh[start];
-        tn = mesh[start + 1];
-#pragma unroll
-        for (int c = 0; c < M; ++c) {
-            yi[c] = y[(size_t)start * M + c];
-            yn[c] = y[(size_t)(start + 1) * M + c];
-        }
-    }
-    for (int k = start; k < start + len; ++k) {
-        tj = tn;
-#pragma unroll
-        for (int c = 0; c < M; ++c) yj[c] = yn[c];
-        if (k + 2 <= start + len) {
-            tn = mesh[k + 2];
-#pragma unroll
-            for (int c = 0; c < M; ++c) yn[c] = y[(size_t)(k + 2) * M + c];
-        }
+    walk_chunk<M, kPfDefect<M>>(mesh, y, start, len, [&](int k, double ti, double tj, const double *yi, const double *yj) {
         const double h = tj - ti;
         double K[TB::s][M], KI[TB::sstar - TB::s][M];
         stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
@@ -1107,10 +1175,7 @@ __device__ inline double defect_pass(const Inst<M> &I, const double *mesh, const
         if (shat) shat[k] = sh;
         smax = fmax(smax, sh);
         ssum += sh;
-        ti = tj;
-#pragma unroll
-        for (int c = 0; c < M; ++c) yi[c] = yj[c];
-    }
+    });
     const bool anybad = wave_any(bad);
     dmax = wave_max(dmax);
     r1 = wave_max(smax);
@@ -1247,15 +1312,19 @@ __device__ inline void transfer(const Inst<M> &I, const double *mesh, const doub
 template <int M> struct Bufs {
     double *mesh[2];
     double *y[2];
+    // buffer c (0 or 1) chosen with a select, so that the array is never indexed
+    // at run time and the pointers keep their global address space
+    __device__ double *mesh_of(int c) const { return c ? mesh[1] : mesh[0]; }
+    __device__ double *y_of(int c) const { return c ? y[1] : y[0]; }
 };
 
 template <int M>
 __device__ inline Bufs<M> bufs_of(const BatchDev &B, int b) {
     Bufs<M> r;
     const size_t cap = (size_t)B.cap;
-    r.mesh[0] = B.mesh + (size_t)b * 2 * cap;
+    r.mesh[0] = as_global(B.mesh + (size_t)b * 2 * cap);
     r.mesh[1] = r.mesh[0] + cap;
-    r.y[0] = B.y + (size_t)b * 2 * cap * M;
+    r.y[0] = as_global(B.y + (size_t)b * 2 * cap * M);
     r.y[1] = r.y[0] + cap * M;
     return r;
 }
@@ -1265,7 +1334,7 @@ __device__ inline Bufs<M> bufs_of(const BatchDev &B, int b) {
 // two-point boundary conditions; every f evaluation then folds to that family's
 // code and the interior-node boundary walk drops out, which lowers the register
 // peak the other paths (sin/cos reduction and the like) impose.
-template <int M, int METHOD, int FAM = -1>
+template <int M, int METHOD, int FAM = -1, bool F32 = false>
 __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<M> &sh, int lane, double *scr, size_t tree_base,
                                double *aux) {
     Inst<M> I;
@@ -1280,27 +1349,63 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     // initial mesh and guess from the packed inputs (or from learned anchors)
     {
         const long long off = B.init_off[b];
+        // U nodes per lane per round: every load of a round is issued before its stores
+        constexpr int U = 4;
         if constexpr (M == 2) {
             if (B.anc) {
                 const double t0 = B.t0[b], t1 = B.t1[b];
-                for (int k = lane; k <= N; k += WAVE) {
-                    const double t = B.init_mesh[off + k];
-                    bf.mesh[0][k] = t;
-                    anchor_hermite(B.anc, B.anc_K, b, t0, t1, t, bf.y[0] + (size_t)k * M);
+                const int K = B.anc_K;
+                for (int k0 = lane; k0 <= N; k0 += U * WAVE) {
+                    double t[U], rec[U][8];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + u * WAVE;
+                        t[u] = k <= N ? B.init_mesh[off + k] : t0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const double *a = as_global(B.anc) + ((size_t)b * K + anchor_interval(K, t0, t1, t[u])) * 4;
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) rec[u][q] = a[q];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + u * WAVE;
+                        if (k <= N) {
+                            bf.mesh[0][k] = t[u];
+                            anchor_eval(rec[u], K, t0, t1, t[u], bf.y[0] + (size_t)k * M);
+                        }
+                    }
                 }
                 __syncthreads();
             }
         }
         if (!(M == 2 && B.anc)) {
-            for (int k = lane; k <= N; k += WAVE) {
-                bf.mesh[0][k] = B.init_mesh[off + k];
+            for (int k0 = lane; k0 <= N; k0 += U * WAVE) {
+                double t[U], v[U][M];
 #pragma unroll
-                for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = B.init_y[(off + k) * M + c];
+                for (int u = 0; u < U; ++u) {
+                    const int k = k0 + u * WAVE;
+                    if (k <= N) {
+                        t[u] = B.init_mesh[off + k];
+#pragma unroll
+                        for (int c = 0; c < M; ++c) v[u][c] = B.init_y[(off + k) * M + c];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = k0 + u * WAVE;
+                    if (k <= N) {
+                        bf.mesh[0][k] = t[u];
+#pragma unroll
+                        for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = v[u][c];
+                    }
+                }
             }
             __syncthreads();
         }
     }
-    double *shat = B.shat + (size_t)b * B.cap;
+    double *shat = as_global(B.shat + (size_t)b * B.cap);
     double *delta = aux;                               // [cap*M]
     double *ytmp = aux + (size_t)B.cap * M;            // [cap*M]
     double *iiscr = aux + (size_t)2 * B.cap * M;       // [cap]
@@ -1308,13 +1413,17 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     int outer = 0, iters = 0, fb = 0;
     double last_defect = NAN;
     long long cnt_it = 0, cnt_def = 0, cnt_tr = 0, cnt_chk = 0;
+    double rn = NAN;
+    bool rn_valid = false;  // rn is max|F| at the current (mesh, y)
     for (;;) {
         ++outer;
-        double *mesh = bf.mesh[cur], *y = bf.y[cur];
-        double *mesh_o = bf.mesh[1 - cur], *y_o = bf.y[1 - cur];
-        const double *u0c = (B.warm && outer == 1) ? B.u0c + (size_t)b * M : nullptr;
-        NewtonOut nl = newton<M, METHOD>(B, I, sh, mesh, y, y_o, delta, ytmp, N, lane, scr, tree_base, u0c);
+        double *mesh = bf.mesh_of(cur), *y = bf.y_of(cur);
+        double *mesh_o = bf.mesh_of(1 - cur), *y_o = bf.y_of(1 - cur);
+        const double *u0c = (B.warm && outer == 1) ? as_global(B.u0c + (size_t)b * M) : nullptr;
+        NewtonOut nl = newton<M, METHOD, F32>(B, I, sh, mesh, y, y_o, delta, ytmp, N, lane, scr, tree_base, u0c);
         iters += nl.iters;
+        rn = nl.fn;
+        rn_valid = true;
         cnt_it += (long long)N * nl.iters;
         cnt_chk += N + (nl.warm_fallback ? N : 0);
         if (nl.fallback) fb |= 1;
@@ -1341,6 +1450,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                     cnt_tr += N + Nn;
                     N = Nn;
                     cur = 1 - cur;
+                    rn_valid = false;
                 }
             }
         } else {
@@ -1354,15 +1464,15 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                 __syncthreads();
                 N = 2 * N;
                 cur = 1 - cur;
+                rn_valid = false;
                 info = BVP_RET_SUCCESS;
             }
         }
         if (!(info == BVP_RET_SUCCESS && error_norm > B.abstol)) break;
         if (outer >= MAX_OUTER) { info = BVP_RET_FAILURE; break; }
     }
-    // final residual norm
-    Rel<M> rel;
-    const double rn = residual_and_blocks<M, METHOD>(I, sh, bf.mesh[cur], bf.y[cur], N, lane, nullptr, rel);
+    // final residual norm: Newton's last check unless the mesh changed after it
+    if (!rn_valid) rn = residual_primal<M, METHOD>(I, bf.mesh_of(cur), bf.y_of(cur), N, lane);
     if (lane == 0) {
         B.retcode[b] = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
         B.N_final[b] = N;
@@ -1395,16 +1505,16 @@ __device__ inline int next_item(unsigned *counter, Shared<M> &sh, int lane) {
 // defect -> mesh loop for it.  WPE = waves per SIMD the register allocation is
 // held to (1: no spills; more: more latency hiding, some spills) -- a variant
 // the autotuner picks by measurement.
-template <int M, int METHOD, int WPE, int FAM = -1>
+template <int M, int METHOD, int WPE, int FAM = -1, bool F32 = false>
 __global__ __launch_bounds__(64, WPE) void k_solve(BatchDev B) {
     __shared__ Shared<M> sh;
     const int lane = threadIdx.x;
-    double *scr = B.scratch + (size_t)blockIdx.x * B.scratch_per_slot;
-    double *aux = B.aux + (size_t)blockIdx.x * B.aux_per_slot;
+    double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
+    double *aux = as_global(B.aux + (size_t)blockIdx.x * B.aux_per_slot);
     for (;;) {
         const int item = next_item<M>(B.counter, sh, lane);
         if (item >= B.n) break;
-        solve_instance<M, METHOD, FAM>(B, B.order[item], sh, lane, scr, B.tree_base, aux);
+        solve_instance<M, METHOD, FAM, F32>(B, B.order[item], sh, lane, scr, B.tree_base, aux);
     }
 }
 
@@ -1414,8 +1524,8 @@ template <int M, int METHOD>
 __global__ __launch_bounds__(64) void k_component(BatchDev B, int op) {
     __shared__ Shared<M> sh;
     const int lane = threadIdx.x;
-    double *scr = B.scratch + (size_t)blockIdx.x * B.scratch_per_slot;
-    double *aux = B.aux + (size_t)blockIdx.x * B.aux_per_slot;
+    double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
+    double *aux = as_global(B.aux + (size_t)blockIdx.x * B.aux_per_slot);
     for (;;) {
         const int b = next_item<M>(B.counter, sh, lane);
         if (b >= B.n) break;
@@ -1466,10 +1576,16 @@ __global__ __launch_bounds__(64) void k_component(BatchDev B, int op) {
             if (lane < M) B.iout[(size_t)b * M + lane] = bc_locate<M>(I, lane, mesh, N).node;
         } else if (op == OP_NEWTON_STEP) {
             Rel<M> rel;
-            const double fn = residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
-            (void)fn;
-            bool ok = solve_correction<M, METHOD>(I, sh, mesh, y, N, lane, scr, B.tree_base, rel,
-                                                  B.out0 + off * M, false);
+            bool ok;
+            if (B.f32rec) {
+                (void)residual_and_blocks<M, METHOD, true>(I, sh, mesh, y, N, lane, scr, rel);
+                ok = solve_correction<M, METHOD, true>(I, sh, mesh, y, N, lane, scr, B.tree_base, rel,
+                                                       B.out0 + off * M, false);
+            } else {
+                (void)residual_and_blocks<M, METHOD>(I, sh, mesh, y, N, lane, scr, rel);
+                ok = solve_correction<M, METHOD>(I, sh, mesh, y, N, lane, scr, B.tree_base, rel,
+                                                 B.out0 + off * M, false);
+            }
             if (lane == 0) B.iout[b] = ok ? 0 : 1;
         } else if (op == OP_DEFECT) {
             double r1, r2;
@@ -1478,7 +1594,7 @@ __global__ __launch_bounds__(64) void k_component(BatchDev B, int op) {
             if (lane == 0) B.out1[b] = dn;
         } else if (op == OP_MESH_SELECT) {
             double r1, r2;
-            double *shat = B.shat + (size_t)b * B.cap;
+            double *shat = as_global(B.shat + (size_t)b * B.cap);
             const double dn = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2);
             (void)dn;
             __syncthreads();
@@ -1513,16 +1629,16 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev B, const long long *out
     Bufs<M> bf = bufs_of<M>(B, b);
     const long long o = out_off[b];
     for (int k = threadIdx.x; k <= N; k += blockDim.x) {
-        out_mesh[o + k] = bf.mesh[cur][k];
+        out_mesh[o + k] = bf.mesh_of(cur)[k];
 #pragma unroll
-        for (int c = 0; c < M; ++c) out_y[(o + k) * M + c] = bf.y[cur][(size_t)k * M + c];
+        for (int c = 0; c < M; ++c) out_y[(o + k) * M + c] = bf.y_of(cur)[(size_t)k * M + c];
     }
 }
 
 // ------------------------------------------------------------------ host launchers
-template <int M, int METHOD, int WPE, int FAM = -1>
+template <int M, int METHOD, int WPE, int FAM = -1, bool F32 = false>
 hipError_t launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
-    k_solve<M, METHOD, WPE, FAM><<<blocks, 64, 0, s>>>(B);
+    k_solve<M, METHOD, WPE, FAM, F32><<<blocks, 64, 0, s>>>(B);
     return hipGetLastError();
 }
 template <int M, int METHOD>
@@ -1531,10 +1647,10 @@ hipError_t launch_component_t(const BatchDev &B, int op, int blocks, hipStream_t
     return hipGetLastError();
 }
 
-template <int M, int METHOD, int WPE, int FAM = -1>
+template <int M, int METHOD, int WPE, int FAM = -1, bool F32 = false>
 int occupancy_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<M, METHOD, WPE, FAM>, 64, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<M, METHOD, WPE, FAM, F32>, 64, 0) != hipSuccess)
         return 4 * WPE;
     return nb;
 }

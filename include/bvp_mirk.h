@@ -73,7 +73,13 @@ extern "C" {
 #define BVP_RET_MAXITERS 2
 #define BVP_RET_UNSTABLE 3   /* non-finite residual */
 
-/* ---- linear-solve precision */
+/* ---- linear-solve precision
+ * M = 8, 16, 32 (block kernels): FP32_REFINE64 factorises in FP32 and refines each
+ * linear solve refine_iters times against the FP64 residual.
+ * M <= 4 (wave kernels): FP32_REFINE64 keeps the elimination in FP64 and stores the
+ * chunk factors (the streamed back-substitution records) in FP32; the refinement is
+ * Newton's own loop, whose residual and convergence test stay FP64 (refine_iters is
+ * not used). */
 #define BVP_PREC_FP64          0  /* block LU in FP64 */
 #define BVP_PREC_FP32_REFINE64 1  /* block LU in FP32 + FP64 iterative refinement */
 
