@@ -3,13 +3,15 @@
 
 A step is one full batched solve (learned guess of every node, then Newton +
 defect control + mesh refinement in one persistent kernel, replayed as a
-hipGraph) of all instances, inputs resident in device memory.  The warm-start
-network is trained first (untimed) on solves of the Bratu family with boundary
-values u(0)=a, u(1)=b that the plain method itself generates.  Prints one JSON
-line with the throughput in BVP solves/s (successful solves with defect <=
-1e-6), the same batch without the network, the plain one-instance-at-a-time
-FP64 GPU rate, the roofline of the solve kernel and the oracle timed on the
-host cores.  Other workloads (--workload) print their own lines.
+hipGraph) of all instances, inputs resident in device memory, with the chunk
+elimination factors stored in FP32 (mixed precision; residuals, elimination and
+the Newton test in FP64).  The warm-start network is trained first (untimed) on
+solves of the Bratu family with boundary values u(0)=a, u(1)=b that the plain
+method itself generates.  Prints one JSON line with the throughput in BVP
+solves/s (successful solves with defect <= 1e-6), the same batch in FP64 with
+and without the network, the plain one-instance-at-a-time FP64 GPU rate, the
+roofline of the solve kernel and the oracle timed on the host cores.  Other
+workloads (--workload) print their own lines.
 """
 import argparse
 import json
@@ -44,12 +46,12 @@ def algorithmic_bytes(M, counters, n0_nodes, nf_nodes, p_j=8):
     return b
 
 
-def read_traffic(workload):
+def read_traffic(workload, key="traffic_bytes_per_launch"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     for p in sorted((ROOT / "profiles").glob("*.md")):
         txt = p.read_text()
         if workload in txt:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+            m = re.search(key + r":\s*([0-9.eE+]+)", txt)
             if m:
                 return float(m.group(1))
     return None
@@ -446,8 +448,10 @@ def main():
                    "headline_run": {"value": value, "guess_ms": tb["guess_ms"], "solve_ms": tb["solve_ms"], **sb},
                    "training": tinfo, "autotune": tune, "autotune_mixed": tune_mx},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("k_solve"),
-                     "kernel": "k_solve<2,4>", "kernel_ms": tb["solve_ms"], "algorithmic_bytes": abytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("k_solve", "traffic_bytes_per_launch" +
+                                                                  ("_fp64" if args.no_mixed else "")),
+                     "kernel": "k_solve<2,4,variant,BRATU," + ("false>" if args.no_mixed else "true>"),
+                     "kernel_ms": tb["solve_ms"], "algorithmic_bytes": abytes},
         "cpu_baseline": cpu,
         "plain_gpu": plain,
     }
