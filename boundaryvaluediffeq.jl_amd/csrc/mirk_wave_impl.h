@@ -266,6 +266,114 @@ __device__ inline void interval_dual(int fam, const Params<M> &P, double ti, dou
         for (int c = 0; c < M; ++c) Kp[r][c] = K[r][c].v;
 }
 
+// f and its Jacobian df/dy at (t, y) for the families the specialised M = 2
+// kernels serve; f is the same arithmetic as f_eval's.
+template <int FAM>
+__device__ inline void f_jac2(const Params<2> &P, double t, const double *y, double *f, double (&J)[2][2]) {
+    static_assert(FAM == BVP_FAMILY_BRATU || FAM == BVP_FAMILY_SINGPERT, "family without an analytic Jacobian");
+    if constexpr (FAM == BVP_FAMILY_BRATU) {  // (y2, -lambda e^y1)
+        const double e = exp(y[0]);
+        f[0] = y[1];
+        f[1] = -P.p[0] * e;
+        J[0][0] = 0.0; J[0][1] = 1.0;
+        J[1][0] = -P.p[0] * e; J[1][1] = 0.0;
+    } else {  // (y2, (y1 - t y2) / eps)
+        f[0] = y[1];
+        f[1] = (y[0] - t * y[1]) / P.p[0];
+        J[0][0] = 0.0; J[0][1] = 1.0;
+        J[1][0] = 1.0 / P.p[0]; J[1][1] = -t / P.p[0];
+    }
+}
+
+// Residual and Jacobian blocks of one interval by the chain rule through the
+// stages (the specialised kernels): dK_r/dy = J_r dY_r/dy with
+// dY_r/dy_i = (1 - v_r) I + h sum_{j<r} x_rj dK_j/dy_i (and v_r I + ... for y_j),
+// S = -I - h sum_r b_r dK_r/dy_i, R = I - h sum_r b_r dK_r/dy_j -- the same
+// blocks interval_dual forms with dual numbers (MIRK/mirk.jl:697-862), in
+// M x M products instead of 2M-direction arithmetic through every operation.
+// The stage values and Phi are the same arithmetic as stages<> and phi_of<>.
+template <int M, int METHOD, int FAM>
+__device__ inline void interval_chain(const Params<M> &P, double ti, double h, const double *yi, const double *yj,
+                                      double *phi, double (&S)[M][M], double (&R)[M][M],
+                                      double (&Kp)[Tab<METHOD>::s][M]) {
+    using TB = Tab<METHOD>;
+    double Di[TB::s][M][M], Dj[TB::s][M][M];
+#pragma unroll
+    for (int r = 0; r < TB::s; ++r) {
+        double Y[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            double a = (1.0 - TB::v(r)) * yi[c];
+            Y[c] = a + TB::v(r) * yj[c];
+        }
+        double Yi[M][M], Yj[M][M];
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                Yi[a][b] = (a == b) ? (1.0 - TB::v(r)) : 0.0;
+                Yj[a][b] = (a == b) ? TB::v(r) : 0.0;
+            }
+        if (r > 0) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                double acc = 0.0;
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < r; ++j) {
+                    if (TB::x(r, j) != 0.0) {
+                        if (!any) { acc = TB::x(r, j) * Kp[j][c]; any = true; }
+                        else acc = acc + TB::x(r, j) * Kp[j][c];
+                    }
+                }
+                if (any) Y[c] = Y[c] + h * acc;
+            }
+#pragma unroll
+            for (int j = 0; j < r; ++j) {
+                if (TB::x(r, j) != 0.0) {
+                    const double hx = h * TB::x(r, j);
+#pragma unroll
+                    for (int a = 0; a < M; ++a)
+#pragma unroll
+                        for (int b = 0; b < M; ++b) {
+                            Yi[a][b] += hx * Di[j][a][b];
+                            Yj[a][b] += hx * Dj[j][a][b];
+                        }
+                }
+            }
+        }
+        double J[M][M];
+        f_jac2<FAM>(P, ti + TB::c(r) * h, Y, Kp[r], J);
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                double si = 0.0, sj = 0.0;
+#pragma unroll
+                for (int q = 0; q < M; ++q) {
+                    si += J[a][q] * Yi[q][b];
+                    sj += J[a][q] * Yj[q][b];
+                }
+                Di[r][a][b] = si;
+                Dj[r][a][b] = sj;
+            }
+    }
+    phi_of<M, METHOD, double>(h, yi, yj, Kp, phi);
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = 0; b < M; ++b) {
+            double ai = TB::b(0) * Di[0][a][b], aj = TB::b(0) * Dj[0][a][b];
+#pragma unroll
+            for (int r = 1; r < TB::s; ++r) {
+                ai = ai + TB::b(r) * Di[r][a][b];
+                aj = aj + TB::b(r) * Dj[r][a][b];
+            }
+            S[a][b] = ((a == b) ? -1.0 : 0.0) - h * ai;
+            R[a][b] = ((a == b) ? 1.0 : 0.0) - h * aj;
+        }
+}
+
 // ------------------------------------------------------------------ elimination
 // Relation E dy_a + F dy_b = g between two nodes.
 template <int M> struct Rel {
@@ -621,7 +729,7 @@ __device__ inline double bc_residual(const Inst<M> &I, int k, const BcLoc &L, co
 // ------------------------------------------------------------------ Newton step
 // One forward pass at y: residual, blocks, chunk elimination.  Returns the
 // lane's max |Phi| (NaN-aware via `bad`), its chunk relation.
-template <int M, int METHOD, bool F32 = false>
+template <int M, int METHOD, bool F32 = false, int FAM = -1>
 __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
                                     double *scr, Rel<M> &rel, double &mx, bool &bad, double *phi_out,
                                     double *S_out, double *R_out) {
@@ -632,7 +740,10 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
     walk_chunk<M, kPfDual<M>>(mesh, y, start, len, [&](int k, double ti, double tj, const double *yi, const double *yj) {
         const double h = tj - ti;
         double phi[M], S[M][M], R[M][M], Kp[Tab<METHOD>::s][M];
-        interval_dual<M, METHOD>(I.fam, I.P, ti, h, yi, yj, phi, S, R, Kp);
+        if constexpr (M == 2 && (FAM == BVP_FAMILY_BRATU || FAM == BVP_FAMILY_SINGPERT))
+            interval_chain<M, METHOD, FAM>(I.P, ti, h, yi, yj, phi, S, R, Kp);
+        else
+            interval_dual<M, METHOD>(I.fam, I.P, ti, h, yi, yj, phi, S, R, Kp);
         double r[M];
 #pragma unroll
         for (int c = 0; c < M; ++c) {
@@ -923,12 +1034,12 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
 
 // Residual evaluation: forward pass + BC rows.  Leaves BC residuals and their
 // nodes in LDS; returns max|F| (NaN if not finite) in all lanes.
-template <int M, int METHOD, bool F32 = false>
+template <int M, int METHOD, bool F32 = false, int FAM = -1>
 __device__ inline double residual_and_blocks(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y,
                                              int N, int lane, double *scr, Rel<M> &rel) {
     double mx;
     bool bad;
-    forward_dual<M, METHOD, F32>(I, mesh, y, N, lane, scr, rel, mx, bad, nullptr, nullptr, nullptr);
+    forward_dual<M, METHOD, F32, FAM>(I, mesh, y, N, lane, scr, rel, mx, bad, nullptr, nullptr, nullptr);
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
         sh.bcnode[lane] = L.node;
@@ -1022,9 +1133,12 @@ struct NewtonOut {
 // NewtonRaphson, then NewtonRaphson with backtracking from the same start
 // (Core/default_internal_solve.jl:37-43).  Stop when max|F| <= tol, checked
 // before each step.
-template <int M, int METHOD, bool F32 = false>
+// restore() writes the starting state into y again (the line-search attempt
+// starts from where the plain attempt started); it regenerates the state from its
+// sources instead of keeping a copy of it.
+template <int M, int METHOD, bool F32 = false, int FAM = -1, typename Restore>
 __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y,
-                            double *ybak, double *delta, double *ytmp, int N, int lane, double *scr,
+                            Restore &&restore, double *delta, double *ytmp, int N, int lane, double *scr,
                             size_t tree_base, const double *u0c) {
     NewtonOut out{BVP_RET_MAXITERS, 0, 0, 0, NAN};
     // u0c != null: y holds a learned warm start; when the first Newton step from it
@@ -1033,17 +1147,16 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
     bool warm = u0c != nullptr;
     double fn_warm = 0.0;
     const int attempts = B.linesearch ? 2 : 1;
-    if (B.linesearch) {
-        for (int k = lane; k <= N; k += WAVE)
-#pragma unroll
-            for (int c = 0; c < M; ++c) ybak[(size_t)k * M + c] = y[(size_t)k * M + c];
-        __syncthreads();
-    }
+    bool start_u0c = false;  // the plain attempt fell back to u0c: the line search starts there too
     for (int attempt = 0; attempt < attempts; ++attempt) {
         if (attempt == 1) {
-            for (int k = lane; k <= N; k += WAVE)
+            if (start_u0c) {
+                for (int k = lane; k <= N; k += WAVE)
 #pragma unroll
-                for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = ybak[(size_t)k * M + c];
+                    for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = u0c[c];
+            } else {
+                restore();
+            }
             __syncthreads();
         }
         int ret = BVP_RET_MAXITERS;
@@ -1052,7 +1165,7 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
             // After a step, max|F| comes from the primal pass; the dual pass that
             // builds the blocks runs only when another step is taken.
             bool blocks = (it == 0);
-            double fn = blocks ? residual_and_blocks<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, rel)
+            double fn = blocks ? residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel)
                                : residual_primal<M, METHOD>(I, mesh, y, N, lane);
             if (warm) {
                 bool fall;
@@ -1068,12 +1181,10 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                     out.warm_fallback = 1;
                     for (int k = lane; k <= N; k += WAVE)
 #pragma unroll
-                        for (int c = 0; c < M; ++c) {
-                            y[(size_t)k * M + c] = u0c[c];
-                            if (B.linesearch) ybak[(size_t)k * M + c] = u0c[c];
-                        }
+                        for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = u0c[c];
+                    start_u0c = true;
                     __syncthreads();
-                    fn = residual_and_blocks<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, rel);
+                    fn = residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
                     blocks = true;
                 }
             }
@@ -1081,7 +1192,7 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
             if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
             if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
             if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
-            if (!blocks) residual_and_blocks<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, rel);
+            if (!blocks) residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
             const bool plain = (attempt == 0);
             if (!solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, plain ? nullptr : delta,
                                              plain)) {
@@ -1091,10 +1202,8 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                     out.iters++;
                     for (int k = lane; k <= N; k += WAVE)
 #pragma unroll
-                        for (int c = 0; c < M; ++c) {
-                            y[(size_t)k * M + c] = u0c[c];
-                            if (B.linesearch) ybak[(size_t)k * M + c] = u0c[c];
-                        }
+                        for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = u0c[c];
+                    start_u0c = true;
                     __syncthreads();
                     it = -1;
                     continue;
@@ -1346,8 +1455,9 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     Bufs<M> bf = bufs_of<M>(B, b);
     int cur = 0;
     int N = B.N0[b];
-    // initial mesh and guess from the packed inputs (or from learned anchors)
-    {
+    // initial mesh and guess from the packed inputs (or the learned guess from the
+    // anchors); with_mesh false rewrites only the state (the line search's restart)
+    auto initial_state = [&](bool with_mesh) {
         const long long off = B.init_off[b];
         // U nodes per lane per round: every load of a round is issued before its stores
         constexpr int U = 4;
@@ -1372,7 +1482,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                     for (int u = 0; u < U; ++u) {
                         const int k = k0 + u * WAVE;
                         if (k <= N) {
-                            bf.mesh[0][k] = t[u];
+                            if (with_mesh) bf.mesh[0][k] = t[u];
                             anchor_eval(rec[u], K, t0, t1, t[u], bf.y[0] + (size_t)k * M);
                         }
                     }
@@ -1396,7 +1506,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                 for (int u = 0; u < U; ++u) {
                     const int k = k0 + u * WAVE;
                     if (k <= N) {
-                        bf.mesh[0][k] = t[u];
+                        if (with_mesh) bf.mesh[0][k] = t[u];
 #pragma unroll
                         for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = v[u][c];
                     }
@@ -1404,7 +1514,8 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
             }
             __syncthreads();
         }
-    }
+    };
+    initial_state(true);
     double *shat = as_global(B.shat + (size_t)b * B.cap);
     double *delta = aux;                               // [cap*M]
     double *ytmp = aux + (size_t)B.cap * M;            // [cap*M]
@@ -1415,12 +1526,28 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     long long cnt_it = 0, cnt_def = 0, cnt_tr = 0, cnt_chk = 0;
     double rn = NAN;
     bool rn_valid = false;  // rn is max|F| at the current (mesh, y)
+    // where the current Newton solve starts: 0 the initial state, 1 the transfer from
+    // the other buffer (prevN intervals), 2 zeros (halving); restore() rebuilds it
+    int start_kind = 0, prevN = 0;
+    auto restore = [&]() {
+        double *yc = bf.y_of(cur);
+        if (start_kind == 0) {
+            initial_state(false);
+        } else if (start_kind == 1) {
+            transfer<M, METHOD>(I, bf.mesh_of(1 - cur), bf.y_of(1 - cur), prevN, bf.mesh_of(cur), yc, N,
+                                B.transfer_mode, lane, iiscr);
+        } else {
+            for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+                for (int c = 0; c < M; ++c) yc[(size_t)k * M + c] = 0.0;
+        }
+    };
     for (;;) {
         ++outer;
         double *mesh = bf.mesh_of(cur), *y = bf.y_of(cur);
         double *mesh_o = bf.mesh_of(1 - cur), *y_o = bf.y_of(1 - cur);
         const double *u0c = (B.warm && outer == 1) ? as_global(B.u0c + (size_t)b * M) : nullptr;
-        NewtonOut nl = newton<M, METHOD, F32>(B, I, sh, mesh, y, y_o, delta, ytmp, N, lane, scr, tree_base, u0c);
+        NewtonOut nl = newton<M, METHOD, F32, FAM>(B, I, sh, mesh, y, restore, delta, ytmp, N, lane, scr, tree_base, u0c);
         iters += nl.iters;
         rn = nl.fn;
         rn_valid = true;
@@ -1448,6 +1575,8 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                 } else {
                     transfer<M, METHOD>(I, mesh, y, N, mesh_o, y_o, Nn, B.transfer_mode, lane, iiscr);
                     cnt_tr += N + Nn;
+                    prevN = N;
+                    start_kind = 1;
                     N = Nn;
                     cur = 1 - cur;
                     rn_valid = false;
@@ -1464,6 +1593,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                 __syncthreads();
                 N = 2 * N;
                 cur = 1 - cur;
+                start_kind = 2;
                 rn_valid = false;
                 info = BVP_RET_SUCCESS;
             }
