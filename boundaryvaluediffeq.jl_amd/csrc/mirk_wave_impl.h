@@ -663,9 +663,21 @@ __device__ inline BcLoc bc_locate(const Inst<M> &I, int k, const double *mesh, i
 }
 
 // ------------------------------------------------------------------ LDS layout
+// Nodes whose state (or intervals whose shat) fit the staging area: all of the
+// initial meshes of the M = 2 workloads; for larger M the area is what the tree
+// leaves (the LDS per wave is not raised for them).
+template <int M> constexpr int kStageNodes = M <= 2 ? 1025 : (2 * WAVE - 1) * kRelFields<M> / M;
 template <int M> struct Shared {
-    double rel[WAVE][kRelFields<M>];  // relation exchange for the tree
-    double tree[(WAVE - 1) * kRelFields<M>];  // back-substitution records of the tree merges
+    union {
+        struct {
+            double rel[WAVE][kRelFields<M>];          // relation exchange for the tree
+            double tree[(WAVE - 1) * kRelFields<M>];  // back-substitution records of the tree merges
+        };
+        // after the tree: the corrected state of the chunk back-substitution, or the
+        // shat of the defect pass, staged so that they reach global memory as whole
+        // lines (a lane's own stores are 8-16 bytes at a stride of its chunk length)
+        double stage[kStageNodes<M> * M];
+    };
     double lo[WAVE][M], hi[WAVE][M];  // corrections at chunk ends
     double bcres[M];
     int bcnode[M];
@@ -967,6 +979,10 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
         __syncthreads();
     }
     // ---- chunk back-substitution
+    // The corrected state goes to the LDS staging area (the tree records are no
+    // longer needed) and then to y in whole lines, when it fits.
+    const bool staged = apply && delta == nullptr && N + 1 <= kStageNodes<M>;
+    double *ynew = staged ? sh.stage : y;
     if (len > 0) {
         double ya[M], yn[M];
 #pragma unroll
@@ -975,7 +991,7 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
 #pragma unroll
             for (int c = 0; c < M; ++c) {
                 if (delta) delta[(size_t)N * M + c] = yn[c];
-                if (apply) y[(size_t)N * M + c] += yn[c];
+                if (apply) ynew[(size_t)N * M + c] = y[(size_t)N * M + c] + yn[c];
             }
         }
         // nodes k = start+len-1 down to start+1; step i is node start+len-1-i, whose record
@@ -1016,7 +1032,7 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
 #pragma unroll
                     for (int c = 0; c < M; ++c) {
                         if (delta) delta[(size_t)k * M + c] = yk[c];
-                        if (apply) y[(size_t)k * M + c] = ycur[c] + yk[c];
+                        if (apply) ynew[(size_t)k * M + c] = ycur[c] + yk[c];
                         yn[c] = yk[c];
                     }
                 }
@@ -1025,10 +1041,14 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
 #pragma unroll
         for (int c = 0; c < M; ++c) {
             if (delta) delta[(size_t)start * M + c] = ya[c];
-            if (apply) y[(size_t)start * M + c] += ya[c];
+            if (apply) ynew[(size_t)start * M + c] = y[(size_t)start * M + c] + ya[c];
         }
     }
     __syncthreads();
+    if (staged) {
+        for (int q = lane; q < (N + 1) * M; q += WAVE) y[q] = sh.stage[q];
+        __syncthreads();
+    }
     return true;
 }
 
@@ -1243,9 +1263,13 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
 // larger max-abs of the two sample points; returns defect norm (NaN-aware) and
 // fills shat (= (max|d_i|/abstol)^(1/(p+1)), MIRK/adaptivity.jl:39-40).
 // forced inline: the specialised kernels fold the family into it
+//
+// stage != null: shat is written through that LDS area (N values) and copied to
+// shat in whole lines at the end.
 template <int M, int METHOD>
 __device__ __forceinline__ double defect_pass(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
-                                     double abstol, double *shat, double *errors, double &r1, double &r2) {
+                                     double abstol, double *shat, double *errors, double &r1, double &r2,
+                                     double *stage = nullptr) {
     using TB = Tab<METHOD>;
     int P, start, len;
     chunk_of(N, lane, P, start, len);
@@ -1281,10 +1305,15 @@ __device__ __forceinline__ double defect_pass(const Inst<M> &I, const double *me
         }
         dmax = fmax(dmax, ei);
         const double sh = pow(ei / abstol, ex);
-        if (shat) shat[k] = sh;
+        if (shat) (stage ? stage : shat)[k] = sh;
         smax = fmax(smax, sh);
         ssum += sh;
     });
+    if (shat && stage) {
+        __syncthreads();
+        for (int k = lane; k < N; k += WAVE) shat[k] = stage[k];
+        __syncthreads();
+    }
     const bool anybad = wave_any(bad);
     dmax = wave_max(dmax);
     r1 = wave_max(smax);
@@ -1561,7 +1590,8 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         info = nl.ret;
         double r1 = 0.0, r2 = 0.0;
         if (info == BVP_RET_SUCCESS) {
-            error_norm = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2);
+            error_norm = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2,
+                                                N <= kStageNodes<M> * M ? sh.stage : nullptr);
             cnt_def += N;
             last_defect = error_norm;
             info = (error_norm > B.defect_threshold || isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
