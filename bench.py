@@ -35,10 +35,12 @@ def log(*a):
 
 def algorithmic_bytes(M, counters, n0_nodes, nf_nodes, p_j=8):
     """Bytes of the streaming model: per interval and Newton iteration
-    40M + 16 + 4 M^2 p_J; per convergence-check pass 16M + 16; per defect
-    interval 8M + 24; per transfer node 8(M+1); plus guess in and solution out."""
+    24M + 16 (state and mesh) + 2 p_J (2M^2 + M) (the elimination record written
+    and read back, p_J bytes per value: 8 FP64, 4 FP32), i.e. 40M + 16 + 4 M^2 p_J
+    for FP64 records; per convergence-check pass 16M + 16; per defect interval
+    8M + 24; per transfer node 8(M+1); plus guess in and solution out."""
     it, dfct, tr, chk = counters
-    b = it * (40 * M + 16 + 4 * M * M * p_j)
+    b = it * (24 * M + 16 + 2 * p_j * (2 * M * M + M))
     b += chk * (16 * M + 16)
     b += dfct * (8 * M + 24)
     b += tr * 8 * (M + 1)

@@ -511,18 +511,14 @@ __device__ inline void load_back(const double *base, int slot, int lane, Back<M>
     for (int i = 0; i < M; ++i) bk.h[i] = p[(f++) * STRIDE];
 }
 
-// Chunk records with the factors X and Z stored in FP32 (h stays FP64): a slot
-// holds M doubles and 2 M^2 floats per lane, [h fields][X, Z float fields] x 64
-// lanes, every field a contiguous run of 64 values.  The elimination itself runs
-// in FP64; Newton's next residual, evaluated in FP64, corrects the rounding of
-// the factors (the correction is an inexact Newton step).
-template <int M> constexpr int kRecDoublesF32 = M + M * M;
+// Chunk records stored in FP32 (X, Z and h): a slot holds 2 M^2 + M floats per
+// lane, [field][64 lanes], every field a contiguous run of 64 values (256 bytes).
+// The elimination itself runs in FP64; Newton's next residual, evaluated in FP64,
+// corrects the rounding of the record (the correction is an inexact Newton step).
+template <int M> constexpr int kRecFloatsF32 = 2 * M * M + M;
 template <int M>
 __device__ inline void store_back_f32(double *base, int slot, int lane, const Back<M> &bk) {
-    double *p = base + (size_t)slot * kRecDoublesF32<M> * WAVE;
-#pragma unroll
-    for (int i = 0; i < M; ++i) p[i * WAVE + lane] = bk.h[i];
-    float *q = reinterpret_cast<float *>(p + M * WAVE) + lane;
+    float *q = reinterpret_cast<float *>(base) + (size_t)slot * kRecFloatsF32<M> * WAVE + lane;
     int f = 0;
 #pragma unroll
     for (int i = 0; i < M; ++i)
@@ -532,13 +528,12 @@ __device__ inline void store_back_f32(double *base, int slot, int lane, const Ba
     for (int i = 0; i < M; ++i)
 #pragma unroll
         for (int c = 0; c < M; ++c) q[(f++) * WAVE] = (float)bk.Z[i][c];
+#pragma unroll
+    for (int i = 0; i < M; ++i) q[(f++) * WAVE] = (float)bk.h[i];
 }
 template <int M>
 __device__ inline void load_back_f32(const double *base, int slot, int lane, Back<M> &bk) {
-    const double *p = base + (size_t)slot * kRecDoublesF32<M> * WAVE;
-#pragma unroll
-    for (int i = 0; i < M; ++i) bk.h[i] = p[i * WAVE + lane];
-    const float *q = reinterpret_cast<const float *>(p + M * WAVE) + lane;
+    const float *q = reinterpret_cast<const float *>(base) + (size_t)slot * kRecFloatsF32<M> * WAVE + lane;
     int f = 0;
 #pragma unroll
     for (int i = 0; i < M; ++i)
@@ -548,6 +543,8 @@ __device__ inline void load_back_f32(const double *base, int slot, int lane, Bac
     for (int i = 0; i < M; ++i)
 #pragma unroll
         for (int c = 0; c < M; ++c) bk.Z[i][c] = (double)q[(f++) * WAVE];
+#pragma unroll
+    for (int i = 0; i < M; ++i) bk.h[i] = (double)q[(f++) * WAVE];
 }
 // chunk record of slot `slot` in the storage precision F32 (a template parameter of
 // the solve path, so that each precision has its own register allocation)
