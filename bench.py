@@ -38,11 +38,12 @@ def algorithmic_bytes(M, counters, n0_nodes, nf_nodes, p_j=8):
     24M + 16 (state and mesh) + 2 p_J (2M^2 + M) (the elimination record written
     and read back, p_J bytes per value: 8 FP64, 4 FP32), i.e. 40M + 16 + 4 M^2 p_J
     for FP64 records; per convergence-check pass 16M + 16; per defect interval
-    8M + 24; per transfer node 8(M+1); plus guess in and solution out."""
+    8M + 16 (shat, 8 more bytes, is formed only when a redistribution follows and
+    is not counted); per transfer node 8(M+1); plus guess in and solution out."""
     it, dfct, tr, chk = counters
     b = it * (24 * M + 16 + 2 * p_j * (2 * M * M + M))
     b += chk * (16 * M + 16)
-    b += dfct * (8 * M + 24)
+    b += dfct * (8 * M + 16)
     b += tr * 8 * (M + 1)
     b += 8 * M * n0_nodes + 8 * (M + 1) * nf_nodes
     return b

@@ -1262,11 +1262,13 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
 // forced inline: the specialised kernels fold the family into it
 //
 // stage != null: shat is written through that LDS area (N values) and copied to
-// shat in whole lines at the end.
+// shat in whole lines at the end.  lazy: shat, r1 and r2 are formed only when the
+// defect norm exceeds abstol (only then does mesh selection read them; r1 = r2 =
+// 0 otherwise), from the interval errors kept meanwhile where shat goes.
 template <int M, int METHOD>
 __device__ __forceinline__ double defect_pass(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
                                      double abstol, double *shat, double *errors, double &r1, double &r2,
-                                     double *stage = nullptr) {
+                                     double *stage = nullptr, bool lazy = false) {
     using TB = Tab<METHOD>;
     int P, start, len;
     chunk_of(N, lane, P, start, len);
@@ -1301,18 +1303,35 @@ __device__ __forceinline__ double defect_pass(const Inst<M> &I, const double *me
             ei = fmax(ei, fabs(d));
         }
         dmax = fmax(dmax, ei);
-        const double sh = pow(ei / abstol, ex);
-        if (shat) (stage ? stage : shat)[k] = sh;
-        smax = fmax(smax, sh);
-        ssum += sh;
+        if (lazy) {
+            (stage ? stage : shat)[k] = ei;
+        } else {
+            const double sh = pow(ei / abstol, ex);
+            if (shat) (stage ? stage : shat)[k] = sh;
+            smax = fmax(smax, sh);
+            ssum += sh;
+        }
     });
-    if (shat && stage) {
+    const bool anybad = wave_any(bad);
+    dmax = wave_max(dmax);
+    bool need = shat != nullptr;
+    if (lazy) {
+        need = !anybad && dmax > abstol;
+        if (need) {  // shat from the kept errors, in the same order as above
+            double *e = stage ? stage : shat;
+            for (int k = start; k < start + len; ++k) {
+                const double sh = pow(e[k] / abstol, ex);
+                e[k] = sh;
+                smax = fmax(smax, sh);
+                ssum += sh;
+            }
+        }
+    }
+    if (need && stage) {
         __syncthreads();
         for (int k = lane; k < N; k += WAVE) shat[k] = stage[k];
         __syncthreads();
     }
-    const bool anybad = wave_any(bad);
-    dmax = wave_max(dmax);
     r1 = wave_max(smax);
     r2 = wave_sum(ssum);
     return anybad ? NAN : dmax;
@@ -1588,7 +1607,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         double r1 = 0.0, r2 = 0.0;
         if (info == BVP_RET_SUCCESS) {
             error_norm = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2,
-                                                N <= kStageNodes<M> * M ? sh.stage : nullptr);
+                                                N <= kStageNodes<M> * M ? sh.stage : nullptr, true);
             cnt_def += N;
             last_defect = error_norm;
             info = (error_norm > B.defect_threshold || isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
