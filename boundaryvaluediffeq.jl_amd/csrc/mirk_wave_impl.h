@@ -1510,6 +1510,39 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
             if (B.anc) {
                 const double t0 = B.t0[b], t1 = B.t1[b];
                 const int K = B.anc_K;
+                if (K <= 4) {
+                    // all anchor records of the instance in registers, loaded together
+                    // with the first node times (no load waits on a node time)
+                    double A[4][4];
+                    const double *ab = as_global(B.anc) + (size_t)b * K * 4;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) A[q][r] = q < K ? ab[q * 4 + r] : 0.0;
+                    for (int k0 = lane; k0 <= N; k0 += U * WAVE) {
+                        double t[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int k = k0 + u * WAVE;
+                            t[u] = k <= N ? B.init_mesh[off + k] : t0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int k = k0 + u * WAVE;
+                            if (k <= N) {
+                                const int j = anchor_interval(K, t0, t1, t[u]);  // 0 .. K-2 <= 2
+                                double rec[8];
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    rec[r] = j == 0 ? A[0][r] : (j == 1 ? A[1][r] : A[2][r]);
+                                    rec[4 + r] = j == 0 ? A[1][r] : (j == 1 ? A[2][r] : A[3][r]);
+                                }
+                                if (with_mesh) bf.mesh[0][k] = t[u];
+                                anchor_eval(rec, K, t0, t1, t[u], bf.y[0] + (size_t)k * M);
+                            }
+                        }
+                    }
+                } else
                 for (int k0 = lane; k0 <= N; k0 += U * WAVE) {
                     double t[U], rec[U][8];
 #pragma unroll
