@@ -1466,9 +1466,12 @@ __device__ inline void transfer(const Inst<M> &I, const double *mesh, const doub
 template <int M> struct Bufs {
     double *mesh[2];
     double *y[2];
+    // mesh 0 as read: the buffer, or the packed initial mesh until a mesh is
+    // written into buffer 0 (the solve path does not copy the initial mesh)
+    double *m0;
     // buffer c (0 or 1) chosen with a select, so that the array is never indexed
     // at run time and the pointers keep their global address space
-    __device__ double *mesh_of(int c) const { return c ? mesh[1] : mesh[0]; }
+    __device__ double *mesh_of(int c) const { return c ? mesh[1] : m0; }
     __device__ double *y_of(int c) const { return c ? y[1] : y[0]; }
 };
 
@@ -1480,6 +1483,7 @@ __device__ inline Bufs<M> bufs_of(const BatchDev &B, int b) {
     r.mesh[1] = r.mesh[0] + cap;
     r.y[0] = as_global(B.y + (size_t)b * 2 * cap * M);
     r.y[1] = r.y[0] + cap * M;
+    r.m0 = r.mesh[0];
     return r;
 }
 
@@ -1500,10 +1504,12 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     Bufs<M> bf = bufs_of<M>(B, b);
     int cur = 0;
     int N = B.N0[b];
-    // initial mesh and guess from the packed inputs (or the learned guess from the
-    // anchors); with_mesh false rewrites only the state (the line search's restart)
-    auto initial_state = [&](bool with_mesh) {
-        const long long off = B.init_off[b];
+    // initial guess from the packed inputs (or the learned guess from the anchors);
+    // the initial mesh is read where it was packed
+    const long long off0 = B.init_off[b];
+    bf.m0 = const_cast<double *>(as_global(B.init_mesh + off0));
+    auto initial_state = [&]() {
+        const long long off = off0;
         // U nodes per lane per round: every load of a round is issued before its stores
         constexpr int U = 4;
         if constexpr (M == 2) {
@@ -1537,8 +1543,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                                     rec[r] = j == 0 ? A[0][r] : (j == 1 ? A[1][r] : A[2][r]);
                                     rec[4 + r] = j == 0 ? A[1][r] : (j == 1 ? A[2][r] : A[3][r]);
                                 }
-                                if (with_mesh) bf.mesh[0][k] = t[u];
-                                anchor_eval(rec, K, t0, t1, t[u], bf.y[0] + (size_t)k * M);
+                                                                anchor_eval(rec, K, t0, t1, t[u], bf.y[0] + (size_t)k * M);
                             }
                         }
                     }
@@ -1560,8 +1565,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                     for (int u = 0; u < U; ++u) {
                         const int k = k0 + u * WAVE;
                         if (k <= N) {
-                            if (with_mesh) bf.mesh[0][k] = t[u];
-                            anchor_eval(rec[u], K, t0, t1, t[u], bf.y[0] + (size_t)k * M);
+                                                        anchor_eval(rec[u], K, t0, t1, t[u], bf.y[0] + (size_t)k * M);
                         }
                     }
                 }
@@ -1584,8 +1588,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
                 for (int u = 0; u < U; ++u) {
                     const int k = k0 + u * WAVE;
                     if (k <= N) {
-                        if (with_mesh) bf.mesh[0][k] = t[u];
-#pragma unroll
+                        #pragma unroll
                         for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = v[u][c];
                     }
                 }
@@ -1593,7 +1596,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
             __syncthreads();
         }
     };
-    initial_state(true);
+    initial_state();
     double *shat = as_global(B.shat + (size_t)b * B.cap);
     double *delta = aux;                               // [cap*M]
     double *ytmp = aux + (size_t)B.cap * M;            // [cap*M]
@@ -1610,7 +1613,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     auto restore = [&]() {
         double *yc = bf.y_of(cur);
         if (start_kind == 0) {
-            initial_state(false);
+            initial_state();
         } else if (start_kind == 1) {
             transfer<M, METHOD>(I, bf.mesh_of(1 - cur), bf.y_of(1 - cur), prevN, bf.mesh_of(cur), yc, N,
                                 B.transfer_mode, lane, iiscr);
@@ -1623,7 +1626,8 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     for (;;) {
         ++outer;
         double *mesh = bf.mesh_of(cur), *y = bf.y_of(cur);
-        double *mesh_o = bf.mesh_of(1 - cur), *y_o = bf.y_of(1 - cur);
+        // the other buffer, as written (a new mesh goes to the buffer itself)
+        double *mesh_o = cur ? bf.mesh[0] : bf.mesh[1], *y_o = bf.y_of(1 - cur);
         const double *u0c = (B.warm && outer == 1) ? as_global(B.u0c + (size_t)b * M) : nullptr;
         NewtonOut nl = newton<M, METHOD, F32, FAM>(B, I, sh, mesh, y, restore, delta, ytmp, N, lane, scr, tree_base, u0c);
         iters += nl.iters;
@@ -1648,6 +1652,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         __syncthreads();
         if (info == BVP_RET_SUCCESS) {
             if (error_norm > B.abstol) {
+                if (cur == 1) bf.m0 = bf.mesh[0];  // buffer 0 receives the new mesh
                 const int Nn = mesh_select(mesh, N, shat, r1, r2, B.max_sub, mesh_o, lane, sh);
                 if (Nn < 0) {
                     info = BVP_RET_FAILURE;
@@ -1665,6 +1670,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
             if (2 * N > B.max_sub) {
                 info = BVP_RET_FAILURE;
             } else {
+                if (cur == 1) bf.m0 = bf.mesh[0];  // buffer 0 receives the halved mesh
                 half_mesh_into(mesh, N, mesh_o, lane);
                 for (int k = lane; k <= 2 * N; k += WAVE)
 #pragma unroll
@@ -1688,7 +1694,8 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         B.newton_iters[b] = iters;
         B.outer_iters[b] = outer;
         B.fallback[b] = fb;
-        B.cur_buf[b] = cur;
+        // 2: the final mesh is the packed initial mesh (state in buffer 0)
+        B.cur_buf[b] = (cur == 0 && bf.m0 != bf.mesh[0]) ? 2 : cur;
         B.defect_norm[b] = last_defect;
         B.resid_norm[b] = rn;
         B.cnt[(size_t)b * 4 + 0] = cnt_it;
@@ -1834,8 +1841,12 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev B, const long long *out
     const int b = blockIdx.x;
     if (b >= B.n) return;
     const int N = B.N_final[b];
-    const int cur = B.cur_buf[b];
+    int cur = B.cur_buf[b];
     Bufs<M> bf = bufs_of<M>(B, b);
+    if (cur == 2) {  // the mesh never left the packed inputs
+        bf.m0 = const_cast<double *>(B.init_mesh + B.init_off[b]);
+        cur = 0;
+    }
     const long long o = out_off[b];
     for (int k = threadIdx.x; k <= N; k += blockDim.x) {
         out_mesh[o + k] = bf.mesh_of(cur)[k];
