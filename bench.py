@@ -274,6 +274,30 @@ def bench_singpert(args, _abi, _lib, synthetic, learned):
                                 "final_N_mean": float(sum(r.N for r in rw) / n), "counters": batch.counters()}
         out["training"] = {"solves": len(tspecs), "samples": int(len(x)), "adam_steps": int(len(loss)),
                            "final_loss": float(loss[-1]), "device_ms": ms}
+    # the same batch with FP32 elimination records (FP64 residual, defect and Newton
+    # test): constant guess, then with the network (the combination this workload names)
+    if not args.no_mixed:
+        omx = _abi.default_opts(_abi.MIRK4)
+        omx.abstol = 1e-6
+        omx.max_num_subintervals = 16384
+        omx.precision = _abi.PREC_FP32_REFINE64
+        batch.close()
+        batch = _lib.Batch(ctx, omx, specs)
+        batch.set_graph(not args.no_graph)
+        if not args.no_tune:
+            cand_ms, (variant, blocks) = batch.autotune(repeats=1)
+            log(f"[bench] autotune (mixed): {['%.1f' % x for x in cand_ms]} ms -> variant {variant}")
+        tm = timed_runs(batch, args.steps, args.warmup, "mixed, constant guess")
+        rm = batch.results()
+        out["mixed_constant_guess"] = {"solves_per_s": n_solved(rm, 1e-6) / tm["wall"], **tm, **learned.summary(rm),
+                                       "final_N_mean": float(sum(r.N for r in rm) / n), "counters": batch.counters()}
+        if not args.no_learned:
+            batch.set_warm_start(net, _abi.FEAT_LOG10_PARAM, 0)
+            tl = timed_runs(batch, args.steps, args.warmup, "mixed, learned guess")
+            rl = batch.results()
+            out["mixed_learned_guess"] = {"solves_per_s": n_solved(rl, 1e-6) / tl["wall"], **tl,
+                                          **learned.summary(rl), "final_N_mean": float(sum(r.N for r in rl) / n),
+                                          "counters": batch.counters()}
     print(json.dumps(out), flush=True)
 
 
