@@ -300,6 +300,25 @@ def test_failure_paths(ctx):
     assert g.retcode == r.retcode != 0 and g.N == r.N and g.outer_iters == r.outer_iters
 
 
+def test_linesearch_restarts_match_oracle(ctx):
+    """Newton capped at one step: the plain attempt fails on every mesh, so the
+    line-search attempt restarts -- from the initial guess on the first mesh and
+    from zeros after each halving -- and the final state, mesh and counts match
+    the oracle's, which restarts from a copy of the starting state."""
+    _say("line-search restarts")
+    for lam, N in ((1.0, 8), (2.5, 12)):
+        s = cases.bratu(lam, N=N)
+        o = _abi.default_opts(4)
+        o.abstol = 1e-8
+        o.max_newton_iters = 1
+        o.max_num_subintervals = 64
+        g = _lib.solve_batch(ctx, o, [s])[0]
+        r = O.solve(O.opts(4, abstol=1e-8, max_newton_iters=1, max_num_subintervals=64), s)
+        assert g.retcode == r.retcode and g.N == r.N and g.outer_iters == r.outer_iters
+        assert g.newton_iters == r.newton_iters
+        assert np.max(np.abs(g.u - r.u) / np.maximum(1.0, np.abs(r.u))) <= 1e-9
+
+
 def test_solve_linear_tanh_small_M(ctx):
     _say("solve linear M=3,4")
     rng = np.random.default_rng(8)
