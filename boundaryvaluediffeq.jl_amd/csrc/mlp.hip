@@ -385,11 +385,15 @@ __device__ inline int instance_of_node(const BatchDev &B, long long gidx) {
 // hidden weight matrices in registers (2 x 4 x 8 operands of 8 BF16 per lane)
 // and streams 64-point tiles through two LDS activation buffers, so that the
 // weights are read from memory once per workgroup instead of once per tile.
+// what a source keeps of a point between reading its features and storing its
+// output (the anchor source: the instance's parameters, time and family)
+constexpr int kStash = 8;
 struct InferTile {
     uint16_t h[2][T][HP];
     float w4[H][NOUT];
     float x[T][NIN];
     float out[T][NOUT];
+    double stash[T][kStash];
     int valid[T];
 };
 
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(256) void k_infer(Net net, Src src, long long npts)
             const long long gi = tile * T + tid;
             const bool v = gi < npts;
             float xr[NIN] = {0.f, 0.f, 0.f, 0.f};
-            if (v) src.features(gi, xr);
+            if (v) src.features(gi, xr, &S.stash[tid][0]);
 #pragma unroll
             for (int q = 0; q < NIN; ++q) S.x[tid][q] = v ? (xr[q] - net.in_shift[q]) * net.in_scale[q] : 0.f;
             S.valid[tid] = v;
@@ -489,18 +493,18 @@ __global__ __launch_bounds__(256) void k_infer(Net net, Src src, long long npts)
             }
         }
         __syncthreads();
-        if (tid < T && S.valid[tid]) src.store(tile * T + tid, S.out[tid]);
+        if (tid < T && S.valid[tid]) src.store(tile * T + tid, S.out[tid], &S.stash[tid][0]);
     }
 }
 
 struct PredictSrc {
     const float *x;
     float *y;
-    __device__ void features(long long gi, float *xr) const {
+    __device__ void features(long long gi, float *xr, double *) const {
 #pragma unroll
         for (int q = 0; q < NIN; ++q) xr[q] = x[gi * NIN + q];
     }
-    __device__ void store(long long gi, const float *o) const {
+    __device__ void store(long long gi, const float *o, const double *) const {
         y[gi * NOUT] = o[0];
         y[gi * NOUT + 1] = o[1];
     }
@@ -519,10 +523,10 @@ struct NodeSrc {
     BatchDev B;
     double *out;
     int kind;
-    __device__ void features(long long gi, float *xr) const {
+    __device__ void features(long long gi, float *xr, double *) const {
         batch_features(B, instance_of_node(B, gi), B.init_mesh[gi], kind, xr);
     }
-    __device__ void store(long long gi, const float *o) const {
+    __device__ void store(long long gi, const float *o, const double *) const {
         out[gi * 2] = (double)o[0];
         out[gi * 2 + 1] = (double)o[1];
     }
@@ -534,19 +538,24 @@ struct AnchorSrc {
     double *anc;
     int K;
     int kind;
-    __device__ void features(long long gi, float *xr) const {
+    // the stash keeps the instance's parameters (5), the anchor time and the family
+    __device__ void features(long long gi, float *xr, double *st) const {
         const int b = (int)(gi / K);
-        batch_features(B, b, anchor_t(B, b, (int)(gi % K), K), kind, xr);
-    }
-    __device__ void store(long long gi, const float *o) const {
-        const int b = (int)(gi / K);
-        bvpdev::Params<2> P;
+        const double t = anchor_t(B, b, (int)(gi % K), K);
+        batch_features(B, b, t, kind, xr);
         const int po = B.param_off[b], pn = B.param_cnt[b];
 #pragma unroll
-        for (int k = 0; k < bvpdev::Params<2>::MAXP; ++k) P.p[k] = k < pn ? B.params[po + k] : 0.0;
+        for (int k = 0; k < bvpdev::Params<2>::MAXP; ++k) st[k] = k < pn ? B.params[po + k] : 0.0;
+        st[5] = t;
+        st[6] = (double)B.family[b];
+    }
+    __device__ void store(long long gi, const float *o, const double *st) const {
+        bvpdev::Params<2> P;
+#pragma unroll
+        for (int k = 0; k < bvpdev::Params<2>::MAXP; ++k) P.p[k] = st[k];
         const double y[2] = {(double)o[0], (double)o[1]};
         double f[2];
-        bvpdev::f_eval<2, double>(B.family[b], P, anchor_t(B, b, (int)(gi % K), K), y, f);
+        bvpdev::f_eval<2, double>((int)st[6], P, st[5], y, f);
         anc[gi * 4] = y[0];
         anc[gi * 4 + 1] = y[1];
         anc[gi * 4 + 2] = f[0];
