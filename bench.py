@@ -286,7 +286,10 @@ def bench_singpert(args, _abi, _lib, synthetic, learned):
         batch.set_graph(not args.no_graph)
         if not args.no_tune:
             cand_ms, (variant, blocks) = batch.autotune(repeats=1)
-            log(f"[bench] autotune (mixed): {['%.1f' % x for x in cand_ms]} ms -> variant {variant}")
+            out["autotune_mixed"] = {"candidates_ms": cand_ms, "variant": variant,
+                                     "records": "fp32" if batch.precision() == _abi.PREC_FP32_REFINE64 else "fp64"}
+            log(f"[bench] autotune (mixed): {['%.1f' % x for x in cand_ms]} ms -> variant {variant}, "
+                f"{out['autotune_mixed']['records']} records")
         tm = timed_runs(batch, args.steps, args.warmup, "mixed, constant guess")
         rm = batch.results()
         out["mixed_constant_guess"] = {"solves_per_s": n_solved(rm, 1e-6) / tm["wall"], **tm, **learned.summary(rm),
@@ -418,8 +421,11 @@ def main():
         batch.set_graph(not args.no_graph)
         if not args.no_tune:
             cand_ms, (variant, blocks) = batch.autotune(repeats=3)
-            tune_mx = {"candidates_ms": cand_ms, "variant": variant, "persistent_waves": blocks}
-            log(f"[bench] autotune (mixed): candidates {['%.3f' % x for x in cand_ms]} ms -> variant {variant}")
+            tune_mx = {"candidates_ms": cand_ms, "variant": variant, "persistent_waves": blocks,
+                       "candidates": "variants 0-2 with FP32 records, then 0-2 with FP64 records",
+                       "records": "fp32" if batch.precision() == _abi.PREC_FP32_REFINE64 else "fp64"}
+            log(f"[bench] autotune (mixed): candidates {['%.3f' % x for x in cand_ms]} ms -> variant {variant}, "
+                f"{tune_mx['records']} records")
         if not args.no_learned:
             batch.set_warm_start(net, _abi.FEAT_PARAM, args.anchors)
         tag = "batched mixed, " + ("constant" if args.no_learned else "learned") + " guess"
@@ -463,8 +469,10 @@ def main():
                                f"(BF16 MLP at {args.anchors} anchors per instance + Hermite, trained untimed on plain "
                                "solves of the Bratu (a, b) family) with fallback to the zero guess",
                    "instances": n, "method": "MIRK4",
-                   "precision": "fp64 factors" if args.no_mixed else
-                   "mixed: chunk factors stored in FP32, FP64 elimination, residual and Newton test",
+                   "precision": "fp64 records" if args.no_mixed else
+                   ("mixed: chunk elimination records stored in FP32, FP64 elimination, residual and Newton test"
+                    if batch.precision() == _abi.PREC_FP32_REFINE64 else
+                    "mixed requested, FP64 records chosen by the autotuner"),
                    "variant": "batched persistent, hipGraph replay" + ("" if args.no_learned else
                                                                        " + learned warm start") +
                    ("" if args.no_mixed else " + mixed precision"),

@@ -60,6 +60,7 @@ def lib():
         "bvp_batch_learned_guess": (C.c_int, [vp, vp, _dp]),
         "bvp_batch_autotune": (C.c_int, [vp, vp, C.c_int32, _fp, _ip]),
         "bvp_batch_config": (C.c_int, [vp, _ip, _ip]),
+        "bvp_batch_precision": (C.c_int, [vp, _ip]),
         "bvp_batch_set_config": (C.c_int, [vp, vp, C.c_int32, C.c_int32]),
         "bvp_mlp_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
         "bvp_mlp_destroy": (None, [vp]),
@@ -88,7 +89,7 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_mlp_create", "bvp_mlp_destroy", "bvp_mlp_set_normalization", "bvp_mlp_get_params",
             "bvp_mlp_set_params", "bvp_mlp_gradient", "bvp_mlp_predict", "bvp_mlp_train",
             "bvp_batch_set_warm_start", "bvp_batch_timing", "bvp_batch_learned_guess", "bvp_batch_autotune",
-            "bvp_batch_config", "bvp_batch_set_config"]
+            "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision"]
 
 
 class Context:
@@ -313,6 +314,12 @@ class Batch:
         v, b = C.c_int32(), C.c_int32()
         lib().bvp_batch_config(self._h, C.byref(v), C.byref(b))
         return int(v.value), int(b.value)
+
+    def precision(self):
+        """The record / factor precision the plan runs with (BVP_PREC_*)."""
+        p = C.c_int32()
+        lib().bvp_batch_precision(self._h, C.byref(p))
+        return int(p.value)
 
     def timing(self):
         """(guess ms, solve ms) of the last synced run."""

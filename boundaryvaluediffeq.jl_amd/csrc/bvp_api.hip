@@ -264,8 +264,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         if (inst[i].family != ufam || inst[i].bc_kind != BVP_BC_TWO_POINT) ufam = -1;
     B->family = ufam;
     const auto key = std::make_tuple(M, o->method, o->precision, blk ? 1 : 0, ufam);
-    int variant = 0, per_cu = 0;
-    if (ctx->tuned.count(key)) std::tie(variant, per_cu) = ctx->tuned[key];
+    int variant = 0, per_cu = 0, tuned_f32 = f32;
+    if (ctx->tuned.count(key)) std::tie(variant, per_cu, tuned_f32) = ctx->tuned[key];
     if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
     if (per_cu < 1)
         per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant, ufam, f32);
@@ -308,7 +308,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->max_slots = slots;
     B->blocks = std::min(slots, std::max(1, std::min(n, ctx->num_cus * per_cu)));
     D.refine_iters = o->refine_iters;
-    D.f32rec = f32;
+    D.f32rec = f32 && tuned_f32;  // the autotuner may have found FP64 records faster for this kind
     if ((rc = dalloc(ctx, A, D.scratch, scratch_doubles)) ||
         (rc = dalloc(ctx, A, D.aux, D.aux_per_slot * slots))) {
         delete B;
@@ -760,24 +760,30 @@ int bvp_batch_learned_guess(bvp_ctx *ctx, bvp_batch *B, double *y) {
 int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_ms, int32_t *n_cand) {
     if (!ctx || !B || repeats < 1) return BVP_EINVAL;
     (void)hipSetDevice(ctx->device);
-    struct Cand { int variant, blocks; };
+    struct Cand { int variant, blocks, f32; };
     std::vector<Cand> cands;
     if (B->block) {
         const int occ = std::max(1, block_blocks_per_cu(B->M, B->method, B->opts.precision));
-        for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc)});
+        for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc), 0});
         if (cands.back().blocks != std::min(B->max_slots, ctx->num_cus * occ))
-            cands.push_back({0, std::min(B->max_slots, ctx->num_cus * occ)});
+            cands.push_back({0, std::min(B->max_slots, ctx->num_cus * occ), 0});
     } else {
-        for (int v = 0; v < wave_variants(B->M); ++v)
-            cands.push_back({v, std::min(B->max_slots,
-                                         ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v, B->family, B->dev.f32rec)))});
+        // FP32_REFINE64 on the wave kernels: FP32 records first, then FP64 records (always
+        // at least as accurate; faster where the inexact corrections cost Newton steps)
+        const bool mixed = B->opts.precision == BVP_PREC_FP32_REFINE64;
+        for (int f = mixed ? 1 : 0; f >= 0; --f)
+            for (int v = 0; v < wave_variants(B->M); ++v)
+                cands.push_back({v, std::min(B->max_slots,
+                                             ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v, B->family, f))),
+                                 f});
     }
-    const int keep_v = B->variant, keep_b = B->blocks;
+    const int keep_v = B->variant, keep_b = B->blocks, keep_f = B->dev.f32rec;
     int best = -1;
     float best_ms = 1e30f;
     for (size_t c = 0; c < cands.size(); ++c) {
         B->variant = cands[c].variant;
         B->blocks = std::max(1, cands[c].blocks);
+        if (!B->block) B->dev.f32rec = cands[c].f32;
         B->drop_graph();
         std::vector<float> t;
         for (int r = 0; r <= repeats; ++r) {  // first run untimed
@@ -786,6 +792,7 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
             if (rc) {
                 B->variant = keep_v;
                 B->blocks = keep_b;
+                B->dev.f32rec = keep_f;
                 B->drop_graph();
                 return rc;
             }
@@ -802,9 +809,17 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     if (n_cand) *n_cand = (int32_t)cands.size();
     B->variant = cands[best].variant;
     B->blocks = std::max(1, cands[best].blocks);
+    if (!B->block) B->dev.f32rec = cands[best].f32;
     B->drop_graph();
     const int per_cu = (B->blocks + ctx->num_cus - 1) / ctx->num_cus;
-    ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision, B->block ? 1 : 0, B->family)] = {B->variant, per_cu};
+    ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision, B->block ? 1 : 0, B->family)] =
+        std::make_tuple(B->variant, per_cu, (int)B->dev.f32rec);
+    return 0;
+}
+
+int bvp_batch_precision(bvp_batch *B, int32_t *precision) {
+    if (!B || !precision) return BVP_EINVAL;
+    *precision = B->block ? B->opts.precision : (B->dev.f32rec ? BVP_PREC_FP32_REFINE64 : BVP_PREC_FP64);
     return 0;
 }
 

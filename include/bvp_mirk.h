@@ -247,10 +247,17 @@ int  bvp_batch_set_graph(bvp_batch *batch, int32_t enable);
  * trading spills for latency hiding) at their full residency, or, for the block
  * kernels, 1, 2, 4 ... waves per CU -- `repeats` timed runs each (median), keeps
  * the fastest for the plan and remembers it in the context for later plans of
- * the same kind (M, method, precision).  cand_ms (optional, room for 16)
- * receives the median of every candidate, *n_cand their number.  The runs
- * overwrite the plan's results. */
+ * the same kind (M, method, precision).  A wave-kernel plan (M <= 4) built with
+ * BVP_PREC_FP32_REFINE64 also times its variants with FP64 records (listed
+ * after the FP32 ones) and keeps FP64 where that is faster: on stiff families
+ * the FP32-record corrections cost extra Newton steps.  cand_ms (optional,
+ * room for 16) receives the median of every candidate, *n_cand their number.
+ * The runs overwrite the plan's results. */
 int  bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *batch, int32_t repeats, float *cand_ms, int32_t *n_cand);
+/* The record / factor precision the plan runs with (BVP_PREC_*): the requested
+ * one, or BVP_PREC_FP64 where the autotuner chose FP64 records (no counterpart
+ * in the reference). */
+int  bvp_batch_precision(bvp_batch *batch, int32_t *precision);
 /* The plan's current launch configuration: kernel variant and persistent waves;
  * and setting it by hand (blocks at most the waves the plan was sized for). */
 int  bvp_batch_config(bvp_batch *batch, int32_t *variant, int32_t *blocks);

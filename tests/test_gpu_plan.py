@@ -96,3 +96,35 @@ def test_block_plan_autotune(ctx):
     b.run()
     b.sync()
     _same(ref, b.results())
+
+
+def test_autotune_chooses_record_precision(ctx):
+    """A mixed-precision wave plan is also timed with FP64 records; the choice is
+    the fastest candidate, and the plan then computes the same bits as a plan
+    built for that precision with the same launch configuration."""
+    _say("autotune precision")
+    specs, _ = synthetic.singpert_batch(1024)
+    o = _abi.default_opts(_abi.MIRK4)
+    o.abstol = 1e-6
+    o.max_num_subintervals = 16384
+    o.precision = _abi.PREC_FP32_REFINE64
+    b = _lib.Batch(ctx, o, specs)
+    ms, (variant, blocks) = b.autotune(repeats=1)
+    _say("  candidates ms", ms, "chosen", variant, blocks, "precision", b.precision())
+    assert len(ms) == 6  # variants 0..2 with FP32 records, then with FP64 records
+    best = int(np.argmin(ms))
+    assert variant == best % 3
+    assert b.precision() == (_abi.PREC_FP32_REFINE64 if best < 3 else _abi.PREC_FP64)
+    b.run()
+    b.sync()
+    got = b.results()
+    o2 = _abi.default_opts(_abi.MIRK4)
+    o2.abstol = 1e-6
+    o2.max_num_subintervals = 16384
+    o2.precision = b.precision()
+    ref_ctx = _lib.Context(0)  # a fresh context: no tuned entry for this kind
+    b2 = _lib.Batch(ref_ctx, o2, specs)
+    b2.set_config(variant, blocks)
+    b2.run()
+    b2.sync()
+    _same(got, b2.results())
