@@ -1468,10 +1468,11 @@ template <int M> struct Bufs {
     double *y[2];
     // mesh 0 as read: the buffer, or the packed initial mesh until a mesh is
     // written into buffer 0 (the solve path does not copy the initial mesh)
-    double *m0;
+    const double *m0;
     // buffer c (0 or 1) chosen with a select, so that the array is never indexed
-    // at run time and the pointers keep their global address space
-    __device__ double *mesh_of(int c) const { return c ? mesh[1] : m0; }
+    // at run time and the pointers keep their global address space; meshes are
+    // written through mesh[] only
+    __device__ const double *mesh_of(int c) const { return c ? mesh[1] : m0; }
     __device__ double *y_of(int c) const { return c ? y[1] : y[0]; }
 };
 
@@ -1507,7 +1508,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     // initial guess from the packed inputs (or the learned guess from the anchors);
     // the initial mesh is read where it was packed
     const long long off0 = B.init_off[b];
-    bf.m0 = const_cast<double *>(as_global(B.init_mesh + off0));
+    bf.m0 = as_global(B.init_mesh + off0);
     auto initial_state = [&]() {
         const long long off = off0;
         // U nodes per lane per round: every load of a round is issued before its stores
@@ -1625,7 +1626,8 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     };
     for (;;) {
         ++outer;
-        double *mesh = bf.mesh_of(cur), *y = bf.y_of(cur);
+        const double *mesh = bf.mesh_of(cur);
+        double *y = bf.y_of(cur);
         // the other buffer, as written (a new mesh goes to the buffer itself)
         double *mesh_o = cur ? bf.mesh[0] : bf.mesh[1], *y_o = bf.y_of(1 - cur);
         const double *u0c = (B.warm && outer == 1) ? as_global(B.u0c + (size_t)b * M) : nullptr;
@@ -1844,7 +1846,7 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev B, const long long *out
     int cur = B.cur_buf[b];
     Bufs<M> bf = bufs_of<M>(B, b);
     if (cur == 2) {  // the mesh never left the packed inputs
-        bf.m0 = const_cast<double *>(B.init_mesh + B.init_off[b]);
+        bf.m0 = B.init_mesh + B.init_off[b];
         cur = 0;
     }
     const long long o = out_off[b];
