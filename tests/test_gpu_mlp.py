@@ -223,3 +223,46 @@ def test_learned_warm_start_saves_newton_iterations(ctx, bratu_net, anchors):
         assert w.defect_norm <= opts.abstol
         checked += 1
     assert checked >= 20
+
+
+def test_learned_mixed_graph_plan(ctx, bratu_net):
+    """The bench's configuration: learned guess (K = 4 anchors), FP32 elimination
+    records, graph replay, autotuned launch.  Against the plain FP64 plan on the
+    same unseen instances: as many solved, fewer Newton iterations, replays give
+    the same bits, and every success passes the oracle's Newton-correction test on
+    its mesh (max|F| <= abstol = 1e-8 with norm(J^-1) ~ 1e2: corrections <= 1e-6)."""
+    _say("learned + mixed + graph plan")
+    eval_specs, _, _ = synthetic.bratu_ab_batch(512, seed=synthetic.SEED_BRATU_AB_EVAL)
+    opts = _plain_opts()
+    b = _lib.Batch(ctx, opts, eval_specs)
+    b.run()
+    b.sync()
+    plain = b.results()
+    om = _plain_opts()
+    om.precision = _abi.PREC_FP32_REFINE64
+    bm = _lib.Batch(ctx, om, eval_specs)
+    bm.set_graph(True)
+    bm.set_warm_start(bratu_net, _abi.FEAT_PARAM, 4)
+    ms, (variant, blocks) = bm.autotune(repeats=1)
+    _say("  candidates", ms, "chosen", variant, "precision", bm.precision())
+    bm.run()
+    bm.sync()
+    first = bm.results()
+    bm.run()
+    bm.sync()
+    again = bm.results()
+    for a, c in zip(first, again):
+        assert a.retcode == c.retcode and a.N == c.N and np.array_equal(a.u, c.u, equal_nan=True)
+    sp, sm = learned.summary(plain), learned.summary(first)
+    _say("  plain", sp, "\n  learned+mixed", sm)
+    assert sm["solved"] >= sp["solved"] - 2
+    assert sm["mean_newton_iters_solved"] < 0.85 * sp["mean_newton_iters_solved"]
+    checked = 0
+    for s, w in zip(eval_specs, first):
+        if w.retcode != 0 or checked >= 40:
+            continue
+        d = O.newton_step(4, s.with_state(w.t, w.u.reshape(-1)))
+        assert np.max(np.abs(d) / np.maximum(1.0, np.abs(w.u.reshape(-1)))) <= 1e-6
+        assert w.defect_norm <= opts.abstol
+        checked += 1
+    assert checked >= 20
