@@ -307,12 +307,27 @@ __global__ __launch_bounds__(256) void k_grad_tile(Net net, const float *x, cons
 }
 
 // sum of the per-tile slots, in parameter order: grad[0..NPARAM) and the loss in grad[NPARAM]
+// Sum of value j over the per-tile gradient slots, in a fixed order (the same bits
+// on every run): eight independent partial sums, so that eight loads of a thread
+// are in flight at a time, combined pairwise.
+__device__ inline float sum_slots(const float *partial, int tiles, int j) {
+    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int t = 0;
+    for (; t + 8 <= tiles; t += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partial[(size_t)(t + u) * NSLOT + j];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += v[u];
+    }
+    for (; t < tiles; ++t) a[0] += partial[(size_t)t * NSLOT + j];
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 __global__ void k_sum_slots(const float *partial, int tiles, float *grad) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > NPARAM) return;
-    float s = 0.f;
-    for (int t = 0; t < tiles; ++t) s += partial[(size_t)t * NSLOT + j];
-    grad[j == NPARAM ? j : slot_param(j)] = s;
+    grad[j == NPARAM ? j : slot_param(j)] = sum_slots(partial, tiles, j);
 }
 
 __device__ inline void requant(const NetBufs &nb, int j, float w) {
@@ -340,8 +355,7 @@ __global__ void k_adam(NetBufs nb, TrainDev td) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > NPARAM) return;
     const int step = td.step[0];
-    float g = 0.f;
-    for (int t = 0; t < td.batch_tiles; ++t) g += td.partial[(size_t)t * NSLOT + j];
+    const float g = sum_slots(td.partial, td.batch_tiles, j);
     if (j == NPARAM) {
         td.loss[step - td.step0] = g;
         return;
