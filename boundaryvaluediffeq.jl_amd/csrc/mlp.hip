@@ -402,13 +402,15 @@ __device__ inline int instance_of_node(const BatchDev &B, long long gidx) {
 // what a source keeps of a point between reading its features and storing its
 // output (the anchor source: the instance's parameters, time and family)
 constexpr int kStash = 8;
+// the features of kAhead tiles are read together, by all 256 threads
+constexpr int kAhead = 4;
 struct InferTile {
     uint16_t h[2][T][HP];
     float w4[H][NOUT];
-    float x[T][NIN];
+    float x[kAhead][T][NIN];
     float out[T][NOUT];
-    double stash[T][kStash];
-    int valid[T];
+    double stash[kAhead][T][kStash];
+    int valid[kAhead][T];
 };
 
 __device__ inline void hidden_reg(const uint16_t *A, uint16_t *Hout, const bf8 (&w)[4][8], const float (&bias)[4],
@@ -465,22 +467,28 @@ __global__ __launch_bounds__(256) void k_infer(Net net, Src src, long long npts)
     S.w4[tid][1] = net.q[OW4 + 2 * tid + 1];
     const float b40 = net.q[OB4], b41 = net.q[OB4 + 1];
     const long long ntiles = (npts + T - 1) / T;
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        if (tid < T) {
-            const long long gi = tile * T + tid;
-            const bool v = gi < npts;
+    static_assert(kAhead * T == 256, "one feature point per thread");
+    for (long long tile0 = blockIdx.x; tile0 < ntiles; tile0 += (long long)kAhead * gridDim.x) {
+        {  // features of this workgroup's next kAhead tiles (tile0 + qa * gridDim.x)
+            const int qa = tid / T, pa = tid % T;
+            const long long tile = tile0 + (long long)qa * gridDim.x;
+            const long long gi = tile * T + pa;
+            const bool v = tile < ntiles && gi < npts;
             float xr[NIN] = {0.f, 0.f, 0.f, 0.f};
-            if (v) src.features(gi, xr, &S.stash[tid][0]);
+            if (v) src.features(gi, xr, &S.stash[qa][pa][0]);
 #pragma unroll
-            for (int q = 0; q < NIN; ++q) S.x[tid][q] = v ? (xr[q] - net.in_shift[q]) * net.in_scale[q] : 0.f;
-            S.valid[tid] = v;
+            for (int q = 0; q < NIN; ++q) S.x[qa][pa][q] = v ? (xr[q] - net.in_shift[q]) * net.in_scale[q] : 0.f;
+            S.valid[qa][pa] = v;
         }
         __syncthreads();
+        for (int qa = 0; qa < kAhead; ++qa) {
+        const long long tile = tile0 + (long long)qa * gridDim.x;
+        if (tile >= ntiles) break;
 #pragma unroll 8
         for (int p = 0; p < T; ++p) {
             float z = b1;
 #pragma unroll
-            for (int q = 0; q < NIN; ++q) z += S.x[p][q] * w1[q];
+            for (int q = 0; q < NIN; ++q) z += S.x[qa][p][q] * w1[q];
             S.h[0][p][tid] = f2bf(tanh_f(z));
         }
         __syncthreads();
@@ -507,7 +515,9 @@ __global__ __launch_bounds__(256) void k_infer(Net net, Src src, long long npts)
             }
         }
         __syncthreads();
-        if (tid < T && S.valid[tid]) src.store(tile * T + tid, S.out[tid], &S.stash[tid][0]);
+        if (tid < T && S.valid[qa][tid]) src.store(tile * T + tid, S.out[tid], &S.stash[qa][tid][0]);
+        }
+        __syncthreads();  // the next feature round overwrites x, stash and valid
     }
 }
 
