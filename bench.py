@@ -27,6 +27,10 @@ from __graft_entry__ import load_package  # noqa: E402
 
 METRIC = "BVP solves/s at defect≤1e-6: batched+learned+mixed vs plain FP64 GPU MIRK"
 HBM_PEAK_GBS = 8000.0
+# Newton tolerance (max|F|) of every leg of the Bratu workloads, GPU and oracle alike:
+# tighter than the reference's default (abstol, MIRK/mirk.jl:51) so that two solves of
+# the same instance agree to 1e-9 relative at every mesh point
+NEWTON_ABSTOL = 1e-11
 
 
 def log(*a):
@@ -190,6 +194,16 @@ def plain_single(ctx, opts, specs, sample, abstol, _lib):
             "sample": f"{len(idx)} instances (every {stride}th), each solved alone in its own launch"}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(specs, abstol, threads, method=4, **kw):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
@@ -200,9 +214,77 @@ def cpu_baseline(specs, abstol, threads, method=4, **kw):
     tcpu = time.perf_counter() - tc
     okc = sum(1 for r in rets if r == 0)
     log(f"[bench] oracle: {okc / tcpu:.1f} solves/s")
-    return {"value": okc / tcpu, "unit": "solves/s", "cores": threads, "kind": "port",
+    return {"value": okc / tcpu, "unit": "solves/s", "cores": threads, "cpu_model": cpu_model(), "kind": "port",
             "sample": f"{len(specs)} of the same instances, one per OpenMP thread (dynamic schedule), "
                       f"{tcpu:.1f} s wall"}
+
+
+def error_vs_oracle(specs, gpu_res, abstol, threads, method=4, chunk=2048, **kw):
+    """Untimed: the oracle's solutions of the same instances (same tolerances), compared
+    with the GPU's at every mesh point, relative to max(1, |u|).  Where the final meshes
+    differ, the distance is the oracle's own Newton correction at the GPU's state on the
+    GPU's mesh (the distance to the discrete solution there)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import pyoracle as O
+    oo = O.opts(method, abstol=abstol, **kw)
+    emax, same, other, retc_diff = 0.0, 0, 0, 0
+    for c0 in range(0, len(specs), chunk):
+        part = specs[c0:c0 + chunk]
+        rs = O.solve_batch(oo, part, nthreads=threads)
+        for s, r, g in zip(part, rs, gpu_res[c0:c0 + chunk]):
+            if r.retcode != g.retcode:
+                retc_diff += 1
+            if r.retcode != 0 or g.retcode != 0:
+                continue
+            if r.N == g.N and np.max(np.abs(r.t - g.t)) <= 1e-12:
+                e = float(np.max(np.abs(g.u - r.u) / np.maximum(1.0, np.abs(r.u))))
+                same += 1
+            else:
+                d = O.newton_step(method, s.with_state(g.t, g.u.reshape(-1))).reshape(-1, s.M)
+                e = float(np.max(np.abs(d) / np.maximum(1.0, np.abs(g.u))))
+                other += 1
+            emax = max(emax, e)
+    return {"max_rel_err_vs_oracle": emax, "compared_same_mesh": same, "compared_on_gpu_mesh": other,
+            "retcode_mismatches": retc_diff, "tolerance": 1e-9, "within_tolerance": emax <= 1e-9}
+
+
+def pendulum_config1(ctx, _abi, _lib, runs=200):
+    """The README pendulum (MIRK4, dt = 0.05, abstol 1e-6), one instance: GPU latency per
+    solve (plan replay, HIP events and wall clock, median of `runs`) and the oracle's
+    single-thread median over `runs` solves."""
+    import statistics
+    import math
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    bvp = load_package()
+    prob = bvp.BVProblem(bvp.PendulumFunction(), [(math.pi / 4, 0, -math.pi / 2), (math.pi / 2, 0, math.pi / 2)],
+                         [math.pi / 2, math.pi / 2], (0.0, math.pi / 2), [9.81])
+    spec = bvp.extract(prob, dt=0.05)
+    o = _abi.default_opts(_abi.MIRK4)
+    b = _lib.Batch(ctx, o, [spec])
+    b.run()
+    b.sync()
+    dev, wall = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        b.run()
+        dev.append(b.sync())
+        wall.append(time.perf_counter() - t0)
+    r = b.results()[0]
+    b.close()
+    oo = O.opts(4)
+    O.solve(oo, spec)
+    cpu = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        rc = O.solve(oo, spec)
+        cpu.append(time.perf_counter() - t0)
+    return {"workload": "README pendulum, MIRK4, dt=0.05 (N=32), abstol 1e-6, single instance",
+            "gpu_us_per_solve_device": statistics.median(dev) * 1e3,
+            "gpu_us_per_solve_wall": statistics.median(wall) * 1e6, "gpu_retcode": r.retcode, "gpu_N": r.N,
+            "cpu_oracle_us_per_solve_1thread": statistics.median(cpu) * 1e6, "cpu_retcode": rc.retcode,
+            "runs": runs}
 
 
 def bench_learned(args, _abi, _lib, synthetic, learned):
@@ -331,6 +413,10 @@ def main():
                     help="bratu: 16,384 Bratu BVPs (MIRK4, FP64, learned warm start); lintanh: 2,048 n=32 "
                          "linear+tanh BVPs (MIRK6, FP32 factors + FP64 refinement); learned: warm start on "
                          "16,384 unseen Bratu (a, b) BVPs")
+    ap.add_argument("--newton-abstol", type=float, default=NEWTON_ABSTOL,
+                    help="Newton tolerance max|F| of every leg (GPU and oracle alike); 0: abstol, the reference's "
+                         "default")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the untimed accuracy legs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -364,6 +450,7 @@ def main():
     workload = f"bratu{n // 1024}k"
     opts = _abi.default_opts(_abi.MIRK4)
     opts.abstol = abstol
+    opts.newton_abstol = args.newton_abstol
     ctx = _lib.context(0)
     log(f"[bench] uploading {n} instances (sum N = {int(Ns.sum())})")
     batch = _lib.Batch(ctx, opts, specs)
@@ -415,6 +502,7 @@ def main():
     if not args.no_mixed:
         omx = _abi.default_opts(_abi.MIRK4)
         omx.abstol = abstol
+        omx.newton_abstol = args.newton_abstol
         omx.precision = _abi.PREC_FP32_REFINE64
         batch.close()
         batch = _lib.Batch(ctx, omx, specs)
@@ -443,13 +531,46 @@ def main():
         f"{abytes / 1e9:.3f} GB -> {achieved:.1f} GB/s; counters {cnt_b}; newton iters/instance "
         f"{sa['mean_newton_iters']:.2f} -> {sb['mean_newton_iters']:.2f}")
 
+    # untimed accuracy of the headline results: the same plan configuration with
+    # report_err (one more FP64 Newton correction per instance; the solution bits are
+    # unchanged, which is checked), then the oracle's solutions of the same instances
+    accuracy = None
+    if not args.no_accuracy:
+        import numpy as np
+        oe = _abi.Opts.from_buffer_copy(batch.opts)
+        oe.report_err = 1
+        be = _lib.Batch(ctx, oe, specs)
+        be.set_config(*batch.config())
+        if not args.no_learned:
+            be.set_warm_start(net, _abi.FEAT_PARAM, args.anchors)
+        be.run()
+        be.sync()
+        res_e = be.results()
+        be.close()
+        ok_e = [r.err_vs_fp64 for r in res_e if r.retcode == 0]
+        accuracy = {"newton_abstol": args.newton_abstol or abstol,
+                    "same_bits_as_timed_run": all(np.array_equal(a.u, c.u) for a, c in zip(res_b, res_e)),
+                    "err_vs_fp64_max": float(max(ok_e)) if ok_e else None,
+                    "err_vs_fp64_p99": float(np.percentile(ok_e, 99)) if ok_e else None,
+                    "err_vs_fp64": "max_i |d_i|/max(1,|u_i|) of the FP64 Newton correction d at each returned "
+                                   "solution, on the device"}
+        log(f"[bench] accuracy: err_vs_fp64 max {accuracy['err_vs_fp64_max']:.3e}, same bits "
+            f"{accuracy['same_bits_as_timed_run']}")
+
     plain = plain_single(ctx, opts, specs, args.plain_sample, abstol, _lib) if args.plain_sample > 0 else None
     if plain:
         plain["speedup_of_value"] = value / plain["value"]
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-        cpu = cpu_baseline(specs, abstol, threads)
+        cpu = cpu_baseline(specs, abstol, threads, newton_abstol=args.newton_abstol)
+        if accuracy is not None:
+            accuracy.update(error_vs_oracle(specs, res_b, abstol, threads, newton_abstol=args.newton_abstol))
+            log(f"[bench] accuracy vs oracle: {accuracy}")
+    pend = None
+    if not args.no_accuracy:
+        pend = pendulum_config1(ctx, _abi, _lib)
+        log(f"[bench] config 1: {pend}")
 
     line = {
         "metric": METRIC,
@@ -462,8 +583,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.no_mixed or batch.precision() != _abi.PREC_FP32_REFINE64 else "f64+f32-records",
         "data": "synthetic",
+        "timed_region": "device solve of the resident batch (learned guess + persistent solve); H2D upload of "
+                        "the batch and D2H of the results are outside it",
         "config": {"workload": f"{workload}: {n} Bratu BVPs u''+lambda e^u=0, lambda~U(0.1,3.4), "
                                "N~U{64..1024}, MIRK4, abstol 1e-8, adaptive; learned warm start "
                                f"(BF16 MLP at {args.anchors} anchors per instance + Hermite, trained untimed on plain "
@@ -489,6 +612,8 @@ def main():
                      "kernel_ms": tb["solve_ms"], "algorithmic_bytes": abytes},
         "cpu_baseline": cpu,
         "plain_gpu": plain,
+        "accuracy": accuracy,
+        "config1_pendulum": pend,
     }
     print(json.dumps(line), flush=True)
 

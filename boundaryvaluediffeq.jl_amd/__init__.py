@@ -9,12 +9,14 @@ from .problems import (BVPFunction, BVProblem, TwoPointBVProblem, DiffEqArray, I
                        LinearTanhFunction, PendulumFunction, BratuFunction, SingPertFunction,
                        ScalarTUFunction, TurningFunction, extract, uniform_mesh, cld, instance_array)
 from .solve import (MIRK4, MIRK6, DefectControl, EnsembleProblem, EnsembleSolution, BVPSolution, solve,
-                    successful_retcode, make_opts)
+                    successful_retcode, make_opts, NewtonRaphson, BackTracking, TrustRegion,
+                    NonlinearSolvePolyAlgorithm, nlsolve_stages)
 
 __all__ = [
     "BVPFunction", "BVProblem", "TwoPointBVProblem", "DiffEqArray", "InstanceSpec",
     "LinearTanhFunction", "PendulumFunction", "BratuFunction", "SingPertFunction",
     "ScalarTUFunction", "TurningFunction", "extract", "uniform_mesh", "cld", "instance_array",
     "MIRK4", "MIRK6", "DefectControl", "EnsembleProblem", "EnsembleSolution", "BVPSolution", "solve",
-    "successful_retcode", "make_opts",
+    "successful_retcode", "make_opts", "NewtonRaphson", "BackTracking", "TrustRegion",
+    "NonlinearSolvePolyAlgorithm", "nlsolve_stages",
 ]

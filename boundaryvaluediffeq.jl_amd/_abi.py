@@ -26,6 +26,11 @@ RETCODE_NAMES = {0: "Success", 1: "Failure", 2: "MaxIters", 3: "Unstable"}
 PREC_FP64 = 0
 PREC_FP32_REFINE64 = 1
 
+STAGE_NEWTON = 1
+STAGE_BACKTRACKING = 2
+STAGE_TRUST_REGION = 4
+STAGES_DEFAULT = 7
+
 TRANSFER_REFERENCE = 0
 TRANSFER_CLEAN = 1
 
@@ -75,11 +80,12 @@ class Opts(C.Structure):
         ("defect_threshold", C.c_double),
         ("max_num_subintervals", C.c_int32),
         ("max_newton_iters", C.c_int32),
-        ("linesearch_fallback", C.c_int32),
+        ("nlsolve_stages", C.c_int32),
         ("transfer_mode", C.c_int32),
         ("precision", C.c_int32),
         ("refine_iters", C.c_int32),
         ("warm_start", C.c_int32),
+        ("report_err", C.c_int32),
     ]
 
 
@@ -93,6 +99,7 @@ class Result(C.Structure):
         ("capacity", C.c_int32),
         ("defect_norm", C.c_double),
         ("resid_norm", C.c_double),
+        ("err_vs_fp64", C.c_double),
         ("mesh", _dp),
         ("u", _dp),
     ]
@@ -124,11 +131,12 @@ def default_opts(method=MIRK4):
     o.defect_threshold = 0.1
     o.max_num_subintervals = 3000
     o.max_newton_iters = 1000
-    o.linesearch_fallback = 1
+    o.nlsolve_stages = STAGES_DEFAULT
     o.transfer_mode = TRANSFER_REFERENCE
     o.precision = PREC_FP64
     o.refine_iters = 0
     o.warm_start = WARM_NONE
+    o.report_err = 0
     return o
 
 

@@ -234,6 +234,7 @@ class RawSolution:
         self.fallback_used = int(r.fallback_used)
         self.defect_norm = float(r.defect_norm)
         self.resid_norm = float(r.resid_norm)
+        self.err_vs_fp64 = float(r.err_vs_fp64)
         self.t = mesh[:self.N + 1].copy()
         self.u = u[:M * (self.N + 1)].reshape(self.N + 1, M).copy()
 

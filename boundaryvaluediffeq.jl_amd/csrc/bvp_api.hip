@@ -163,6 +163,15 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         return BVP_EINVAL;
     }
     if (!(o->abstol > 0)) { set_err(ctx, "abstol must be positive"); return BVP_EINVAL; }
+    if (o->nlsolve_stages < 1 || o->nlsolve_stages > 7) {
+        set_err(ctx, "nlsolve_stages must be a non-empty mask of bits 0 (NewtonRaphson), 1 (BackTracking), "
+                     "2 (TrustRegion)");
+        return BVP_EINVAL;
+    }
+    if (blk && !(o->nlsolve_stages & 3)) {
+        set_err(ctx, "the block kernels (M >= 8) run the NewtonRaphson stages (bits 0 and 1) only");
+        return BVP_ENOTSUP;
+    }
     int maxN0 = 0;
     for (int i = 0; i < n; ++i) maxN0 = std::max(maxN0, inst[i].N);
     int cap = std::max(cap_req, std::max(o->max_num_subintervals, maxN0) + 1);
@@ -248,7 +257,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         (rc = dalloc(ctx, A, D.N_final, n)) || (rc = dalloc(ctx, A, D.newton_iters, n)) ||
         (rc = dalloc(ctx, A, D.outer_iters, n)) || (rc = dalloc(ctx, A, D.fallback, n)) ||
         (rc = dalloc(ctx, A, D.cur_buf, n)) || (rc = dalloc(ctx, A, D.defect_norm, n)) ||
-        (rc = dalloc(ctx, A, D.resid_norm, n)) || (rc = dalloc(ctx, A, D.cnt, (size_t)n * 4)) ||
+        (rc = dalloc(ctx, A, D.resid_norm, n)) || (rc = dalloc(ctx, A, D.err, n)) ||
+        (rc = dalloc(ctx, A, D.cnt, (size_t)n * 4)) ||
         (rc = dalloc(ctx, A, D.counter, 1))) {
         delete B;
         return rc;
@@ -282,7 +292,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         const size_t lmax = (size_t)(cap + 63) / 64 + 1;
         D.tree_base = lmax * NF * 64;
         D.scratch_per_slot = D.tree_base + (size_t)64 * NF;
-        D.aux_per_slot = (size_t)cap * (2 * M + 1);
+        D.aux_per_slot = (size_t)cap * (3 * M + 1);
         scratch_doubles = D.scratch_per_slot * slots;
         D.rec_cap = cap;
     } else {
@@ -318,7 +328,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     D.adaptive = o->adaptive;
     D.max_sub = o->max_num_subintervals;
     D.max_newton = o->max_newton_iters;
-    D.linesearch = o->linesearch_fallback;
+    D.stages = o->nlsolve_stages;
+    D.report_err = o->report_err ? 1 : 0;
     D.transfer_mode = o->transfer_mode;
     D.abstol = o->abstol;
     D.newton_tol = o->newton_abstol > 0 ? o->newton_abstol : o->abstol;
@@ -438,7 +449,7 @@ int batch_results(bvp_ctx *ctx, bvp_batch *B, bvp_result *res) {
     const int n = B->n, M = B->M;
     const BatchDev &D = B->dev;
     std::vector<int> ret(n), Nf(n), it(n), ou(n), fb(n);
-    std::vector<double> dn(n), rn(n);
+    std::vector<double> dn(n), rn(n), er(n);
     HIP_TRY(ctx, hipMemcpy(ret.data(), D.retcode, n * sizeof(int), hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipMemcpy(Nf.data(), D.N_final, n * sizeof(int), hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipMemcpy(it.data(), D.newton_iters, n * sizeof(int), hipMemcpyDeviceToHost));
@@ -446,6 +457,7 @@ int batch_results(bvp_ctx *ctx, bvp_batch *B, bvp_result *res) {
     HIP_TRY(ctx, hipMemcpy(fb.data(), D.fallback, n * sizeof(int), hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipMemcpy(dn.data(), D.defect_norm, n * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(ctx, hipMemcpy(rn.data(), D.resid_norm, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(er.data(), D.err, n * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<long long> off(n);
     long long tot = 0;
     int maxn = 0;
@@ -476,6 +488,7 @@ int batch_results(bvp_ctx *ctx, bvp_batch *B, bvp_result *res) {
         r.fallback_used = fb[i];
         r.defect_norm = dn[i];
         r.resid_norm = rn[i];
+        r.err_vs_fp64 = er[i];
         if (r.capacity < Nf[i] + 1) { bad = 1; continue; }
         if (r.mesh) std::memcpy(r.mesh, &hm[(size_t)off[i]], sizeof(double) * (Nf[i] + 1));
         if (r.u) std::memcpy(r.u, &hy[(size_t)off[i] * M], sizeof(double) * (size_t)M * (Nf[i] + 1));
@@ -495,11 +508,12 @@ void bvp_opts_default(bvp_opts *o) {
     o->defect_threshold = 0.1;
     o->max_num_subintervals = 3000;
     o->max_newton_iters = 1000;
-    o->linesearch_fallback = 1;
+    o->nlsolve_stages = 7;
     o->transfer_mode = BVP_TRANSFER_REFERENCE;
     o->precision = BVP_PREC_FP64;
     o->refine_iters = 0;
     o->warm_start = BVP_WARM_NONE;
+    o->report_err = 0;
 }
 
 int bvp_ctx_create(int device, bvp_ctx **out) {

@@ -713,27 +713,36 @@ __device__ __noinline__ void brefine_residual(const BInst<M> &I, BShared<M, METH
 }
 
 // ------------------------------------------------------------------ Newton (block)
+// NewtonRaphson, then NewtonRaphson + BackTracking from the same start (bits 0
+// and 1 of B.stages; Core/default_internal_solve.jl:37-43).  The TrustRegion
+// stage (bit 2) is run by the wave kernels only.
 template <int M, int METHOD, typename LT>
 __device__ NewtonOut bnewton(const BatchDev &B, const BInst<M> &I, BShared<M, METHOD, LT> &sh, const double *mesh,
                              double *y, double *ybak, double *delta, double *rbuf, double *ytmp, int N, int lane,
                              LT *rec) {
-    NewtonOut out{BVP_RET_MAXITERS, 0, 0};
-    const int attempts = B.linesearch ? 2 : 1;
+    NewtonOut out{BVP_RET_MAXITERS, 0, -1, 0, NAN};
     const bool fp32 = sizeof(LT) == 4;
     const int refine = fp32 ? (B.refine_iters > 0 ? B.refine_iters : 0) : 0;
-    if (B.linesearch) {
+    const int stages = B.stages & 3;
+    const bool restart = stages == 3;
+    if (restart) {
         for (int e = lane; e < (N + 1) * M; e += WAVE) ybak[e] = y[e];
         __syncthreads();
     }
-    for (int attempt = 0; attempt < attempts; ++attempt) {
-        if (attempt == 1) {
+    bool first = true;
+    for (int stage = 0; stage < 2; ++stage) {
+        if (!((stages >> stage) & 1)) continue;
+        if (!first) {
             for (int e = lane; e < (N + 1) * M; e += WAVE) y[e] = ybak[e];
             __syncthreads();
         }
+        first = false;
+        out.stage = stage;
         int ret = BVP_RET_MAXITERS;
         for (int it = 0;; ++it) {
             double ss;
             const double fn = bresidual<M, METHOD, LT>(I, sh, mesh, y, nullptr, 0.0, N, lane, &ss);
+            out.fn = fn;
             if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
             if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
             if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
@@ -748,19 +757,19 @@ __device__ NewtonOut bnewton(const BatchDev &B, const BInst<M> &I, BShared<M, ME
             if (!ok) { ret = BVP_RET_FAILURE; break; }
             out.iters++;
             double alpha = 1.0;
-            if (attempt == 1) {
-                for (int ls = 0; ls < 12; ++ls) {
+            if (stage == 1) {
+                // BackTracking on |F(y + a d)|^2 / 2 with phi'(0) = -|F|^2 (Newton direction)
+                auto phi = [&](double a) {
                     double ph;
-                    bresidual<M, METHOD, LT>(I, sh, mesh, y, delta, alpha, N, lane, &ph);
-                    if (ph <= (1.0 - 2e-4 * alpha) * ss) break;
-                    alpha *= 0.5;
-                }
+                    bresidual<M, METHOD, LT>(I, sh, mesh, y, delta, a, N, lane, &ph);
+                    return 0.5 * ph;
+                };
+                if (!backtracking(phi, 0.5 * ss, -ss, alpha)) { ret = BVP_RET_FAILURE; break; }
             }
             for (int e = lane; e < (N + 1) * M; e += WAVE) y[e] += alpha * delta[e];
             __syncthreads();
         }
         out.ret = ret;
-        out.fallback = attempt;
         if (ret == BVP_RET_SUCCESS) break;
     }
     return out;
@@ -900,7 +909,7 @@ __device__ void bsolve_instance(const BatchDev &B, int b, BShared<M, METHOD, LT>
         if (N > B.rec_cap) {
             nl.ret = BVP_RET_FAILURE;
             nl.iters = 0;
-            nl.fallback = 0;
+            nl.stage = -1;
             fb |= 8;  // records of this instance do not fit the per-wave factor storage
         } else {
             nl = bnewton<M, METHOD, LT>(B, I, sh, mesh, y, y_o, delta, rbuf, ytmp, N, lane, rec);
@@ -908,7 +917,7 @@ __device__ void bsolve_instance(const BatchDev &B, int b, BShared<M, METHOD, LT>
         iters += nl.iters;
         cnt_it += (long long)N * nl.iters;
         cnt_chk += N;
-        if (nl.fallback) fb |= 1;
+        if (nl.stage == 1) fb |= 1;
         nl_ret = nl.ret;
         double error_norm = 2.0 * B.abstol;
         if (!B.adaptive || (fb & 8)) { info = nl.ret; break; }
@@ -950,7 +959,28 @@ __device__ void bsolve_instance(const BatchDev &B, int b, BShared<M, METHOD, LT>
     }
     double ss;
     const double rn = bresidual<M, METHOD, LT>(I, sh, mesh_b[cur], y_b[cur], nullptr, 0.0, N, lane, &ss);
+    // distance of the returned state from the discrete solution on its mesh: one more
+    // Newton correction (factors in the plan's precision, refined as in the solve)
+    double err = NAN;
+    if (B.report_err && N <= B.rec_cap) {
+        const double *mesh = mesh_b[cur], *y = y_b[cur];
+        bool ok = bsolve<M, METHOD, LT>(I, sh, mesh, y, N, lane, rec, true, nullptr, delta);
+        const int refine = sizeof(LT) == 4 ? (B.refine_iters > 0 ? B.refine_iters : 0) : 0;
+        for (int rI = 0; ok && rI < refine; ++rI) {
+            brefine_residual<M, METHOD, LT>(I, sh, mesh, y, delta, N, lane, rbuf);
+            ok = bsolve<M, METHOD, LT>(I, sh, mesh, y, N, lane, rec, false, rbuf, ytmp);
+            for (int e = lane; e < (N + 1) * M; e += WAVE) delta[e] += ytmp[e];
+            __syncthreads();
+        }
+        if (ok) {
+            double e_ = 0.0;
+            for (int e = lane; e < (N + 1) * M; e += WAVE) e_ = fmax(e_, fabs(delta[e]) / fmax(1.0, fabs(y[e])));
+            err = wave_max(e_);
+        }
+        __syncthreads();
+    }
     if (lane == 0) {
+        B.err[b] = err;
         B.retcode[b] = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
         B.N_final[b] = N;
         B.newton_iters[b] = iters;

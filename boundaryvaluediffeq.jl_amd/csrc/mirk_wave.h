@@ -41,14 +41,15 @@ struct BatchDev {
 
     // per-instance results
     int *retcode, *N_final, *newton_iters, *outer_iters, *fallback, *cur_buf;
-    double *defect_norm, *resid_norm;
+    double *defect_norm, *resid_norm, *err;
     long long *cnt;  // [n][4]: interval-iterations, defect intervals, transfer nodes, check passes
 
     // per-slot (persistent wave) scratch
     double *scratch;
     size_t scratch_per_slot;  // doubles
     size_t tree_base;         // offset of the tree records inside a slot
-    double *aux;              // [slot][cap*(2M+1)]: delta, trial state, transfer indices
+    double *aux;              // [slot][cap*(3M+1)]: delta, trial state, transfer indices, J'F
+                              //   (block kernels: their own layout)
     size_t aux_per_slot;
     unsigned *counter;        // work-queue head
 
@@ -68,7 +69,8 @@ struct BatchDev {
     int f32rec;               // wave kernels: elimination factors stored in FP32 (mixed precision)
 
     // options
-    int adaptive, max_sub, max_newton, linesearch, transfer_mode;
+    int adaptive, max_sub, max_newton, stages, transfer_mode;
+    int report_err;           // fill err[] after each solve (one more Newton correction)
     double abstol, newton_tol, defect_threshold;
 };
 

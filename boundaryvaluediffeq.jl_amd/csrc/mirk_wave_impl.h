@@ -54,6 +54,14 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 __device__ inline bool wave_any(bool p) { return __any(p ? 1 : 0) != 0; }
+// v as a wave-uniform value (lane 0's; after a reduction every lane holds the same
+// bits), so that the compiler can keep it in scalar registers
+__device__ inline double uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffLL));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // p as a pointer into global memory.  Pointers that reach the compiler as
 // generic (flat) pointers -- read back from a runtime-indexed array, or derived
@@ -1108,65 +1116,303 @@ __device__ inline double residual_primal(const Inst<M> &I, const double *mesh, c
     return anybad ? NAN : mx;
 }
 
-// Primal residual at y + alpha*delta: returns sum of squares (NaN if not finite).
+// Primal residual statistics at state y: returns sum of squares of all residual
+// rows (Phi and BC rows); *mx receives max|F|.  Both NaN if a value is not finite.
 template <int M, int METHOD>
-__device__ inline double residual_sq_trial(const Inst<M> &I, const double *mesh, const double *y,
-                                           const double *delta, double alpha, int N, int lane, double *ytmp) {
-    // trial state into ytmp (same layout as y)
-    for (int k = lane; k <= N; k += WAVE)
-#pragma unroll
-        for (int c = 0; c < M; ++c) ytmp[(size_t)k * M + c] = y[(size_t)k * M + c] + alpha * delta[(size_t)k * M + c];
-    __syncthreads();
+__device__ inline double residual_stats(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
+                                        double *mx_out) {
     int P, start, len;
     chunk_of(N, lane, P, start, len);
-    double acc = 0.0;
-    for (int k = start; k < start + len; ++k) {
-        const double ti = mesh[k], h = mesh[k + 1] - ti;
-        double yi[M], yj[M];
-#pragma unroll
-        for (int c = 0; c < M; ++c) { yi[c] = ytmp[(size_t)k * M + c]; yj[c] = ytmp[(size_t)(k + 1) * M + c]; }
+    double acc = 0.0, mx = 0.0;
+    walk_chunk<M, kPfPrimal<M>>(mesh, y, start, len, [&](int, double ti, double tj, const double *yi, const double *yj) {
+        const double h = tj - ti;
         double K[Tab<METHOD>::s][M], ph[M];
         stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
         phi_of<M, METHOD, double>(h, yi, yj, K, ph);
 #pragma unroll
-        for (int c = 0; c < M; ++c) acc += ph[c] * ph[c];
-    }
+        for (int c = 0; c < M; ++c) {
+            acc += ph[c] * ph[c];
+            mx = fmax(mx, fabs(ph[c]));
+        }
+    });
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
-        const double r = bc_residual<M, METHOD>(I, lane, L, mesh, ytmp);
+        const double r = bc_residual<M, METHOD>(I, lane, L, mesh, y);
         acc += r * r;
+        mx = fmax(mx, fabs(r));
     }
     const bool bad = wave_any(!(acc <= 1.7976931348623157e308));
-    acc = wave_sum(acc);
+    acc = uniform(wave_sum(acc));
+    mx = uniform(wave_max(mx));
     __syncthreads();
+    if (mx_out) *mx_out = bad ? NAN : mx;
     return bad ? NAN : acc;
 }
 
+// ytmp = y + a*d1 + b*d2 (d2 may be null), then the residual statistics there.
+template <int M, int METHOD>
+__device__ inline double residual_sq_trial(const Inst<M> &I, const double *mesh, const double *y, const double *d1,
+                                           double a, const double *d2, double b, int N, int lane, double *ytmp,
+                                           double *mx_out = nullptr) {
+    for (int q = lane; q < (N + 1) * M; q += WAVE) {
+        double v = y[q] + a * d1[q];
+        if (d2) v += b * d2[q];
+        ytmp[q] = v;
+    }
+    __syncthreads();
+    return residual_stats<M, METHOD>(I, mesh, ytmp, N, lane, mx_out);
+}
+
+// Phi_k and the Jacobian blocks of one interval (dual numbers, or the chain rule
+// in the kernels specialised to one family).
+template <int M, int METHOD, int FAM>
+__device__ inline void interval_blocks(const Inst<M> &I, double ti, double h, const double *yi, const double *yj,
+                                       double (&phi)[M], double (&S)[M][M], double (&R)[M][M]) {
+    double Kp[Tab<METHOD>::s][M];
+    if constexpr (M == 2 && (FAM == BVP_FAMILY_BRATU || FAM == BVP_FAMILY_SINGPERT))
+        interval_chain<M, METHOD, FAM>(I.P, ti, h, yi, yj, phi, S, R, Kp);
+    else
+        interval_dual<M, METHOD>(I.fam, I.P, ti, h, yi, yj, phi, S, R, Kp);
+}
+
+// g = J' F at y (point-major, M*(N+1)); the BC rows' residuals and nodes are the
+// ones residual_and_blocks left in LDS (frozen-stage unit rows).
+template <int M, int METHOD, int FAM>
+__device__ inline void jt_times_f(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y, int N,
+                                  int lane, double *g) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double carry[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) carry[c] = 0.0;
+    walk_chunk<M, kPfDual<M>>(mesh, y, start, len, [&](int k, double ti, double tj, const double *yi, const double *yj) {
+        double phi[M], S[M][M], R[M][M];
+        interval_blocks<M, METHOD, FAM>(I, ti, tj - ti, yi, yj, phi, S, R);
+#pragma unroll
+        for (int b = 0; b < M; ++b) {
+            double gs = carry[b], gr = 0.0;
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                gs += S[a][b] * phi[a];
+                gr += R[a][b] * phi[a];
+            }
+            g[(size_t)k * M + b] = gs;
+            carry[b] = gr;
+        }
+    });
+    // the carry of a chunk belongs to the first node of the next one
+    if (len > 0) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) sh.lo[lane][c] = carry[c];
+    }
+    __syncthreads();
+    if (len > 0 && lane > 0) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) g[(size_t)start * M + c] += sh.lo[lane - 1][c];
+    }
+    if (lane == P - 1) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) g[(size_t)N * M + c] = carry[c];
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) g[(size_t)sh.bcnode[k] * M + sel<M>(I.bcc, k)] += sh.bcres[k];
+    }
+    __syncthreads();
+}
+
+// |J v|^2 at y (Phi rows and the frozen-stage BC rows); all lanes.
+template <int M, int METHOD, int FAM>
+__device__ inline double jac_norm2(const Inst<M> &I, const Shared<M> &sh, const double *mesh, const double *y,
+                                   const double *v, int N, int lane) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double acc = 0.0;
+    double vi[M];
+    if (len > 0) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) vi[c] = v[(size_t)start * M + c];
+    }
+    walk_chunk<M, kPfDual<M>>(mesh, y, start, len, [&](int k, double ti, double tj, const double *yi, const double *yj) {
+        double phi[M], S[M][M], R[M][M], vj[M];
+        interval_blocks<M, METHOD, FAM>(I, ti, tj - ti, yi, yj, phi, S, R);
+#pragma unroll
+        for (int c = 0; c < M; ++c) vj[c] = v[(size_t)(k + 1) * M + c];
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+            double r = 0.0;
+#pragma unroll
+            for (int b = 0; b < M; ++b) r += S[a][b] * vi[b] + R[a][b] * vj[b];
+            acc += r * r;
+        }
+#pragma unroll
+        for (int c = 0; c < M; ++c) vi[c] = vj[c];
+    });
+    if (lane < M) {
+        const double r = v[(size_t)sh.bcnode[lane] * M + sel<M>(I.bcc, lane)];
+        acc += r * r;
+    }
+    acc = uniform(wave_sum(acc));
+    __syncthreads();
+    return acc;
+}
+
 struct NewtonOut {
-    int ret, iters, fallback, warm_fallback;
+    int ret, iters, stage, warm_fallback;
     double fn;  // max|F| at the state Newton returns
 };
 
-// NewtonRaphson, then NewtonRaphson with backtracking from the same start
-// (Core/default_internal_solve.jl:37-43).  Stop when max|F| <= tol, checked
-// before each step.
-// restore() writes the starting state into y again (the line-search attempt
-// starts from where the plain attempt started); it regenerates the state from its
-// sources instead of keeping a copy of it.
+// LineSearches.jl's BackTracking, order 3 (the restatement the oracle follows):
+// phi(a) = |F(y + a d)|^2 / 2, Armijo constant 1e-4, quadratic then cubic
+// interpolation, each new step kept within [0.1, 0.5] of the previous one, a
+// non-finite trial halved first (at most 52 times), at most 1000 reductions.
+// phi returns the same value in every lane, so every branch here is uniform.
+// Returns false when the reductions ran out.
+template <typename Phi>
+__device__ inline bool backtracking(Phi &&phi, double phi0, double dphi0, double &alpha) {
+    const double c1 = 1e-4, rho_hi = 0.5, rho_lo = 0.1;
+    double a1 = 1.0, a2 = 1.0;
+    double fx0 = phi0, fx1 = phi(a1);
+    for (int f = 0; !(fabs(fx1) <= 1.7976931348623157e308) && f < 52; ++f) {
+        a1 = a2;
+        a2 = a1 * 0.5;
+        fx1 = phi(a2);
+    }
+    for (int iteration = 1; fx1 > phi0 + c1 * a2 * dphi0; ++iteration) {
+        if (iteration > 1000) { alpha = a2; return false; }
+        double at;
+        if (iteration == 1) {
+            at = -(dphi0 * a2 * a2) / (2.0 * (fx1 - phi0 - dphi0 * a2));
+        } else {
+            const double div = 1.0 / (a1 * a1 * a2 * a2 * (a2 - a1));
+            const double e1 = fx1 - phi0 - dphi0 * a2, e0 = fx0 - phi0 - dphi0 * a1;
+            const double a = (a1 * a1 * e1 - a2 * a2 * e0) * div;
+            const double b = (-a1 * a1 * a1 * e1 + a2 * a2 * a2 * e0) * div;
+            if (fabs(a) <= 2.220446049250313e-16) {
+                at = dphi0 / (2.0 * b);
+            } else {
+                const double d = fmax(b * b - 3.0 * a * dphi0, 0.0);
+                at = (-b + sqrt(d)) / (3.0 * a);
+            }
+        }
+        a1 = a2;
+        const double hi = a2 * rho_hi, lo = a2 * rho_lo;
+        at = isnan(at) ? hi : fmin(at, hi);
+        a2 = isnan(at) ? lo : fmax(at, lo);
+        fx0 = fx1;
+        fx1 = phi(a2);
+    }
+    alpha = a2;
+    return true;
+}
+
+// The TrustRegion stage (dogleg step, "Simple" radius update; the restatement the
+// oracle follows): from y, with buffers dn (Newton step), g (J'F) and ytmp (trial).
+template <int M, int METHOD, bool F32, int FAM>
+__device__ inline int trust_region(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y,
+                                   double *dn, double *g, double *ytmp, int N, int lane, double *scr,
+                                   size_t tree_base, NewtonOut &out) {
+    double mx;
+    double ss = residual_stats<M, METHOD>(I, mesh, y, N, lane, &mx);
+    double ymax = -INFINITY, ymin = INFINITY;
+    for (int q = lane; q < (N + 1) * M; q += WAVE) {
+        ymax = fmax(ymax, y[q]);
+        ymin = fmin(ymin, y[q]);
+    }
+    ymax = uniform(wave_max(ymax));
+    ymin = uniform(-wave_max(-ymin));
+    __syncthreads();
+    double radius = fmax(sqrt(ss), ymax - ymin) / 11.0;
+    int shrinks = 0;
+    bool fresh = true;
+    double dn2 = 0.0, l2 = 0.0, gdn = 0.0, jg2 = 0.0;
+    for (int it = 0;; ++it) {
+        out.fn = mx;
+        if (!(mx <= 1.7976931348623157e308)) return BVP_RET_UNSTABLE;
+        if (mx <= B.newton_tol) return BVP_RET_SUCCESS;
+        if (it >= B.max_newton) return BVP_RET_MAXITERS;
+        if (fresh) {
+            Rel<M> rel;
+            residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
+            if (!solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, dn, false))
+                return BVP_RET_FAILURE;
+            jt_times_f<M, METHOD, FAM>(I, sh, mesh, y, N, lane, g);
+            double a = 0.0, b = 0.0, c = 0.0;
+            for (int q = lane; q < (N + 1) * M; q += WAVE) {
+                a += dn[q] * dn[q];
+                b += g[q] * g[q];
+                c += g[q] * dn[q];
+            }
+            dn2 = uniform(wave_sum(a));
+            l2 = uniform(wave_sum(b));
+            gdn = uniform(wave_sum(c));
+            __syncthreads();
+            jg2 = jac_norm2<M, METHOD, FAM>(I, sh, mesh, y, g, N, lane);
+            fresh = false;
+        }
+        out.iters++;
+        // step d = cn * dn + cg * g and the predicted reduction of the linear model
+        double cn, cg, pred;
+        if (sqrt(dn2) <= radius) {
+            cn = 1.0; cg = 0.0; pred = ss;
+        } else {
+            const double lg = sqrt(l2), dcauchy = l2 * lg / jg2;
+            if (dcauchy >= radius) {
+                cn = 0.0; cg = -radius / lg;
+                pred = 2.0 * radius * lg - radius * radius * jg2 / l2;
+            } else {
+                const double beta = l2 / jg2;
+                // |a|^2, 2 dc.a for a = dn - dc, dc = -beta g
+                const double A2 = dn2 + 2.0 * beta * gdn + beta * beta * l2;
+                const double B2 = -2.0 * (beta * gdn + beta * beta * l2);
+                const double C2 = dcauchy * dcauchy - radius * radius;
+                const double tau = (-B2 + sqrt(fmax(0.0, B2 * B2 - 4.0 * A2 * C2))) / (2.0 * A2);
+                cn = tau; cg = -(1.0 - tau) * beta;
+                pred = ss - (1.0 - tau) * (1.0 - tau) * (ss - l2 * l2 / jg2);
+            }
+        }
+        double mxt;
+        const double sst = residual_sq_trial<M, METHOD>(I, mesh, y, dn, cn, g, cg, N, lane, ytmp, &mxt);
+        const double rho = (ss - sst) / pred;
+        if (rho > 1e-4) {
+            for (int q = lane; q < (N + 1) * M; q += WAVE) y[q] = ytmp[q];
+            __syncthreads();
+            ss = sst;
+            mx = mxt;
+            fresh = true;
+        }
+        if (rho < 0.25) {
+            radius *= 0.25;
+            if (++shrinks > 32) { out.fn = mx; return BVP_RET_FAILURE; }
+        } else {
+            shrinks = 0;
+            if (rho > 0.75) radius *= 2.0;
+        }
+    }
+}
+
+// The default polyalgorithm NewtonRaphson -> NewtonRaphson + BackTracking ->
+// TrustRegion (Core/default_internal_solve.jl:37-43), stages selected by the bit
+// mask B.stages, each restarting from the same start; stop when max|F| <= tol,
+// checked before each step.
+// restore() writes the starting state into y again; it regenerates the state from
+// its sources instead of keeping a copy of it.
 template <int M, int METHOD, bool F32 = false, int FAM = -1, typename Restore>
 __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y,
-                            Restore &&restore, double *delta, double *ytmp, int N, int lane, double *scr,
+                            Restore &&restore, double *delta, double *ytmp, double *gbuf, int N, int lane, double *scr,
                             size_t tree_base, const double *u0c) {
-    NewtonOut out{BVP_RET_MAXITERS, 0, 0, 0, NAN};
+    NewtonOut out{BVP_RET_MAXITERS, 0, -1, 0, NAN};
     // u0c != null: y holds a learned warm start; when the first Newton step from it
     // does not contract max|F| (or the guess is unusable), restart from the constant
     // guess u0c, as the plain method would have started.
     bool warm = u0c != nullptr;
     double fn_warm = 0.0;
-    const int attempts = B.linesearch ? 2 : 1;
-    bool start_u0c = false;  // the plain attempt fell back to u0c: the line search starts there too
-    for (int attempt = 0; attempt < attempts; ++attempt) {
-        if (attempt == 1) {
+    bool start_u0c = false;  // the first stage fell back to u0c: later stages start there too
+    bool first = true;
+    for (int stage = 0; stage < 3; ++stage) {
+        if (!((B.stages >> stage) & 1)) continue;
+        if (!first) {
             if (start_u0c) {
                 for (int k = lane; k <= N; k += WAVE)
 #pragma unroll
@@ -1176,83 +1422,99 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
             }
             __syncthreads();
         }
+        first = false;
+        out.stage = stage;
         int ret = BVP_RET_MAXITERS;
-        for (int it = 0;; ++it) {
-            Rel<M> rel;
-            // After a step, max|F| comes from the primal pass; the dual pass that
-            // builds the blocks runs only when another step is taken.
-            bool blocks = (it == 0);
-            double fn = blocks ? residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel)
-                               : residual_primal<M, METHOD>(I, mesh, y, N, lane);
-            if (warm) {
-                bool fall;
-                if (it == 0) {
-                    fn_warm = fn;
-                    fall = !(fn <= 1.7976931348623157e308);
-                } else {
-                    fall = !(fn < fn_warm);
-                    warm = false;
-                }
-                if (fall) {
-                    warm = false;
-                    out.warm_fallback = 1;
-                    for (int k = lane; k <= N; k += WAVE)
+        if (stage == 2) {
+            ret = trust_region<M, METHOD, F32, FAM>(B, I, sh, mesh, y, delta, gbuf, ytmp, N, lane, scr, tree_base, out);
+        } else {
+            for (int it = 0;; ++it) {
+                Rel<M> rel;
+                // After a step, max|F| comes from the primal pass; the dual pass that
+                // builds the blocks runs only when another step is taken.
+                bool blocks = (it == 0);
+                double fn = blocks ? residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel)
+                                   : residual_primal<M, METHOD>(I, mesh, y, N, lane);
+                if (warm) {
+                    bool fall;
+                    if (it == 0) {
+                        fn_warm = fn;
+                        fall = !(fn <= 1.7976931348623157e308);
+                    } else {
+                        fall = !(fn < fn_warm);
+                        warm = false;
+                    }
+                    if (fall) {
+                        warm = false;
+                        out.warm_fallback = 1;
+                        for (int k = lane; k <= N; k += WAVE)
 #pragma unroll
-                        for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = u0c[c];
-                    start_u0c = true;
+                            for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = u0c[c];
+                        start_u0c = true;
+                        __syncthreads();
+                        fn = residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
+                        blocks = true;
+                        it = 0;  // the restart has the full iteration budget, as the plain method
+                    }
+                }
+                out.fn = fn;  // every exit below leaves y at the state fn was computed for
+                if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
+                if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
+                if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
+                if (!blocks) residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
+                const bool plain = (stage == 0);
+                if (!solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel,
+                                                      plain ? nullptr : delta, plain)) {
+                    if (warm) {  // singular at the warm start: restart from the constant guess
+                        warm = false;
+                        out.warm_fallback = 1;
+                        out.iters++;
+                        for (int k = lane; k <= N; k += WAVE)
+#pragma unroll
+                            for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = u0c[c];
+                        start_u0c = true;
+                        __syncthreads();
+                        it = -1;
+                        continue;
+                    }
+                    ret = BVP_RET_FAILURE;
+                    break;
+                }
+                out.iters++;
+                if (!plain) {
+                    // phi(0) with the same primal code as the trials; phi'(0) = -|F|^2
+                    // for the Newton direction
+                    const double ss = residual_stats<M, METHOD>(I, mesh, y, N, lane, nullptr);
+                    double alpha = 1.0;
+                    auto phi = [&](double a) {
+                        return 0.5 * residual_sq_trial<M, METHOD>(I, mesh, y, delta, a, nullptr, 0.0, N, lane, ytmp);
+                    };
+                    if (!backtracking(phi, 0.5 * ss, -ss, alpha)) { ret = BVP_RET_FAILURE; break; }
+                    for (int q = lane; q < (N + 1) * M; q += WAVE) y[q] += alpha * delta[q];
                     __syncthreads();
-                    fn = residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
-                    blocks = true;
                 }
-            }
-            out.fn = fn;  // every exit below leaves y at the state fn was computed for
-            if (!(fn <= 1.7976931348623157e308)) { ret = BVP_RET_UNSTABLE; break; }
-            if (fn <= B.newton_tol) { ret = BVP_RET_SUCCESS; break; }
-            if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
-            if (!blocks) residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
-            const bool plain = (attempt == 0);
-            if (!solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, plain ? nullptr : delta,
-                                             plain)) {
-                if (warm) {  // singular at the warm start: restart from the constant guess
-                    warm = false;
-                    out.warm_fallback = 1;
-                    out.iters++;
-                    for (int k = lane; k <= N; k += WAVE)
-#pragma unroll
-                        for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = u0c[c];
-                    start_u0c = true;
-                    __syncthreads();
-                    it = -1;
-                    continue;
-                }
-                ret = BVP_RET_FAILURE;
-                break;
-            }
-            out.iters++;
-            if (!plain) {
-                // backtracking: halve alpha until 0.5||F||^2 decreases by the Armijo factor
-                double phi0 = 0.0;
-                {
-                    // recompute ||F(y)||_2^2 with the same primal code as the trials
-                    phi0 = residual_sq_trial<M, METHOD>(I, mesh, y, delta, 0.0, N, lane, ytmp);
-                }
-                double alpha = 1.0;
-                for (int ls = 0; ls < 12; ++ls) {
-                    const double ph = residual_sq_trial<M, METHOD>(I, mesh, y, delta, alpha, N, lane, ytmp);
-                    if (ph <= (1.0 - 2e-4 * alpha) * phi0) break;
-                    alpha *= 0.5;
-                }
-                for (int k = lane; k <= N; k += WAVE)
-#pragma unroll
-                    for (int c = 0; c < M; ++c) y[(size_t)k * M + c] += alpha * delta[(size_t)k * M + c];
-                __syncthreads();
             }
         }
         out.ret = ret;
-        out.fallback = attempt;
         if (ret == BVP_RET_SUCCESS) break;
     }
     return out;
+}
+
+// max_i |d_i| / max(1, |y_i|) for the Newton correction d at (mesh, y): the
+// first-order distance of y from the discrete solution on this mesh (FP64
+// residual and elimination).  NaN if the matrix is singular.
+template <int M, int METHOD, bool F32, int FAM>
+__device__ inline double correction_norm(const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y, double *d,
+                                         int N, int lane, double *scr, size_t tree_base) {
+    Rel<M> rel;
+    residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
+    if (!solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, d, false)) return NAN;
+    double e = 0.0;
+    for (int q = lane; q < (N + 1) * M; q += WAVE) e = fmax(e, fabs(d[q]) / fmax(1.0, fabs(y[q])));
+    e = wave_max(e);
+    __syncthreads();
+    return e;
 }
 
 // ------------------------------------------------------------------ defect
@@ -1602,6 +1864,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     double *delta = aux;                               // [cap*M]
     double *ytmp = aux + (size_t)B.cap * M;            // [cap*M]
     double *iiscr = aux + (size_t)2 * B.cap * M;       // [cap]
+    double *gbuf = aux + (size_t)2 * B.cap * M + B.cap;  // [cap*M] J'F of the trust-region stage
     int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
     int outer = 0, iters = 0, fb = 0;
     double last_defect = NAN;
@@ -1631,13 +1894,15 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         // the other buffer, as written (a new mesh goes to the buffer itself)
         double *mesh_o = cur ? bf.mesh[0] : bf.mesh[1], *y_o = bf.y_of(1 - cur);
         const double *u0c = (B.warm && outer == 1) ? as_global(B.u0c + (size_t)b * M) : nullptr;
-        NewtonOut nl = newton<M, METHOD, F32, FAM>(B, I, sh, mesh, y, restore, delta, ytmp, N, lane, scr, tree_base, u0c);
+        NewtonOut nl = newton<M, METHOD, F32, FAM>(B, I, sh, mesh, y, restore, delta, ytmp, gbuf, N, lane, scr,
+                                                   tree_base, u0c);
         iters += nl.iters;
         rn = nl.fn;
         rn_valid = true;
         cnt_it += (long long)N * nl.iters;
         cnt_chk += N + (nl.warm_fallback ? N : 0);
-        if (nl.fallback) fb |= 1;
+        if (nl.stage == 1) fb |= 1;
+        if (nl.stage == 2) fb |= 16;
         if (nl.warm_fallback) fb |= 2;
         nl_ret = nl.ret;
         double error_norm = 2.0 * B.abstol;
@@ -1690,7 +1955,12 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
     }
     // final residual norm: Newton's last check unless the mesh changed after it
     if (!rn_valid) rn = residual_primal<M, METHOD>(I, bf.mesh_of(cur), bf.y_of(cur), N, lane);
+    // distance of the returned state from the FP64 discrete solution on its mesh
+    const double err = B.report_err ? correction_norm<M, METHOD, F32, FAM>(I, sh, bf.mesh_of(cur), bf.y_of(cur), delta,
+                                                                            N, lane, scr, tree_base)
+                                    : NAN;
     if (lane == 0) {
+        B.err[b] = err;
         B.retcode[b] = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
         B.N_final[b] = N;
         B.newton_iters[b] = iters;

@@ -17,6 +17,54 @@ class DefectControl:
         self.defect_threshold = float(defect_threshold)
 
 
+class BackTracking:
+    """LineSearch.jl's BackTracking (Armijo c1 = 1e-4, quadratic then cubic
+    interpolation, steps kept within [0.1, 0.5] of the previous one)."""
+
+
+class NewtonRaphson:
+    """NonlinearSolve's NewtonRaphson, undamped or with a BackTracking line search."""
+
+    def __init__(self, linesearch=None):
+        if linesearch is not None and not isinstance(linesearch, BackTracking):
+            raise ValueError("the line search must be BackTracking()")
+        self.linesearch = linesearch
+
+    @property
+    def stages(self):
+        return _abi.STAGE_BACKTRACKING if self.linesearch is not None else _abi.STAGE_NEWTON
+
+
+class TrustRegion:
+    """NonlinearSolve's TrustRegion (dogleg step, "Simple" radius update, defaults)."""
+    stages = _abi.STAGE_TRUST_REGION
+
+
+class NonlinearSolvePolyAlgorithm:
+    """Stages tried in order, each from the same start; the default nonlinear solver
+    is NewtonRaphson -> NewtonRaphson(BackTracking) -> TrustRegion
+    (Core/default_internal_solve.jl:31-45).  The stages run in that fixed order."""
+
+    def __init__(self, algs):
+        self.algs = tuple(algs)
+
+    @property
+    def stages(self):
+        m = 0
+        for a in self.algs:
+            if a.stages <= m:
+                raise ValueError("stages must be NewtonRaphson, NewtonRaphson(BackTracking), TrustRegion in order")
+            m |= a.stages
+        return m
+
+
+def nlsolve_stages(nlsolve):
+    """bvp_opts.nlsolve_stages of an `nlsolve` choice (None: the default polyalgorithm)."""
+    if nlsolve is None:
+        return _abi.STAGES_DEFAULT
+    return nlsolve.stages
+
+
 class _MIRK:
     order = 0
 
@@ -54,6 +102,7 @@ class BVPSolution:
         self.u = [raw.u[i].copy() for i in range(raw.u.shape[0])]
         self.retcode = _abi.RETCODE_NAMES.get(raw.retcode, str(raw.retcode))
         self.resid = raw.resid_norm
+        self.err_vs_fp64 = raw.err_vs_fp64
         self.stats = {"newton_iters": raw.newton_iters, "outer_iters": raw.outer_iters,
                       "defect_norm": raw.defect_norm, "N": raw.N, "fallback_used": raw.fallback_used}
 
@@ -80,7 +129,7 @@ class EnsembleSolution(list):
         return all(successful_retcode(s) for s in self)
 
 
-def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=None):
+def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=None, report_err=False):
     o = _abi.default_opts(alg.order)
     o.abstol = float(abstol)
     nk = dict(nlsolve_kwargs or {})
@@ -92,13 +141,16 @@ def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=N
     o.precision = alg.precision
     o.refine_iters = alg.refine_iters
     o.transfer_mode = alg.transfer_mode
+    o.nlsolve_stages = nlsolve_stages(alg.nlsolve)
+    o.report_err = 1 if report_err else 0
     return o
 
 
 def solve(prob, alg, *, dt=0.0, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=None,
-          trajectories=None, device=0):
-    """Solve a BVProblem / TwoPointBVProblem (or an EnsembleProblem) with MIRK4/MIRK6 on the GPU."""
-    opts = make_opts(alg, abstol, adaptive, controller, nlsolve_kwargs)
+          trajectories=None, device=0, report_err=False):
+    """Solve a BVProblem / TwoPointBVProblem (or an EnsembleProblem) with MIRK4/MIRK6 on the GPU.
+    report_err: also return, per solution, err_vs_fp64 (the FP64 Newton correction at the result)."""
+    opts = make_opts(alg, abstol, adaptive, controller, nlsolve_kwargs, report_err)
     ctx = _lib.context(device)
     if isinstance(prob, EnsembleProblem):
         if trajectories is None:

@@ -83,6 +83,15 @@ extern "C" {
 #define BVP_PREC_FP64          0  /* block LU in FP64 */
 #define BVP_PREC_FP32_REFINE64 1  /* block LU in FP32 + FP64 iterative refinement */
 
+/* ---- stages of the default nonlinear polyalgorithm (bvp_opts.nlsolve_stages;
+ * Core/default_internal_solve.jl:31-45): NewtonRaphson, NewtonRaphson with the
+ * BackTracking line search, TrustRegion (dogleg), tried in this order, each from
+ * the same start */
+#define BVP_STAGE_NEWTON       1
+#define BVP_STAGE_BACKTRACKING 2
+#define BVP_STAGE_TRUST_REGION 4
+#define BVP_STAGES_DEFAULT     7
+
 /* ---- mesh-transfer mode after a redistribution (MIRK/mirk.jl:364-372) */
 #define BVP_TRANSFER_REFERENCE 0  /* in place, as the reference writes y0 while reading it */
 #define BVP_TRANSFER_CLEAN     1  /* interpolate from an untouched copy of the old solution */
@@ -130,14 +139,22 @@ typedef struct {
     double  defect_threshold;     /* 0.1             Core/calc_errors.jl:57     */
     int32_t max_num_subintervals; /* 3000            MIRK/algorithms.jl:60      */
     int32_t max_newton_iters;     /* 1000            (NonlinearSolve default)   */
-    int32_t linesearch_fallback;  /* 1: NR -> NR+backtracking, as the default
-                                     polyalgorithm Core/default_internal_solve.jl:31-45 */
+    int32_t nlsolve_stages;       /* bit mask of the stages of the default nonlinear
+                                     polyalgorithm run in order, each from the same
+                                     start (Core/default_internal_solve.jl:31-45):
+                                     bit0 NewtonRaphson, bit1 NewtonRaphson + BackTracking
+                                     line search, bit2 TrustRegion (dogleg); default 7.
+                                     Block kernels (M >= 8) run bits 0 and 1 only. */
     int32_t transfer_mode;        /* BVP_TRANSFER_*                              */
     int32_t precision;            /* BVP_PREC_*                                  */
     int32_t refine_iters;         /* FP64 refinement steps per linear solve (FP32 LU) */
     int32_t warm_start;           /* BVP_WARM_MLP: guess holds a learned guess;
                                      the solve falls back to u0 when its first
                                      Newton step does not contract (see below) */
+    int32_t report_err;           /* 1: fill bvp_result.err_vs_fp64 (one more Newton
+                                     correction per instance after the solve; the
+                                     returned solution is unchanged).  No counterpart
+                                     in the reference; default 0. */
 } bvp_opts;
 
 /* Per-instance result.  mesh/u are caller-allocated with room for
@@ -147,11 +164,18 @@ typedef struct {
     int32_t N_final;             /* intervals of the returned mesh */
     int32_t newton_iters;        /* Newton iterations, summed over mesh passes */
     int32_t outer_iters;         /* mesh passes (MIRK/mirk.jl:299-307) */
-    int32_t fallback_used;       /* bit0: line-search fallback, bit1: warm-start fallback,
-                                    bit2: FP64 fallback after an FP32 refinement stall */
+    int32_t fallback_used;       /* bit0: the NewtonRaphson + BackTracking stage ran,
+                                    bit1: warm-start fallback, bit2: FP64 fallback after an
+                                    FP32 refinement stall, bit3: factor records did not fit
+                                    (block kernels), bit4: the TrustRegion stage ran */
     int32_t capacity;            /* in: node capacity of mesh and u */
     double  defect_norm;         /* last defect estimate (MIRK/adaptivity.jl:410) */
     double  resid_norm;          /* max-norm of the final collocation residual */
+    double  err_vs_fp64;         /* with opts.report_err: max_i |d_i| / max(1, |u_i|) for
+                                    the FP64 Newton correction d = -J^-1 F at the returned
+                                    (mesh, u): the distance, to first order, from u to the
+                                    FP64 discrete solution on that mesh.  NaN otherwise.
+                                    No counterpart in the reference. */
     double *mesh;                /* out: [N_final+1] */
     double *u;                   /* out: [M*(N_final+1)] point-major */
 } bvp_result;

@@ -33,14 +33,16 @@ class Opts(C.Structure):
     _fields_ = [("method", C.c_int32), ("abstol", C.c_double), ("newton_abstol", C.c_double),
                 ("adaptive", C.c_int32), ("defect_threshold", C.c_double),
                 ("max_num_subintervals", C.c_int32), ("max_newton_iters", C.c_int32),
-                ("linesearch_fallback", C.c_int32), ("transfer_mode", C.c_int32),
-                ("precision", C.c_int32), ("refine_iters", C.c_int32), ("warm_start", C.c_int32)]
+                ("nlsolve_stages", C.c_int32), ("transfer_mode", C.c_int32),
+                ("precision", C.c_int32), ("refine_iters", C.c_int32), ("warm_start", C.c_int32),
+                ("report_err", C.c_int32)]
 
 
 class _Result(C.Structure):
     _fields_ = [("retcode", C.c_int32), ("N_final", C.c_int32), ("newton_iters", C.c_int32),
                 ("outer_iters", C.c_int32), ("fallback_used", C.c_int32), ("capacity", C.c_int32),
-                ("defect_norm", C.c_double), ("resid_norm", C.c_double), ("mesh", _dp), ("u", _dp)]
+                ("defect_norm", C.c_double), ("resid_norm", C.c_double), ("err_vs_fp64", C.c_double),
+                ("mesh", _dp), ("u", _dp)]
 
 
 def _d(a):
@@ -77,6 +79,8 @@ def lib():
         L.ref_solve_batch.argtypes = [C.POINTER(Opts), C.c_int, C.POINTER(_Instance), C.POINTER(_Result),
                                       C.c_int]
         L.ref_opts_default.argtypes = [C.POINTER(Opts)]
+        L.ref_backtracking_poly.argtypes = [_dp, C.c_int, _dp, _dp, C.c_int, _ip]
+        L.ref_dogleg.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
         _lib = L
     return _lib
 
@@ -234,6 +238,27 @@ def mesh_select_errors(order, abstol, max_sub, mesh, errors):
     return out[:r + 1].copy()
 
 
+def backtracking_poly(coef, max_trace=64):
+    """The oracle's BackTracking line search on phi(a) = sum_k coef[k] a^k:
+    (return code, accepted step, trial steps in order)."""
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    a = np.zeros(1)
+    tr = np.zeros(max_trace)
+    nt = np.zeros(1, dtype=np.int32)
+    rc = lib().ref_backtracking_poly(_d(coef), coef.shape[0], _d(a), _d(tr), max_trace, nt.ctypes.data_as(_ip))
+    return rc, float(a[0]), tr[:min(int(nt[0]), max_trace)].copy()
+
+
+def dogleg(J, F, radius):
+    """The oracle's dogleg step of a dense system: (step, predicted reduction)."""
+    J = np.ascontiguousarray(J, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    n = F.shape[0]
+    d, pred = np.zeros(n), np.zeros(1)
+    _check(lib().ref_dogleg(n, _d(J), _d(F), radius, _d(d), _d(pred)), "dogleg")
+    return d, float(pred[0])
+
+
 class Solution:
     def __init__(self, res, M, mesh, u):
         self.retcode = res.retcode
@@ -243,6 +268,7 @@ class Solution:
         self.fallback_used = res.fallback_used
         self.defect_norm = res.defect_norm
         self.resid_norm = res.resid_norm
+        self.err_vs_fp64 = res.err_vs_fp64
         self.t = mesh[:self.N + 1].copy()
         self.u = u[:M * (self.N + 1)].reshape(self.N + 1, M).copy()
 

@@ -584,56 +584,264 @@ int newton_solve(const Ctx &C, int N, const double *mesh, const double *y,
 }
 
 // ---------------------------------------------------------------- Newton
-struct NlOut { int ret; int iters; double fnorm; int fallback; };
+// The default nonlinear solver is the polyalgorithm
+//   NewtonRaphson -> NewtonRaphson(linesearch = BackTracking()) -> TrustRegion()
+// (Core/default_internal_solve.jl:31-45), each stage restarting from the same
+// initial guess with its own iteration budget, stopping when max|F| <= tol,
+// checked before every step (so maxiters = 0 returns the guess untouched,
+// MIRKt/Core/mirk_basic_tests.jl:678-720).  `stages` is a bit mask of the
+// stages to run (bit0 NR, bit1 NR + BackTracking, bit2 TrustRegion).
+//
+// The stages are external to the reference (NonlinearSolveFirstOrder "1.2, 2",
+// LineSearch "0.1.4"); they are restated here from the published algorithms:
+//  * BackTracking: LineSearches.jl's backtracking line search, order 3, on the
+//    merit function phi(a) = |F(y + a d)|_2^2 / 2 with phi'(0) = F'J d = -|F|^2
+//    for the Newton direction d: Armijo test phi(a) <= phi(0) + c1 a phi'(0),
+//    c1 = 1e-4; the first reduction minimises the quadratic through phi(0),
+//    phi'(0), phi(a), later ones the cubic through phi(0), phi'(0) and the last
+//    two trials; each new step is kept within [rho_lo, rho_hi] = [0.1, 0.5] of
+//    the previous one; a non-finite trial is halved first (at most 52 times);
+//    more than 1000 reductions fail the stage.
+//  * TrustRegion: the dogleg step (Newton step inside the region; else the
+//    Cauchy point of the model |F + J d|^2 / 2 along -J'F, cut at the region
+//    boundary or joined to the Newton step where the dogleg path leaves the
+//    region) with the "Simple" radius update and TrustRegion()'s documented
+//    defaults: initial radius max(|F(u0)|_2, max(u0) - min(u0)) / 11, step
+//    accepted when rho = actual / predicted reduction > 1e-4, radius times 1/4
+//    when rho < 1/4 (failure after more than 32 shrinks in a row), times 2 when
+//    rho > 3/4.
+// The Jacobian of the boundary rows is the frozen-stage one of the Newton
+// matrix (MIRK/mirk.jl:565-572) in J d and J'F as well.
+struct NlOut { int ret; int iters; double fnorm; int stage; };
 
-// NewtonRaphson (undamped) followed, on failure, by NewtonRaphson with a
-// backtracking line search from the same start (Core/default_internal_solve.jl:37-43).
-// Termination: max|F| <= tol (checked before every step, so maxiters = 0
-// returns the guess untouched, MIRKt/Core/mirk_basic_tests.jl:678-720).
-NlOut newton(const Ctx &C, int N, const double *mesh, std::vector<double> &y,
-             const std::vector<BcRow> &rows, double tol, int maxiters, bool fallback) {
-    const int n = C.M * (N + 1);
-    std::vector<double> y0 = y, delta(n), yt(n);
-    Eval E, Et;
-    Band B;
-    NlOut out{BVP_RET_MAXITERS, 0, 0, 0};
-    const bool interp_all = false;
-    for (int attempt = 0; attempt < (fallback ? 2 : 1); ++attempt) {
-        if (attempt == 1) y = y0;
-        int ret = BVP_RET_MAXITERS;
-        double fn = 0;
-        for (int it = 0;; ++it) {
-            evaluate(C, N, mesh, y.data(), rows, E, interp_all);
-            fn = max_abs_residual(E);
-            if (!std::isfinite(fn)) { ret = BVP_RET_UNSTABLE; break; }
-            if (fn <= tol) { ret = BVP_RET_SUCCESS; break; }
-            if (it >= maxiters) { ret = BVP_RET_MAXITERS; break; }
-            if (newton_solve(C, N, mesh, y.data(), rows, E, delta.data(), B)) { ret = BVP_RET_FAILURE; break; }
-            out.iters++;
-            if (attempt == 0) {
-                for (int i = 0; i < n; ++i) y[i] += delta[i];
+double sumsq_residual(const Eval &E) {
+    double s = 0;
+    for (double v : E.phi) s += v * v;
+    for (double v : E.bc) s += v * v;
+    return s;
+}
+
+// LineSearches.jl BackTracking (order 3).  phi(a) evaluates the merit function.
+// Returns 0 and the accepted step in *alpha, or 1 when the reductions ran out.
+template <class Phi>
+int backtracking(Phi &&phi, double phi0, double dphi0, double *alpha, double *phi_alpha) {
+    const double c1 = 1e-4, rho_hi = 0.5, rho_lo = 0.1;
+    const int iterations = 1000, iterfinitemax = 52;  // -log2(eps(Float64))
+    auto nanmin = [](double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : std::min(a, b)); };
+    auto nanmax = [](double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : std::max(a, b)); };
+    double a1 = 1.0, a2 = 1.0;
+    double fx0 = phi0, fx1 = phi(a1);
+    int iterfinite = 0;
+    while (!std::isfinite(fx1) && iterfinite < iterfinitemax) {
+        ++iterfinite;
+        a1 = a2;
+        a2 = a1 / 2;
+        fx1 = phi(a2);
+    }
+    int iteration = 0;
+    while (fx1 > phi0 + c1 * a2 * dphi0) {
+        if (++iteration > iterations) { *alpha = a2; *phi_alpha = fx1; return 1; }
+        double at;
+        if (iteration == 1) {
+            at = -(dphi0 * a2 * a2) / (2 * (fx1 - phi0 - dphi0 * a2));
+        } else {
+            const double div = 1.0 / (a1 * a1 * a2 * a2 * (a2 - a1));
+            const double e1 = fx1 - phi0 - dphi0 * a2, e0 = fx0 - phi0 - dphi0 * a1;
+            const double a = (a1 * a1 * e1 - a2 * a2 * e0) * div;
+            const double b = (-a1 * a1 * a1 * e1 + a2 * a2 * a2 * e0) * div;
+            if (std::fabs(a) <= 2.220446049250313e-16) {
+                at = dphi0 / (2 * b);
             } else {
-                // Armijo backtracking on 0.5*||F||_2^2 with step halving
-                double phi0 = 0;
-                for (double v : E.phi) phi0 += v * v;
-                for (double v : E.bc) phi0 += v * v;
-                double alpha = 1.0;
-                for (int ls = 0; ls < 12; ++ls) {
-                    for (int i = 0; i < n; ++i) yt[i] = y[i] + alpha * delta[i];
-                    evaluate(C, N, mesh, yt.data(), rows, Et, interp_all);
-                    double phi = 0;
-                    for (double v : Et.phi) phi += v * v;
-                    for (double v : Et.bc) phi += v * v;
-                    if (std::isfinite(phi) && phi <= (1.0 - 2e-4 * alpha) * phi0) break;
-                    alpha *= 0.5;
-                }
-                for (int i = 0; i < n; ++i) y[i] += alpha * delta[i];
+                const double d = std::max(b * b - 3 * a * dphi0, 0.0);
+                at = (-b + std::sqrt(d)) / (3 * a);
             }
         }
-        out.ret = ret;
-        out.fnorm = fn;
-        out.fallback = attempt;
-        if (ret == BVP_RET_SUCCESS) break;
+        a1 = a2;
+        at = nanmin(at, a2 * rho_hi);
+        a2 = nanmax(at, a2 * rho_lo);
+        fx0 = fx1;
+        fx1 = phi(a2);
+    }
+    *alpha = a2;
+    *phi_alpha = fx1;
+    return 0;
+}
+
+// Products with the Newton matrix at the stages in E.  Residual rows are
+// ordered [Phi_1..Phi_N ; bc rows in input order].
+void jac_mul(const Ctx &C, int N, const double *mesh, const Eval &E, const std::vector<BcRow> &rows,
+             const double *v, double *out) {
+    const int M = C.M, MM = M * M;
+    std::vector<double> S(MM), R(MM);
+    for (int i = 0; i < N; ++i) {
+        interval_jac(C, mesh[i], mesh[i + 1] - mesh[i], &E.Y[(size_t)i * C.T.s * M], S.data(), R.data());
+        for (int a = 0; a < M; ++a) {
+            double acc = 0;
+            for (int b = 0; b < M; ++b) acc += S[a * M + b] * v[i * M + b] + R[a * M + b] * v[(i + 1) * M + b];
+            out[(size_t)i * M + a] = acc;
+        }
+    }
+    for (size_t k = 0; k < rows.size(); ++k) out[(size_t)N * M + k] = v[rows[k].node * M + rows[k].comp];
+}
+
+void jac_tmul(const Ctx &C, int N, const double *mesh, const Eval &E, const std::vector<BcRow> &rows,
+              const double *w, double *out) {
+    const int M = C.M, MM = M * M;
+    std::vector<double> S(MM), R(MM);
+    std::fill(out, out + (size_t)M * (N + 1), 0.0);
+    for (int i = 0; i < N; ++i) {
+        interval_jac(C, mesh[i], mesh[i + 1] - mesh[i], &E.Y[(size_t)i * C.T.s * M], S.data(), R.data());
+        for (int b = 0; b < M; ++b) {
+            double sa = 0, sr = 0;
+            for (int a = 0; a < M; ++a) {
+                sa += S[a * M + b] * w[(size_t)i * M + a];
+                sr += R[a * M + b] * w[(size_t)i * M + a];
+            }
+            out[i * M + b] += sa;
+            out[(i + 1) * M + b] += sr;
+        }
+    }
+    for (size_t k = 0; k < rows.size(); ++k) out[rows[k].node * M + rows[k].comp] += w[(size_t)N * M + k];
+}
+
+// Dogleg step of radius `radius` from the Newton step dn, the steepest-descent
+// direction g = J'F with |g|^2 = l2 and |J g|^2 = jg2, and ss = |F|^2.  Writes
+// the step into d and returns the predicted reduction |F|^2 - |F + J d|^2 of the
+// linear model (F + J dn = 0 for the Newton step):
+//   Newton step:            pred = ss
+//   cut steepest descent:   d = -(radius/|g|) g, pred = 2 radius |g| - radius^2 jg2 / l2
+//   dogleg, d = dc + tau (dn - dc), dc = -(l2/jg2) g:
+//                           pred = ss - (1 - tau)^2 (ss - l2^2 / jg2)
+// dn_norm2 = |dn|^2 (the caller's test |dn| <= radius selects the Newton step).
+double dogleg(int n, const double *dn, const double *g, double l2, double jg2, double ss, double radius,
+              double *d) {
+    const double lg = std::sqrt(l2);
+    const double dcauchy = l2 * lg / jg2;
+    if (dcauchy >= radius) {
+        const double lam = radius / lg;
+        for (int i = 0; i < n; ++i) d[i] = -lam * g[i];
+        return 2 * radius * lg - radius * radius * jg2 / l2;
+    }
+    const double beta = l2 / jg2;
+    double a = 0, b = 0;
+    for (int i = 0; i < n; ++i) {
+        const double dc = -beta * g[i], di = dn[i] - dc;
+        a += di * di;
+        b += 2 * dc * di;
+    }
+    const double c = dcauchy * dcauchy - radius * radius;
+    const double tau = (-b + std::sqrt(std::max(0.0, b * b - 4 * a * c))) / (2 * a);
+    for (int i = 0; i < n; ++i) {
+        const double dc = -beta * g[i];
+        d[i] = dc + tau * (dn[i] - dc);
+    }
+    return ss - (1 - tau) * (1 - tau) * (ss - l2 * l2 / jg2);
+}
+
+// One NewtonRaphson stage (linesearch: with BackTracking).
+int newton_stage(const Ctx &C, int N, const double *mesh, std::vector<double> &y, const std::vector<BcRow> &rows,
+                 double tol, int maxiters, bool linesearch, NlOut &out) {
+    const int n = C.M * (N + 1);
+    std::vector<double> delta(n), yt(n);
+    Eval E, Et;
+    Band B;
+    for (int it = 0;; ++it) {
+        evaluate(C, N, mesh, y.data(), rows, E, false);
+        out.fnorm = max_abs_residual(E);
+        if (!std::isfinite(out.fnorm)) return BVP_RET_UNSTABLE;
+        if (out.fnorm <= tol) return BVP_RET_SUCCESS;
+        if (it >= maxiters) return BVP_RET_MAXITERS;
+        if (newton_solve(C, N, mesh, y.data(), rows, E, delta.data(), B)) return BVP_RET_FAILURE;
+        out.iters++;
+        double alpha = 1.0;
+        if (linesearch) {
+            const double ss = sumsq_residual(E);
+            auto phi = [&](double a) {
+                for (int i = 0; i < n; ++i) yt[i] = y[i] + a * delta[i];
+                evaluate(C, N, mesh, yt.data(), rows, Et, false);
+                return 0.5 * sumsq_residual(Et);
+            };
+            double pa;
+            if (backtracking(phi, 0.5 * ss, -ss, &alpha, &pa)) return BVP_RET_FAILURE;
+        }
+        for (int i = 0; i < n; ++i) y[i] += alpha * delta[i];
+    }
+}
+
+int trust_region_stage(const Ctx &C, int N, const double *mesh, std::vector<double> &y,
+                       const std::vector<BcRow> &rows, double tol, int maxiters, NlOut &out) {
+    const int M = C.M, n = M * (N + 1), nr = M * N + (int)rows.size();
+    std::vector<double> dn(n), g(n), d(n), yt(n), F(nr), Jg(nr);
+    Eval E, Et;
+    Band B;
+    evaluate(C, N, mesh, y.data(), rows, E, false);
+    double ss = sumsq_residual(E);
+    double ymax = -INFINITY, ymin = INFINITY;
+    for (double v : y) { ymax = std::max(ymax, v); ymin = std::min(ymin, v); }
+    double radius = std::max(std::sqrt(ss), ymax - ymin) / 11;
+    int shrinks = 0;
+    bool fresh = true;  // dn, g belong to the current y
+    double l2 = 0, jg2 = 0, dn2 = 0;
+    for (int it = 0;; ++it) {
+        out.fnorm = max_abs_residual(E);
+        if (!std::isfinite(out.fnorm)) return BVP_RET_UNSTABLE;
+        if (out.fnorm <= tol) return BVP_RET_SUCCESS;
+        if (it >= maxiters) return BVP_RET_MAXITERS;
+        if (fresh) {
+            if (newton_solve(C, N, mesh, y.data(), rows, E, dn.data(), B)) return BVP_RET_FAILURE;
+            dn2 = 0;
+            for (double v : dn) dn2 += v * v;
+            for (int i = 0; i < M * N; ++i) F[i] = E.phi[i];
+            for (size_t k = 0; k < rows.size(); ++k) F[(size_t)M * N + k] = E.bc[k];
+            jac_tmul(C, N, mesh, E, rows, F.data(), g.data());
+            l2 = 0;
+            for (double v : g) l2 += v * v;
+            jac_mul(C, N, mesh, E, rows, g.data(), Jg.data());
+            jg2 = 0;
+            for (double v : Jg) jg2 += v * v;
+            fresh = false;
+        }
+        out.iters++;
+        double pred;
+        if (std::sqrt(dn2) <= radius) {
+            d = dn;
+            pred = ss;
+        } else {
+            pred = dogleg(n, dn.data(), g.data(), l2, jg2, ss, radius, d.data());
+        }
+        for (int i = 0; i < n; ++i) yt[i] = y[i] + d[i];
+        evaluate(C, N, mesh, yt.data(), rows, Et, false);
+        const double sst = sumsq_residual(Et);
+        const double rho = (ss - sst) / pred;
+        if (rho > 1e-4) {  // accept
+            y = yt;
+            std::swap(E, Et);
+            ss = sst;
+            fresh = true;
+        }
+        if (rho < 0.25) {
+            radius *= 0.25;
+            if (++shrinks > 32) { out.fnorm = max_abs_residual(E); return BVP_RET_FAILURE; }
+        } else {
+            shrinks = 0;
+            if (rho > 0.75) radius *= 2;
+        }
+    }
+}
+
+NlOut newton(const Ctx &C, int N, const double *mesh, std::vector<double> &y,
+             const std::vector<BcRow> &rows, double tol, int maxiters, int stages) {
+    const std::vector<double> y0 = y;
+    NlOut out{BVP_RET_MAXITERS, 0, 0, -1};
+    bool first = true;
+    for (int st = 0; st < 3; ++st) {
+        if (!(stages & (1 << st))) continue;
+        if (!first) y = y0;
+        first = false;
+        out.stage = st;
+        out.ret = st < 2 ? newton_stage(C, N, mesh, y, rows, tol, maxiters, st == 1, out)
+                         : trust_region_stage(C, N, mesh, y, rows, tol, maxiters, out);
+        if (out.ret == BVP_RET_SUCCESS) break;
     }
     return out;
 }
@@ -818,9 +1026,10 @@ int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
     for (;;) {
         ++outer;
         if (resolve_bc(P, N, mesh.data(), rows)) return BVP_EINVAL;
-        NlOut nl = newton(C, N, mesh.data(), y, rows, tol, o.max_newton_iters, o.linesearch_fallback != 0);
+        NlOut nl = newton(C, N, mesh.data(), y, rows, tol, o.max_newton_iters, o.nlsolve_stages);
         newton_iters += nl.iters;
-        if (nl.fallback) fb |= 1;
+        if (nl.stage == 1) fb |= 1;
+        if (nl.stage == 2) fb |= 16;
         nl_ret = nl.ret;
         error_norm = 2 * o.abstol;
         if (!o.adaptive) { info = nl.ret; break; }
@@ -863,6 +1072,7 @@ int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
     res.outer_iters = outer;
     res.fallback_used = fb;
     res.defect_norm = last_defect;
+    res.err_vs_fp64 = NAN;
     if (resolve_bc(P, N, mesh.data(), rows) == 0) {
         evaluate(C, N, mesh.data(), y.data(), rows, E, false);
         res.resid_norm = max_abs_residual(E);
@@ -888,11 +1098,72 @@ void ref_opts_default(bvp_opts *o) {
     o->defect_threshold = 0.1;
     o->max_num_subintervals = 3000;
     o->max_newton_iters = 1000;
-    o->linesearch_fallback = 1;
+    o->nlsolve_stages = 7;
     o->transfer_mode = BVP_TRANSFER_REFERENCE;
     o->precision = BVP_PREC_FP64;
     o->refine_iters = 0;
     o->warm_start = BVP_WARM_NONE;
+    o->report_err = 0;
+}
+
+// BackTracking on the polynomial merit function phi(a) = sum_k coef[k] a^k
+// (phi(0) = coef[0], phi'(0) = coef[1]).  trace (room for max_trace) receives
+// the trial steps in order, *n_trace their number.  Returns 0, or 1 when the
+// reductions ran out.
+int ref_backtracking_poly(const double *coef, int ncoef, double *alpha, double *trace, int max_trace,
+                          int *n_trace) {
+    int nt = 0;
+    auto phi = [&](double a) {
+        double v = 0, p = 1;
+        for (int k = 0; k < ncoef; ++k) { v += coef[k] * p; p *= a; }
+        if (nt < max_trace) trace[nt] = a;
+        ++nt;
+        return v;
+    };
+    double pa;
+    const int rc = backtracking(phi, coef[0], coef[1], alpha, &pa);
+    *n_trace = nt;
+    return rc;
+}
+
+// Dogleg step for a dense system (J n x n row-major, F) of radius `radius`:
+// d and the predicted reduction |F|^2 - |F + J d|^2 of the linear model.
+int ref_dogleg(int n, const double *J, const double *F, double radius, double *d, double *pred) {
+    std::vector<double> A(J, J + (size_t)n * n), dn(n), g(n, 0.0), Jg(n, 0.0);
+    // Newton step by Gaussian elimination with partial pivoting
+    for (int i = 0; i < n; ++i) dn[i] = -F[i];
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        for (int i = j + 1; i < n; ++i) if (std::fabs(A[(size_t)i * n + j]) > std::fabs(A[(size_t)p * n + j])) p = i;
+        if (A[(size_t)p * n + j] == 0.0) return 1;
+        for (int c = 0; c < n; ++c) std::swap(A[(size_t)j * n + c], A[(size_t)p * n + c]);
+        std::swap(dn[j], dn[p]);
+        for (int i = j + 1; i < n; ++i) {
+            const double m = A[(size_t)i * n + j] / A[(size_t)j * n + j];
+            for (int c = j; c < n; ++c) A[(size_t)i * n + c] -= m * A[(size_t)j * n + c];
+            dn[i] -= m * dn[j];
+        }
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        for (int c = j + 1; c < n; ++c) dn[j] -= A[(size_t)j * n + c] * dn[c];
+        dn[j] /= A[(size_t)j * n + j];
+    }
+    double ss = 0, dn2 = 0, l2 = 0, jg2 = 0;
+    for (int i = 0; i < n; ++i) { ss += F[i] * F[i]; dn2 += dn[i] * dn[i]; }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) g[i] += J[(size_t)k * n + i] * F[k];
+    for (int i = 0; i < n; ++i) l2 += g[i] * g[i];
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < n; ++k) Jg[i] += J[(size_t)i * n + k] * g[k];
+        jg2 += Jg[i] * Jg[i];
+    }
+    if (std::sqrt(dn2) <= radius) {
+        for (int i = 0; i < n; ++i) d[i] = dn[i];
+        *pred = ss;
+        return 0;
+    }
+    *pred = dogleg(n, dn.data(), g.data(), l2, jg2, ss, radius, d);
+    return 0;
 }
 
 void ref_uniform_mesh(double t0, double t1, int N, double *mesh) { uniform_mesh(t0, t1, N, mesh); }

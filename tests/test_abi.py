@@ -55,6 +55,8 @@ def test_constants_match_header():
         "BVP_WARM_NONE": _abi.WARM_NONE, "BVP_WARM_MLP": _abi.WARM_MLP,
         "BVP_MLP_NPARAM": _abi.MLP_NPARAM, "BVP_FEAT_PARAM": _abi.FEAT_PARAM,
         "BVP_FEAT_LOG10_PARAM": _abi.FEAT_LOG10_PARAM,
+        "BVP_STAGE_NEWTON": _abi.STAGE_NEWTON, "BVP_STAGE_BACKTRACKING": _abi.STAGE_BACKTRACKING,
+        "BVP_STAGE_TRUST_REGION": _abi.STAGE_TRUST_REGION, "BVP_STAGES_DEFAULT": _abi.STAGES_DEFAULT,
     }
     for k, v in pairs.items():
         assert d[k] == v, k
@@ -123,3 +125,45 @@ def test_context_fails_loudly_without_gpu():
 def test_mlp_parameter_count():
     """4 -> 256 -> 256 -> 256 -> 2 with biases."""
     assert _abi.MLP_NPARAM == 4 * 256 + 256 + 2 * (256 * 256 + 256) + 256 * 2 + 2
+
+
+_JULIA_TYPES = {"Int32": "int32_t", "Int64": "int64_t", "UInt64": "uint64_t", "Float64": "double",
+                "Float32": "float", "Cint": "int32_t"}
+
+
+def _c_struct_fields(name):
+    """Field names and C types of a typedef'd struct of the header, in order."""
+    text = re.sub(r"/\*.*?\*/", "", HEADERS[0].read_text(), flags=re.S)
+    m = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + r"\s*;", text)
+    assert m, name
+    out = []
+    for decl in m.group(1).split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        ptr = "*" in decl
+        decl = decl.replace("*", " ").replace("const ", "")
+        parts = decl.replace(",", " , ").split()
+        ctype = parts[0]
+        names = [p for p in parts[1:] if p != ","]
+        for nm in names:
+            out.append((nm, "ptr" if ptr else ctype))
+    return out
+
+
+def _julia_struct_fields(jname):
+    text = (ROOT / "INTEGRATION.md").read_text()
+    m = re.search(r"struct " + jname + r"\b[^\n]*\n(.*?)\nend", text, re.S)
+    assert m, jname
+    out = []
+    for nm, jt in re.findall(r"(\w+)::([\w{}]+)", m.group(1)):
+        out.append((nm, "ptr" if jt.startswith("Ptr") else _JULIA_TYPES[jt]))
+    return out
+
+
+@pytest.mark.parametrize("jname,cname", [("BvpInstance", "bvp_instance"), ("BvpOpts", "bvp_opts"),
+                                         ("BvpResult", "bvp_result"), ("BvpTrainOpts", "bvp_train_opts")])
+def test_integration_julia_structs_match_header(jname, cname):
+    """The Julia mirrors in INTEGRATION.md have the header's fields, in order, with matching types
+    (a Julia caller passes these structs by reference to the library)."""
+    assert _julia_struct_fields(jname) == _c_struct_fields(cname)
