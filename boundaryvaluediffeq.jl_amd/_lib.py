@@ -47,6 +47,7 @@ def lib():
         "bvp_mirk_newton_step": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, Inst, _dp]),
         "bvp_mirk_defect": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp]),
         "bvp_mirk_mesh_select": (C.c_int, [vp, Opts, C.c_int32, Inst, C.c_int32, _ip, _ip, _dp, _dp]),
+        "bvp_mirk_interpolate": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _ip, _dp, _dp, _dp]),
         "bvp_batch_create": (C.c_int, [vp, Opts, C.c_int32, Inst, C.POINTER(vp)]),
         "bvp_batch_run": (C.c_int, [vp, vp]),
         "bvp_batch_sync": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
@@ -89,7 +90,7 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_mlp_create", "bvp_mlp_destroy", "bvp_mlp_set_normalization", "bvp_mlp_get_params",
             "bvp_mlp_set_params", "bvp_mlp_gradient", "bvp_mlp_predict", "bvp_mlp_train",
             "bvp_batch_set_warm_start", "bvp_batch_timing", "bvp_batch_learned_guess", "bvp_batch_autotune",
-            "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision"]
+            "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision", "bvp_mirk_interpolate"]
 
 
 class Context:
@@ -194,6 +195,27 @@ def defect(ctx, method, specs):
     for i, s in enumerate(specs):
         out.append((err[o:o + s.N * M].reshape(s.N, M).copy(), float(dn[i])))
         o += s.N * M
+    return out
+
+
+def interpolate(ctx, method, specs, tvals, derivative=True):
+    """Dense output: the continuous extension of each instance's (mesh, guess) at the
+    points tvals[i]; returns [(u (k, M), du (k, M) or None)] per instance."""
+    arr = instance_array(specs)
+    M = specs[0].M
+    ts = [np.ascontiguousarray(t, dtype=np.float64).reshape(-1) for t in tvals]
+    nt = np.array([len(t) for t in ts], dtype=np.int32)
+    tt = np.concatenate(ts) if len(ts) else np.zeros(0)
+    u = np.zeros(max(1, len(tt)) * M)
+    du = np.zeros(max(1, len(tt)) * M) if derivative else None
+    ctx.check(lib().bvp_mirk_interpolate(ctx.handle, method, len(specs), arr, nt.ctypes.data_as(_ip),
+                                         tt.ctypes.data_as(_dp), u.ctypes.data_as(_dp),
+                                         du.ctypes.data_as(_dp) if derivative else _dp()), "interpolate")
+    out, o = [], 0
+    for k in nt:
+        out.append((u[o * M:(o + k) * M].reshape(k, M).copy(),
+                    du[o * M:(o + k) * M].reshape(k, M).copy() if derivative else None))
+        o += k
     return out
 
 

@@ -697,6 +697,52 @@ int bvp_mirk_defect(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance 
     return rc;
 }
 
+int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, const int32_t *n_t,
+                         const double *t, double *u, double *du) {
+    if (!ctx || !n_t || !u) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    long long nq = 0;
+    for (int i = 0; i < n; ++i) {
+        if (n_t[i] < 0) { set_err(ctx, "instance %d: negative point count", i); return BVP_EINVAL; }
+        nq += n_t[i];
+    }
+    if (nq > 0 && !t) return BVP_EINVAL;
+    if (n > 0 && !wave_supports(inst[0].M, inst[0].family)) {
+        set_err(ctx, "dense output serves the M <= 4 families");
+        return BVP_ENOTSUP;
+    }
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, method, OP_RESIDUAL, n, inst, 0, nullptr, &B);
+    if (rc) return rc;
+    if (B->block) {
+        delete B;
+        set_err(ctx, "dense output serves the M <= 4 families");
+        return BVP_ENOTSUP;
+    }
+    std::vector<int> qi((size_t)nq);
+    for (int i = 0, q = 0; i < n; ++i)
+        for (int k = 0; k < n_t[i]; ++k) qi[q++] = i;
+    std::vector<double> tv(t, t + nq);
+    int *dqi = nullptr;
+    double *dt = nullptr, *du_ = nullptr, *ddu = nullptr;
+    const int M = B->M;
+    if (nq > 0 &&
+        ((rc = upload(ctx, B->arena, dqi, qi)) || (rc = upload(ctx, B->arena, dt, tv)) ||
+         (rc = dalloc(ctx, B->arena, du_, (size_t)nq * M)) || (du && (rc = dalloc(ctx, B->arena, ddu, (size_t)nq * M))))) {
+        delete B;
+        return rc;
+    }
+    if (nq > 0) {
+        hipError_t e = wave_launch_interp(B->dev, method, dqi, dt, du_, ddu, nq, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(u, du_, (size_t)nq * M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && du) e = hipMemcpy(du, ddu, (size_t)nq * M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "interpolate: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+    }
+    delete B;
+    return rc;
+}
+
 int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const bvp_instance *inst, int32_t capacity,
                          int32_t *new_N, int32_t *info, double *new_mesh, double *new_y) {
     if (!ctx || !opts || !new_N || !info || !new_mesh || !new_y) return BVP_EINVAL;

@@ -84,6 +84,10 @@ hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blo
 hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s);
 hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, double *out_mesh, double *out_y,
                               int max_nodes, hipStream_t s);
+// dense output at nq points: qinst[q] instance of point q, tq[q] its t; u (and du
+// when non-null) receive M values per point
+hipError_t wave_launch_interp(const BatchDev &B, int method, const int *qinst, const double *tq, double *u,
+                              double *du, long long nq, hipStream_t s);
 bool wave_supports(int M, int family);
 int wave_rel_fields(int M);
 

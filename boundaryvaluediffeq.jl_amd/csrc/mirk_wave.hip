@@ -119,6 +119,17 @@ hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, doubl
     return hipGetLastError();
 }
 
+hipError_t wave_launch_interp(const BatchDev &B, int method, const int *qinst, const double *tq, double *u,
+                              double *du, long long nq, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nq + 255) / 256));
+#define CALL_INTERP(m, me)                                                      \
+    k_interp<m, me><<<grid, 256, 0, s>>>(B, qinst, tq, u, du, nq);              \
+    return hipGetLastError()
+    DISPATCH_M_METHOD(B.M, method, CALL_INTERP);
+#undef CALL_INTERP
+}
+
 bool wave_supports(int M, int family) {
     switch (M) {
     case 1: return family_supported<1>(family);

@@ -2127,6 +2127,44 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev B, const long long *out
     }
 }
 
+// Dense output (MIRKInterpolation, MIRK/interpolation.jl:17-206): one thread per
+// query point.  The solution of instance b is its packed (mesh, y); point q lies
+// in interval(mesh, t) (a node belongs to the interval on its left; points
+// outside [t0, t1] extrapolate the end intervals), tau = (t - t_i)/h, and
+//   u(t) = y_i + h sum_r w_r(tau) K_r,   u'(t) = sum_r w'_r(tau) K_r
+// over the discrete stages K at (y_i, y_{i+1}) and the extra interpolation stages.
+template <int M, int METHOD>
+__global__ __launch_bounds__(256) void k_interp(BatchDev B, const int *qinst, const double *tq, double *uo,
+                                                double *duo, long long nq) {
+    using TB = Tab<METHOD>;
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    const int b = qinst[q];
+    Inst<M> I;
+    load_inst<M>(B, b, I);
+    const long long off = B.init_off[b];
+    const double *mesh = as_global(B.init_mesh + off);
+    const double *y = as_global(B.init_y + off * M);
+    const int N = B.N0[b];
+    const double t = tq[q];
+    const int i = interval_of(mesh, N, t);
+    const double ti = mesh[i], h = mesh[i + 1] - ti, tau = (t - ti) / h;
+    double yi[M], yj[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)i * M + c]; yj[c] = y[(size_t)(i + 1) * M + c]; }
+    double K[TB::s][M], KI[TB::sstar - TB::s][M];
+    stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
+    interp_stages<M, METHOD>(I.fam, I.P, ti, h, yi, yj, K, KI);
+    double w[TB::sstar], wp[TB::sstar], z[M], zp[M];
+    interp_weights<METHOD>(tau, w, wp);
+    sum_stages<M, METHOD>(h, yi, K, KI, w, wp, z, zp);
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        uo[q * M + c] = z[c];
+        if (duo) duo[q * M + c] = zp[c];
+    }
+}
+
 // ------------------------------------------------------------------ host launchers
 template <int M, int METHOD, int WPE, int FAM = -1, bool F32 = false>
 hipError_t launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {

@@ -19,6 +19,11 @@ SEED_BRATU_AB_TRAIN = 0xB0A70004
 SEED_BRATU_AB_EVAL = 0xB0A70044
 SEED_SINGPERT_TRAIN = 0xB0A70055
 
+# Newton tolerance max|F| of the Bratu workloads when solutions are compared at 1e-9:
+# the reference stops Newton at max|F| <= abstol (MIRK/mirk.jl:51), which leaves two
+# solves of one instance up to norm(J^-1) * abstol apart; both sides stop here instead
+HEADLINE_NEWTON_ABSTOL = 1e-11
+
 
 def _u01(seed, n, ndraw):
     """n streams x ndraw uniforms in [0, 1) (splitmix64, vectorised with wrapping uint64)."""

@@ -238,6 +238,18 @@ int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
                          int32_t *new_N, int32_t *info, double *new_mesh,
                          double *new_y);
 
+/* Dense output: the continuous MIRK extension of a solution and its first
+ * derivative at arbitrary points -- sol(t) and sol(t, Val{1}) of
+ * MIRKInterpolation (MIRK/interpolation.jl:17-206).  The solution of instance i
+ * is its (mesh, guess): N intervals, the N+1 nodes and the point-major state, as
+ * a solve returns them.  n_t[i] points of instance i are packed in t (instance
+ * order); u receives M values per point (packed the same way), du (optional,
+ * NULL to skip) the derivative.  A point lies in interval(mesh, t)
+ * (Core/utils.jl:119-121: a node belongs to the interval on its left; points
+ * outside [t0, t1] extrapolate the end intervals).  M <= 4 families. */
+int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst,
+                         const int32_t *n_t, const double *t, double *u, double *du);
+
 /* ---- batch plans: upload once, solve many times (no counterpart in the
  * reference, whose batches are EnsembleProblem solves run one per thread,
  * MIRKt/Core/ensemble_tests.jl:7-41).  For callers that time or repeat the

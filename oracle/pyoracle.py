@@ -81,6 +81,7 @@ def lib():
         L.ref_opts_default.argtypes = [C.POINTER(Opts)]
         L.ref_backtracking_poly.argtypes = [_dp, C.c_int, _dp, _dp, C.c_int, _ip]
         L.ref_dogleg.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
+        L.ref_interpolate.argtypes = [C.c_int, C.POINTER(_Instance), C.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -236,6 +237,15 @@ def mesh_select_errors(order, abstol, max_sub, mesh, errors):
     if r < 0:
         return None
     return out[:r + 1].copy()
+
+
+def interpolate(method, spec, t):
+    """Dense output of (mesh, guess) at the points t: (u (k, M), du (k, M))."""
+    P = _Pinned(spec)
+    t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+    u, du = np.zeros(max(1, len(t)) * spec.M), np.zeros(max(1, len(t)) * spec.M)
+    _check(lib().ref_interpolate(method, C.byref(P.inst), len(t), _d(t), _d(u), _d(du)), "interpolate")
+    return u[:len(t) * spec.M].reshape(-1, spec.M), du[:len(t) * spec.M].reshape(-1, spec.M)
 
 
 def backtracking_poly(coef, max_trace=64):

@@ -1288,6 +1288,29 @@ int ref_defect(int method, const bvp_instance *P, double *errors, double *defect
     return 0;
 }
 
+// Dense output at (mesh, guess): sol(t) and sol(t, Val{1}) of MIRKInterpolation
+// (MIRK/interpolation.jl:77-196): interval(mesh, t), tau = (t - t_i)/h,
+// u = y_i + h (K w + KI w*), u' = K w' + KI w'*, stages at (y_i, y_{i+1}).
+int ref_interpolate(int method, const bvp_instance *P, int nt, const double *t, double *u, double *du) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    Ctx C(*P, method);
+    std::vector<double> mesh, y;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    const int M = P->M, s = C.T.s, ns = C.T.sstar - C.T.s, N = P->N;
+    std::vector<double> K(s * M), Y(s * M), KI(std::max(1, ns) * M), w(9), wp(9);
+    for (int q = 0; q < nt; ++q) {
+        const int i = interval_of(mesh.data(), N, t[q]);
+        const double h = mesh[i + 1] - mesh[i], tau = (t[q] - mesh[i]) / h;
+        interval_stages(C, mesh[i], h, &y[(size_t)i * M], &y[(size_t)(i + 1) * M], K.data(), Y.data());
+        interval_interp_stages(C, mesh[i], h, &y[(size_t)i * M], &y[(size_t)(i + 1) * M], K.data(), KI.data());
+        interp_weights(method, tau, w.data(), wp.data());
+        sum_stages(C, h, &y[(size_t)i * M], K.data(), KI.data(), w.data(), wp.data(), u + (size_t)q * M,
+                   du ? du + (size_t)q * M : nullptr);
+    }
+    return 0;
+}
+
 int ref_mesh_select(const bvp_opts *o, const bvp_instance *P, int capacity, int32_t *new_N, int32_t *info,
                     double *new_mesh, double *new_y) {
     int rc = check_instance(*P);

@@ -1,0 +1,181 @@
+"""The benchmark's own configurations against the oracle at 1e-9.
+
+Headline (16,384 Bratu BVPs): learned guess at 4 anchors, FP32 elimination
+records, hipGraph replay, autotuned launch.  Config 5 (adaptive singular
+perturbation): a network on log10(eps) attached and FP32 records requested.
+
+Both sides stop Newton at the same tightened max|F| (the reference stops at
+abstol, which leaves two solves of one instance up to norm(J^-1) * abstol
+apart).  Every success must lie within 1e-9 relative (scale max(1, |u|)) of the
+oracle: directly where the final meshes agree, else as the oracle's own Newton
+correction at the GPU state on the GPU's mesh (its distance from the discrete
+solution there).  The per-result err_vs_fp64 the library reports must be within
+the same bound and agree with the oracle's correction.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from boundaryvaluediffeq_amd import _abi, _lib, learned, synthetic
+import pyoracle as O
+import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return _lib.context(0)
+
+
+def _check_against_oracle(specs, got, oracle_opts, method=4):
+    ref = O.solve_batch(oracle_opts, specs, nthreads=0)
+    worst, same, other = 0.0, 0, 0
+    for s, g, r in zip(specs, got, ref):
+        assert g.retcode == r.retcode, (g.retcode, r.retcode, s.params, s.N)
+        if r.retcode != 0:
+            continue
+        assert g.defect_norm <= oracle_opts.abstol
+        d = O.newton_step(method, s.with_state(g.t, g.u.reshape(-1))).reshape(-1, s.M)
+        corr = float(np.max(np.abs(d) / np.maximum(1.0, np.abs(g.u))))
+        if g.N == r.N and np.max(np.abs(g.t - r.t)) <= 1e-12:
+            e = float(np.max(np.abs(g.u - r.u) / np.maximum(1.0, np.abs(r.u))))
+            same += 1
+        else:
+            e = corr
+            other += 1
+        assert e <= TOL, (e, s.params, s.N, g.N, r.N)
+        # the library's own per-result figure is that correction, computed on the device
+        assert g.err_vs_fp64 <= TOL
+        assert abs(g.err_vs_fp64 - corr) <= 1e-2 * corr + 1e-15, (g.err_vs_fp64, corr)
+        worst = max(worst, e)
+    return worst, same, other
+
+
+@pytest.fixture(scope="module")
+def bratu_ab_net(ctx):
+    """A network trained on 4,096 plain solves of the Bratu (a, b) family."""
+    _say("training (Bratu a, b)")
+    o = _abi.default_opts(_abi.MIRK4)
+    o.abstol = 1e-8
+    o.max_newton_iters = 50
+    specs, _, _ = synthetic.bratu_ab_batch(4096, seed=synthetic.SEED_BRATU_AB_TRAIN)
+    x, y, _ = learned.harvest(ctx, specs, o, per_solve=64)
+    net, loss, _ = learned.train(ctx, x, y, epochs=30, batch=4096, lr=2e-3, lr_min=2e-5, warmup=50)
+    _say("  loss", loss[0], "->", loss[-1])
+    return net
+
+
+def _headline_plan(ctx, specs, net, **kw):
+    o = _abi.default_opts(_abi.MIRK4)
+    o.abstol = 1e-8
+    o.newton_abstol = synthetic.HEADLINE_NEWTON_ABSTOL
+    o.precision = _abi.PREC_FP32_REFINE64
+    o.report_err = 1
+    for k, v in kw.items():
+        setattr(o, k, v)
+    b = _lib.Batch(ctx, o, specs)
+    b.set_graph(True)
+    b.set_warm_start(net, _abi.FEAT_PARAM, 4)
+    return b, o
+
+
+def test_headline_configuration_within_1e9(ctx, bratu_ab_net):
+    """Config-2 instances (lambda ~ U(0.1, 3.4), N ~ U{64..1024}, the headline's
+    generator) with the headline's plan, after autotuning; two replays give the same bits."""
+    _say("headline configuration")
+    specs, _, Ns = synthetic.bratu_batch(512)
+    assert Ns.max() > 1000
+    b, o = _headline_plan(ctx, specs, bratu_ab_net)
+    ms, cfg = b.autotune(repeats=1)
+    _say("  candidates", ms, "chosen", cfg, "records", b.precision())
+    b.run()
+    b.sync()
+    first = b.results()
+    b.run()
+    b.sync()
+    again = b.results()
+    for a, c in zip(first, again):
+        assert a.retcode == c.retcode and a.N == c.N and np.array_equal(a.u, c.u)
+    assert all(r.retcode == 0 for r in first)
+    assert np.mean([r.newton_iters for r in first]) < 4.0
+    oo = O.opts(4, abstol=1e-8, newton_abstol=synthetic.HEADLINE_NEWTON_ABSTOL)
+    worst, same, other = _check_against_oracle(specs, first, oo)
+    _say(f"  worst {worst:.3e}, same mesh {same}, on the GPU mesh {other}")
+
+
+def test_headline_plan_on_bratu_ab_within_1e9(ctx, bratu_ab_net):
+    """The same plan on unseen Bratu (a, b) instances (the network's own family, part of
+    it without a solution: those end like the oracle's, Newton capped at 60 steps)."""
+    _say("headline plan, Bratu (a, b)")
+    specs, _, _ = synthetic.bratu_ab_batch(256, seed=synthetic.SEED_BRATU_AB_EVAL)
+    b, o = _headline_plan(ctx, specs, bratu_ab_net, max_newton_iters=60)
+    b.run()
+    b.sync()
+    got = b.results()
+    oo = O.opts(4, abstol=1e-8, newton_abstol=synthetic.HEADLINE_NEWTON_ABSTOL, max_newton_iters=60)
+    worst, same, other = _check_against_oracle(specs, got, oo)
+    _say(f"  worst {worst:.3e}, same mesh {same}, on the GPU mesh {other}, "
+         f"solved {sum(r.retcode == 0 for r in got)}/{len(got)}")
+    assert same + other >= 200
+
+
+@pytest.fixture(scope="module")
+def singpert_net(ctx):
+    """A network on (log10 eps, t) trained on 2,000 plain solves of the family."""
+    _say("training (singular perturbation)")
+    o = _abi.default_opts(_abi.MIRK4)
+    o.abstol = 1e-6
+    o.max_num_subintervals = 16384
+    specs, _ = synthetic.singpert_batch(2000, seed=synthetic.SEED_SINGPERT_TRAIN)
+    x, y, _ = learned.harvest(ctx, specs, o, per_solve=64, kind=_abi.FEAT_LOG10_PARAM)
+    net, loss, _ = learned.train(ctx, x, y, epochs=20, batch=4096, lr=2e-3, lr_min=2e-5, warmup=50)
+    _say("  loss", loss[0], "->", loss[-1])
+    return net
+
+
+@pytest.mark.parametrize("autotune", [True, False], ids=["autotuned", "fp32-records"])
+def test_config5_learned_mixed_adaptive(ctx, singpert_net, autotune):
+    """Config 5 as named: eps u'' + t u' - u = 0, eps in {1e-4, 3e-4, 1e-3, 1e-2, 1e-1},
+    adaptive from N = 32 to <= 16,384 at abstol 1e-6, network attached (features log10
+    eps, t), FP32 records requested; the autotuned plan (which may keep FP64 records on
+    this stiff family) and the plan held to FP32 records."""
+    _say("config 5", autotune)
+    eps = [1e-4, 3e-4, 1e-3, 1e-2, 1e-1]
+    specs = [cases.singpert(e, N=32) for e in eps]
+    o = _abi.default_opts(_abi.MIRK4)
+    o.abstol = 1e-6
+    o.newton_abstol = 1e-10
+    o.max_num_subintervals = 16384
+    o.precision = _abi.PREC_FP32_REFINE64
+    o.report_err = 1
+    b = _lib.Batch(ctx, o, specs)
+    b.set_warm_start(singpert_net, _abi.FEAT_LOG10_PARAM, 0)
+    if autotune:
+        ms, cfg = b.autotune(repeats=1)
+        _say("  candidates", ms, "records", b.precision())
+    else:
+        assert b.precision() == _abi.PREC_FP32_REFINE64
+    b.run()
+    b.sync()
+    got = b.results()
+    for s, g in zip(specs, got):
+        assert g.retcode == 0 and g.defect_norm <= 1e-6
+        d = O.newton_step(4, s.with_state(g.t, g.u.reshape(-1))).reshape(-1, 2)
+        corr = float(np.max(np.abs(d) / np.maximum(1.0, np.abs(g.u))))
+        _say(f"  eps {s.params[0]:.0e}: N {g.N}, iters {g.newton_iters}, fallback {g.fallback_used}, "
+             f"oracle correction {corr:.3e}, err_vs_fp64 {g.err_vs_fp64:.3e}")
+        assert corr <= TOL
+        assert g.err_vs_fp64 <= TOL
+    ref = O.solve_batch(O.opts(4, abstol=1e-6, newton_abstol=1e-10, max_num_subintervals=16384), specs)
+    for g, r in zip(got, ref):
+        assert r.retcode == 0
+        assert abs(g.N - r.N) <= max(4, r.N // 4)  # adaptive mesh sequences are rounding-sensitive
