@@ -34,12 +34,15 @@ def log(*a):
 
 
 def algorithmic_bytes(M, counters, n0_nodes, nf_nodes, p_j=8):
-    """Bytes of the streaming model: per interval and Newton iteration
-    24M + 16 (state and mesh) + 2 p_J (2M^2 + M) (the elimination record written
-    and read back, p_J bytes per value: 8 FP64, 4 FP32), i.e. 40M + 16 + 4 M^2 p_J
-    for FP64 records; per convergence-check pass 16M + 16; per defect interval
-    8M + 16 (shat, 8 more bytes, is formed only when a redistribution follows and
-    is not counted); per transfer node 8(M+1); plus guess in and solution out."""
+    """Bytes of the streaming model, per launch.  Per interval and Newton iteration:
+    the dual pass reads y and the mesh (8M + 8) and writes the elimination record, the
+    back-substitution reads the record, y and the mesh (the residual of the corrected
+    state -- the next convergence check -- and its defect are evaluated there) and
+    writes y (16M + 8): 24M + 16 + 2 p_J (2M^2 + M), p_J bytes per record value (8
+    FP64, 4 FP32 records).  Per interval of a separate residual pass (line-search and
+    trust-region stages) 16M + 16; per interval of a separate defect pass (when mesh
+    selection follows) 8M + 16 (shat, 8 more bytes, not counted); per transfer node
+    8(M+1); plus guess in and solution out."""
     it, dfct, tr, chk = counters
     b = it * (24 * M + 16 + 2 * p_j * (2 * M * M + M))
     b += chk * (16 * M + 16)
@@ -462,7 +465,8 @@ def main():
     if not args.no_tune:  # untimed: pick the launch configuration by measurement
         cand_ms, (variant, blocks) = batch.autotune(repeats=3)
         tune = {"candidates_ms": cand_ms, "variant": variant, "persistent_waves": blocks,
-                "variants": "solve kernel register allocation held to 1, 2, 3 waves per SIMD"}
+                "variants": "solve kernel register allocation held to 1, 2, 3 waves per SIMD, then at least 2, 4, "
+                            "8 intervals per lane for the fastest", "min_intervals_per_lane": batch.chunking()}
         log(f"[bench] autotune: candidates {['%.3f' % x for x in cand_ms]} ms -> variant {variant}, {blocks} waves")
 
     # batched, constant guess
@@ -510,8 +514,10 @@ def main():
         if not args.no_tune:
             cand_ms, (variant, blocks) = batch.autotune(repeats=3)
             tune_mx = {"candidates_ms": cand_ms, "variant": variant, "persistent_waves": blocks,
-                       "candidates": "variants 0-2 with FP32 records, then 0-2 with FP64 records",
-                       "records": "fp32" if batch.precision() == _abi.PREC_FP32_REFINE64 else "fp64"}
+                       "candidates": "variants 0-2 with FP32 records, then 0-2 with FP64 records, then at least "
+                                     "2, 4, 8 intervals per lane for the fastest",
+                       "records": "fp32" if batch.precision() == _abi.PREC_FP32_REFINE64 else "fp64",
+                       "min_intervals_per_lane": batch.chunking()}
             log(f"[bench] autotune (mixed): candidates {['%.3f' % x for x in cand_ms]} ms -> variant {variant}, "
                 f"{tune_mx['records']} records")
         if not args.no_learned:
@@ -541,6 +547,7 @@ def main():
         oe.report_err = 1
         be = _lib.Batch(ctx, oe, specs)
         be.set_config(*batch.config())
+        be.set_chunking(batch.chunking())
         if not args.no_learned:
             be.set_warm_start(net, _abi.FEAT_PARAM, args.anchors)
         be.run()

@@ -31,6 +31,13 @@ STAGE_BACKTRACKING = 2
 STAGE_TRUST_REGION = 4
 STAGES_DEFAULT = 7
 
+CONTROL_DEFECT = 0
+CONTROL_GLOBAL = 1
+CONTROL_SEQUENTIAL = 2
+CONTROL_HYBRID = 3
+GLOBAL_HO = 0
+GLOBAL_RE = 1
+
 TRANSFER_REFERENCE = 0
 TRANSFER_CLEAN = 1
 
@@ -86,6 +93,10 @@ class Opts(C.Structure):
         ("refine_iters", C.c_int32),
         ("warm_start", C.c_int32),
         ("report_err", C.c_int32),
+        ("controller", C.c_int32),
+        ("global_method", C.c_int32),
+        ("hybrid_de", C.c_double),
+        ("hybrid_ge", C.c_double),
     ]
 
 
@@ -137,6 +148,10 @@ def default_opts(method=MIRK4):
     o.refine_iters = 0
     o.warm_start = WARM_NONE
     o.report_err = 0
+    o.controller = CONTROL_DEFECT
+    o.global_method = GLOBAL_HO
+    o.hybrid_de = 1.0
+    o.hybrid_ge = 1.0
     return o
 
 

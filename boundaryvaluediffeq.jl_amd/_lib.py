@@ -62,6 +62,8 @@ def lib():
         "bvp_batch_autotune": (C.c_int, [vp, vp, C.c_int32, _fp, _ip]),
         "bvp_batch_config": (C.c_int, [vp, _ip, _ip]),
         "bvp_batch_precision": (C.c_int, [vp, _ip]),
+        "bvp_batch_chunking": (C.c_int, [vp, _ip]),
+        "bvp_batch_set_chunking": (C.c_int, [vp, C.c_int32]),
         "bvp_batch_set_config": (C.c_int, [vp, vp, C.c_int32, C.c_int32]),
         "bvp_mlp_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
         "bvp_mlp_destroy": (None, [vp]),
@@ -90,7 +92,8 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_mlp_create", "bvp_mlp_destroy", "bvp_mlp_set_normalization", "bvp_mlp_get_params",
             "bvp_mlp_set_params", "bvp_mlp_gradient", "bvp_mlp_predict", "bvp_mlp_train",
             "bvp_batch_set_warm_start", "bvp_batch_timing", "bvp_batch_learned_guess", "bvp_batch_autotune",
-            "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision", "bvp_mirk_interpolate"]
+            "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision", "bvp_mirk_interpolate",
+            "bvp_batch_chunking", "bvp_batch_set_chunking"]
 
 
 class Context:
@@ -337,6 +340,17 @@ class Batch:
         v, b = C.c_int32(), C.c_int32()
         lib().bvp_batch_config(self._h, C.byref(v), C.byref(b))
         return int(v.value), int(b.value)
+
+    def chunking(self):
+        """Minimum intervals per lane of the wave kernels' chunk mapping."""
+        m = C.c_int32()
+        lib().bvp_batch_chunking(self._h, C.byref(m))
+        return int(m.value)
+
+    def set_chunking(self, min_chunk):
+        rc = lib().bvp_batch_set_chunking(self._h, int(min_chunk))
+        if rc != 0:
+            raise BVPError(f"batch_set_chunking failed ({rc})")
 
     def precision(self):
         """The record / factor precision the plan runs with (BVP_PREC_*)."""

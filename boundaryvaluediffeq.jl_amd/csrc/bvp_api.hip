@@ -172,6 +172,14 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         set_err(ctx, "the block kernels (M >= 8) run the NewtonRaphson stages (bits 0 and 1) only");
         return BVP_ENOTSUP;
     }
+    if (o->controller < BVP_CONTROL_DEFECT || o->controller > BVP_CONTROL_HYBRID) {
+        set_err(ctx, "controller must be one of BVP_CONTROL_*");
+        return BVP_EINVAL;
+    }
+    if (o->controller != BVP_CONTROL_DEFECT) {
+        set_err(ctx, "the GPU path runs DefectControl only (the global-error controllers are in the oracle)");
+        return BVP_ENOTSUP;
+    }
     int maxN0 = 0;
     for (int i = 0; i < n; ++i) maxN0 = std::max(maxN0, inst[i].N);
     int cap = std::max(cap_req, std::max(o->max_num_subintervals, maxN0) + 1);
@@ -274,8 +282,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         if (inst[i].family != ufam || inst[i].bc_kind != BVP_BC_TWO_POINT) ufam = -1;
     B->family = ufam;
     const auto key = std::make_tuple(M, o->method, o->precision, blk ? 1 : 0, ufam);
-    int variant = 0, per_cu = 0, tuned_f32 = f32;
-    if (ctx->tuned.count(key)) std::tie(variant, per_cu, tuned_f32) = ctx->tuned[key];
+    int variant = 0, per_cu = 0, tuned_f32 = f32, tuned_minc = 1;
+    if (ctx->tuned.count(key)) std::tie(variant, per_cu, tuned_f32, tuned_minc) = ctx->tuned[key];
     if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
     if (per_cu < 1)
         per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant, ufam, f32);
@@ -289,7 +297,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     size_t scratch_doubles;
     if (!blk) {
         const int NF = wave_rel_fields(M);
-        const size_t lmax = (size_t)(cap + 63) / 64 + 1;
+        // records per lane: a chunk holds at most max(ceil(cap / 64), 2 kMaxMinChunk) intervals
+        const size_t lmax = std::max((size_t)(cap + 63) / 64, (size_t)(2 * kMaxMinChunk)) + 1;
         D.tree_base = lmax * NF * 64;
         D.scratch_per_slot = D.tree_base + (size_t)64 * NF;
         D.aux_per_slot = (size_t)cap * (3 * M + 1);
@@ -319,6 +328,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->blocks = std::min(slots, std::max(1, std::min(n, ctx->num_cus * per_cu)));
     D.refine_iters = o->refine_iters;
     D.f32rec = f32 && tuned_f32;  // the autotuner may have found FP64 records faster for this kind
+    D.min_chunk = blk ? 1 : std::max(1, std::min(kMaxMinChunk, tuned_minc));
     if ((rc = dalloc(ctx, A, D.scratch, scratch_doubles)) ||
         (rc = dalloc(ctx, A, D.aux, D.aux_per_slot * slots))) {
         delete B;
@@ -514,6 +524,10 @@ void bvp_opts_default(bvp_opts *o) {
     o->refine_iters = 0;
     o->warm_start = BVP_WARM_NONE;
     o->report_err = 0;
+    o->controller = BVP_CONTROL_DEFECT;
+    o->global_method = BVP_GLOBAL_HO;
+    o->hybrid_de = 1.0;
+    o->hybrid_ge = 1.0;
 }
 
 int bvp_ctx_create(int device, bvp_ctx **out) {
@@ -604,6 +618,13 @@ static int run_component(bvp_ctx *ctx, int method, int op, int32_t n, const bvp_
     o.method = method;
     int rc = batch_build(ctx, &o, n, inst, cap_req, outB);
     if (rc) return rc;
+    // a component call computes in the precision it asks for, with the default chunk
+    // mapping, whatever the autotuner chose for plans of this kind
+    bvp_batch *B = *outB;
+    if (!B->block) {
+        B->dev.f32rec = (o.precision == BVP_PREC_FP32_REFINE64) ? 1 : 0;
+        B->dev.min_chunk = 1;
+    }
     return 0;
 }
 
@@ -820,13 +841,13 @@ int bvp_batch_learned_guess(bvp_ctx *ctx, bvp_batch *B, double *y) {
 int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_ms, int32_t *n_cand) {
     if (!ctx || !B || repeats < 1) return BVP_EINVAL;
     (void)hipSetDevice(ctx->device);
-    struct Cand { int variant, blocks, f32; };
+    struct Cand { int variant, blocks, f32, minc; };
     std::vector<Cand> cands;
     if (B->block) {
         const int occ = std::max(1, block_blocks_per_cu(B->M, B->method, B->opts.precision));
-        for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc), 0});
+        for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc), 0, 1});
         if (cands.back().blocks != std::min(B->max_slots, ctx->num_cus * occ))
-            cands.push_back({0, std::min(B->max_slots, ctx->num_cus * occ), 0});
+            cands.push_back({0, std::min(B->max_slots, ctx->num_cus * occ), 0, 1});
     } else {
         // FP32_REFINE64 on the wave kernels: FP32 records first, then FP64 records (always
         // at least as accurate; faster where the inexact corrections cost Newton steps)
@@ -835,45 +856,79 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
             for (int v = 0; v < wave_variants(B->M); ++v)
                 cands.push_back({v, std::min(B->max_slots,
                                              ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v, B->family, f))),
-                                 f});
+                                 f, 1});
     }
-    const int keep_v = B->variant, keep_b = B->blocks, keep_f = B->dev.f32rec;
+    const int keep_v = B->variant, keep_b = B->blocks, keep_f = B->dev.f32rec, keep_m = B->dev.min_chunk;
     int best = -1;
     float best_ms = 1e30f;
-    for (size_t c = 0; c < cands.size(); ++c) {
-        B->variant = cands[c].variant;
-        B->blocks = std::max(1, cands[c].blocks);
-        if (!B->block) B->dev.f32rec = cands[c].f32;
+    auto time_cand = [&](const Cand &cd, float &med) -> int {
+        B->variant = cd.variant;
+        B->blocks = std::max(1, cd.blocks);
+        if (!B->block) {
+            B->dev.f32rec = cd.f32;
+            B->dev.min_chunk = cd.minc;
+        }
         B->drop_graph();
         std::vector<float> t;
         for (int r = 0; r <= repeats; ++r) {  // first run untimed
             int rc = batch_run(ctx, B);
             if (!rc) rc = batch_sync(ctx, B, nullptr);
+            if (rc) return rc;
+            if (r > 0) t.push_back(B->last_solve_ms);
+        }
+        std::sort(t.begin(), t.end());
+        med = t[t.size() / 2];
+        return 0;
+    };
+    // phase 1: kernel variant and record precision; phase 2 (wave kernels): the
+    // interval-per-lane mapping of the fastest one (at least 2, 4, 8 intervals per lane)
+    for (int phase = 0; phase < (B->block ? 1 : 2); ++phase) {
+        if (phase == 1) {
+            const Cand b0 = cands[best];
+            for (int m = 2; m <= kMaxMinChunk; m *= 2) cands.push_back({b0.variant, b0.blocks, b0.f32, m});
+        }
+        for (size_t c = (phase == 0 ? 0 : cands.size() - 3); c < cands.size(); ++c) {
+            float med = 0.f;
+            const int rc = time_cand(cands[c], med);
             if (rc) {
                 B->variant = keep_v;
                 B->blocks = keep_b;
                 B->dev.f32rec = keep_f;
+                B->dev.min_chunk = keep_m;
                 B->drop_graph();
                 return rc;
             }
-            if (r > 0) t.push_back(B->last_solve_ms);
-        }
-        std::sort(t.begin(), t.end());
-        const float med = t[t.size() / 2];
-        if (cand_ms && (int)c < 16) cand_ms[c] = med;
-        if (med < best_ms) {
-            best_ms = med;
-            best = (int)c;
+            if (cand_ms && (int)c < 16) cand_ms[c] = med;
+            if (med < best_ms) {
+                best_ms = med;
+                best = (int)c;
+            }
         }
     }
-    if (n_cand) *n_cand = (int32_t)cands.size();
+    if (n_cand) *n_cand = (int32_t)std::min<size_t>(cands.size(), 16);
     B->variant = cands[best].variant;
     B->blocks = std::max(1, cands[best].blocks);
-    if (!B->block) B->dev.f32rec = cands[best].f32;
+    if (!B->block) {
+        B->dev.f32rec = cands[best].f32;
+        B->dev.min_chunk = cands[best].minc;
+    }
     B->drop_graph();
     const int per_cu = (B->blocks + ctx->num_cus - 1) / ctx->num_cus;
     ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision, B->block ? 1 : 0, B->family)] =
-        std::make_tuple(B->variant, per_cu, (int)B->dev.f32rec);
+        std::make_tuple(B->variant, per_cu, (int)B->dev.f32rec, (int)B->dev.min_chunk);
+    return 0;
+}
+
+int bvp_batch_chunking(bvp_batch *B, int32_t *min_chunk) {
+    if (!B || !min_chunk) return BVP_EINVAL;
+    *min_chunk = B->dev.min_chunk;
+    return 0;
+}
+
+int bvp_batch_set_chunking(bvp_batch *B, int32_t min_chunk) {
+    if (!B || min_chunk < 1 || min_chunk > kMaxMinChunk || (B->block && min_chunk != 1)) return BVP_EINVAL;
+    B->dev.min_chunk = min_chunk;
+    B->drop_graph();
     return 0;
 }
 

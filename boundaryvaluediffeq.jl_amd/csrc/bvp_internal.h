@@ -23,8 +23,9 @@ struct bvp_ctx {
     int blocks_per_cu = 0;         // 0: from the occupancy calculator
     std::string err;
     // autotuner choices, (M, method, precision, block kernel, family or -1) -> (variant,
-    // waves per CU, FP32 records); later plans of the same kind start from them
-    std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>> tuned;
+    // waves per CU, FP32 records, minimum intervals per lane); later plans of the same
+    // kind start from them
+    std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int, int>> tuned;
 };
 
 // device allocations owned by one batch; freed together

@@ -176,8 +176,135 @@ __device__ inline double bf_of_vd(BShared<M, METHOD, LT> &sh, int lane) {
 // S = -I - h G[:, :M] / R = I - h G[:, M:] straight into the stacked rows in LDS:
 //   first == false: rows M..2M-1 get [S | 0 | R | .]   (a new block row)
 //   first == true : rows 0..M-1  get [R | S | 0 | .]   (interval 0 is the first relation)
+// The same blocks for the FP32 factorisation at M >= 16 with the stage products
+// J_r dY_r = A dY_r + c sech^2(Y_r) o dY_r on the matrix cores: the M x 2M tangent
+// matrix D = dY_r is staged in LDS (the rows the new block row will occupy, free
+// until the end) and A D runs as 16 x 16 x 4 FP32 MFMA tiles
+// (v_mfma_f32_16x16x4_f32).  Every lane keeps its part of [S | R] and of the
+// last-stage accumulator in the MFMA output layout -- lane l holds, for each
+// 16 x 16 tile, rows 4 (l / 16) .. +3 of column l % 16 -- so the elementwise terms
+// (the identity seeds, the first two stages, the sech^2 diagonal) are formed in
+// that layout and the blocks are written to the stacked rows once at the end.
+template <int M, int METHOD>
+__device__ __noinline__ void btangent_blocks_mfma(BShared<M, METHOD, float> &sh, int lane, double hd, bool first) {
+    using TB = Tab<METHOD>;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int RT = M / 16, CT = 2 * M / 16, KS = M / 4;
+    const float h = (float)hd;
+    const float c = (float)sh.c;
+    const int rb = first ? 0 : M;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    float *D = &sh.w.stk[M][0];  // D[q][col] at D[q * SW + col]
+    constexpr int SW = BShared<M, METHOD, float>::SW;
+    // A operands: A[16 rt + l16][4 ks + l4]
+    float aop[RT][KS];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) aop[rt][ks] = sh.Al[16 * rt + l16][4 * ks + l4];
+    float G[RT][CT][4], q4[RT][CT][4];
+    // first two stages: G = +-I - h b_r0 J_r0 (r0 = 0 for the y_k columns, 1 for y_{k+1})
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int col = 16 * ct + l16;
+            const bool left = col < M;
+            const int jc = left ? col : col - M;
+            const int r0 = left ? 0 : 1;
+            const float sj = c * sh.sech2[r0][jc];
+            const float hb0 = h * (float)(left ? TB::b(0) : TB::b(1));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * rt + 4 * l4 + v;
+                const float jv = sh.Al[i][jc] + ((i == jc) ? sj : 0.f);
+                const float e = (i == jc) ? (left ? -1.f : 1.f) : 0.f;
+                G[rt][ct][v] = e - hb0 * jv;
+                q4[rt][ct][v] = 0.f;
+            }
+        }
+#pragma unroll
+    for (int r = 2; r < TB::s; ++r) {
+        // dY_r in the output layout, into LDS
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const int col = 16 * ct + l16;
+                const bool left = col < M;
+                const int jc = left ? col : col - M;
+                const int r0 = left ? 0 : 1;
+                const float sj = c * sh.sech2[r0][jc];
+                const float seed = left ? (float)(1.0 - TB::v(r)) : (float)TB::v(r);
+                const float xr = left ? (float)TB::x(r, 0) : (float)TB::x(r, 1);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 16 * rt + 4 * l4 + v;
+                    const float jv = sh.Al[i][jc] + ((i == jc) ? sj : 0.f);
+                    float d = ((i == jc) ? seed : 0.f) + h * xr * jv;
+                    if (r == TB::s - 1 && TB::s > 3) d += h * q4[rt][ct][v];
+                    D[i * SW + col] = d;
+                }
+            }
+        __syncthreads();
+        // A dY_r on the matrix cores
+        f4 acc[RT][CT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            float bop[CT];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) bop[ct] = D[(4 * ks + l4) * SW + 16 * ct + l16];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[rt][ks], bop[ct], acc[rt][ct], 0, 0, 0);
+        }
+        const float hb = h * (float)TB::b(r);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 16 * rt + 4 * l4 + v;
+                    const float jd = acc[rt][ct][v] + c * sh.sech2[r][i] * D[i * SW + 16 * ct + l16];
+                    G[rt][ct][v] -= hb * jd;
+                    if (TB::s > 3 && r < TB::s - 1) q4[rt][ct][v] += (float)TB::x(TB::s - 1, r) * jd;
+                }
+        __syncthreads();  // D is read by every lane before the next stage overwrites it
+    }
+    // [S | R] into the stacked rows (target column as in btangent_blocks); the left
+    // lanes' middle block is cleared
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int col = 16 * ct + l16;
+            const bool left = col < M;
+            const int jc = left ? col : col - M;
+            const int tc = first ? (left ? M + jc : jc) : (left ? jc : 2 * M + jc);
+            const int zc = first ? 2 * M + jc : M + jc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * rt + 4 * l4 + v;
+                sh.w.stk[rb + i][tc] = G[rt][ct][v];
+                if (left) sh.w.stk[rb + i][zc] = 0.f;
+            }
+        }
+    __syncthreads();
+}
+
 template <int M, int METHOD, typename LT>
 __device__ inline void btangent_blocks(BShared<M, METHOD, LT> &sh, int lane, double hd, bool first) {
+    if constexpr (sizeof(LT) == 4 && M >= 16) {
+        btangent_blocks_mfma<M, METHOD>(sh, lane, hd, first);
+        return;
+    }
     using TB = Tab<METHOD>;
     using SHT = BShared<M, METHOD, LT>;
     const LT h = (LT)hd;

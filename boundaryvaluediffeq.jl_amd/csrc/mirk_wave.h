@@ -67,12 +67,18 @@ struct BatchDev {
     int rec_cap;              // block kernels: intervals whose factors fit a wave's storage
     int refine_iters;         // FP64 refinement steps per linear solve with FP32 factors
     int f32rec;               // wave kernels: elimination factors stored in FP32 (mixed precision)
+    int min_chunk;            // wave kernels: at least this many intervals per lane's chunk
+                              //   (1 ... kMaxMinChunk; chosen by the autotuner)
 
     // options
     int adaptive, max_sub, max_newton, stages, transfer_mode;
     int report_err;           // fill err[] after each solve (one more Newton correction)
     double abstol, newton_tol, defect_threshold;
 };
+
+// largest interval-per-lane minimum the autotuner tries (the chunk records of a
+// slot are sized for chunks of up to 2 * kMaxMinChunk intervals)
+constexpr int kMaxMinChunk = 8;
 
 hipError_t wave_reset_counter(unsigned *counter, hipStream_t s);
 // variant v of the persistent solve kernel (register allocation held to v + 1

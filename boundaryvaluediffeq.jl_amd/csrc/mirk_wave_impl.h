@@ -83,8 +83,11 @@ __device__ inline T sel(const T (&a)[N], int k) {
     return r;
 }
 
-__device__ inline void chunk_of(int N, int lane, int &P, int &start, int &len) {
-    P = N < WAVE ? N : WAVE;
+// minc: at least this many intervals per chunk (the interval-per-lane mapping the
+// autotuner chooses; 1 spreads the intervals over up to 64 lanes)
+__device__ inline void chunk_of(int N, int lane, int &P, int &start, int &len, int minc = 1) {
+    P = N / minc;
+    P = P < 1 ? 1 : (P > WAVE ? WAVE : P);
     if (lane >= P) { start = N; len = 0; return; }
     const int q = N / P, r = N % P;
     len = q + (lane < r ? 1 : 0);
@@ -615,6 +618,7 @@ __device__ inline bool dense_solve(double (&A)[NN][NN], double (&b)[NN]) {
 // ------------------------------------------------------------------ instance view
 template <int M> struct Inst {
     int fam;
+    int minc;  // chunk_of's minimum intervals per lane
     Params<M> P;
     int bc_kind;
     double bct[M], bcv[M];
@@ -625,6 +629,7 @@ template <int M> struct Inst {
 template <int M>
 __device__ inline void load_inst(const BatchDev &B, int b, Inst<M> &I) {
     I.fam = B.family[b];
+    I.minc = B.min_chunk > 0 ? B.min_chunk : 1;
     const int po = B.param_off[b], pn = B.param_cnt[b];
 #pragma unroll
     for (int k = 0; k < Params<M>::MAXP; ++k) I.P.p[k] = k < pn ? B.params[po + k] : 0.0;
@@ -751,7 +756,7 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
                                     double *scr, Rel<M> &rel, double &mx, bool &bad, double *phi_out,
                                     double *S_out, double *R_out) {
     int P, start, len;
-    chunk_of(N, lane, P, start, len);
+    chunk_of(N, lane, P, start, len, I.minc);
     mx = 0.0;
     bad = false;
     walk_chunk<M, kPfDual<M>>(mesh, y, start, len, [&](int k, double ti, double tj, const double *yi, const double *yj) {
@@ -804,12 +809,18 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
 // Tree up-sweep, root solve (lane 0), tree down-sweep and chunk back-substitution.
 // Writes the correction into `delta` (point-major) when non-null and, when
 // `apply` is set, adds it to y.
-template <int M, int METHOD, bool F32 = false>
+// FUSE (with apply and no delta): while the chunk is walked backwards the corrected
+// values of both ends of each interval are in registers, so the residual of the new
+// state is evaluated there -- *fn_out = max|F| of the corrected state (BC rows
+// included; NaN if not finite), the convergence check of the next Newton iteration
+// without another pass over HBM -- and with FUSE == 2 also the defect estimate,
+// *dn_out = max_i |d_i| (MIRK/adaptivity.jl:416-461; NaN if not finite).
+template <int M, int METHOD, bool F32 = false, int FUSE = 0>
 __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y, int N,
                                         int lane, double *scr, size_t tree_base, Rel<M> &rel, double *delta,
-                                        bool apply) {
+                                        bool apply, double *fn_out = nullptr, double *dn_out = nullptr) {
     int P, start, len;
-    chunk_of(N, lane, P, start, len);
+    chunk_of(N, lane, P, start, len, I.minc);
     int D = 0;
     while ((1 << D) < P) ++D;
     double *tree = sh.tree;  // tree records stay on chip (read back level by level in the down-sweep)
@@ -870,7 +881,7 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
                     const int span = 1 << d;
                     if (l + span >= P) continue;
                     int ps, plen, pP;
-                    chunk_of(N, l + span, pP, ps, plen);
+                    chunk_of(N, l + span, pP, ps, plen, I.minc);
                     Back<M> bk;
                     load_back<M, 1>(tree, tree_index(d, l), 0, bk);
                     double mb[M][W];
@@ -904,7 +915,7 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
                 }
                 if (!found) {
                     int ls, llen, lP;
-                    chunk_of(N, l, lP, ls, llen);
+                    chunk_of(N, l, lP, ls, llen, I.minc);
                     if (node == ls) {
 #pragma unroll
                         for (int i = 0; i < M; ++i)
@@ -988,10 +999,59 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
     // longer needed) and then to y in whole lines, when it fits.
     const bool staged = apply && delta == nullptr && N + 1 <= kStageNodes<M>;
     double *ynew = staged ? sh.stage : y;
+    using TB = Tab<METHOD>;
+    double fmx = 0.0, dmx = 0.0;
+    bool fbad = false, dbad = false;
+    // residual (and defect) of interval [ti, tj] at the corrected state yi, yj
+    auto check = [&](double ti, double tj, const double *yi, const double *yj) {
+        const double h = tj - ti;
+        double K[TB::s][M], ph[M];
+        stages<M, METHOD, double>(I.fam, I.P, ti, h, yi, yj, K);
+        phi_of<M, METHOD, double>(h, yi, yj, K, ph);
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const double a = fabs(ph[c]);
+            fbad |= !(a <= 1.7976931348623157e308);
+            fmx = fmax(fmx, a);
+        }
+        if constexpr (FUSE == 2) {
+            double w[TB::sstar], wp[TB::sstar];
+            double KI[TB::sstar - TB::s][M], z[M], zp[M], f[M];
+            interp_stages<M, METHOD>(I.fam, I.P, ti, h, yi, yj, K, KI);
+            double e1 = 0.0, e2 = 0.0, d1[M], d2[M];
+            interp_weights<METHOD>(TB::tau(), w, wp);
+            sum_stages<M, METHOD>(h, yi, K, KI, w, wp, z, zp);
+            f_eval<M, double>(I.fam, I.P, ti + TB::tau() * h, z, f);
+#pragma unroll
+            for (int c = 0; c < M; ++c) { d1[c] = (zp[c] - f[c]) / (fabs(f[c]) + 1.0); e1 = fmax(e1, fabs(d1[c])); }
+            interp_weights<METHOD>(1.0 - TB::tau(), w, wp);
+            sum_stages<M, METHOD>(h, yi, K, KI, w, wp, z, zp);
+            f_eval<M, double>(I.fam, I.P, ti + (1.0 - TB::tau()) * h, z, f);
+#pragma unroll
+            for (int c = 0; c < M; ++c) { d2[c] = (zp[c] - f[c]) / (fabs(f[c]) + 1.0); e2 = fmax(e2, fabs(d2[c])); }
+            const bool first = e1 > e2;
+            double ei = 0.0;
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                const double d = first ? d1[c] : d2[c];
+                dbad |= !(fabs(d) <= 1.7976931348623157e308);
+                ei = fmax(ei, fabs(d));
+            }
+            dmx = fmax(dmx, ei);
+        }
+    };
     if (len > 0) {
         double ya[M], yn[M];
 #pragma unroll
         for (int c = 0; c < M; ++c) { ya[c] = sh.lo[lane][c]; yn[c] = sh.hi[lane][c]; }
+        // FUSE: t and corrected state of the node right of the next interval to check
+        // (the chunk's end node: read before any lane writes the state)
+        double tr = 0.0, yr[M];
+        if constexpr (FUSE > 0) {
+            tr = mesh[start + len];
+#pragma unroll
+            for (int c = 0; c < M; ++c) yr[c] = y[(size_t)(start + len) * M + c] + yn[c];
+        }
         if (lane == P - 1) {
 #pragma unroll
             for (int c = 0; c < M; ++c) {
@@ -1000,11 +1060,12 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
             }
         }
         // nodes k = start+len-1 down to start+1; step i is node start+len-1-i, whose record
-        // is slot len-2-i.  Records (and y) are read PB steps ahead into a register ring.
+        // is slot len-2-i.  Records (and y, and with FUSE the node times) are read PB
+        // steps ahead into a register ring.
         constexpr int PB = kPfBack<M>;
         const int ns = len - 1;
         Back<M> rb[PB];
-        double ry[PB][M];
+        double ry[PB][M], rt[PB];
 #pragma unroll
         for (int j = 0; j < PB; ++j) {
             if (j < ns) {
@@ -1013,6 +1074,7 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
 #pragma unroll
                     for (int c = 0; c < M; ++c) ry[j][c] = y[(size_t)(start + len - 1 - j) * M + c];
                 }
+                if constexpr (FUSE > 0) rt[j] = mesh[start + len - 1 - j];
             }
         }
         for (int i0 = 0; i0 < ns; i0 += PB) {
@@ -1025,33 +1087,62 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
                     double ycur[M];
 #pragma unroll
                     for (int c = 0; c < M; ++c) ycur[c] = ry[j][c];
+                    const double tk = rt[j];
                     if (i + PB < ns) {
                         get_rec<M, F32>(scr, len - 2 - (i + PB), lane, rb[j]);
                         if (apply) {
 #pragma unroll
                             for (int c = 0; c < M; ++c) ry[j][c] = y[(size_t)(k - PB) * M + c];
                         }
+                        if constexpr (FUSE > 0) rt[j] = mesh[k - PB];
                     }
-                    double yk[M];
+                    double yk[M], yl[M];
                     back_apply<M>(bk, ya, yn, yk);
 #pragma unroll
                     for (int c = 0; c < M; ++c) {
+                        yl[c] = ycur[c] + yk[c];
                         if (delta) delta[(size_t)k * M + c] = yk[c];
-                        if (apply) ynew[(size_t)k * M + c] = ycur[c] + yk[c];
+                        if (apply) ynew[(size_t)k * M + c] = yl[c];
                         yn[c] = yk[c];
+                    }
+                    if constexpr (FUSE > 0) {
+                        check(tk, tr, yl, yr);
+                        tr = tk;
+#pragma unroll
+                        for (int c = 0; c < M; ++c) yr[c] = yl[c];
                     }
                 }
             }
         }
+        double yl[M];
 #pragma unroll
         for (int c = 0; c < M; ++c) {
+            yl[c] = y[(size_t)start * M + c] + ya[c];
             if (delta) delta[(size_t)start * M + c] = ya[c];
-            if (apply) ynew[(size_t)start * M + c] = y[(size_t)start * M + c] + ya[c];
+            if (apply) ynew[(size_t)start * M + c] = yl[c];
         }
+        if constexpr (FUSE > 0) check(mesh[start], tr, yl, yr);
     }
     __syncthreads();
     if (staged) {
         for (int q = lane; q < (N + 1) * M; q += WAVE) y[q] = sh.stage[q];
+        __syncthreads();
+    }
+    if constexpr (FUSE > 0) {
+        if (lane < M) {
+            const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+            const double a = fabs(bc_residual<M, METHOD>(I, lane, L, mesh, y));
+            fbad |= !(a <= 1.7976931348623157e308);
+            fmx = fmax(fmx, a);
+        }
+        const bool fb = wave_any(fbad);
+        fmx = uniform(wave_max(fmx));
+        *fn_out = fb ? NAN : fmx;
+        if constexpr (FUSE == 2) {
+            const bool db = wave_any(dbad);
+            dmx = uniform(wave_max(dmx));
+            *dn_out = db ? NAN : dmx;
+        }
         __syncthreads();
     }
     return true;
@@ -1089,7 +1180,7 @@ __device__ inline double residual_and_blocks(const Inst<M> &I, Shared<M> &sh, co
 template <int M, int METHOD>
 __device__ inline double residual_primal(const Inst<M> &I, const double *mesh, const double *y, int N, int lane) {
     int P, start, len;
-    chunk_of(N, lane, P, start, len);
+    chunk_of(N, lane, P, start, len, I.minc);
     double mx = 0.0;
     bool bad = false;
     walk_chunk<M, kPfPrimal<M>>(mesh, y, start, len, [&](int, double ti, double tj, const double *yi, const double *yj) {
@@ -1122,7 +1213,7 @@ template <int M, int METHOD>
 __device__ inline double residual_stats(const Inst<M> &I, const double *mesh, const double *y, int N, int lane,
                                         double *mx_out) {
     int P, start, len;
-    chunk_of(N, lane, P, start, len);
+    chunk_of(N, lane, P, start, len, I.minc);
     double acc = 0.0, mx = 0.0;
     walk_chunk<M, kPfPrimal<M>>(mesh, y, start, len, [&](int, double ti, double tj, const double *yi, const double *yj) {
         const double h = tj - ti;
@@ -1181,7 +1272,7 @@ template <int M, int METHOD, int FAM>
 __device__ inline void jt_times_f(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y, int N,
                                   int lane, double *g) {
     int P, start, len;
-    chunk_of(N, lane, P, start, len);
+    chunk_of(N, lane, P, start, len, I.minc);
     double carry[M];
 #pragma unroll
     for (int c = 0; c < M; ++c) carry[c] = 0.0;
@@ -1227,7 +1318,7 @@ template <int M, int METHOD, int FAM>
 __device__ inline double jac_norm2(const Inst<M> &I, const Shared<M> &sh, const double *mesh, const double *y,
                                    const double *v, int N, int lane) {
     int P, start, len;
-    chunk_of(N, lane, P, start, len);
+    chunk_of(N, lane, P, start, len, I.minc);
     double acc = 0.0;
     double vi[M];
     if (len > 0) {
@@ -1261,6 +1352,8 @@ __device__ inline double jac_norm2(const Inst<M> &I, const Shared<M> &sh, const 
 struct NewtonOut {
     int ret, iters, stage, warm_fallback;
     double fn;  // max|F| at the state Newton returns
+    double dn;  // defect estimate of that state when dn_valid (evaluated with its last correction)
+    int dn_valid;
 };
 
 // LineSearches.jl's BackTracking, order 3 (the restatement the oracle follows):
@@ -1402,7 +1495,7 @@ template <int M, int METHOD, bool F32 = false, int FAM = -1, typename Restore>
 __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y,
                             Restore &&restore, double *delta, double *ytmp, double *gbuf, int N, int lane, double *scr,
                             size_t tree_base, const double *u0c) {
-    NewtonOut out{BVP_RET_MAXITERS, 0, -1, 0, NAN};
+    NewtonOut out{BVP_RET_MAXITERS, 0, -1, 0, NAN, NAN, 0};
     // u0c != null: y holds a learned warm start; when the first Newton step from it
     // does not contract max|F| (or the guess is unusable), restart from the constant
     // guess u0c, as the plain method would have started.
@@ -1425,16 +1518,21 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
         first = false;
         out.stage = stage;
         int ret = BVP_RET_MAXITERS;
+        out.dn_valid = 0;
         if (stage == 2) {
             ret = trust_region<M, METHOD, F32, FAM>(B, I, sh, mesh, y, delta, gbuf, ytmp, N, lane, scr, tree_base, out);
         } else {
+            // after a plain step max|F| (and the defect) of the new state come from its
+            // back-substitution; after a line-search step from a primal pass.  The dual
+            // pass that builds the blocks runs only when another step is taken.
+            double fn_next = NAN, dn_next = NAN;
             for (int it = 0;; ++it) {
                 Rel<M> rel;
-                // After a step, max|F| comes from the primal pass; the dual pass that
-                // builds the blocks runs only when another step is taken.
                 bool blocks = (it == 0);
                 double fn = blocks ? residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel)
-                                   : residual_primal<M, METHOD>(I, mesh, y, N, lane);
+                                   : (stage == 0 ? fn_next : residual_primal<M, METHOD>(I, mesh, y, N, lane));
+                out.dn_valid = (!blocks && stage == 0) ? 1 : 0;
+                out.dn = dn_next;
                 if (warm) {
                     bool fall;
                     if (it == 0) {
@@ -1454,6 +1552,7 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                         __syncthreads();
                         fn = residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
                         blocks = true;
+                        out.dn_valid = 0;
                         it = 0;  // the restart has the full iteration budget, as the plain method
                     }
                 }
@@ -1463,8 +1562,11 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                 if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
                 if (!blocks) residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
                 const bool plain = (stage == 0);
-                if (!solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel,
-                                                      plain ? nullptr : delta, plain)) {
+                const bool solved =
+                    plain ? solve_correction<M, METHOD, F32, 2>(I, sh, mesh, y, N, lane, scr, tree_base, rel, nullptr,
+                                                                true, &fn_next, &dn_next)
+                          : solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, delta, false);
+                if (!solved) {
                     if (warm) {  // singular at the warm start: restart from the constant guess
                         warm = false;
                         out.warm_fallback = 1;
@@ -1533,7 +1635,7 @@ __device__ __forceinline__ double defect_pass(const Inst<M> &I, const double *me
                                      double *stage = nullptr, bool lazy = false) {
     using TB = Tab<METHOD>;
     int P, start, len;
-    chunk_of(N, lane, P, start, len);
+    chunk_of(N, lane, P, start, len, I.minc);
     double w1[TB::sstar], w1p[TB::sstar], w2[TB::sstar], w2p[TB::sstar];
     interp_weights<METHOD>(TB::tau(), w1, w1p);
     interp_weights<METHOD>(1.0 - TB::tau(), w2, w2p);
@@ -1900,7 +2002,9 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         rn = nl.fn;
         rn_valid = true;
         cnt_it += (long long)N * nl.iters;
-        cnt_chk += N + (nl.warm_fallback ? N : 0);
+        // separate primal residual passes: the checks of the line-search and trust-region
+        // stages (the plain stage checks inside its back-substitution)
+        if (nl.stage > 0) cnt_chk += (long long)N * nl.iters;
         if (nl.stage == 1) fb |= 1;
         if (nl.stage == 2) fb |= 16;
         if (nl.warm_fallback) fb |= 2;
@@ -1910,9 +2014,14 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         info = nl.ret;
         double r1 = 0.0, r2 = 0.0;
         if (info == BVP_RET_SUCCESS) {
-            error_norm = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2,
-                                                N <= kStageNodes<M> * M ? sh.stage : nullptr, true);
-            cnt_def += N;
+            if (nl.dn_valid && !(nl.dn > B.abstol)) {
+                // the defect came with the last correction and no mesh selection follows
+                error_norm = nl.dn;
+            } else {
+                error_norm = defect_pass<M, METHOD>(I, mesh, y, N, lane, B.abstol, shat, nullptr, r1, r2,
+                                                    N <= kStageNodes<M> * M ? sh.stage : nullptr, true);
+                cnt_def += N;
+            }
             last_defect = error_norm;
             info = (error_norm > B.defect_threshold || isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
         }

@@ -92,6 +92,16 @@ extern "C" {
 #define BVP_STAGE_TRUST_REGION 4
 #define BVP_STAGES_DEFAULT     7
 
+/* ---- error controllers (bvp_opts.controller; Core/calc_errors.jl:54-110,
+ * MIRK/adaptivity.jl:23-243, 370-567) */
+#define BVP_CONTROL_DEFECT     0  /* DefectControl (the default) */
+#define BVP_CONTROL_GLOBAL     1  /* GlobalErrorControl(method) */
+#define BVP_CONTROL_SEQUENTIAL 2  /* SequentialErrorControl: defect, then global error */
+#define BVP_CONTROL_HYBRID     3  /* HybridErrorControl: DE * defect + GE * global error */
+/* global-error methods (bvp_opts.global_method) */
+#define BVP_GLOBAL_HO 0  /* HOErrorControl: MIRK(p+2) on the same mesh (MIRK4 only: no MIRK8) */
+#define BVP_GLOBAL_RE 1  /* REErrorControl: Richardson extrapolation on the halved mesh */
+
 /* ---- mesh-transfer mode after a redistribution (MIRK/mirk.jl:364-372) */
 #define BVP_TRANSFER_REFERENCE 0  /* in place, as the reference writes y0 while reading it */
 #define BVP_TRANSFER_CLEAN     1  /* interpolate from an untouched copy of the old solution */
@@ -155,6 +165,9 @@ typedef struct {
                                      correction per instance after the solve; the
                                      returned solution is unchanged).  No counterpart
                                      in the reference; default 0. */
+    int32_t controller;           /* BVP_CONTROL_*; default DEFECT  MIRK/mirk.jl:51 */
+    int32_t global_method;        /* BVP_GLOBAL_*; default HO  Core/calc_errors.jl:70 */
+    double  hybrid_de, hybrid_ge; /* HybridErrorControl weights; 1, 1  Core/calc_errors.jl:100 */
 } bvp_opts;
 
 /* Per-instance result.  mesh/u are caller-allocated with room for
@@ -287,9 +300,20 @@ int  bvp_batch_set_graph(bvp_batch *batch, int32_t enable);
  * BVP_PREC_FP32_REFINE64 also times its variants with FP64 records (listed
  * after the FP32 ones) and keeps FP64 where that is faster: on stiff families
  * the FP32-record corrections cost extra Newton steps.  cand_ms (optional,
- * room for 16) receives the median of every candidate, *n_cand their number.
+ * room for 16) receives the median of every candidate, *n_cand their number;
+ * a wave-kernel plan then also times the interval-per-lane mapping of its fastest
+ * candidate (bvp_batch_chunking; three more candidates).
  * The runs overwrite the plan's results. */
 int  bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *batch, int32_t repeats, float *cand_ms, int32_t *n_cand);
+/* The interval-per-wavefront mapping of a wave-kernel plan (no counterpart in the
+ * reference): an instance's N intervals are split into P = clamp(N / min_chunk, 1,
+ * 64) contiguous chunks, one per lane of its wavefront (chunk elimination, then a
+ * merge tree across the lanes).  The autotuner times min_chunk = 2, 4, 8 against 1
+ * for its fastest kernel variant and keeps the fastest; it can be read or set
+ * (1 ... 8; block-kernel plans: 1).  Different mappings give the same solution to
+ * rounding (the elimination order differs). */
+int  bvp_batch_chunking(bvp_batch *batch, int32_t *min_chunk);
+int  bvp_batch_set_chunking(bvp_batch *batch, int32_t min_chunk);
 /* The record / factor precision the plan runs with (BVP_PREC_*): the requested
  * one, or BVP_PREC_FP64 where the autotuner chose FP64 records (no counterpart
  * in the reference). */

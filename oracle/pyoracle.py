@@ -35,7 +35,8 @@ class Opts(C.Structure):
                 ("max_num_subintervals", C.c_int32), ("max_newton_iters", C.c_int32),
                 ("nlsolve_stages", C.c_int32), ("transfer_mode", C.c_int32),
                 ("precision", C.c_int32), ("refine_iters", C.c_int32), ("warm_start", C.c_int32),
-                ("report_err", C.c_int32)]
+                ("report_err", C.c_int32), ("controller", C.c_int32), ("global_method", C.c_int32),
+                ("hybrid_de", C.c_double), ("hybrid_ge", C.c_double)]
 
 
 class _Result(C.Structure):
@@ -82,6 +83,7 @@ def lib():
         L.ref_backtracking_poly.argtypes = [_dp, C.c_int, _dp, _dp, C.c_int, _ip]
         L.ref_dogleg.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
         L.ref_interpolate.argtypes = [C.c_int, C.POINTER(_Instance), C.c_int, _dp, _dp, _dp]
+        L.ref_global_error.argtypes = [C.POINTER(Opts), C.POINTER(_Instance), _dp, _dp]
         _lib = L
     return _lib
 
@@ -246,6 +248,15 @@ def interpolate(method, spec, t):
     u, du = np.zeros(max(1, len(t)) * spec.M), np.zeros(max(1, len(t)) * spec.M)
     _check(lib().ref_interpolate(method, C.byref(P.inst), len(t), _d(t), _d(u), _d(du)), "interpolate")
     return u[:len(t) * spec.M].reshape(-1, spec.M), du[:len(t) * spec.M].reshape(-1, spec.M)
+
+
+def global_error(o, spec):
+    """The global error estimate of o.global_method at (mesh, guess): (per-interval
+    errors (N, M), the norm compared with abstol)."""
+    P = _Pinned(spec)
+    e, nrm = np.zeros(spec.N * spec.M), np.zeros(1)
+    _check(lib().ref_global_error(C.byref(o), C.byref(P.inst), _d(e), _d(nrm)), "global_error")
+    return e.reshape(spec.N, spec.M), float(nrm[0])
 
 
 def backtracking_poly(coef, max_trace=64):

@@ -899,24 +899,30 @@ long long round_even(double x) {
 
 // mesh_selector!(::DefectControl) + redistribute!  MIRK/adaptivity.jl:23-75, 250-278.
 // Returns BVP_RET_SUCCESS/FAILURE; fills new_mesh.
-int mesh_select(int order, double abstol, int max_sub, const std::vector<double> &mesh,
-                const std::vector<double> &errors, int M, std::vector<double> &new_mesh) {
+// max-abs of each interval's error vector
+std::vector<double> interval_max(const std::vector<double> &errors, int n, int M) {
+    std::vector<double> m(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < M; ++c) m[i] = std::max(m[i], std::fabs(errors[(size_t)i * M + c]));
+    return m;
+}
+
+// The common part of the four mesh selectors (MIRK/adaptivity.jl:23-243): shat[i]
+// is the controller's per-interval indicator, rho its halving threshold (1 for
+// DefectControl, 2 for the global, sequential and hybrid controllers).
+int mesh_select_shat(double rho, int max_sub, const std::vector<double> &mesh, std::vector<double> shat,
+                     std::vector<double> &new_mesh) {
     const int Nn = (int)mesh.size();  // length(mesh)
     const int n = Nn - 1;
-    std::vector<double> dt(n), shat(n);
+    std::vector<double> dt(n);
     for (int i = 0; i < n; ++i) dt[i] = mesh[i + 1] - mesh[i];
-    for (int i = 0; i < n; ++i) {
-        double m = 0;
-        for (int c = 0; c < M; ++c) m = std::max(m, std::fabs(errors[(size_t)i * M + c]));
-        shat[i] = std::pow(m / abstol, 1.0 / (order + 1));
-    }
     double r1 = shat[0], r2 = 0;
     for (int i = 0; i < n; ++i) { r1 = std::max(r1, shat[i]); r2 += shat[i]; }
     const double r3 = r2 / (Nn - 1);
     long long n_predict = round_even(1.3 * r2 + 1);
     const double n_ = 0.1 * n;
     if (std::fabs((double)(n_predict - n)) < n_) n_predict = round_even(n + n_);
-    if (r1 <= 1.0 * r3) {
+    if (r1 <= rho * r3) {
         const long long Nsub = 2LL * n;
         if (Nsub > max_sub) return BVP_RET_FAILURE;
         half_mesh(mesh, new_mesh);
@@ -953,6 +959,15 @@ int mesh_select(int order, double abstol, int max_sub, const std::vector<double>
     }
     new_mesh[(size_t)Nsub] = mesh[n];
     return BVP_RET_SUCCESS;
+}
+
+// mesh_selector!(::DefectControl): shat = (max|d_i| / abstol)^(1/(p+1)), rho = 1
+int mesh_select(int order, double abstol, int max_sub, const std::vector<double> &mesh,
+                const std::vector<double> &errors, int M, std::vector<double> &new_mesh) {
+    const int n = (int)mesh.size() - 1;
+    std::vector<double> shat = interval_max(errors, n, M);
+    for (double &v : shat) v = std::pow(v / abstol, 1.0 / (order + 1));
+    return mesh_select_shat(1.0, max_sub, mesh, shat, new_mesh);
 }
 
 // Solution transfer: y_new(node) = continuous extension of the old solve.
@@ -1006,6 +1021,64 @@ int check_instance(const bvp_instance &P) {
     return 0;
 }
 
+// ---------------------------------------------------------------- global error
+// global_error + GE_subinterval! (MIRK/adaptivity.jl:540-556): per node the relative
+// difference e_i = (high_i - low_i) / (1 + |low_i|); per interval the one of its two
+// end-node vectors with the larger max-abs (the left one on ties); returns the
+// max-abs over all intervals.
+double global_error_of(const std::vector<double> &high, const std::vector<double> &low, int n, int M,
+                       std::vector<double> &errors) {
+    std::vector<double> e((size_t)(n + 1) * M);
+    for (size_t q = 0; q < e.size(); ++q) e[q] = (high[q] - low[q]) / (1.0 + std::fabs(low[q]));
+    errors.assign((size_t)n * M, 0.0);
+    double norm = 0;
+    for (int i = 0; i < n; ++i) {
+        double a = 0, b = 0;
+        for (int c = 0; c < M; ++c) {
+            a = std::max(a, std::fabs(e[(size_t)i * M + c]));
+            b = std::max(b, std::fabs(e[(size_t)(i + 1) * M + c]));
+        }
+        const int src = (a >= b) ? i : i + 1;
+        for (int c = 0; c < M; ++c) errors[(size_t)i * M + c] = e[(size_t)src * M + c];
+        norm = std::max(norm, std::max(a, b));
+    }
+    return norm;
+}
+
+// error_estimate!(::GlobalErrorControl, ::REErrorControl) (MIRK/adaptivity.jl:511-525):
+// the same method, non-adaptive, on the halved mesh from halve_sol (:311-333) of the
+// current solution, solved with the same nonlinear solver and tolerance (its return
+// code is not consulted, as in the reference); the global error of the current
+// solution at the old nodes, its norm scaled by 2^p / (2^p - 1).
+// error_estimate!(::GlobalErrorControl, ::HOErrorControl) (:527-538): MIRK(p+2)
+// (MIRK6 for MIRK4) on the same mesh from the current solution; norm unscaled.
+double global_error_estimate(const bvp_instance &P, const bvp_opts &o, const Ctx &C, const std::vector<double> &mesh,
+                             const std::vector<double> &y, double tol, std::vector<double> &errors) {
+    const int M = P.M, n = (int)mesh.size() - 1;
+    std::vector<BcRow> rows;
+    if (o.global_method == BVP_GLOBAL_RE) {
+        std::vector<double> mh, yh((size_t)(2 * n + 1) * M);
+        half_mesh(mesh, mh);
+        for (int i = 0; i <= n; ++i)
+            for (int c = 0; c < M; ++c) yh[(size_t)(2 * i) * M + c] = y[(size_t)i * M + c];
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < M; ++c)
+                yh[(size_t)(2 * i + 1) * M + c] = (yh[(size_t)(2 * i + 2) * M + c] + yh[(size_t)(2 * i) * M + c]) / 2.0;
+        resolve_bc(P, 2 * n, mh.data(), rows);
+        newton(C, 2 * n, mh.data(), yh, rows, tol, o.max_newton_iters, o.nlsolve_stages);
+        std::vector<double> hi((size_t)(n + 1) * M);
+        for (int i = 0; i <= n; ++i)
+            for (int c = 0; c < M; ++c) hi[(size_t)i * M + c] = yh[(size_t)(2 * i) * M + c];
+        const double p2 = std::pow(2.0, C.T.order);
+        return global_error_of(hi, y, n, M, errors) * p2 / (p2 - 1.0);
+    }
+    Ctx Ch(P, C.method + 2);
+    std::vector<double> yh = y;
+    resolve_bc(P, n, mesh.data(), rows);
+    newton(Ch, n, mesh.data(), yh, rows, tol, o.max_newton_iters, o.nlsolve_stages);
+    return global_error_of(yh, y, n, M, errors);
+}
+
 int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
     int rc = check_instance(P);
     if (rc) return rc;
@@ -1014,6 +1087,9 @@ int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
     rc = instance_mesh_and_guess(P, mesh, y);
     if (rc) return rc;
     const double tol = o.newton_abstol > 0 ? o.newton_abstol : o.abstol;
+    if (o.controller < BVP_CONTROL_DEFECT || o.controller > BVP_CONTROL_HYBRID) return BVP_EINVAL;
+    if (o.controller != BVP_CONTROL_DEFECT && o.global_method == BVP_GLOBAL_HO && o.method != BVP_MIRK4)
+        return BVP_ENOTSUP;  // MIRK(p+2) for MIRK6 would be MIRK8
     const int M = P.M;
     int N = P.N;
     int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
@@ -1034,15 +1110,50 @@ int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
         error_norm = 2 * o.abstol;
         if (!o.adaptive) { info = nl.ret; break; }
         info = nl.ret;
+        // error_estimate! of the controller (MIRK/adaptivity.jl:370-525) and the
+        // indicator its mesh_selector! equidistributes (:23-243)
+        std::vector<double> shat;
+        double rho = 1.0;
         if (info == BVP_RET_SUCCESS) {
+            const int p = C.T.order;
             evaluate(C, N, mesh.data(), y.data(), rows, E, false);
-            error_norm = defect(C, N, mesh.data(), y.data(), E, errors);
-            last_defect = error_norm;
-            info = (error_norm > o.defect_threshold || std::isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
+            if (o.controller == BVP_CONTROL_GLOBAL) {
+                error_norm = global_error_estimate(P, o, C, mesh, y, tol, errors);
+                shat = interval_max(errors, N, M);
+                for (double &v : shat) v = std::pow(v / o.abstol, 1.0 / p);
+                rho = 2.0;
+                evaluate(C, N, mesh.data(), y.data(), rows, E, true);  // stages for the transfer
+            } else if (o.controller == BVP_CONTROL_SEQUENTIAL) {
+                error_norm = defect(C, N, mesh.data(), y.data(), E, errors);
+                last_defect = error_norm;
+                info = (error_norm > o.defect_threshold || std::isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
+                if (info == BVP_RET_SUCCESS && error_norm <= o.abstol)
+                    error_norm = global_error_estimate(P, o, C, mesh, y, tol, errors);
+                shat = interval_max(errors, N, M);
+                for (double &v : shat) v = std::pow(v / o.abstol, 1.0 / (p + 1));
+                rho = 2.0;
+            } else if (o.controller == BVP_CONTROL_HYBRID) {
+                std::vector<double> ge;
+                const double dn = defect(C, N, mesh.data(), y.data(), E, errors);
+                last_defect = dn;
+                const double gn = global_error_estimate(P, o, C, mesh, y, tol, ge);
+                error_norm = o.hybrid_de * dn + o.hybrid_ge * gn;
+                std::vector<double> s1 = interval_max(errors, N, M), s2 = interval_max(ge, N, M);
+                shat.resize(N);
+                for (int i = 0; i < N; ++i)
+                    shat[i] = std::pow(s1[i] / o.abstol, 1.0 / (p + 1)) + std::pow(s2[i] / o.abstol, 1.0 / (p + 1));
+                rho = 2.0;
+            } else {
+                error_norm = defect(C, N, mesh.data(), y.data(), E, errors);
+                last_defect = error_norm;
+                info = (error_norm > o.defect_threshold || std::isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
+                shat = interval_max(errors, N, M);
+                for (double &v : shat) v = std::pow(v / o.abstol, 1.0 / (p + 1));
+            }
         }
         if (info == BVP_RET_SUCCESS) {
             if (error_norm > o.abstol) {
-                info = mesh_select(C.T.order, o.abstol, o.max_num_subintervals, mesh, errors, M, new_mesh);
+                info = mesh_select_shat(rho, o.max_num_subintervals, mesh, shat, new_mesh);
                 if (info == BVP_RET_SUCCESS) {
                     transfer(C, o.transfer_mode, mesh, y, E, new_mesh, ynew);
                     mesh = new_mesh;
@@ -1104,6 +1215,25 @@ void ref_opts_default(bvp_opts *o) {
     o->refine_iters = 0;
     o->warm_start = BVP_WARM_NONE;
     o->report_err = 0;
+    o->controller = BVP_CONTROL_DEFECT;
+    o->global_method = BVP_GLOBAL_HO;
+    o->hybrid_de = 1.0;
+    o->hybrid_ge = 1.0;
+}
+
+// The global error estimate of opts.global_method at (mesh, guess): per-interval
+// error vectors (N*M) and the norm the controller compares with abstol.
+int ref_global_error(const bvp_opts *o, const bvp_instance *P, double *errors, double *norm) {
+    int rc = check_instance(*P);
+    if (rc) return rc;
+    if (o->global_method == BVP_GLOBAL_HO && o->method != BVP_MIRK4) return BVP_ENOTSUP;
+    Ctx C(*P, o->method);
+    std::vector<double> mesh, y, e;
+    if ((rc = instance_mesh_and_guess(*P, mesh, y))) return rc;
+    const double tol = o->newton_abstol > 0 ? o->newton_abstol : o->abstol;
+    *norm = global_error_estimate(*P, *o, C, mesh, y, tol, e);
+    std::memcpy(errors, e.data(), sizeof(double) * e.size());
+    return 0;
 }
 
 // BackTracking on the polynomial merit function phi(a) = sum_k coef[k] a^k

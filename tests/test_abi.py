@@ -57,6 +57,9 @@ def test_constants_match_header():
         "BVP_FEAT_LOG10_PARAM": _abi.FEAT_LOG10_PARAM,
         "BVP_STAGE_NEWTON": _abi.STAGE_NEWTON, "BVP_STAGE_BACKTRACKING": _abi.STAGE_BACKTRACKING,
         "BVP_STAGE_TRUST_REGION": _abi.STAGE_TRUST_REGION, "BVP_STAGES_DEFAULT": _abi.STAGES_DEFAULT,
+        "BVP_CONTROL_DEFECT": _abi.CONTROL_DEFECT, "BVP_CONTROL_GLOBAL": _abi.CONTROL_GLOBAL,
+        "BVP_CONTROL_SEQUENTIAL": _abi.CONTROL_SEQUENTIAL, "BVP_CONTROL_HYBRID": _abi.CONTROL_HYBRID,
+        "BVP_GLOBAL_HO": _abi.GLOBAL_HO, "BVP_GLOBAL_RE": _abi.GLOBAL_RE,
     }
     for k, v in pairs.items():
         assert d[k] == v, k

@@ -3,8 +3,10 @@ MIRKInterpolation, MIRK/interpolation.jl:17-206) against the oracle, and the
 reference's interpolation tests (MIRKt/Core/mirk_basic_tests.jl:173-313) run
 end to end on the GPU: solve, then evaluate the extension.
 
-Tolerance against the oracle: 1e-12 relative to max(1, |u|) (FP64 both sides,
-different FMA contraction and summation order).
+Tolerance against the oracle, relative to max(1, |u|): 1e-11 for u, 1e-10 for u'
+(FP64 both sides with different FMA contraction and summation order; the MIRK6
+weight polynomials have coefficients of order 10-100, and the points include
+extrapolation beyond the end nodes, tau > 1, where they grow further).
 """
 import math
 import sys
@@ -57,8 +59,8 @@ def test_interpolate_parity(ctx, method, name):
         ru, rdu = O.interpolate(method, s, t)
         assert u.shape == ru.shape
         if len(t):
-            assert np.max(np.abs(u - ru) / np.maximum(1.0, np.abs(ru))) <= 1e-12
-            assert np.max(np.abs(du - rdu) / np.maximum(1.0, np.abs(rdu))) <= 1e-12
+            assert np.max(np.abs(u - ru) / np.maximum(1.0, np.abs(ru))) <= 1e-11
+            assert np.max(np.abs(du - rdu) / np.maximum(1.0, np.abs(rdu))) <= 1e-10
 
 
 @pytest.mark.parametrize("method", [4, 6])

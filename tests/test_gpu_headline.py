@@ -157,7 +157,8 @@ def test_config5_learned_mixed_adaptive(ctx, singpert_net, autotune):
     o.max_num_subintervals = 16384
     o.precision = _abi.PREC_FP32_REFINE64
     o.report_err = 1
-    b = _lib.Batch(ctx, o, specs)
+    # held to FP32 records: a fresh context, with no tuned choice for this kind of plan
+    b = _lib.Batch(ctx if autotune else _lib.Context(0), o, specs)
     b.set_warm_start(singpert_net, _abi.FEAT_LOG10_PARAM, 0)
     if autotune:
         ms, cfg = b.autotune(repeats=1)

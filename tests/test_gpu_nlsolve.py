@@ -6,8 +6,8 @@ The line search and the trust region take decisions on sums of squares whose
 low digits differ between the two sides (summation order), so a trajectory may
 part from the oracle's in its last digits; the checks are the return code, the
 stage that succeeded, the branch reached and the solution at 1e-9 on the same
-mesh.  Undamped Newton is deterministic to rounding, so its iteration counts
-must agree exactly.
+mesh.  Converging undamped Newton is deterministic to rounding, so its iteration
+counts must agree exactly.
 """
 import json
 import pathlib
@@ -65,7 +65,9 @@ def test_polyalgorithm_matches_oracle(ctx, lam, c):
         r = O.solve(O.opts(4, adaptive=0, abstol=1e-10, nlsolve_stages=stages), spec)
         assert g.retcode == r.retcode, (stages, g.retcode, r.retcode)
         assert (g.fallback_used & 17) == (r.fallback_used & 17), stages
-        if stages == 1:
+        if stages == 1 and r.retcode == 0:
+            # converging Newton is deterministic to rounding; a diverging one may reach a
+            # non-finite residual one step apart
             assert g.newton_iters == r.newton_iters
         if r.retcode == 0:
             _close(g, r)
@@ -76,13 +78,14 @@ def test_polyalgorithm_matches_oracle(ctx, lam, c):
 @pytest.mark.parametrize("entry", GOLDEN["bratu"], ids=lambda e: f"lam{e['lam']}")
 def test_each_stage_alone_lower_branch(ctx, entry):
     _say("stage alone", entry["lam"])
-    spec = cases.bratu(entry["lam"], N=96)
+    spec = cases.bratu(entry["lam"], N=64)  # dt = 1/64 is exact: node 32 is t = 1/2
     for stages in (1, 2, 4):
         g = _lib.solve_batch(ctx, _opts(stages), [spec])[0]
         r = O.solve(O.opts(4, adaptive=0, abstol=1e-10, nlsolve_stages=stages), spec)
         assert g.retcode == r.retcode == 0
         _close(g, r)
-        assert abs(g.u[48, 0] - entry["u_half"]) < 2e-6 * max(1.0, entry["u_half"])
+        assert g.t[32] == 0.5
+        assert abs(g.u[32, 0] - entry["u_half"]) < 2e-6 * max(1.0, entry["u_half"])
 
 
 def test_polyalgorithm_batch_mixed_outcomes(ctx):

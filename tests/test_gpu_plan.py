@@ -63,20 +63,33 @@ def test_autotuner_variants_agree_and_choice_is_fastest(ctx):
     b.sync()
     ref = b.results()
     ms, (variant, blocks) = b.autotune(repeats=2)
-    _say("  candidates ms", ms, "chosen", variant, blocks)
-    assert len(ms) == 3  # M = 2: 1, 2 or 3 waves per SIMD
-    assert ms[variant] == min(ms)
+    minc = b.chunking()
+    _say("  candidates ms", ms, "chosen", variant, blocks, "min intervals per lane", minc)
+    # M = 2: 1, 2 or 3 waves per SIMD, then at least 2, 4, 8 intervals per lane for the fastest
+    assert len(ms) == 6
+    assert min(ms) == ms[variant] if minc == 1 else min(ms) == ms[3 + [2, 4, 8].index(minc)]
+    assert ms[variant] == min(ms[:3])
     assert blocks >= 1
-    b.run()
-    b.sync()
-    _same(ref, b.results())
-    # every variant computes the same bits, at any number of persistent waves
-    for v in range(len(ms)):
-        for nb in (blocks, 7):
-            b.set_config(v, nb)
-            b.run()
-            b.sync()
-            _same(ref, b.results())
+    # every variant computes the same bits, at any number of persistent waves, for a
+    # given interval-per-lane mapping; mappings agree to rounding
+    for m in sorted({1, minc}):
+        b.set_chunking(m)
+        b.set_config(variant, blocks)
+        b.run()
+        b.sync()
+        first = b.results()
+        if m == 1:
+            _same(ref, first)
+        else:
+            for x, y in zip(ref, first):
+                assert x.retcode == y.retcode and x.N == y.N
+                assert np.max(np.abs(x.u - y.u) / np.maximum(1.0, np.abs(x.u))) <= 1e-10
+        for v in range(3):
+            for nb in (blocks, 7):
+                b.set_config(v, nb)
+                b.run()
+                b.sync()
+                _same(first, b.results())
 
 
 def test_block_plan_autotune(ctx):
@@ -110,9 +123,10 @@ def test_autotune_chooses_record_precision(ctx):
     o.precision = _abi.PREC_FP32_REFINE64
     b = _lib.Batch(ctx, o, specs)
     ms, (variant, blocks) = b.autotune(repeats=1)
-    _say("  candidates ms", ms, "chosen", variant, blocks, "precision", b.precision())
-    assert len(ms) == 6  # variants 0..2 with FP32 records, then with FP64 records
-    best = int(np.argmin(ms))
+    _say("  candidates ms", ms, "chosen", variant, blocks, "precision", b.precision(), "chunking", b.chunking())
+    # variants 0..2 with FP32 records, then with FP64 records, then the mappings
+    assert len(ms) == 9
+    best = int(np.argmin(ms[:6]))
     assert variant == best % 3
     assert b.precision() == (_abi.PREC_FP32_REFINE64 if best < 3 else _abi.PREC_FP64)
     b.run()
@@ -125,6 +139,7 @@ def test_autotune_chooses_record_precision(ctx):
     ref_ctx = _lib.Context(0)  # a fresh context: no tuned entry for this kind
     b2 = _lib.Batch(ref_ctx, o2, specs)
     b2.set_config(variant, blocks)
+    b2.set_chunking(b.chunking())
     b2.run()
     b2.sync()
     _same(got, b2.results())
