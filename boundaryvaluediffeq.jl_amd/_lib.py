@@ -42,6 +42,7 @@ def lib():
         "bvp_ctx_destroy": (None, [vp]),
         "bvp_last_error": (C.c_char_p, [vp]),
         "bvp_mirk_solve_batch": (C.c_int, [vp, Opts, C.c_int32, Inst, Res]),
+        "bvp_mirk_solve_batch_noalloc": (C.c_int, [vp, Opts, C.c_int32, Inst, Res]),
         "bvp_mirk_residual": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp]),
         "bvp_mirk_jacobian_blocks": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp, _ip]),
         "bvp_mirk_newton_step": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, Inst, _dp]),
@@ -93,7 +94,7 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_mlp_set_params", "bvp_mlp_gradient", "bvp_mlp_predict", "bvp_mlp_train",
             "bvp_batch_set_warm_start", "bvp_batch_timing", "bvp_batch_learned_guess", "bvp_batch_autotune",
             "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision", "bvp_mirk_interpolate",
-            "bvp_batch_chunking", "bvp_batch_set_chunking"]
+            "bvp_mirk_solve_batch_noalloc", "bvp_batch_chunking", "bvp_batch_set_chunking"]
 
 
 class Context:
@@ -282,6 +283,17 @@ def solve_batch(ctx, opts, specs):
     cap = max(opts.max_num_subintervals, max(s.N for s in specs)) + 1
     res, bufs = _result_array(specs, cap)
     ctx.check(lib().bvp_mirk_solve_batch(ctx.handle, C.byref(opts), len(specs), arr, res), "solve_batch")
+    return [RawSolution(res[i], specs[i].M, *bufs[i]) for i in range(len(specs))]
+
+
+def solve_batch_noalloc(ctx, opts, specs):
+    """bvp_mirk_solve_batch_noalloc: the same solve with no device allocation (the
+    library's static scratch array holds the call's device memory)."""
+    arr = instance_array(specs)
+    cap = max(opts.max_num_subintervals, max(s.N for s in specs)) + 1
+    res, bufs = _result_array(specs, cap)
+    ctx.check(lib().bvp_mirk_solve_batch_noalloc(ctx.handle, C.byref(opts), len(specs), arr, res),
+              "solve_batch_noalloc")
     return [RawSolution(res[i], specs[i].M, *bufs[i]) for i in range(len(specs))]
 
 

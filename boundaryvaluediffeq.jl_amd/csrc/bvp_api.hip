@@ -103,6 +103,18 @@ template <typename T>
 int dalloc(bvp_ctx *ctx, DevArena &arena, T *&p, size_t count) {
     p = nullptr;
     if (count == 0) count = 1;
+    if (arena.bump) {
+        const size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
+        if (arena.bump_used + bytes > arena.bump_cap) {
+            set_err(ctx, "the call needs more than the %zu bytes of the no-allocation scratch (%zu more bytes "
+                         "requested with %zu in use): solve fewer instances per call or lower "
+                         "max_num_subintervals", arena.bump_cap, bytes, arena.bump_used);
+            return BVP_ENOMEM;
+        }
+        p = (T *)(arena.bump + arena.bump_used);
+        arena.bump_used += bytes;
+        return 0;
+    }
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, count * sizeof(T));
     if (e != hipSuccess) {
@@ -128,12 +140,26 @@ DevArena::~DevArena() {
     for (void *p : ptrs) (void)hipFree(p);
 }
 
+// ------------------------------------------------------------------ no-allocation scratch
+// Device memory of bvp_mirk_solve_batch_noalloc: the packed inputs, working
+// state, elimination records and result staging of one call are carved from this
+// array, which each device holds from module load until the process ends.  Host
+// code reaches it only through the address hipGetSymbolAddress returns for the
+// context's device (bvp_ctx::noalloc); calls on one device share it.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 // ------------------------------------------------------------------ batch build
-int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *inst, int cap_req, bvp_batch **out) {
+int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *inst, int cap_req, bvp_batch **out,
+                char *bump, size_t bump_cap) {
     int M;
     int rc = validate(ctx, n, inst, M);
     if (rc) return rc;
     const bool blk = !wave_supports(M, inst[0].family);
+    if (blk && bump) {
+        set_err(ctx, "the no-allocation entry point serves the M <= 4 families");
+        return BVP_ENOTSUP;
+    }
     if (blk) {
         for (int i = 0; i < n; ++i) {
             if (!block_supports(M, inst[i].family)) {
@@ -184,6 +210,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     for (int i = 0; i < n; ++i) maxN0 = std::max(maxN0, inst[i].N);
     int cap = std::max(cap_req, std::max(o->max_num_subintervals, maxN0) + 1);
     auto *B = new bvp_batch();
+    B->arena.bump = bump;
+    B->arena.bump_cap = bump_cap;
     B->n = n;
     B->M = M;
     B->method = o->method;
@@ -477,6 +505,10 @@ int batch_results(bvp_ctx *ctx, bvp_batch *B, bvp_result *res) {
         maxn = std::max(maxn, Nf[i] + 1);
     }
     DevArena tmp;
+    if (B->arena.bump) {  // the gather buffers follow the plan's own part of the region
+        tmp.bump = B->arena.bump + B->arena.bump_used;
+        tmp.bump_cap = B->arena.bump_cap - B->arena.bump_used;
+    }
     long long *doff;
     double *dm, *dy;
     int rc;
@@ -551,6 +583,13 @@ int bvp_ctx_create(int device, bvp_ctx **out) {
     }
     const char *env = std::getenv("BVP_BLOCKS_PER_CU");
     c->blocks_per_cu = env ? std::atoi(env) : 0;
+    // this device's copy of the no-allocation scratch (the device is current here)
+    void *sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) {
+        bvp_ctx_destroy(c);
+        return BVP_EHIP;
+    }
+    c->noalloc = (char *)sp;
     *out = c;
     return 0;
 }
@@ -572,8 +611,22 @@ int bvp_mirk_solve_batch(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_i
     if (!ctx || !o || !res) return BVP_EINVAL;
     (void)hipSetDevice(ctx->device);
     bvp_batch *B = nullptr;
-    int rc = batch_build(ctx, o, n, inst, 0, &B);
+    int rc = batch_build(ctx, o, n, inst, 0, &B, nullptr, 0);
     if (rc) return rc;
+    if ((rc = batch_run(ctx, B)) == 0 && (rc = batch_sync(ctx, B, nullptr)) == 0) rc = batch_results(ctx, B, res);
+    delete B;
+    return rc;
+}
+
+int bvp_mirk_solve_batch_noalloc(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *inst,
+                                 bvp_result *res) {
+    if (!ctx || !o || !res) return BVP_EINVAL;
+    if (!ctx->noalloc) { set_err(ctx, "no-allocation scratch unavailable on this device"); return BVP_EHIP; }
+    (void)hipSetDevice(ctx->device);
+    bvp_batch *B = nullptr;
+    int rc = batch_build(ctx, o, n, inst, 0, &B, ctx->noalloc, kNoAllocBytes);
+    if (rc) return rc;
+    B->use_graph = false;  // launched directly: instantiating a graph may allocate
     if ((rc = batch_run(ctx, B)) == 0 && (rc = batch_sync(ctx, B, nullptr)) == 0) rc = batch_results(ctx, B, res);
     delete B;
     return rc;
@@ -583,7 +636,7 @@ int bvp_mirk_solve_batch(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_i
 int bvp_batch_create(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *inst, bvp_batch **out) {
     if (!ctx || !o || !out) return BVP_EINVAL;
     (void)hipSetDevice(ctx->device);
-    return batch_build(ctx, o, n, inst, 0, out);
+    return batch_build(ctx, o, n, inst, 0, out, nullptr, 0);
 }
 int bvp_batch_run(bvp_ctx *ctx, bvp_batch *B) { return (ctx && B) ? batch_run(ctx, B) : BVP_EINVAL; }
 int bvp_batch_sync(bvp_ctx *ctx, bvp_batch *B, float *ms) { return (ctx && B) ? batch_sync(ctx, B, ms) : BVP_EINVAL; }
@@ -616,7 +669,7 @@ static int run_component(bvp_ctx *ctx, int method, int op, int32_t n, const bvp_
     if (o_in) o = *o_in;
     else bvp_opts_default(&o);
     o.method = method;
-    int rc = batch_build(ctx, &o, n, inst, cap_req, outB);
+    int rc = batch_build(ctx, &o, n, inst, cap_req, outB, nullptr, 0);
     if (rc) return rc;
     // a component call computes in the precision it asks for, with the default chunk
     // mapping, whatever the autotuner chose for plans of this kind

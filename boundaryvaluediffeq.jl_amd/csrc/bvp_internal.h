@@ -22,15 +22,21 @@ struct bvp_ctx {
     hipEvent_t evm = nullptr;      // between the learned guess and the solve of a plan run
     int blocks_per_cu = 0;         // 0: from the occupancy calculator
     std::string err;
+    // device address of the no-allocation entry point's scratch array on this
+    // context's device (hipGetSymbolAddress at creation)
+    char *noalloc = nullptr;
     // autotuner choices, (M, method, precision, block kernel, family or -1) -> (variant,
     // waves per CU, FP32 records, minimum intervals per lane); later plans of the same
     // kind start from them
     std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int, int>> tuned;
 };
 
-// device allocations owned by one batch; freed together
+// device allocations owned by one batch; freed together.  In bump mode (bump set)
+// allocations are carved from a region the arena does not own and nothing is freed.
 struct DevArena {
     std::vector<void *> ptrs;
+    char *bump = nullptr;
+    size_t bump_cap = 0, bump_used = 0;
     DevArena() = default;
     DevArena(const DevArena &) = delete;
     DevArena &operator=(const DevArena &) = delete;

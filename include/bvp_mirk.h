@@ -206,6 +206,20 @@ const char *bvp_last_error(const bvp_ctx *ctx);
 int bvp_mirk_solve_batch(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
                          const bvp_instance *inst, bvp_result *res);
 
+/* bvp_mirk_solve_batch without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * packed inputs, working state, elimination records and result staging of the call
+ * live in a static device array of 64 MiB (held by each device from library load
+ * until the process ends); the call allocates no device memory and returns
+ * BVP_ENOMEM with a message when the batch does not fit (about 100 instances at
+ * max_num_subintervals = 3000 for M = 2; fewer intervals, more instances).  M <= 4
+ * families only (BVP_ENOTSUP otherwise).  Results are the same bits as
+ * bvp_mirk_solve_batch.  All calls on one device share the array: calls that
+ * overlap on one device (other host threads, other contexts) race. */
+int bvp_mirk_solve_batch_noalloc(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
+                                 const bvp_instance *inst, bvp_result *res);
+
 /* Collocation residual at (mesh, guess) of each instance, reference layout
  * (TWO_POINT: [bca; Phi; bcb], STANDARD: [bc; Phi]).  Slot: M*(N+1) doubles.
  * Replaces the loss functions MIRK/mirk.jl:471-534 with
