@@ -8,7 +8,8 @@ from . import _abi
 from .problems import (BVPFunction, BVProblem, TwoPointBVProblem, DiffEqArray, InstanceSpec,
                        LinearTanhFunction, PendulumFunction, BratuFunction, SingPertFunction,
                        ScalarTUFunction, TurningFunction, extract, uniform_mesh, cld, instance_array)
-from .solve import (MIRK4, MIRK6, DefectControl, EnsembleProblem, EnsembleSolution, BVPSolution, solve,
+from .solve import (MIRK4, MIRK6, DefectControl, GlobalErrorControl, SequentialErrorControl, HybridErrorControl,
+                    HOErrorControl, REErrorControl, EnsembleProblem, EnsembleSolution, BVPSolution, solve,
                     successful_retcode, make_opts, NewtonRaphson, BackTracking, TrustRegion,
                     NonlinearSolvePolyAlgorithm, nlsolve_stages)
 
@@ -16,7 +17,8 @@ __all__ = [
     "BVPFunction", "BVProblem", "TwoPointBVProblem", "DiffEqArray", "InstanceSpec",
     "LinearTanhFunction", "PendulumFunction", "BratuFunction", "SingPertFunction",
     "ScalarTUFunction", "TurningFunction", "extract", "uniform_mesh", "cld", "instance_array",
-    "MIRK4", "MIRK6", "DefectControl", "EnsembleProblem", "EnsembleSolution", "BVPSolution", "solve",
+    "MIRK4", "MIRK6", "DefectControl", "GlobalErrorControl", "SequentialErrorControl", "HybridErrorControl",
+    "HOErrorControl", "REErrorControl", "EnsembleProblem", "EnsembleSolution", "BVPSolution", "solve",
     "successful_retcode", "make_opts", "NewtonRaphson", "BackTracking", "TrustRegion",
     "NonlinearSolvePolyAlgorithm", "nlsolve_stages",
 ]

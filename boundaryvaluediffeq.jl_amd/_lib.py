@@ -49,6 +49,7 @@ def lib():
         "bvp_mirk_defect": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp]),
         "bvp_mirk_mesh_select": (C.c_int, [vp, Opts, C.c_int32, Inst, C.c_int32, _ip, _ip, _dp, _dp]),
         "bvp_mirk_interpolate": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _ip, _dp, _dp, _dp]),
+        "bvp_mirk_global_error": (C.c_int, [vp, Opts, C.c_int32, Inst, _dp, _dp]),
         "bvp_batch_create": (C.c_int, [vp, Opts, C.c_int32, Inst, C.POINTER(vp)]),
         "bvp_batch_run": (C.c_int, [vp, vp]),
         "bvp_batch_sync": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
@@ -94,6 +95,7 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_mlp_set_params", "bvp_mlp_gradient", "bvp_mlp_predict", "bvp_mlp_train",
             "bvp_batch_set_warm_start", "bvp_batch_timing", "bvp_batch_learned_guess", "bvp_batch_autotune",
             "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision", "bvp_mirk_interpolate",
+            "bvp_mirk_global_error",
             "bvp_mirk_solve_batch_noalloc", "bvp_batch_chunking", "bvp_batch_set_chunking"]
 
 
@@ -198,6 +200,22 @@ def defect(ctx, method, specs):
     out, o = [], 0
     for i, s in enumerate(specs):
         out.append((err[o:o + s.N * M].reshape(s.N, M).copy(), float(dn[i])))
+        o += s.N * M
+    return out
+
+
+def global_error(ctx, opts, specs):
+    """The global-error estimate of opts.global_method at each instance's (mesh,
+    guess): [(errors (N, M), norm)]."""
+    arr = instance_array(specs)
+    M = specs[0].M
+    err = np.zeros(sum(s.N for s in specs) * M)
+    nrm = np.zeros(len(specs))
+    ctx.check(lib().bvp_mirk_global_error(ctx.handle, C.byref(opts), len(specs), arr, err.ctypes.data_as(_dp),
+                                          nrm.ctypes.data_as(_dp)), "global_error")
+    out, o = [], 0
+    for i, s in enumerate(specs):
+        out.append((err[o:o + s.N * M].reshape(s.N, M).copy(), float(nrm[i])))
         o += s.N * M
     return out
 

@@ -8,7 +8,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "_build"
 LIB = OUT / "libbvp_mirk.so"
-SOURCES = ["bvp_api.hip", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m2v.hip", "wave_m2_bratu.hip", "wave_m2_singpert.hip", "wave_m3.hip", "wave_m4.hip",
+SOURCES = ["bvp_api.hip", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m2v.hip", "wave_m2_bratu.hip", "wave_m2_singpert.hip", "wave_m3.hip", "wave_m4.hip", "wave_ge.hip",
            "wave_f32_m2.hip", "wave_f32_m2v.hip", "wave_f32_bratu.hip", "wave_f32_singpert.hip", "wave_f32_m134.hip",
            "block_dispatch.hip", "block_m8.hip", "block_m16.hip", "block_m32.hip", "mlp.hip"]
 
@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BVP_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    # packed FP32 instructions stay off (see DESIGN.md, MI355X notes)
+    # packed FP32 instructions stay off: some of their forms have given wrong results on this chip
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]

@@ -203,8 +203,18 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         return BVP_EINVAL;
     }
     if (o->controller != BVP_CONTROL_DEFECT) {
-        set_err(ctx, "the GPU path runs DefectControl only (the global-error controllers are in the oracle)");
-        return BVP_ENOTSUP;
+        if (blk) {
+            set_err(ctx, "the global-error controllers run on the M <= 4 kernels");
+            return BVP_ENOTSUP;
+        }
+        if (o->global_method != BVP_GLOBAL_HO && o->global_method != BVP_GLOBAL_RE) {
+            set_err(ctx, "global_method must be BVP_GLOBAL_HO or BVP_GLOBAL_RE");
+            return BVP_EINVAL;
+        }
+        if (o->global_method == BVP_GLOBAL_HO && o->method != BVP_MIRK4) {
+            set_err(ctx, "HOErrorControl needs MIRK(p+2): MIRK4 only (no MIRK8)");
+            return BVP_ENOTSUP;
+        }
     }
     int maxN0 = 0;
     for (int i = 0; i < n; ++i) maxN0 = std::max(maxN0, inst[i].N);
@@ -309,7 +319,9 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     for (int i = 0; i < n; ++i)
         if (inst[i].family != ufam || inst[i].bc_kind != BVP_BC_TWO_POINT) ufam = -1;
     B->family = ufam;
-    const auto key = std::make_tuple(M, o->method, o->precision, blk ? 1 : 0, ufam);
+    // kernel kind: 0 wave, 1 block, 2 wave under a global-error controller
+    const auto key = std::make_tuple(M, o->method, o->precision,
+                                     blk ? 1 : (o->controller != BVP_CONTROL_DEFECT ? 2 : 0), ufam);
     int variant = 0, per_cu = 0, tuned_f32 = f32, tuned_minc = 1;
     if (ctx->tuned.count(key)) std::tie(variant, per_cu, tuned_f32, tuned_minc) = ctx->tuned[key];
     if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
@@ -323,10 +335,13 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     if (slots < 1) slots = 1;
     B->variant = variant;
     size_t scratch_doubles;
+    const bool ge = o->controller != BVP_CONTROL_DEFECT;
     if (!blk) {
         const int NF = wave_rel_fields(M);
-        // records per lane: a chunk holds at most max(ceil(cap / 64), 2 kMaxMinChunk) intervals
-        const size_t lmax = std::max((size_t)(cap + 63) / 64, (size_t)(2 * kMaxMinChunk)) + 1;
+        // records per lane: a chunk holds at most max(ceil(cap / 64), 2 kMaxMinChunk)
+        // intervals; the Richardson estimate solves on the halved mesh (2 cap)
+        const size_t rcap = (size_t)cap * (ge ? 2 : 1);
+        const size_t lmax = std::max((rcap + 63) / 64, (size_t)(2 * kMaxMinChunk)) + 1;
         D.tree_base = lmax * NF * 64;
         D.scratch_per_slot = D.tree_base + (size_t)64 * NF;
         D.aux_per_slot = (size_t)cap * (3 * M + 1);
@@ -334,15 +349,17 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         D.rec_cap = cap;
     } else {
         // factor records of one instance live in its wave's slot: rec_cap intervals.
-        // Room for meshes up to twice the largest initial mesh when memory allows.
         const size_t esz = (o->precision == BVP_PREC_FP32_REFINE64) ? 4 : 8;
         const size_t rv = block_record_values(M);
         // budget: 80% of the device memory free at this point (the problem data is
-        // already allocated), so that meshes can grow to twice the largest initial one
+        // already allocated)
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) free_b = (size_t)120 << 30;
         const size_t budget = free_b / 10 * 8;
-        int rec_cap = std::min(o->max_num_subintervals > maxN0 ? o->max_num_subintervals : maxN0, 2 * maxN0);
+        // room for every mesh the solve may reach (max_num_subintervals) when it fits,
+        // else for meshes up to twice the largest initial one, else for that one
+        int rec_cap = std::max(o->max_num_subintervals, maxN0);
+        if ((size_t)rec_cap * rv * esz * slots > budget) rec_cap = std::min(rec_cap, 2 * maxN0);
         if ((size_t)rec_cap * rv * esz * slots > budget) rec_cap = maxN0;
         const size_t per_slot = (size_t)rec_cap * rv;  // in factor-type elements
         if (per_slot * esz * slots > budget) slots = std::max(1, (int)(budget / (per_slot * esz)));
@@ -355,10 +372,17 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->max_slots = slots;
     B->blocks = std::min(slots, std::max(1, std::min(n, ctx->num_cus * per_cu)));
     D.refine_iters = o->refine_iters;
-    D.f32rec = f32 && tuned_f32;  // the autotuner may have found FP64 records faster for this kind
+    D.f32rec = f32 && tuned_f32 && !ge;  // the autotuner may have found FP64 records faster for this kind;
+                                         // the global-error kernel runs FP64 records
+    D.controller = o->controller;
+    D.global_method = o->global_method;
+    D.hybrid_de = o->hybrid_de;
+    D.hybrid_ge = o->hybrid_ge;
+    D.ge_per_slot = ge ? wave_ge_doubles(cap, M) : 0;
     D.min_chunk = blk ? 1 : std::max(1, std::min(kMaxMinChunk, tuned_minc));
     if ((rc = dalloc(ctx, A, D.scratch, scratch_doubles)) ||
-        (rc = dalloc(ctx, A, D.aux, D.aux_per_slot * slots))) {
+        (rc = dalloc(ctx, A, D.aux, D.aux_per_slot * slots)) ||
+        (ge && (rc = dalloc(ctx, A, D.ge, D.ge_per_slot * slots)))) {
         delete B;
         return rc;
     }
@@ -416,6 +440,8 @@ static int enqueue_solve(bvp_ctx *ctx, bvp_batch *B) {
     }
     if (B->block)
         HIP_TRY(ctx, block_launch_solve(D, B->method, B->opts.precision, B->blocks, ctx->stream));
+    else if (D.controller != BVP_CONTROL_DEFECT)
+        HIP_TRY(ctx, wave_launch_solve_ge(D, B->method, B->blocks, ctx->stream));
     else
         HIP_TRY(ctx, wave_launch_solve(D, B->method, B->variant, B->blocks, B->family, ctx->stream));
     return 0;
@@ -771,6 +797,29 @@ int bvp_mirk_defect(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance 
     return rc;
 }
 
+int bvp_mirk_global_error(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const bvp_instance *inst, double *errors,
+                          double *norm) {
+    if (!ctx || !opts || !errors || !norm) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    bvp_opts o = *opts;
+    if (o.controller == BVP_CONTROL_DEFECT) o.controller = BVP_CONTROL_GLOBAL;  // sizes the global-error buffers
+    o.precision = BVP_PREC_FP64;
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, o.method, -1, n, inst, 0, &o, &B);
+    if (rc) return rc;
+    const size_t nint = (size_t)(B->total_nodes - n);
+    if ((rc = dalloc(ctx, B->arena, B->dev.out0, nint * B->M)) == 0 && (rc = dalloc(ctx, B->arena, B->dev.out1, n)) == 0) {
+        hipError_t e = wave_reset_counter(B->dev.counter, ctx->stream);
+        if (e == hipSuccess) e = wave_launch_global_error(B->dev, B->method, B->blocks, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(errors, B->dev.out0, nint * B->M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(norm, B->dev.out1, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "global error: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+    }
+    delete B;
+    return rc;
+}
+
 int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, const int32_t *n_t,
                          const double *t, double *u, double *du) {
     if (!ctx || !n_t || !u) return BVP_EINVAL;
@@ -904,9 +953,11 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     } else {
         // FP32_REFINE64 on the wave kernels: FP32 records first, then FP64 records (always
         // at least as accurate; faster where the inexact corrections cost Newton steps)
-        const bool mixed = B->opts.precision == BVP_PREC_FP32_REFINE64;
+        // (the global-error kernel: one variant, FP64 records)
+        const bool ge = B->dev.controller != BVP_CONTROL_DEFECT;
+        const bool mixed = B->opts.precision == BVP_PREC_FP32_REFINE64 && !ge;
         for (int f = mixed ? 1 : 0; f >= 0; --f)
-            for (int v = 0; v < wave_variants(B->M); ++v)
+            for (int v = 0; v < (ge ? 1 : wave_variants(B->M)); ++v)
                 cands.push_back({v, std::min(B->max_slots,
                                              ctx->num_cus * std::max(1, wave_blocks_per_cu(B->M, B->method, v, B->family, f))),
                                  f, 1});
@@ -967,7 +1018,8 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     }
     B->drop_graph();
     const int per_cu = (B->blocks + ctx->num_cus - 1) / ctx->num_cus;
-    ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision, B->block ? 1 : 0, B->family)] =
+    ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision,
+                               B->block ? 1 : (B->dev.controller != BVP_CONTROL_DEFECT ? 2 : 0), B->family)] =
         std::make_tuple(B->variant, per_cu, (int)B->dev.f32rec, (int)B->dev.min_chunk);
     return 0;
 }

@@ -25,7 +25,8 @@ struct bvp_ctx {
     // device address of the no-allocation entry point's scratch array on this
     // context's device (hipGetSymbolAddress at creation)
     char *noalloc = nullptr;
-    // autotuner choices, (M, method, precision, block kernel, family or -1) -> (variant,
+    // autotuner choices, (M, method, precision, kernel kind (0 wave, 1 block, 2 wave
+    // under a global-error controller), family or -1) -> (variant,
     // waves per CU, FP32 records, minimum intervals per lane); later plans of the same
     // kind start from them
     std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int, int>> tuned;

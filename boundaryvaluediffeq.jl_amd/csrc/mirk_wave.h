@@ -70,6 +70,14 @@ struct BatchDev {
     int min_chunk;            // wave kernels: at least this many intervals per lane's chunk
                               //   (1 ... kMaxMinChunk; chosen by the autotuner)
 
+    // error controller (MIRK/adaptivity.jl:77-243, 463-567): BVP_CONTROL_* and
+    // BVP_GLOBAL_*; the global-error solve's buffers, per slot: the halved mesh
+    // [2 cap], its state, Newton correction, trial state and J'F [2 cap][M] each
+    int controller, global_method;
+    double hybrid_de, hybrid_ge;
+    double *ge;
+    size_t ge_per_slot;
+
     // options
     int adaptive, max_sub, max_newton, stages, transfer_mode;
     int report_err;           // fill err[] after each solve (one more Newton correction)
@@ -88,6 +96,14 @@ int wave_variants(int M);
 // to one family exists for some (M, method, family))
 hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blocks, int fam, hipStream_t s);
 hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s);
+// the persistent solve with a global-error controller (B.controller != DEFECT;
+// generic family code, FP64 records, one variant)
+hipError_t wave_launch_solve_ge(const BatchDev &B, int method, int blocks, hipStream_t s);
+// the controller's global-error estimate of each instance at its packed state:
+// out0 the interval errors [sum N][M], out1 the norm per instance
+hipError_t wave_launch_global_error(const BatchDev &B, int method, int blocks, hipStream_t s);
+// doubles per slot of the global-error buffers for node capacity cap
+inline size_t wave_ge_doubles(int cap, int M) { return (size_t)2 * cap * (4 * M + 1); }
 hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, double *out_mesh, double *out_y,
                               int max_nodes, hipStream_t s);
 // dense output at nq points: qinst[q] instance of point q, tq[q] its t; u (and du

@@ -4,7 +4,9 @@
 
 namespace bvpdev {
 #define DECL(m, me)                                                                              \
-    extern template hipError_t launch_component_t<m, me>(const BatchDev &, int, int, hipStream_t);
+    extern template hipError_t launch_component_t<m, me>(const BatchDev &, int, int, hipStream_t); \
+    extern template hipError_t launch_solve_ge_t<m, me>(const BatchDev &, int, hipStream_t);      \
+    extern template hipError_t launch_global_error_t<m, me>(const BatchDev &, int, hipStream_t);
 #define DECLV(m, me, w, f32)                                                                     \
     extern template hipError_t launch_solve_t<m, me, w, -1, f32>(const BatchDev &, int, hipStream_t); \
     extern template int occupancy_t<m, me, w, -1, f32>();
@@ -100,6 +102,18 @@ hipError_t wave_launch_component(const BatchDev &B, int method, int op, int bloc
 #define CALL_COMP(m, me) return launch_component_t<m, me>(B, op, blocks, s)
     DISPATCH_M_METHOD(B.M, method, CALL_COMP);
 #undef CALL_COMP
+}
+
+hipError_t wave_launch_solve_ge(const BatchDev &B, int method, int blocks, hipStream_t s) {
+#define CALL_GE(m, me) return launch_solve_ge_t<m, me>(B, blocks, s)
+    DISPATCH_M_METHOD(B.M, method, CALL_GE);
+#undef CALL_GE
+}
+
+hipError_t wave_launch_global_error(const BatchDev &B, int method, int blocks, hipStream_t s) {
+#define CALL_GEST(m, me) return launch_global_error_t<m, me>(B, blocks, s)
+    DISPATCH_M_METHOD(B.M, method, CALL_GEST);
+#undef CALL_GEST
 }
 
 hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, double *out_mesh, double *out_y,

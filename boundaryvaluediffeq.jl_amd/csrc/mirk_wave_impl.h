@@ -1711,19 +1711,21 @@ __device__ inline void half_mesh_into(const double *mesh, int N, double *out, in
     for (int i = lane; i < N; i += WAVE) out[2 * i + 1] = (mesh[i + 1] + mesh[i]) / 2.0;
 }
 
-// mesh_selector!(::DefectControl) + redistribute! (MIRK/adaptivity.jl:23-75, 250-278).
-// shat holds (max|d|/abstol)^(1/(p+1)) per interval on entry (it is divided by h
-// in place).  Writes the new mesh into new_mesh and returns the new N, or -1 on
-// failure (cap exceeded).
+// mesh_selector! + redistribute! (MIRK/adaptivity.jl:23-243, 250-278).  shat
+// holds the controller's indicator per interval on entry ((max|d|/abstol)^(1/(p+1))
+// for DefectControl; it is divided by h in place), r1 and r2 its max and sum; rho
+// the halving threshold (1 for DefectControl, 2 for the global-error controllers).
+// Writes the new mesh into new_mesh and returns the new N, or -1 on failure (cap
+// exceeded).
 template <typename SH>
 __device__ inline int mesh_select(const double *mesh, int N, double *shat, double r1, double r2, int max_sub,
-                                  double *new_mesh, int lane, SH &sh) {
+                                  double *new_mesh, int lane, SH &sh, double rho = 1.0) {
     const int n = N;
     const double r3 = r2 / (double)n;
     long long npred = round_even(1.3 * r2 + 1.0);
     const double n_ = 0.1 * n;
     if (fabs((double)(npred - n)) < n_) npred = round_even((double)n + n_);
-    if (r1 <= 1.0 * r3) {
+    if (r1 <= rho * r3) {
         const long long Ns = 2LL * n;
         if (Ns > max_sub) return -1;
         half_mesh_into(mesh, N, new_mesh, lane);
@@ -1826,6 +1828,144 @@ __device__ inline void transfer(const Inst<M> &I, const double *mesh, const doub
     __syncthreads();
 }
 
+// ------------------------------------------------------------------ global error
+// Julia's max: NaN if either argument is NaN
+__device__ inline double jmax(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : fmax(a, b); }
+
+// error_estimate!(::GlobalErrorControl) (MIRK/adaptivity.jl:511-538) with
+// global_error + GE_subinterval! (:540-556) at (mesh, y).  RE: the same method,
+// non-adaptive, on the halved mesh from halve_sol (:311-333) of y; HO (MIRK4 only):
+// MIRK6 on the same mesh from y.  The nonlinear solve is the same polyalgorithm with
+// the same tolerance, its return code is not consulted.  Per node e = (high - low)
+// / (1 + |low|); per interval the end-node vector with the larger max-abs (the
+// left one on ties; the right one when a max-abs is NaN), whose max-abs goes to
+// ind[i] and, when errs is set, whose components go to errs[i][*].  Returns the
+// max over the intervals (NaN-propagating), RE: times 2^p / (2^p - 1).
+// ge: the slot's global-error buffers (wave_ge_doubles).
+template <int M, int METHOD>
+__device__ double global_error(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh,
+                               const double *y, int N, int lane, double *scr, size_t tree_base, double *ge,
+                               double *ind, double *errs, long long &cnt_it) {
+    const size_t c2 = (size_t)2 * B.cap;
+    double *gm = ge, *gy = ge + c2, *dl = gy + c2 * M, *yt = dl + c2 * M, *gb = yt + c2 * M;
+    int hs = 1;  // node i of the current mesh is node hs * i of the high-order solution
+    if (B.global_method == BVP_GLOBAL_RE) {
+        half_mesh_into(mesh, N, gm, lane);
+        auto fill = [&]() {
+            for (int i = lane; i <= N; i += WAVE)
+#pragma unroll
+                for (int c = 0; c < M; ++c) gy[(size_t)2 * i * M + c] = y[(size_t)i * M + c];
+            for (int i = lane; i < N; i += WAVE)
+#pragma unroll
+                for (int c = 0; c < M; ++c)
+                    gy[(size_t)(2 * i + 1) * M + c] = (y[(size_t)(i + 1) * M + c] + y[(size_t)i * M + c]) / 2.0;
+            __syncthreads();
+        };
+        fill();
+        const NewtonOut nl = newton<M, METHOD>(B, I, sh, gm, gy, fill, dl, yt, gb, 2 * N, lane, scr, tree_base, nullptr);
+        cnt_it += 2LL * N * nl.iters;
+        hs = 2;
+    } else if constexpr (METHOD == 4) {
+        auto fill = [&]() {
+            for (int q = lane; q < (N + 1) * M; q += WAVE) gy[q] = y[q];
+            __syncthreads();
+        };
+        fill();
+        const NewtonOut nl = newton<M, 6>(B, I, sh, mesh, gy, fill, dl, yt, gb, N, lane, scr, tree_base, nullptr);
+        cnt_it += (long long)N * nl.iters;
+    }
+    double nrm = 0.0;
+    bool bad = false;
+    for (int i = lane; i < N; i += WAVE) {
+        double ea[M], eb[M], a = 0.0, b = 0.0;
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const double lo = y[(size_t)i * M + c], lo1 = y[(size_t)(i + 1) * M + c];
+            ea[c] = (gy[(size_t)hs * i * M + c] - lo) / (1.0 + fabs(lo));
+            eb[c] = (gy[(size_t)hs * (i + 1) * M + c] - lo1) / (1.0 + fabs(lo1));
+            a = jmax(a, fabs(ea[c]));
+            b = jmax(b, fabs(eb[c]));
+        }
+        const bool left = a >= b;
+        const double e = left ? a : b;
+        ind[i] = e;
+        if (errs)
+#pragma unroll
+            for (int c = 0; c < M; ++c) errs[(size_t)i * M + c] = left ? ea[c] : eb[c];
+        bad |= isnan(e);
+        nrm = fmax(nrm, e);
+    }
+    const bool anybad = wave_any(bad);
+    nrm = wave_max(nrm);
+    __syncthreads();
+    if (anybad) return NAN;
+    if (B.global_method == BVP_GLOBAL_RE) {
+        const double p2 = pow(2.0, (double)Tab<METHOD>::order);
+        return nrm * p2 / (p2 - 1.0);
+    }
+    return nrm;
+}
+
+// ind[i] -> (ind[i] / abstol)^ex (add: added to what shat holds), r1 and r2 the max
+// and sum over the intervals (chunk order, as the defect pass)
+__device__ inline void indicator(double *shat, const double *ind, int N, int lane, double abstol, double ex, bool add,
+                                 int minc, double &r1, double &r2) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len, minc);
+    double smax = 0.0, ssum = 0.0;
+    for (int k = start; k < start + len; ++k) {
+        double v = pow(ind[k] / abstol, ex);
+        if (add) v = shat[k] + v;
+        shat[k] = v;
+        smax = fmax(smax, v);
+        ssum += v;
+    }
+    r1 = wave_max(smax);
+    r2 = wave_sum(ssum);
+    __syncthreads();
+}
+
+// error_estimate! of the global-error controllers at a converged state
+// (MIRK/adaptivity.jl:463-525, Core/calc_errors.jl:67-163) and the indicator their
+// mesh_selector! equidistributes (MIRK/adaptivity.jl:77-243; halving threshold 2):
+//   GLOBAL      shat = (ge / abstol)^(1/p)
+//   SEQUENTIAL  the defect first (its threshold test sets info); when it is <= abstol
+//               the global error replaces it; shat = (error / abstol)^(1/(p+1))
+//   HYBRID      DE * defect + GE * global error; shat = (d / abstol)^(1/(p+1)) +
+//               (ge / abstol)^(1/(p+1)); no threshold test
+// Returns the error norm the outer loop compares with abstol.
+template <int M, int METHOD>
+__device__ double control_estimate(const BatchDev &B, const Inst<M> &I, Shared<M> &sh, const double *mesh,
+                                   const double *y, int N, int lane, double *scr, size_t tree_base, double *shat,
+                                   double *ge, double &last_defect, int &info, double &r1, double &r2,
+                                   long long &cnt_it, long long &cnt_def) {
+    constexpr int p = Tab<METHOD>::order;
+    const double abstol = B.abstol;
+    // interval indicators of the global error: past the nested solve's buffers
+    double *ind = ge + (size_t)2 * B.cap * (3 * M + 1);
+    if (B.controller == BVP_CONTROL_GLOBAL) {
+        const double gn = global_error<M, METHOD>(B, I, sh, mesh, y, N, lane, scr, tree_base, ge, ind, nullptr, cnt_it);
+        indicator(shat, ind, N, lane, abstol, 1.0 / p, false, I.minc, r1, r2);
+        return gn;
+    }
+    const double dn = defect_pass<M, METHOD>(I, mesh, y, N, lane, abstol, shat, nullptr, r1, r2);
+    cnt_def += N;
+    last_defect = dn;
+    __syncthreads();
+    if (B.controller == BVP_CONTROL_SEQUENTIAL) {
+        info = (dn > B.defect_threshold || isnan(dn)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
+        if (info == BVP_RET_SUCCESS && dn <= abstol) {
+            const double gn = global_error<M, METHOD>(B, I, sh, mesh, y, N, lane, scr, tree_base, ge, ind, nullptr, cnt_it);
+            indicator(shat, ind, N, lane, abstol, 1.0 / (p + 1), false, I.minc, r1, r2);
+            return gn;
+        }
+        return dn;
+    }
+    const double gn = global_error<M, METHOD>(B, I, sh, mesh, y, N, lane, scr, tree_base, ge, ind, nullptr, cnt_it);
+    indicator(shat, ind, N, lane, abstol, 1.0 / (p + 1), true, I.minc, r1, r2);
+    return B.hybrid_de * dn + B.hybrid_ge * gn;
+}
+
 // ------------------------------------------------------------------ instance buffers
 template <int M> struct Bufs {
     double *mesh[2];
@@ -1857,9 +1997,11 @@ __device__ inline Bufs<M> bufs_of(const BatchDev &B, int b) {
 // two-point boundary conditions; every f evaluation then folds to that family's
 // code and the interior-node boundary walk drops out, which lowers the register
 // peak the other paths (sin/cos reduction and the like) impose.
-template <int M, int METHOD, int FAM = -1, bool F32 = false>
+// GE: the batch runs a global-error controller (B.controller != DEFECT) with the
+// slot's global-error buffers ge.
+template <int M, int METHOD, int FAM = -1, bool F32 = false, bool GE = false>
 __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<M> &sh, int lane, double *scr, size_t tree_base,
-                               double *aux) {
+                               double *aux, double *ge = nullptr) {
     Inst<M> I;
     load_inst<M>(B, b, I);
     if constexpr (FAM >= 0) {
@@ -2012,8 +2154,14 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         double error_norm = 2.0 * B.abstol;
         if (!B.adaptive) { info = nl.ret; break; }
         info = nl.ret;
-        double r1 = 0.0, r2 = 0.0;
-        if (info == BVP_RET_SUCCESS) {
+        double r1 = 0.0, r2 = 0.0, rho = 1.0;
+        if constexpr (GE) {
+            if (info == BVP_RET_SUCCESS) {
+                error_norm = control_estimate<M, METHOD>(B, I, sh, mesh, y, N, lane, scr, tree_base, shat, ge,
+                                                         last_defect, info, r1, r2, cnt_it, cnt_def);
+                rho = 2.0;
+            }
+        } else if (info == BVP_RET_SUCCESS) {
             if (nl.dn_valid && !(nl.dn > B.abstol)) {
                 // the defect came with the last correction and no mesh selection follows
                 error_norm = nl.dn;
@@ -2029,7 +2177,7 @@ __device__ __forceinline__ void solve_instance(const BatchDev &B, int b, Shared<
         if (info == BVP_RET_SUCCESS) {
             if (error_norm > B.abstol) {
                 if (cur == 1) bf.m0 = bf.mesh[0];  // buffer 0 receives the new mesh
-                const int Nn = mesh_select(mesh, N, shat, r1, r2, B.max_sub, mesh_o, lane, sh);
+                const int Nn = mesh_select(mesh, N, shat, r1, r2, B.max_sub, mesh_o, lane, sh, rho);
                 if (Nn < 0) {
                     info = BVP_RET_FAILURE;
                 } else {
@@ -2112,6 +2260,49 @@ __global__ __launch_bounds__(64, WPE) void k_solve(BatchDev B) {
         const int item = next_item<M>(B.counter, sh, lane);
         if (item >= B.n) break;
         solve_instance<M, METHOD, FAM, F32>(B, B.order[item], sh, lane, scr, B.tree_base, aux);
+    }
+}
+
+// The persistent solve under a global-error controller: the generic family code,
+// FP64 records, the register allocation of one wave per SIMD.
+template <int M, int METHOD>
+__global__ __launch_bounds__(64, 1) void k_solve_ge(BatchDev B) {
+    __shared__ Shared<M> sh;
+    const int lane = threadIdx.x;
+    double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
+    double *aux = as_global(B.aux + (size_t)blockIdx.x * B.aux_per_slot);
+    double *ge = as_global(B.ge + (size_t)blockIdx.x * B.ge_per_slot);
+    for (;;) {
+        const int item = next_item<M>(B.counter, sh, lane);
+        if (item >= B.n) break;
+        solve_instance<M, METHOD, -1, false, true>(B, B.order[item], sh, lane, scr, B.tree_base, aux, ge);
+    }
+}
+
+// The global-error estimate of each instance at its packed state (mesh and guess):
+// out0 the interval error vectors at the instance's interval offset, out1 the norm.
+template <int M, int METHOD>
+__global__ __launch_bounds__(64, 1) void k_global_error(BatchDev B) {
+    __shared__ Shared<M> sh;
+    const int lane = threadIdx.x;
+    double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
+    double *ge = as_global(B.ge + (size_t)blockIdx.x * B.ge_per_slot);
+    for (;;) {
+        const int b = next_item<M>(B.counter, sh, lane);
+        if (b >= B.n) break;
+        Inst<M> I;
+        load_inst<M>(B, b, I);
+        const int N = B.N0[b];
+        const long long off = B.init_off[b];
+        const double *mesh = as_global(B.init_mesh + off);
+        const double *y = as_global(B.init_y + off * M);
+        const long long ioff = off - b;  // interval offset = node offset - instance index
+        double *ind = ge + (size_t)2 * B.cap * (3 * M + 1);
+        long long cnt = 0;
+        const double g = global_error<M, METHOD>(B, I, sh, mesh, y, N, lane, scr, B.tree_base, ge, ind,
+                                                 B.out0 + ioff * M, cnt);
+        if (lane == 0) B.out1[b] = g;
+        __syncthreads();
     }
 }
 
@@ -2283,6 +2474,16 @@ hipError_t launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
 template <int M, int METHOD>
 hipError_t launch_component_t(const BatchDev &B, int op, int blocks, hipStream_t s) {
     k_component<M, METHOD><<<blocks, 64, 0, s>>>(B, op);
+    return hipGetLastError();
+}
+template <int M, int METHOD>
+hipError_t launch_solve_ge_t(const BatchDev &B, int blocks, hipStream_t s) {
+    k_solve_ge<M, METHOD><<<blocks, 64, 0, s>>>(B);
+    return hipGetLastError();
+}
+template <int M, int METHOD>
+hipError_t launch_global_error_t(const BatchDev &B, int blocks, hipStream_t s) {
+    k_global_error<M, METHOD><<<blocks, 64, 0, s>>>(B);
     return hipGetLastError();
 }
 

@@ -12,9 +12,63 @@ from .problems import BVProblem, extract
 
 class DefectControl:
     """Core/calc_errors.jl:54-60."""
+    code = _abi.CONTROL_DEFECT
 
     def __init__(self, defect_threshold=0.1):
         self.defect_threshold = float(defect_threshold)
+
+
+class HOErrorControl:
+    """Core/calc_errors.jl:131: MIRK(p+2) on the same mesh (MIRK4 only here: there is no MIRK8)."""
+    code = _abi.GLOBAL_HO
+
+
+class REErrorControl:
+    """Core/calc_errors.jl:145: Richardson extrapolation with the same method on the halved mesh."""
+    code = _abi.GLOBAL_RE
+
+
+class GlobalErrorControl:
+    """Core/calc_errors.jl:67-73."""
+    code = _abi.CONTROL_GLOBAL
+
+    def __init__(self, method=None):
+        self.method = method if method is not None else HOErrorControl()
+
+
+class SequentialErrorControl:
+    """Core/calc_errors.jl:80-87: the defect first, then the global error once the defect is satisfied."""
+    code = _abi.CONTROL_SEQUENTIAL
+
+    def __init__(self, defect=None, global_error=None):
+        self.defect = defect if defect is not None else DefectControl()
+        self.global_error = global_error if global_error is not None else GlobalErrorControl()
+
+
+class HybridErrorControl:
+    """Core/calc_errors.jl:94-107: DE * defect + GE * global error."""
+    code = _abi.CONTROL_HYBRID
+
+    def __init__(self, DE=1.0, GE=1.0, defect=None, global_error=None):
+        self.DE = float(DE)
+        self.GE = float(GE)
+        self.defect = defect if defect is not None else DefectControl()
+        self.global_error = global_error if global_error is not None else GlobalErrorControl()
+
+
+def _controller_fields(controller):
+    """(controller, global_method, defect_threshold, DE, GE) of bvp_opts for a controller object."""
+    c = controller if controller is not None else DefectControl()
+    if isinstance(c, DefectControl):
+        return c.code, _abi.GLOBAL_HO, c.defect_threshold, 1.0, 1.0
+    if isinstance(c, GlobalErrorControl):
+        return c.code, c.method.code, DefectControl().defect_threshold, 1.0, 1.0
+    if isinstance(c, SequentialErrorControl):
+        return c.code, c.global_error.method.code, c.defect.defect_threshold, 1.0, 1.0
+    if isinstance(c, HybridErrorControl):
+        return c.code, c.global_error.method.code, c.defect.defect_threshold, c.DE, c.GE
+    raise TypeError("controller must be DefectControl, GlobalErrorControl, SequentialErrorControl or "
+                    "HybridErrorControl")
 
 
 class BackTracking:
@@ -136,7 +190,7 @@ def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=N
     o.newton_abstol = float(nk.get("abstol", abstol))   # MIRK/mirk.jl:51
     o.max_newton_iters = int(nk.get("maxiters", 1000))
     o.adaptive = 1 if adaptive else 0
-    o.defect_threshold = (controller or DefectControl()).defect_threshold
+    (o.controller, o.global_method, o.defect_threshold, o.hybrid_de, o.hybrid_ge) = _controller_fields(controller)
     o.max_num_subintervals = alg.max_num_subintervals
     o.precision = alg.precision
     o.refine_iters = alg.refine_iters

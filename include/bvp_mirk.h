@@ -165,7 +165,9 @@ typedef struct {
                                      correction per instance after the solve; the
                                      returned solution is unchanged).  No counterpart
                                      in the reference; default 0. */
-    int32_t controller;           /* BVP_CONTROL_*; default DEFECT  MIRK/mirk.jl:51 */
+    int32_t controller;           /* BVP_CONTROL_*; default DEFECT  MIRK/mirk.jl:51.
+                                     The global-error controllers run on the M <= 4
+                                     kernels with FP64 records (one kernel variant). */
     int32_t global_method;        /* BVP_GLOBAL_*; default HO  Core/calc_errors.jl:70 */
     double  hybrid_de, hybrid_ge; /* HybridErrorControl weights; 1, 1  Core/calc_errors.jl:100 */
 } bvp_opts;
@@ -264,6 +266,17 @@ int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
                          const bvp_instance *inst, int32_t capacity,
                          int32_t *new_N, int32_t *info, double *new_mesh,
                          double *new_y);
+
+/* The global-error estimate of GlobalErrorControl at (mesh, guess) of each
+ * instance (error_estimate! with REErrorControl / HOErrorControl and
+ * global_error + GE_subinterval!, MIRK/adaptivity.jl:511-556): opts.global_method
+ * picks the method (RE: the same method on the halved mesh; HO: MIRK6 on the same
+ * mesh, MIRK4 only), solved with opts' Newton tolerance, iteration cap and stages.
+ * Outputs: errors [sum N][M] (per interval the end-node relative error vector with
+ * the larger max-abs, instance order), norm [n] (RE: scaled by 2^p / (2^p - 1)).
+ * M <= 4 families. */
+int bvp_mirk_global_error(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const bvp_instance *inst,
+                          double *errors, double *norm);
 
 /* Dense output: the continuous MIRK extension of a solution and its first
  * derivative at arbitrary points -- sol(t) and sol(t, Val{1}) of

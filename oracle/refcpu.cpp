@@ -1022,10 +1022,14 @@ int check_instance(const bvp_instance &P) {
 }
 
 // ---------------------------------------------------------------- global error
+// Julia's max: NaN if either argument is NaN
+double jmax(double a, double b) { return (a != a || b != b) ? NAN : std::max(a, b); }
+
 // global_error + GE_subinterval! (MIRK/adaptivity.jl:540-556): per node the relative
 // difference e_i = (high_i - low_i) / (1 + |low_i|); per interval the one of its two
-// end-node vectors with the larger max-abs (the left one on ties); returns the
-// max-abs over all intervals.
+// end-node vectors with the larger max-abs (the left one on ties, the right one when
+// a max-abs is NaN: ifelse(a >= b, ...)); returns the max-abs over all intervals
+// (NaN-propagating, as Julia's maximum).
 double global_error_of(const std::vector<double> &high, const std::vector<double> &low, int n, int M,
                        std::vector<double> &errors) {
     std::vector<double> e((size_t)(n + 1) * M);
@@ -1035,12 +1039,12 @@ double global_error_of(const std::vector<double> &high, const std::vector<double
     for (int i = 0; i < n; ++i) {
         double a = 0, b = 0;
         for (int c = 0; c < M; ++c) {
-            a = std::max(a, std::fabs(e[(size_t)i * M + c]));
-            b = std::max(b, std::fabs(e[(size_t)(i + 1) * M + c]));
+            a = jmax(a, std::fabs(e[(size_t)i * M + c]));
+            b = jmax(b, std::fabs(e[(size_t)(i + 1) * M + c]));
         }
-        const int src = (a >= b) ? i : i + 1;
-        for (int c = 0; c < M; ++c) errors[(size_t)i * M + c] = e[(size_t)src * M + c];
-        norm = std::max(norm, std::max(a, b));
+        const bool left = a >= b;
+        for (int c = 0; c < M; ++c) errors[(size_t)i * M + c] = e[(size_t)(left ? i : i + 1) * M + c];
+        norm = jmax(norm, left ? a : b);
     }
     return norm;
 }

@@ -105,6 +105,23 @@ def test_opts_default_matches_reference_defaults():
     assert o.abstol == 1e-6 and o.defect_threshold == 0.1 and o.max_num_subintervals == 3000
 
 
+def test_controller_objects_fill_opts():
+    """solve(...; controller) maps the reference's controller structs (Core/calc_errors.jl:54-107,
+    their defaults included) onto bvp_opts."""
+    from boundaryvaluediffeq_amd.solve import make_opts
+    cases = [(bvp.DefectControl(defect_threshold=0.3), (0, 0, 0.3, 1.0, 1.0)),
+             (bvp.GlobalErrorControl(), (1, 0, 0.1, 1.0, 1.0)),
+             (bvp.GlobalErrorControl(method=bvp.REErrorControl()), (1, 1, 0.1, 1.0, 1.0)),
+             (bvp.SequentialErrorControl(defect=bvp.DefectControl(defect_threshold=0.2)), (2, 0, 0.2, 1.0, 1.0)),
+             (bvp.HybridErrorControl(DE=0.5, GE=2.0, global_error=bvp.GlobalErrorControl(method=bvp.REErrorControl())),
+              (3, 1, 0.1, 0.5, 2.0))]
+    for ctl, want in cases:
+        o = make_opts(bvp.MIRK4(), controller=ctl)
+        assert (o.controller, o.global_method, o.defect_threshold, o.hybrid_de, o.hybrid_ge) == want, ctl
+    with pytest.raises(TypeError):
+        make_opts(bvp.MIRK4(), controller=object())
+
+
 def test_dt_to_mesh_like_reference():
     """N = cld(t1 - t0, dt); nodes of range(t0, t1, N+1) (Core/utils.jl:354-362, 694-700)."""
     import math

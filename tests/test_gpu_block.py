@@ -131,3 +131,30 @@ def test_block_solve_mirk6(ctx, M, prec, refine):
             _compare(g, r)
         d = O.newton_step(6, s.with_state(g.t, g.u.reshape(-1)))
         assert np.max(np.abs(d) / np.maximum(1.0, np.abs(g.u.reshape(-1)))) <= 1e-9
+
+
+@pytest.mark.parametrize("n,abstol", [(512, 1e-6), (640, 1e-6), (96, 1e-9), (40, 1e-8)])
+def test_block_solve_config3_sizes(ctx, n, abstol):
+    """Config 3's own sizes (M = 32, MIRK6, FP32 factors + 2 FP64 refinement steps,
+    N0 >= 512), and tighter tolerances whose adaptive meshes grow past twice the
+    initial one (96 -> 227, 40 -> 140 intervals in the oracle): the factor records of
+    every mesh fit, and each solve ends where the oracle's does."""
+    _say("block solve config-3 size", n, abstol)
+    rng = np.random.default_rng(10 + n)
+    spec = cases.linear_tanh_c3(rng, M=32, N=n, spread=(0.5, 50.0))
+    o = _abi.default_opts(6)
+    o.abstol = abstol
+    o.precision = _abi.PREC_FP32_REFINE64
+    o.refine_iters = 2
+    o.newton_abstol = 1e-11
+    o.max_num_subintervals = 8192
+    g = _lib.solve_batch(ctx, o, [spec])[0]
+    r = O.solve(O.opts(6, abstol=abstol, newton_abstol=1e-11, max_num_subintervals=8192), spec)
+    assert r.retcode == 0 and g.retcode == 0, (g.retcode, g.fallback_used)
+    assert not (g.fallback_used & 8)
+    assert g.defect_norm <= abstol
+    if g.N == r.N and np.max(np.abs(g.t - r.t)) <= 1e-10:
+        _compare(g, r)
+    d = O.newton_step(6, spec.with_state(g.t, g.u.reshape(-1)))
+    assert np.max(np.abs(d) / np.maximum(1.0, np.abs(g.u.reshape(-1)))) <= 1e-9
+    _say(f"  N {g.N} (oracle {r.N}), iters {g.newton_iters} ({r.newton_iters})")
