@@ -7,7 +7,7 @@ SciML/BoundaryValueDiffEq.jl; the compute runs in the HIP library
 from . import _abi
 from .problems import (BVPFunction, BVProblem, TwoPointBVProblem, DiffEqArray, InstanceSpec,
                        LinearTanhFunction, PendulumFunction, BratuFunction, SingPertFunction,
-                       ScalarTUFunction, TurningFunction, extract, uniform_mesh, cld, instance_array)
+                       ScalarTUFunction, TurningFunction, SwirlFunction, extract, uniform_mesh, cld, instance_array)
 from .solve import (MIRK4, MIRK6, DefectControl, GlobalErrorControl, SequentialErrorControl, HybridErrorControl,
                     HOErrorControl, REErrorControl, EnsembleProblem, EnsembleSolution, BVPSolution, solve,
                     successful_retcode, make_opts, NewtonRaphson, BackTracking, TrustRegion,
@@ -16,7 +16,7 @@ from .solve import (MIRK4, MIRK6, DefectControl, GlobalErrorControl, SequentialE
 __all__ = [
     "BVPFunction", "BVProblem", "TwoPointBVProblem", "DiffEqArray", "InstanceSpec",
     "LinearTanhFunction", "PendulumFunction", "BratuFunction", "SingPertFunction",
-    "ScalarTUFunction", "TurningFunction", "extract", "uniform_mesh", "cld", "instance_array",
+    "ScalarTUFunction", "TurningFunction", "SwirlFunction", "extract", "uniform_mesh", "cld", "instance_array",
     "MIRK4", "MIRK6", "DefectControl", "GlobalErrorControl", "SequentialErrorControl", "HybridErrorControl",
     "HOErrorControl", "REErrorControl", "EnsembleProblem", "EnsembleSolution", "BVPSolution", "solve",
     "successful_retcode", "make_opts", "NewtonRaphson", "BackTracking", "TrustRegion",

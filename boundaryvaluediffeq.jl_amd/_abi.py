@@ -13,6 +13,7 @@ FAMILY_BRATU = 2
 FAMILY_SINGPERT = 3
 FAMILY_SCALAR_TU = 4
 FAMILY_TURNING = 5
+FAMILY_SWIRL = 6
 
 BC_TWO_POINT = 0
 BC_STANDARD = 1

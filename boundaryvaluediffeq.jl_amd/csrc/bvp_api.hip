@@ -157,7 +157,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     if (rc) return rc;
     const bool blk = !wave_supports(M, inst[0].family);
     if (blk && bump) {
-        set_err(ctx, "the no-allocation entry point serves the M <= 4 families");
+        set_err(ctx, "the no-allocation entry point serves the wave-kernel state dimensions (M = 1..4, 6)");
         return BVP_ENOTSUP;
     }
     if (blk) {
@@ -203,7 +203,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         return BVP_EINVAL;
     }
     if (o->controller != BVP_CONTROL_DEFECT) {
-        if (blk) {
+        if (blk || M > 4) {
             set_err(ctx, "the global-error controllers run on the M <= 4 kernels");
             return BVP_ENOTSUP;
         }
@@ -314,7 +314,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     // the most waves any variant can keep resident, so that the autotuner can try them
     // wave kernels: FP32 storage of the chunk elimination factors (the records are
     // smaller than the FP64 slots they are laid out in)
-    const int f32 = (!blk && o->precision == BVP_PREC_FP32_REFINE64) ? 1 : 0;
+    // (M = 6: FP64 records only)
+    const int f32 = (!blk && M <= 4 && o->precision == BVP_PREC_FP32_REFINE64) ? 1 : 0;
     int ufam = inst[0].family;  // the batch's single family (two-point BCs only), or -1
     for (int i = 0; i < n; ++i)
         if (inst[i].family != ufam || inst[i].bc_kind != BVP_BC_TWO_POINT) ufam = -1;
@@ -831,7 +832,7 @@ int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_inst
     }
     if (nq > 0 && !t) return BVP_EINVAL;
     if (n > 0 && !wave_supports(inst[0].M, inst[0].family)) {
-        set_err(ctx, "dense output serves the M <= 4 families");
+        set_err(ctx, "dense output serves the wave-kernel families (M = 1..4, 6)");
         return BVP_ENOTSUP;
     }
     bvp_batch *B = nullptr;
@@ -839,7 +840,7 @@ int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_inst
     if (rc) return rc;
     if (B->block) {
         delete B;
-        set_err(ctx, "dense output serves the M <= 4 families");
+        set_err(ctx, "dense output serves the wave-kernel families (M = 1..4, 6)");
         return BVP_ENOTSUP;
     }
     std::vector<int> qi((size_t)nq);
@@ -955,7 +956,7 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
         // at least as accurate; faster where the inexact corrections cost Newton steps)
         // (the global-error kernel: one variant, FP64 records)
         const bool ge = B->dev.controller != BVP_CONTROL_DEFECT;
-        const bool mixed = B->opts.precision == BVP_PREC_FP32_REFINE64 && !ge;
+        const bool mixed = B->opts.precision == BVP_PREC_FP32_REFINE64 && !ge && B->M <= 4;
         for (int f = mixed ? 1 : 0; f >= 0; --f)
             for (int v = 0; v < (ge ? 1 : wave_variants(B->M)); ++v)
                 cands.push_back({v, std::min(B->max_slots,

@@ -278,6 +278,17 @@ __device__ inline void f_eval(int fam, const Params<M> &P, double t, const T *y,
             return;
         }
     }
+    if constexpr (M == 6) {
+        if (fam == BVP_FAMILY_SWIRL) {
+            out[0] = y[1];
+            out[1] = (y[0] * y[3] - y[2] * y[1]) / P.p[0];
+            out[2] = y[3];
+            out[3] = y[4];
+            out[4] = y[5];
+            out[5] = (-y[2] * y[5] - y[0] * y[1]) / P.p[0];
+            return;
+        }
+    }
     // BVP_FAMILY_LINEAR_TANH: f = A y + c tanh(y)
     {
         const double c = P.p[0];
@@ -297,6 +308,7 @@ __host__ __device__ inline bool family_supported(int fam) {
     if (M == 2) return fam == BVP_FAMILY_BRATU || fam == BVP_FAMILY_PENDULUM || fam == BVP_FAMILY_SINGPERT ||
                        fam == BVP_FAMILY_TURNING;
     if (M == 1) return fam == BVP_FAMILY_SCALAR_TU;
+    if (M == 6) return fam == BVP_FAMILY_SWIRL;
     return false;
 }
 

@@ -16,6 +16,12 @@ namespace bvpdev {
     DECLV(2, 4, 2, f32) DECLV(2, 6, 2, f32) DECLV(2, 4, 3, f32) DECLV(2, 6, 3, f32)
 DECL(1, 4) DECL(1, 6) DECL(2, 4) DECL(2, 6) DECL(3, 4) DECL(3, 6) DECL(4, 4) DECL(4, 6)
 DECLP(false) DECLP(true)
+// M = 6: FP64 records and the component kernels only (wave_m6_mirk4/6.hip)
+#define DECL6(me)                                                                                \
+    extern template hipError_t launch_component_t<6, me>(const BatchDev &, int, int, hipStream_t); \
+    DECLV(6, me, 1, false)
+DECL6(4) DECL6(6)
+#undef DECL6
 #define DECLF(w, f, f32)                                                                         \
     extern template hipError_t launch_solve_t<2, 4, w, f, f32>(const BatchDev &, int, hipStream_t); \
     extern template int occupancy_t<2, 4, w, f, f32>();
@@ -84,6 +90,10 @@ static hipError_t launch_solve_p(const BatchDev &B, int method, int variant, int
              : w == 2 ? launch_solve_t<2, 4, 2, BVP_FAMILY_SINGPERT, F32>(B, blocks, s)
                       : launch_solve_t<2, 4, 3, BVP_FAMILY_SINGPERT, F32>(B, blocks, s);
     }
+    if (B.M == 6) {
+        if constexpr (F32) return hipErrorInvalidValue;  // FP64 records only at M = 6
+        else return method == 6 ? launch_solve_t<6, 6, 1, -1, false>(B, blocks, s) : launch_solve_t<6, 4, 1, -1, false>(B, blocks, s);
+    }
     if (B.M == 2 && variant == 1)
         return method == 6 ? launch_solve_t<2, 6, 2, -1, F32>(B, blocks, s) : launch_solve_t<2, 4, 2, -1, F32>(B, blocks, s);
     if (B.M == 2 && variant == 2)
@@ -99,6 +109,7 @@ hipError_t wave_launch_solve(const BatchDev &B, int method, int variant, int blo
 }
 
 hipError_t wave_launch_component(const BatchDev &B, int method, int op, int blocks, hipStream_t s) {
+    if (B.M == 6) return method == 6 ? launch_component_t<6, 6>(B, op, blocks, s) : launch_component_t<6, 4>(B, op, blocks, s);
 #define CALL_COMP(m, me) return launch_component_t<m, me>(B, op, blocks, s)
     DISPATCH_M_METHOD(B.M, method, CALL_COMP);
 #undef CALL_COMP
@@ -125,6 +136,7 @@ hipError_t wave_launch_gather(const BatchDev &B, const long long *out_off, doubl
     case 2: k_gather<2><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     case 3: k_gather<3><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     case 4: k_gather<4><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
+    case 6: k_gather<6><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     case 8: k_gather<8><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     case 16: k_gather<16><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
     case 32: k_gather<32><<<grid, 256, 0, s>>>(B, out_off, out_mesh, out_y); break;
@@ -137,6 +149,11 @@ hipError_t wave_launch_interp(const BatchDev &B, int method, const int *qinst, c
                               double *du, long long nq, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     const dim3 grid((unsigned)((nq + 255) / 256));
+    if (B.M == 6) {
+        if (method == 6) k_interp<6, 6><<<grid, 256, 0, s>>>(B, qinst, tq, u, du, nq);
+        else k_interp<6, 4><<<grid, 256, 0, s>>>(B, qinst, tq, u, du, nq);
+        return hipGetLastError();
+    }
 #define CALL_INTERP(m, me)                                                      \
     k_interp<m, me><<<grid, 256, 0, s>>>(B, qinst, tq, u, du, nq);              \
     return hipGetLastError()
@@ -150,6 +167,7 @@ bool wave_supports(int M, int family) {
     case 2: return family_supported<2>(family);
     case 3: return family_supported<3>(family);
     case 4: return family_supported<4>(family);
+    case 6: return family_supported<6>(family);
     }
     return false;
 }
@@ -177,6 +195,7 @@ static int blocks_per_cu_p(int M, int method, int variant, int fam) {
     case 2: return six ? occupancy_t<2, 6, 1, -1, F32>() : occupancy_t<2, 4, 1, -1, F32>();
     case 3: return six ? occupancy_t<3, 6, 1, -1, F32>() : occupancy_t<3, 4, 1, -1, F32>();
     case 4: return six ? occupancy_t<4, 6, 1, -1, F32>() : occupancy_t<4, 4, 1, -1, F32>();
+    case 6: return six ? occupancy_t<6, 6, 1, -1, false>() : occupancy_t<6, 4, 1, -1, false>();
     }
     return 4;
 }

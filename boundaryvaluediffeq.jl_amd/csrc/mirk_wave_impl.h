@@ -1349,6 +1349,10 @@ __device__ inline double jac_norm2(const Inst<M> &I, const Shared<M> &sh, const 
     return acc;
 }
 
+// What the plain Newton step evaluates at the corrected state inside its
+// back-substitution walk: 1 the residual max|F|, 2 also the defect.
+constexpr int kFusePlain = 2;
+
 struct NewtonOut {
     int ret, iters, stage, warm_fallback;
     double fn;  // max|F| at the state Newton returns
@@ -1531,7 +1535,7 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                 bool blocks = (it == 0);
                 double fn = blocks ? residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel)
                                    : (stage == 0 ? fn_next : residual_primal<M, METHOD>(I, mesh, y, N, lane));
-                out.dn_valid = (!blocks && stage == 0) ? 1 : 0;
+                out.dn_valid = (!blocks && stage == 0 && kFusePlain == 2) ? 1 : 0;
                 out.dn = dn_next;
                 if (warm) {
                     bool fall;
@@ -1563,7 +1567,7 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                 if (!blocks) residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
                 const bool plain = (stage == 0);
                 const bool solved =
-                    plain ? solve_correction<M, METHOD, F32, 2>(I, sh, mesh, y, N, lane, scr, tree_base, rel, nullptr,
+                    plain ? solve_correction<M, METHOD, F32, kFusePlain>(I, sh, mesh, y, N, lane, scr, tree_base, rel, nullptr,
                                                                 true, &fn_next, &dn_next)
                           : solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, delta, false);
                 if (!solved) {

@@ -55,6 +55,9 @@ class BVPFunction:
             pi = math.pi
             return np.array([u[1], -t / p[0] * u[1] - pi * pi * math.cos(pi * t)
                              - pi * t / p[0] * math.sin(pi * t)])
+        if fam == _abi.FAMILY_SWIRL:
+            return np.array([u[1], (u[0] * u[3] - u[2] * u[1]) / p[0], u[3], u[4], u[5],
+                             (-u[2] * u[5] - u[0] * u[1]) / p[0]])
         raise ValueError(f"unknown family {fam}")
 
 
@@ -86,6 +89,11 @@ def ScalarTUFunction():
 def TurningFunction():
     """u'' = -t u'/eps - pi^2 cos(pi t) - pi t sin(pi t)/eps; p = [eps]."""
     return BVPFunction(_abi.FAMILY_TURNING, 2, "turning")
+
+
+def SwirlFunction():
+    """Swirling Flow III (MIRKt/Core/mirk_basic_tests.jl:318-327), M = 6; p = [eps]."""
+    return BVPFunction(_abi.FAMILY_SWIRL, 6, "swirl")
 
 
 class DiffEqArray:

@@ -38,7 +38,9 @@ extern "C" {
 #define BVP_MIRK6 6
 
 /* ---- problem families: the right-hand side f(y, p, t) is compiled into the
- * device code, since a host closure cannot run on the GPU.
+ * device code, since a host closure cannot run on the GPU.  State dimensions:
+ * M = 1..4 and 6 on the wave-per-instance kernels, M = 8, 16, 32 on the block
+ * kernels (LINEAR_TANH).
  *   LINEAR_TANH : f = A y + c tanh(y);   params = [c, A(M*M, row-major)]
  *   PENDULUM    : f = [y2, -g sin(y1)];   params = [g]        (M = 2)
  *   BRATU       : f = [y2, -lambda e^y1]; params = [lambda]   (M = 2)
@@ -46,13 +48,17 @@ extern "C" {
  *   SCALAR_TU   : f = t y;                params = []         (M = 1)
  *   TURNING     : f = [y2, -t y2/eps - pi^2 cos(pi t) - pi t sin(pi t)/eps];
  *                 params = [eps]  (M = 2; the turning-point problem of the
- *                 reference's adaptivity test, test/misc/adaptivity_tests.jl:7-10) */
+ *                 reference's adaptivity test, test/misc/adaptivity_tests.jl:7-10)
+ *   SWIRL       : f = [y2, (y1 y4 - y3 y2)/eps, y4, y5, y6, (-y3 y6 - y1 y2)/eps];
+ *                 params = [eps]  (M = 6; the reference's Swirling Flow III test,
+ *                 MIRKt/Core/mirk_basic_tests.jl:315-345) */
 #define BVP_FAMILY_LINEAR_TANH 0
 #define BVP_FAMILY_PENDULUM    1
 #define BVP_FAMILY_BRATU       2
 #define BVP_FAMILY_SINGPERT    3
 #define BVP_FAMILY_SCALAR_TU   4
 #define BVP_FAMILY_TURNING     5
+#define BVP_FAMILY_SWIRL       6
 
 /* ---- boundary-condition kinds
  *   TWO_POINT: TwoPointBVProblem.  Row k is y(bc_t[k])[bc_comp[k]] - bc_val[k]
@@ -79,7 +85,8 @@ extern "C" {
  * M <= 4 (wave kernels): FP32_REFINE64 keeps the elimination in FP64 and stores the
  * chunk factors (the streamed back-substitution records) in FP32; the refinement is
  * Newton's own loop, whose residual and convergence test stay FP64 (refine_iters is
- * not used). */
+ * not used).  M = 6: FP64 records whatever the request (bvp_batch_precision reports
+ * it). */
 #define BVP_PREC_FP64          0  /* block LU in FP64 */
 #define BVP_PREC_FP32_REFINE64 1  /* block LU in FP32 + FP64 iterative refinement */
 
@@ -215,8 +222,8 @@ int bvp_mirk_solve_batch(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
  * live in a static device array of 64 MiB (held by each device from library load
  * until the process ends); the call allocates no device memory and returns
  * BVP_ENOMEM with a message when the batch does not fit (about 100 instances at
- * max_num_subintervals = 3000 for M = 2; fewer intervals, more instances).  M <= 4
- * families only (BVP_ENOTSUP otherwise).  Results are the same bits as
+ * max_num_subintervals = 3000 for M = 2; fewer intervals, more instances).  The
+ * wave-kernel families only (M = 1..4, 6; BVP_ENOTSUP otherwise).  Results are the same bits as
  * bvp_mirk_solve_batch.  All calls on one device share the array: calls that
  * overlap on one device (other host threads, other contexts) race. */
 int bvp_mirk_solve_batch_noalloc(bvp_ctx *ctx, const bvp_opts *opts, int32_t n,
@@ -286,7 +293,8 @@ int bvp_mirk_global_error(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const b
  * order); u receives M values per point (packed the same way), du (optional,
  * NULL to skip) the derivative.  A point lies in interval(mesh, t)
  * (Core/utils.jl:119-121: a node belongs to the interval on its left; points
- * outside [t0, t1] extrapolate the end intervals).  M <= 4 families. */
+ * outside [t0, t1] extrapolate the end intervals).  The wave-kernel families
+ * (M = 1..4, 6). */
 int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst,
                          const int32_t *n_t, const double *t, double *u, double *du);
 

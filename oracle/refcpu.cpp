@@ -18,10 +18,15 @@
 //   linear solve ........ banded Gaussian elimination with partial pivoting
 //                         (the reference hands the banded / almost-banded
 //                         Jacobian to LinearSolve, Core/internal_problems.jl:107-169)
-//   Newton .............. NewtonRaphson, then NewtonRaphson + backtracking
-//                         (Core/default_internal_solve.jl:31-45); restated from
-//                         NonlinearSolveFirstOrder "1.2, 2" / NonlinearSolveBase
-//                         "2.11.2": undamped steps, stop when max|F| <= abstol
+//   Newton .............. the default polyalgorithm NewtonRaphson -> NewtonRaphson
+//                         + BackTracking -> TrustRegion (Core/default_internal_solve.jl:
+//                         31-45), each stage from the same start; restated from the
+//                         published algorithms of NonlinearSolveFirstOrder "1.2, 2" /
+//                         LineSearch (BackTracking of order 3: Armijo 1e-4, quadratic
+//                         then cubic steps clamped to [0.1, 0.5]) and a dogleg trust
+//                         region ("Simple" radius update); stop when max|F| <= tol
+//   dense output ........ MIRK/interpolation.jl:17-206 (sol(t), sol(t, Val{1}))
+//   global error ........ MIRK/adaptivity.jl:77-243, 463-567; Core/calc_errors.jl:54-163
 //   interp_setup! ....... MIRK/interpolation.jl:300-372
 //   defect .............. MIRK/adaptivity.jl:370-461, 623-666
 //   mesh_selector! ...... MIRK/adaptivity.jl:23-75
@@ -179,6 +184,17 @@ void f_eval(const bvp_instance &P, double t, const double *y, double *out) {
         out[1] = -t / p[0] * y[1] - pi * pi * std::cos(pi * t) - pi * t / p[0] * std::sin(pi * t);
         break;
     }
+    case BVP_FAMILY_SWIRL: {
+        // Swirling Flow III (MIRKt/Core/mirk_basic_tests.jl:318-327)
+        const double e = p[0];
+        out[0] = y[1];
+        out[1] = (y[0] * y[3] - y[2] * y[1]) / e;
+        out[2] = y[3];
+        out[3] = y[4];
+        out[4] = y[5];
+        out[5] = (-y[2] * y[5] - y[0] * y[1]) / e;
+        break;
+    }
     }
 }
 
@@ -211,6 +227,17 @@ void J_eval(const bvp_instance &P, double t, const double *y, double *J) {
     case BVP_FAMILY_TURNING:
         J[0] = 0; J[1] = 1; J[2] = 0; J[3] = -t / p[0];
         break;
+    case BVP_FAMILY_SWIRL: {
+        const double e = p[0];
+        for (int i = 0; i < 36; ++i) J[i] = 0.0;
+        J[0 * 6 + 1] = 1.0;
+        J[1 * 6 + 0] = y[3] / e; J[1 * 6 + 1] = -y[2] / e; J[1 * 6 + 2] = -y[1] / e; J[1 * 6 + 3] = y[0] / e;
+        J[2 * 6 + 3] = 1.0;
+        J[3 * 6 + 4] = 1.0;
+        J[4 * 6 + 5] = 1.0;
+        J[5 * 6 + 0] = -y[1] / e; J[5 * 6 + 1] = -y[0] / e; J[5 * 6 + 2] = -y[5] / e; J[5 * 6 + 5] = -y[2] / e;
+        break;
+    }
     }
 }
 
@@ -222,6 +249,7 @@ bool family_ok(const bvp_instance &P) {
     case BVP_FAMILY_SINGPERT:
     case BVP_FAMILY_TURNING: return P.M == 2 && P.n_params >= 1;
     case BVP_FAMILY_SCALAR_TU: return P.M == 1;
+    case BVP_FAMILY_SWIRL: return P.M == 6 && P.n_params >= 1;
     }
     return false;
 }

@@ -37,6 +37,14 @@ def turning(eps, dt=0.01):
     return bvp.extract(prob, dt=dt)
 
 
+def swirl(eps=0.01, dt=0.01, u0=None):
+    """Swirling Flow III (MIRKt/Core/mirk_basic_tests.jl:315-345): M = 6 on [0, 1],
+    u(0)[1] = -1, u(0)[3] = u(0)[4] = 0, u(1)[1] = 1, u(1)[3] = u(1)[4] = 0, zero guess."""
+    bc = [(0.0, 0, -1.0), (0.0, 2, 0.0), (0.0, 3, 0.0), (1.0, 0, 1.0), (1.0, 2, 0.0), (1.0, 3, 0.0)]
+    prob = bvp.BVProblem(bvp.SwirlFunction(), bc, list(u0 if u0 is not None else np.zeros(6)), (0.0, 1.0), [eps])
+    return bvp.extract(prob, dt=dt)
+
+
 def linear(A, c=0.0, tspan=(0.0, 5.0), bc=None, dt=0.2, u0=None, standard=False):
     A = np.asarray(A, dtype=np.float64)
     M = A.shape[0]

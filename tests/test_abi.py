@@ -46,6 +46,7 @@ def test_constants_match_header():
         "BVP_FAMILY_LINEAR_TANH": _abi.FAMILY_LINEAR_TANH, "BVP_FAMILY_PENDULUM": _abi.FAMILY_PENDULUM,
         "BVP_FAMILY_BRATU": _abi.FAMILY_BRATU, "BVP_FAMILY_SINGPERT": _abi.FAMILY_SINGPERT,
         "BVP_FAMILY_SCALAR_TU": _abi.FAMILY_SCALAR_TU, "BVP_FAMILY_TURNING": _abi.FAMILY_TURNING,
+        "BVP_FAMILY_SWIRL": _abi.FAMILY_SWIRL,
         "BVP_BC_TWO_POINT": _abi.BC_TWO_POINT, "BVP_BC_STANDARD": _abi.BC_STANDARD,
         "BVP_RET_SUCCESS": _abi.RET_SUCCESS, "BVP_RET_FAILURE": _abi.RET_FAILURE,
         "BVP_RET_MAXITERS": _abi.RET_MAXITERS, "BVP_RET_UNSTABLE": _abi.RET_UNSTABLE,

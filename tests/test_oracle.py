@@ -148,6 +148,7 @@ FAMILY_CASES = {
     "pendulum": lambda: cases.pendulum(dt=0.4),
     "singpert": lambda: cases.singpert(0.05, N=6),
     "turning": lambda: cases.turning(0.05, dt=0.4),
+    "swirl": lambda: cases.swirl(0.05, dt=0.25),
     "linear_tanh3": lambda: cases.linear([[0.3, 1.0, -0.2], [-1.0, 0.1, 0.4], [0.5, -0.3, -0.6]], c=0.7,
                                          tspan=(0, 1), bc=([(0, 1.0), (2, -1.0)], [(1, 0.5)]), dt=0.25),
 }
