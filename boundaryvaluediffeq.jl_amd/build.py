@@ -8,7 +8,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "_build"
 LIB = OUT / "libbvp_mirk.so"
-SOURCES = ["bvp_api.hip", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m2v.hip", "wave_m2_bratu.hip", "wave_m2_singpert.hip", "wave_m3.hip", "wave_m4.hip", "wave_m6_mirk4.hip", "wave_m6_mirk6.hip", "wave_ge_m12.hip", "wave_ge_m34.hip",
+SOURCES = ["bvp_api.hip", "bvp_host.cpp", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m2v.hip", "wave_m2_bratu.hip", "wave_m2_singpert.hip", "wave_m3.hip", "wave_m4.hip", "wave_m6_mirk4.hip", "wave_m6_mirk6.hip", "wave_ge_m12.hip", "wave_ge_m34.hip",
            "wave_f32_m2.hip", "wave_f32_m2v.hip", "wave_f32_bratu.hip", "wave_f32_singpert.hip", "wave_f32_m134.hip",
            "block_dispatch.hip", "block_m8.hip", "block_m16.hip", "block_m32.hip", "mlp.hip"]
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/bvp_mirk.h"
+#include "bvp_host.h"
 #include "bvp_internal.h"
 #include "mirk_wave.h"
 
@@ -40,63 +41,16 @@ void set_err(bvp_ctx *ctx, const char *fmt, ...) {
         }                                                                                  \
     } while (0)
 
-int params_needed(int family, int M) {
-    switch (family) {
-    case BVP_FAMILY_LINEAR_TANH: return 1 + M * M;
-    case BVP_FAMILY_SCALAR_TU: return 0;
-    default: return 1;
-    }
-}
+using bvphost::params_needed;
+using bvphost::uniform_mesh;
 
-// range(t0; stop=t1, length=N+1) (Core/utils.jl:694-700), correctly rounded
-// through an extended-precision intermediate.
-void uniform_mesh(double t0, double t1, int N, double *m) {
-    for (int k = 0; k <= N; ++k)
-        m[k] = (double)(((long double)t0 * (N - k) + (long double)t1 * k) / (long double)N);
-    m[0] = t0;
-    m[N] = t1;
-}
+bool any_kernel_supports(int M, int family) { return wave_supports(M, family) || blockm_supports(M, family); }
 
 int validate(bvp_ctx *ctx, int32_t n, const bvp_instance *inst, int &M) {
-    if (n <= 0 || !inst) { set_err(ctx, "empty batch"); return BVP_EINVAL; }
-    M = inst[0].M;
-    for (int i = 0; i < n; ++i) {
-        const bvp_instance &P = inst[i];
-        if (P.M != M) { set_err(ctx, "instance %d: all instances of a batch must share M", i); return BVP_EINVAL; }
-        if (P.M < 1) { set_err(ctx, "instance %d: M must be >= 1", i); return BVP_EINVAL; }
-        if (!wave_supports(M, P.family) && !blockm_supports(M, P.family)) {
-            set_err(ctx, "instance %d: family %d with M=%d is not supported", i, P.family, M);
-            return BVP_ENOTSUP;
-        }
-        if (P.n_params < params_needed(P.family, M) || (params_needed(P.family, M) > 0 && !P.params)) {
-            set_err(ctx, "instance %d: family %d needs %d parameters", i, P.family, params_needed(P.family, M));
-            return BVP_EINVAL;
-        }
-        if (P.n_bc != M || !P.bc_t || !P.bc_comp || !P.bc_val) {
-            set_err(ctx, "instance %d: need exactly M boundary rows", i);
-            return BVP_EINVAL;
-        }
-        if (P.bc_kind != BVP_BC_TWO_POINT && P.bc_kind != BVP_BC_STANDARD) {
-            set_err(ctx, "instance %d: bad bc_kind", i);
-            return BVP_EINVAL;
-        }
-        for (int k = 0; k < M; ++k) {
-            if (P.bc_comp[k] < 0 || P.bc_comp[k] >= M) { set_err(ctx, "instance %d: bc_comp out of range", i); return BVP_EINVAL; }
-            if (P.bc_kind == BVP_BC_TWO_POINT && P.bc_t[k] != P.t0 && P.bc_t[k] != P.t1) {
-                set_err(ctx, "instance %d: two-point BC row %d not at t0 or t1", i, k);
-                return BVP_EINVAL;
-            }
-            if (!(P.bc_t[k] >= P.t0 && P.bc_t[k] <= P.t1)) { set_err(ctx, "instance %d: BC point outside tspan", i); return BVP_EINVAL; }
-        }
-        if (P.N < 1) { set_err(ctx, "instance %d: N must be >= 1", i); return BVP_EINVAL; }
-        if (!(P.t1 > P.t0)) { set_err(ctx, "instance %d: need t1 > t0", i); return BVP_EINVAL; }
-        if (!P.guess && !P.u0) { set_err(ctx, "instance %d: need a guess or u0", i); return BVP_EINVAL; }
-        if (P.mesh) {
-            for (int k = 0; k < P.N; ++k)
-                if (!(P.mesh[k + 1] > P.mesh[k])) { set_err(ctx, "instance %d: mesh not increasing", i); return BVP_EINVAL; }
-        }
-    }
-    return 0;
+    std::string err;
+    const int rc = bvphost::validate_instances(n, inst, M, any_kernel_supports, err);
+    if (rc) set_err(ctx, "%s", err.c_str());
+    return rc;
 }
 
 template <typename T>
@@ -161,61 +115,21 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         return BVP_ENOTSUP;
     }
     if (blk) {
-        for (int i = 0; i < n; ++i) {
+        for (int i = 0; i < n; ++i)
             if (!block_supports(M, inst[i].family)) {
                 set_err(ctx, "instance %d: family %d with M=%d is not supported", i, inst[i].family, M);
                 return BVP_ENOTSUP;
             }
-            for (int k = 0; k < M; ++k)
-                if (inst[i].bc_t[k] != inst[i].t0 && inst[i].bc_t[k] != inst[i].t1) {
-                    set_err(ctx, "instance %d: for M=%d boundary rows must sit at t0 or t1", i, M);
-                    return BVP_ENOTSUP;
-                }
-        }
-    } else if (o->precision != BVP_PREC_FP64 && o->precision != BVP_PREC_FP32_REFINE64) {
-        set_err(ctx, "precision must be BVP_PREC_FP64 or BVP_PREC_FP32_REFINE64");
-        return BVP_EINVAL;
     }
-    if (o->method != BVP_MIRK4 && o->method != BVP_MIRK6) { set_err(ctx, "method must be 4 or 6"); return BVP_EINVAL; }
+    {
+        std::string err;
+        if ((rc = bvphost::validate_opts(o, n, inst, M, blk, err))) {
+            set_err(ctx, "%s", err.c_str());
+            return rc;
+        }
+    }
     bool has_u0 = true;
     for (int i = 0; i < n; ++i) has_u0 &= inst[i].u0 != nullptr;
-    if (o->warm_start == BVP_WARM_MLP) {
-        if (blk || M != 2) { set_err(ctx, "the learned warm start serves M = 2 problems"); return BVP_ENOTSUP; }
-        if (!has_u0) { set_err(ctx, "warm start: every instance needs u0 (the fallback guess)"); return BVP_EINVAL; }
-        for (int i = 0; i < n; ++i)
-            if (!inst[i].guess) { set_err(ctx, "warm start: instance %d has no guess", i); return BVP_EINVAL; }
-    } else if (o->warm_start != BVP_WARM_NONE) {
-        set_err(ctx, "unknown warm_start");
-        return BVP_EINVAL;
-    }
-    if (!(o->abstol > 0)) { set_err(ctx, "abstol must be positive"); return BVP_EINVAL; }
-    if (o->nlsolve_stages < 1 || o->nlsolve_stages > 7) {
-        set_err(ctx, "nlsolve_stages must be a non-empty mask of bits 0 (NewtonRaphson), 1 (BackTracking), "
-                     "2 (TrustRegion)");
-        return BVP_EINVAL;
-    }
-    if (blk && !(o->nlsolve_stages & 3)) {
-        set_err(ctx, "the block kernels (M >= 8) run the NewtonRaphson stages (bits 0 and 1) only");
-        return BVP_ENOTSUP;
-    }
-    if (o->controller < BVP_CONTROL_DEFECT || o->controller > BVP_CONTROL_HYBRID) {
-        set_err(ctx, "controller must be one of BVP_CONTROL_*");
-        return BVP_EINVAL;
-    }
-    if (o->controller != BVP_CONTROL_DEFECT) {
-        if (blk || M > 4) {
-            set_err(ctx, "the global-error controllers run on the M <= 4 kernels");
-            return BVP_ENOTSUP;
-        }
-        if (o->global_method != BVP_GLOBAL_HO && o->global_method != BVP_GLOBAL_RE) {
-            set_err(ctx, "global_method must be BVP_GLOBAL_HO or BVP_GLOBAL_RE");
-            return BVP_EINVAL;
-        }
-        if (o->global_method == BVP_GLOBAL_HO && o->method != BVP_MIRK4) {
-            set_err(ctx, "HOErrorControl needs MIRK(p+2): MIRK4 only (no MIRK8)");
-            return BVP_ENOTSUP;
-        }
-    }
     int maxN0 = 0;
     for (int i = 0; i < n; ++i) maxN0 = std::max(maxN0, inst[i].N);
     int cap = std::max(cap_req, std::max(o->max_num_subintervals, maxN0) + 1);
@@ -230,49 +144,16 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->block = blk;
     B->has_u0 = has_u0;
     // ---- pack the problem description
-    std::vector<int> fam(n), poff(n), pcnt(n), bck(n), N0(n), bcc((size_t)n * M), order(n);
-    std::vector<double> t0(n), t1(n), params, bct((size_t)n * M), bcv((size_t)n * M);
-    std::vector<long long> ioff(n);
-    long long nodes = 0;
-    for (int i = 0; i < n; ++i) {
-        const bvp_instance &P = inst[i];
-        fam[i] = P.family;
-        t0[i] = P.t0;
-        t1[i] = P.t1;
-        poff[i] = (int)params.size();
-        pcnt[i] = P.n_params;
-        for (int k = 0; k < P.n_params; ++k) params.push_back(P.params[k]);
-        bck[i] = P.bc_kind;
-        for (int k = 0; k < M; ++k) {
-            bct[(size_t)i * M + k] = P.bc_t[k];
-            bcc[(size_t)i * M + k] = P.bc_comp[k];
-            bcv[(size_t)i * M + k] = P.bc_val[k];
-        }
-        N0[i] = P.N;
-        ioff[i] = nodes;
-        nodes += P.N + 1;
-    }
-    B->N0 = N0;
-    B->init_off = ioff;
-    B->total_nodes = nodes;
-    std::vector<double> imesh((size_t)nodes), iy((size_t)nodes * M), u0c((size_t)n * M, 0.0);
-    for (int i = 0; i < n; ++i)
-        if (inst[i].u0)
-            for (int c = 0; c < M; ++c) u0c[(size_t)i * M + c] = inst[i].u0[c];
-    for (int i = 0; i < n; ++i) {
-        const bvp_instance &P = inst[i];
-        double *m = &imesh[(size_t)ioff[i]];
-        if (P.mesh) std::memcpy(m, P.mesh, sizeof(double) * (P.N + 1));
-        else uniform_mesh(P.t0, P.t1, P.N, m);
-        double *y = &iy[(size_t)ioff[i] * M];
-        if (P.guess) std::memcpy(y, P.guess, sizeof(double) * (size_t)M * (P.N + 1));
-        else
-            for (int k = 0; k <= P.N; ++k)
-                for (int c = 0; c < M; ++c) y[(size_t)k * M + c] = P.u0[c];
-    }
-    // queue order: largest meshes first (longest-processing-time first)
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return N0[a] > N0[b]; });
+    bvphost::Packed pk;
+    bvphost::pack_instances(n, inst, M, pk);
+    B->N0 = pk.N0;
+    B->init_off = pk.off;
+    B->total_nodes = pk.nodes;
+    std::vector<int> &fam = pk.fam, &poff = pk.poff, &pcnt = pk.pcnt, &bck = pk.bck, &N0 = pk.N0, &bcc = pk.bcc,
+                     &order = pk.order;
+    std::vector<double> &t0 = pk.t0, &t1 = pk.t1, &params = pk.params, &bct = pk.bct, &bcv = pk.bcv,
+                        &imesh = pk.mesh, &iy = pk.y, &u0c = pk.u0c;
+    std::vector<long long> &ioff = pk.off;
 
     BatchDev &D = B->dev;
     std::memset(&D, 0, sizeof D);
