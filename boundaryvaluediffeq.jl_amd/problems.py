@@ -12,7 +12,9 @@ Problem details are extracted as the reference does in
 (Core/utils.jl:330-388, 694-704, 750-777).
 """
 from fractions import Fraction
+import inspect
 import math
+import warnings
 
 import numpy as np
 
@@ -214,18 +216,49 @@ def instance_array(specs):
     return arr
 
 
+def _initial_guess(f, p, t):
+    """An initial-guess function at t with the (p, t) calling convention, or the
+    deprecated single-argument t form (Core/utils.jl:395-410)."""
+    try:
+        sig = inspect.signature(f)
+        npos = sum(1 for q in sig.parameters.values()
+                   if q.kind in (q.POSITIONAL_ONLY, q.POSITIONAL_OR_KEYWORD) and q.default is q.empty)
+        var = any(q.kind == q.VAR_POSITIONAL for q in sig.parameters.values())
+    except (TypeError, ValueError):
+        npos, var = 2, False
+    if npos == 2 or var:
+        return f(p, t)
+    if npos == 1:
+        warnings.warn("initial guess function must take 2 inputs `(p, t)` instead of just `t`",
+                      DeprecationWarning, stacklevel=3)
+        return f(t)
+    raise TypeError("`initial_guess` must be a function of the form `f(p, t)`")
+
+
 def extract(prob, dt=0.0):
-    """Problem details, mesh and guess of ``prob`` (Core/utils.jl:330-388, 694-704, 750-777)."""
+    """Problem details, mesh and guess of ``prob`` (Core/utils.jl:330-388, 694-704, 750-777).
+
+    u0 may be a constant vector (or number), a vector of per-node vectors, a
+    DiffEqArray or any solution object carrying ``t`` and ``u`` (ODESolution: its
+    nodes become the mesh), or a function ``f(p, t)`` (or the deprecated ``f(t)``)
+    evaluated at the nodes of the mesh from dt (Core/utils.jl:750-777)."""
     t0, t1 = prob.tspan
     M = prob.f.M
     u0 = prob.u0
     mesh = None
     guess = None
     const = None
-    if isinstance(u0, DiffEqArray):
+    if isinstance(u0, DiffEqArray) or (hasattr(u0, "t") and hasattr(u0, "u")):
         N = len(u0.u) - 1
-        mesh = u0.t.copy()
+        mesh = np.asarray(u0.t, dtype=np.float64).copy()
         guess = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in u0.u])
+    elif callable(u0):
+        if not dt > 0:
+            raise ValueError("dt must be positive")
+        N = cld(t1 - t0, dt)
+        nodes = uniform_mesh(t0, t1, N)
+        guess = np.concatenate([np.asarray(_initial_guess(u0, prob.p, float(t)), dtype=np.float64).reshape(M)
+                                for t in nodes])
     else:
         a = np.asarray(u0, dtype=np.float64)
         if a.ndim == 2 or (a.ndim == 1 and isinstance(u0, (list, tuple)) and len(u0) > 0

@@ -188,3 +188,43 @@ def test_integration_julia_structs_match_header(jname, cname):
     """The Julia mirrors in INTEGRATION.md have the header's fields, in order, with matching types
     (a Julia caller passes these structs by reference to the library)."""
     assert _julia_struct_fields(jname) == _c_struct_fields(cname)
+
+
+def test_initial_guess_forms_like_reference():
+    """u0 as a function f(p, t) (and the deprecated f(t)) is evaluated at the nodes of the
+    mesh from dt; a solution object (t, u) gives the mesh and the guess
+    (Core/utils.jl:395-410, 683-704, 750-777); the oracle solves all forms alike."""
+    import math
+    import warnings
+    import numpy as np
+    import pyoracle as O
+    f = bvp.BratuFunction()
+    bc = ([(0, 0.0)], [(0, 0.0)])
+    guess_pt = lambda p, t: [0.1 * p[0] * t * (1 - t), 0.1 * p[0] * (1 - 2 * t)]
+    prob = bvp.TwoPointBVProblem(f, bc, guess_pt, (0.0, 1.0), [2.0])
+    s = bvp.extract(prob, dt=0.1)
+    assert s.N == 10 and s.u0 is None
+    nodes = bvp.uniform_mesh(0.0, 1.0, 10)
+    assert np.allclose(s.guess.reshape(-1, 2), np.array([guess_pt([2.0], t) for t in nodes]), rtol=0, atol=0)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        s1 = bvp.extract(bvp.TwoPointBVProblem(f, bc, lambda t: guess_pt([2.0], t), (0.0, 1.0), [2.0]), dt=0.1)
+    assert any(issubclass(x.category, DeprecationWarning) for x in w)
+    assert np.array_equal(s1.guess, s.guess)
+    with pytest.raises(TypeError):
+        bvp.extract(bvp.TwoPointBVProblem(f, bc, lambda a, b, c: 0, (0.0, 1.0), [2.0]), dt=0.1)
+    # a solution object as the guess: its nodes become the mesh
+    r = O.solve(O.opts(4, abstol=1e-8), s)
+    class Sol:
+        pass
+    sol = Sol()
+    sol.t = r.t
+    sol.u = [r.u[k] for k in range(r.u.shape[0])]
+    s2 = bvp.extract(bvp.TwoPointBVProblem(f, bc, sol, (0.0, 1.0), [2.0]), dt=0.5)
+    assert s2.N == r.N and np.array_equal(s2.mesh, r.t)
+    r2 = O.solve(O.opts(4, abstol=1e-8, max_newton_iters=0, adaptive=0), s2)
+    assert r2.retcode == 0 and np.array_equal(r2.u, r.u)
+    # a number as the constant guess of a scalar problem
+    s3 = bvp.extract(bvp.BVProblem(bvp.ScalarTUFunction(), [(0.5, 0, 1.0)], 0.0, (0.0, 1.0), []), dt=0.25)
+    assert s3.N == 4 and s3.u0.shape == (1,)
+    assert math.isfinite(float(s3.u0[0]))
