@@ -1350,7 +1350,12 @@ __device__ inline double jac_norm2(const Inst<M> &I, const Shared<M> &sh, const 
 }
 
 // What the plain Newton step evaluates at the corrected state inside its
-// back-substitution walk: 1 the residual max|F|, 2 also the defect.
+// back-substitution walk: 1 the residual max|F|, 2 also the defect -- the defect
+// only when the step is predicted to converge: from max|F| = fn, quadratic
+// convergence C fn^2 <= tol with C taken as 1e-4 (fn <= 100 sqrt(tol); on the
+// Bratu family at tol 1e-8 nine in ten first steps from the learned guess start above
+// sqrt(tol) and still converge).  A step that converges without the defect is
+// followed by a separate defect pass, which computes the same values.
 constexpr int kFusePlain = 2;
 
 struct NewtonOut {
@@ -1530,12 +1535,14 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
             // back-substitution; after a line-search step from a primal pass.  The dual
             // pass that builds the blocks runs only when another step is taken.
             double fn_next = NAN, dn_next = NAN;
+            bool dn_fused = false;  // the last plain step evaluated the defect of its result
+            const double fuse_defect_below = 100.0 * sqrt(B.newton_tol);
             for (int it = 0;; ++it) {
                 Rel<M> rel;
                 bool blocks = (it == 0);
                 double fn = blocks ? residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel)
                                    : (stage == 0 ? fn_next : residual_primal<M, METHOD>(I, mesh, y, N, lane));
-                out.dn_valid = (!blocks && stage == 0 && kFusePlain == 2) ? 1 : 0;
+                out.dn_valid = (!blocks && stage == 0 && dn_fused) ? 1 : 0;
                 out.dn = dn_next;
                 if (warm) {
                     bool fall;
@@ -1566,10 +1573,14 @@ __device__ __forceinline__ NewtonOut newton(const BatchDev &B, const Inst<M> &I,
                 if (it >= B.max_newton) { ret = BVP_RET_MAXITERS; break; }
                 if (!blocks) residual_and_blocks<M, METHOD, F32, FAM>(I, sh, mesh, y, N, lane, scr, rel);
                 const bool plain = (stage == 0);
+                dn_fused = plain && kFusePlain == 2 && fn <= fuse_defect_below;
+                dn_next = NAN;
                 const bool solved =
-                    plain ? solve_correction<M, METHOD, F32, kFusePlain>(I, sh, mesh, y, N, lane, scr, tree_base, rel, nullptr,
-                                                                true, &fn_next, &dn_next)
-                          : solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, delta, false);
+                    !plain ? solve_correction<M, METHOD, F32>(I, sh, mesh, y, N, lane, scr, tree_base, rel, delta, false)
+                    : dn_fused ? solve_correction<M, METHOD, F32, 2>(I, sh, mesh, y, N, lane, scr, tree_base, rel,
+                                                                     nullptr, true, &fn_next, &dn_next)
+                               : solve_correction<M, METHOD, F32, 1>(I, sh, mesh, y, N, lane, scr, tree_base, rel,
+                                                                     nullptr, true, &fn_next, nullptr);
                 if (!solved) {
                     if (warm) {  // singular at the warm start: restart from the constant guess
                         warm = false;
