@@ -393,7 +393,7 @@ def main():
     ap.add_argument("--instances", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=128, help="instances of the oracle run (lintanh workload)")
     ap.add_argument("--no-learned", action="store_true", help="skip the warm-start network (constant guess only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (variant 0 at full residency)")
