@@ -7,6 +7,30 @@ import ctypes as C
 MIRK4 = 4
 MIRK6 = 6
 
+
+def RADAU_IIA(s):
+    return 100 + s
+
+
+def LOBATTO_IIIA(s):
+    return 200 + s
+
+
+def LOBATTO_IIIB(s):
+    return 300 + s
+
+
+def LOBATTO_IIIC(s):
+    return 400 + s
+
+
+def is_firk(method):
+    return method >= 100
+
+
+def firk_stages(method):
+    return method % 100
+
 FAMILY_LINEAR_TANH = 0
 FAMILY_PENDULUM = 1
 FAMILY_BRATU = 2
@@ -114,6 +138,7 @@ class Result(C.Structure):
         ("err_vs_fp64", C.c_double),
         ("mesh", _dp),
         ("u", _dp),
+        ("stages", _dp),
     ]
 
 

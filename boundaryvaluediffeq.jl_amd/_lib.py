@@ -77,6 +77,10 @@ def lib():
         "bvp_mlp_train": (C.c_int, [vp, C.c_int64, _fp, _fp, C.POINTER(_abi.TrainOpts), _fp, _ip,
                                     C.POINTER(C.c_float)]),
         "bvp_batch_set_warm_start": (C.c_int, [vp, vp, C.c_int32, C.c_int32]),
+        "bvp_firk_residual": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp]),
+        "bvp_firk_newton_step": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp]),
+        "bvp_firk_defect": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _dp, _dp]),
+        "bvp_firk_interpolate": (C.c_int, [vp, C.c_int32, C.c_int32, Inst, _dp, _ip, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -96,7 +100,8 @@ EXPORTED = ["bvp_opts_default", "bvp_ctx_create", "bvp_ctx_destroy", "bvp_last_e
             "bvp_batch_set_warm_start", "bvp_batch_timing", "bvp_batch_learned_guess", "bvp_batch_autotune",
             "bvp_batch_config", "bvp_batch_set_config", "bvp_batch_precision", "bvp_mirk_interpolate",
             "bvp_mirk_global_error",
-            "bvp_mirk_solve_batch_noalloc", "bvp_batch_chunking", "bvp_batch_set_chunking"]
+            "bvp_mirk_solve_batch_noalloc", "bvp_batch_chunking", "bvp_batch_set_chunking",
+            "bvp_firk_residual", "bvp_firk_newton_step", "bvp_firk_defect", "bvp_firk_interpolate"]
 
 
 class Context:
@@ -261,6 +266,87 @@ def mesh_select(ctx, opts, specs, capacity=None):
     return out
 
 
+# ---- FIRK components.  stages: None (the reference's initial stages K_ij = y_{i+1}) or
+# a list of (N, s, M) arrays, one per instance.
+def _firk_stages_arg(method, specs, stages):
+    if stages is None:
+        return None
+    s = _abi.firk_stages(method)
+    parts = [np.ascontiguousarray(k, dtype=np.float64).reshape(sp.N * s * sp.M) for k, sp in zip(stages, specs)]
+    return np.concatenate(parts) if parts else np.zeros(1)
+
+
+def _nx(method, spec):
+    return spec.M * (spec.N * (_abi.firk_stages(method) + 1) + 1)
+
+
+def firk_residual(ctx, method, specs, stages=None):
+    arr = instance_array(specs)
+    st = _firk_stages_arg(method, specs, stages)
+    out = np.zeros(sum(_nx(method, s) for s in specs))
+    ctx.check(lib().bvp_firk_residual(ctx.handle, method, len(specs), arr,
+                                      st.ctypes.data_as(_dp) if st is not None else _dp(),
+                                      out.ctypes.data_as(_dp)), "firk_residual")
+    res, o = [], 0
+    for s in specs:
+        k = _nx(method, s)
+        res.append(out[o:o + k].copy())
+        o += k
+    return res
+
+
+def firk_newton_step(ctx, method, specs, stages=None):
+    arr = instance_array(specs)
+    st = _firk_stages_arg(method, specs, stages)
+    out = np.zeros(sum(_nx(method, s) for s in specs))
+    ctx.check(lib().bvp_firk_newton_step(ctx.handle, method, len(specs), arr,
+                                         st.ctypes.data_as(_dp) if st is not None else _dp(),
+                                         out.ctypes.data_as(_dp)), "firk_newton_step")
+    res, o = [], 0
+    for s in specs:
+        k = _nx(method, s)
+        res.append(out[o:o + k].copy())
+        o += k
+    return res
+
+
+def firk_defect(ctx, method, specs, stages=None):
+    arr = instance_array(specs)
+    st = _firk_stages_arg(method, specs, stages)
+    M = specs[0].M
+    err = np.zeros(sum(s.N for s in specs) * M)
+    dn = np.zeros(len(specs))
+    ctx.check(lib().bvp_firk_defect(ctx.handle, method, len(specs), arr,
+                                    st.ctypes.data_as(_dp) if st is not None else _dp(),
+                                    err.ctypes.data_as(_dp), dn.ctypes.data_as(_dp)), "firk_defect")
+    out, o = [], 0
+    for i, s in enumerate(specs):
+        out.append((err[o:o + s.N * M].reshape(s.N, M).copy(), float(dn[i])))
+        o += s.N * M
+    return out
+
+
+def firk_interpolate(ctx, method, specs, tvals, stages=None, derivative=True):
+    arr = instance_array(specs)
+    st = _firk_stages_arg(method, specs, stages)
+    M = specs[0].M
+    ts = [np.ascontiguousarray(t, dtype=np.float64).reshape(-1) for t in tvals]
+    nt = np.array([len(t) for t in ts], dtype=np.int32)
+    tt = np.concatenate(ts) if len(ts) else np.zeros(0)
+    u = np.zeros(max(1, len(tt)) * M)
+    du = np.zeros(max(1, len(tt)) * M) if derivative else None
+    ctx.check(lib().bvp_firk_interpolate(ctx.handle, method, len(specs), arr,
+                                         st.ctypes.data_as(_dp) if st is not None else _dp(),
+                                         nt.ctypes.data_as(_ip), tt.ctypes.data_as(_dp), u.ctypes.data_as(_dp),
+                                         du.ctypes.data_as(_dp) if derivative else _dp()), "firk_interpolate")
+    out, o = [], 0
+    for k in nt:
+        out.append((u[o * M:(o + k) * M].reshape(k, M).copy(),
+                    du[o * M:(o + k) * M].reshape(k, M).copy() if derivative else None))
+        o += k
+    return out
+
+
 def copy_probe(ctx, nbytes, repeats=3):
     ms = C.c_float()
     ctx.check(lib().bvp_copy_probe(ctx.handle, int(nbytes), int(repeats), C.byref(ms)), "copy_probe")
@@ -281,27 +367,41 @@ class RawSolution:
         self.err_vs_fp64 = float(r.err_vs_fp64)
         self.t = mesh[:self.N + 1].copy()
         self.u = u[:M * (self.N + 1)].reshape(self.N + 1, M).copy()
+        self.stages = None
 
 
-def _result_array(specs, cap):
+def _result_array(specs, cap, stages=0):
     n = len(specs)
     res = (_abi.Result * n)()
     bufs = []
     for i, s in enumerate(specs):
         m, u = np.zeros(cap), np.zeros(cap * s.M)
-        bufs.append((m, u))
+        k = np.zeros(max(1, cap * stages * s.M)) if stages else None
+        bufs.append((m, u, k))
         res[i].capacity = cap
         res[i].mesh = m.ctypes.data_as(_dp)
         res[i].u = u.ctypes.data_as(_dp)
+        if stages:
+            res[i].stages = k.ctypes.data_as(_dp)
     return res, bufs
 
 
+def _raw(res, spec, buf, stages):
+    r = RawSolution(res, spec.M, buf[0], buf[1])
+    if stages:
+        r.stages = buf[2][:r.N * stages * spec.M].reshape(r.N, stages, spec.M).copy()
+    return r
+
+
 def solve_batch(ctx, opts, specs):
+    """bvp_mirk_solve_batch; for a FIRK method each solution also carries its final
+    stage values (N, s, M)."""
     arr = instance_array(specs)
     cap = max(opts.max_num_subintervals, max(s.N for s in specs)) + 1
-    res, bufs = _result_array(specs, cap)
+    st = _abi.firk_stages(opts.method) if _abi.is_firk(opts.method) else 0
+    res, bufs = _result_array(specs, cap, st)
     ctx.check(lib().bvp_mirk_solve_batch(ctx.handle, C.byref(opts), len(specs), arr, res), "solve_batch")
-    return [RawSolution(res[i], specs[i].M, *bufs[i]) for i in range(len(specs))]
+    return [_raw(res[i], specs[i], bufs[i], st) for i in range(len(specs))]
 
 
 def solve_batch_noalloc(ctx, opts, specs):
@@ -309,10 +409,11 @@ def solve_batch_noalloc(ctx, opts, specs):
     library's static scratch array holds the call's device memory)."""
     arr = instance_array(specs)
     cap = max(opts.max_num_subintervals, max(s.N for s in specs)) + 1
-    res, bufs = _result_array(specs, cap)
+    st = _abi.firk_stages(opts.method) if _abi.is_firk(opts.method) else 0
+    res, bufs = _result_array(specs, cap, st)
     ctx.check(lib().bvp_mirk_solve_batch_noalloc(ctx.handle, C.byref(opts), len(specs), arr, res),
               "solve_batch_noalloc")
-    return [RawSolution(res[i], specs[i].M, *bufs[i]) for i in range(len(specs))]
+    return [_raw(res[i], specs[i], bufs[i], st) for i in range(len(specs))]
 
 
 class Batch:
@@ -337,9 +438,10 @@ class Batch:
 
     def results(self):
         cap = max(self.opts.max_num_subintervals, max(s.N for s in self.specs)) + 1
-        res, bufs = _result_array(self.specs, cap)
+        st = _abi.firk_stages(self.opts.method) if _abi.is_firk(self.opts.method) else 0
+        res, bufs = _result_array(self.specs, cap, st)
         self.ctx.check(lib().bvp_batch_results(self.ctx.handle, self._h, res), "batch_results")
-        return [RawSolution(res[i], self.specs[i].M, *bufs[i]) for i in range(len(self.specs))]
+        return [_raw(res[i], self.specs[i], bufs[i], st) for i in range(len(self.specs))]
 
     def learned_guess(self):
         """The attached network's guess at every node, one (N+1, M) array per instance."""

@@ -128,6 +128,9 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
             return rc;
         }
     }
+    const bool firk = bvphost::is_firk(o->method);
+    bvphost::FirkTableau ftab;
+    if (firk) bvphost::firk_tableau(o->method, ftab);  // validate_opts accepted the code
     bool has_u0 = true;
     for (int i = 0; i < n; ++i) has_u0 &= inst[i].u0 != nullptr;
     int maxN0 = 0;
@@ -142,6 +145,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     B->cap = cap;
     B->opts = *o;
     B->block = blk;
+    B->firk = firk;
+    B->firk_s = firk ? ftab.s : 0;
     B->has_u0 = has_u0;
     // ---- pack the problem description
     bvphost::Packed pk;
@@ -196,22 +201,24 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     // wave kernels: FP32 storage of the chunk elimination factors (the records are
     // smaller than the FP64 slots they are laid out in)
     // (M = 6: FP64 records only)
-    const int f32 = (!blk && M <= 4 && o->precision == BVP_PREC_FP32_REFINE64) ? 1 : 0;
+    const int f32 = (!blk && !firk && M <= 4 && o->precision == BVP_PREC_FP32_REFINE64) ? 1 : 0;
     int ufam = inst[0].family;  // the batch's single family (two-point BCs only), or -1
     for (int i = 0; i < n; ++i)
         if (inst[i].family != ufam || inst[i].bc_kind != BVP_BC_TWO_POINT) ufam = -1;
     B->family = ufam;
-    // kernel kind: 0 wave, 1 block, 2 wave under a global-error controller
+    // kernel kind: 0 wave, 1 block, 2 wave under a global-error controller, 3 FIRK
     const auto key = std::make_tuple(M, o->method, o->precision,
-                                     blk ? 1 : (o->controller != BVP_CONTROL_DEFECT ? 2 : 0), ufam);
+                                     blk ? 1 : (firk ? 3 : (o->controller != BVP_CONTROL_DEFECT ? 2 : 0)), ufam);
     int variant = 0, per_cu = 0, tuned_f32 = f32, tuned_minc = 1;
     if (ctx->tuned.count(key)) std::tie(variant, per_cu, tuned_f32, tuned_minc) = ctx->tuned[key];
     if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
+    // the FIRK kernels hold one wave per SIMD (register allocation of 512)
     if (per_cu < 1)
-        per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision) : wave_blocks_per_cu(M, o->method, variant, ufam, f32);
+        per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision)
+                     : (firk ? 4 : wave_blocks_per_cu(M, o->method, variant, ufam, f32));
     if (per_cu < 1) per_cu = 1;
-    int slots = std::min(n, ctx->num_cus * per_cu);
-    if (!blk)
+    int slots = std::min(n, ctx->num_cus * (firk ? 4 : per_cu));
+    if (!blk && !firk)
         for (int v = 0; v < wave_variants(M); ++v)
             slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v, ufam, f32))));
     if (slots < 1) slots = 1;
@@ -226,7 +233,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         const size_t lmax = std::max((rcap + 63) / 64, (size_t)(2 * kMaxMinChunk)) + 1;
         D.tree_base = lmax * NF * 64;
         D.scratch_per_slot = D.tree_base + (size_t)64 * NF;
-        D.aux_per_slot = (size_t)cap * (3 * M + 1);
+        D.aux_per_slot = firk ? firk_aux_doubles(cap, M, ftab.s) : (size_t)cap * (3 * M + 1);
         scratch_doubles = D.scratch_per_slot * slots;
         D.rec_cap = cap;
     } else {
@@ -267,6 +274,23 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         (ge && (rc = dalloc(ctx, A, D.ge, D.ge_per_slot * slots)))) {
         delete B;
         return rc;
+    }
+    if (firk) {
+        FirkTab tab{};
+        std::memcpy(tab.A, ftab.A, sizeof tab.A);
+        std::memcpy(tab.b, ftab.b, sizeof tab.b);
+        std::memcpy(tab.c, ftab.c, sizeof tab.c);
+        std::memcpy(tab.Q, ftab.Q, sizeof tab.Q);
+        tab.tau = ftab.tau;
+        tab.s = ftab.s;
+        tab.order = ftab.order;
+        FirkTab *dtab = nullptr;
+        if ((rc = upload(ctx, A, dtab, std::vector<FirkTab>(1, tab))) ||
+            (rc = dalloc(ctx, A, D.stK, (size_t)n * cap * ftab.s * M))) {
+            delete B;
+            return rc;
+        }
+        D.firk_tab = dtab;
     }
     // options
     D.adaptive = o->adaptive;
@@ -320,7 +344,9 @@ static int enqueue_solve(bvp_ctx *ctx, bvp_batch *B) {
         }
         D.warm = 1;
     }
-    if (B->block)
+    if (B->firk)
+        HIP_TRY(ctx, firk_launch_solve(D, B->firk_s, B->blocks, ctx->stream));
+    else if (B->block)
         HIP_TRY(ctx, block_launch_solve(D, B->method, B->opts.precision, B->blocks, ctx->stream));
     else if (D.controller != BVP_CONTROL_DEFECT)
         HIP_TRY(ctx, wave_launch_solve_ge(D, B->method, B->blocks, ctx->stream));
@@ -444,6 +470,29 @@ int batch_results(bvp_ctx *ctx, bvp_batch *B, bvp_result *res) {
         if (r.u) std::memcpy(r.u, &hy[(size_t)off[i] * M], sizeof(double) * (size_t)M * (Nf[i] + 1));
     }
     if (bad) { set_err(ctx, "result capacity smaller than the final mesh"); return BVP_EINVAL; }
+    bool want_stages = false;
+    for (int i = 0; i < n; ++i) want_stages |= B->firk && res[i].stages != nullptr;
+    if (want_stages) {
+        // final stage values, instances back to back at interval offsets
+        const int S = B->firk_s;
+        std::vector<long long> ioff(n);
+        long long nint = 0;
+        for (int i = 0; i < n; ++i) {
+            ioff[i] = nint;
+            nint += Nf[i];
+        }
+        long long *dio;
+        double *dk;
+        if ((rc = upload(ctx, tmp, dio, ioff)) || (rc = dalloc(ctx, tmp, dk, (size_t)std::max(1LL, nint) * S * M)))
+            return rc;
+        HIP_TRY(ctx, firk_launch_gather(D, S, dio, dk, ctx->stream));
+        std::vector<double> hk((size_t)nint * S * M);
+        HIP_TRY(ctx, hipMemcpyAsync(hk.data(), dk, hk.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; ++i)
+            if (res[i].stages)
+                std::memcpy(res[i].stages, &hk[(size_t)ioff[i] * S * M], sizeof(double) * (size_t)Nf[i] * S * M);
+    }
     return 0;
 }
 
@@ -572,11 +621,26 @@ void bvp_batch_destroy(bvp_batch *B) { delete B; }
 
 // ---- component entry points
 static int run_component(bvp_ctx *ctx, int method, int op, int32_t n, const bvp_instance *inst, int cap_req,
-                         const bvp_opts *o_in, bvp_batch **outB) {
+                         const bvp_opts *o_in, bvp_batch **outB, bool firk_entry = false) {
+    if (bvphost::is_firk(method) != firk_entry) {
+        set_err(ctx, firk_entry ? "method %d is not a FIRK method" : "method %d: FIRK methods have bvp_firk_* entry points",
+                method);
+        return BVP_EINVAL;
+    }
     bvp_opts o;
     if (o_in) o = *o_in;
     else bvp_opts_default(&o);
     o.method = method;
+    if (firk_entry) {
+        // the component kernels use neither adaptivity nor the nonlinear solver's options;
+        // the stage state is sized for the given meshes
+        o.adaptive = 0;
+        o.nlsolve_stages = BVP_STAGE_NEWTON | BVP_STAGE_BACKTRACKING;
+        o.precision = BVP_PREC_FP64;
+        int maxN = 1;
+        for (int i = 0; i < n && inst; ++i) maxN = std::max(maxN, inst[i].N);
+        o.max_num_subintervals = maxN;
+    }
     int rc = batch_build(ctx, &o, n, inst, cap_req, outB, nullptr, 0);
     if (rc) return rc;
     // a component call computes in the precision it asks for, with the default chunk
@@ -748,6 +812,123 @@ int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_inst
     return rc;
 }
 
+// ---- FIRK components
+// The stage values of the component kernels: the caller's, or extend_y's
+// (K_ij = y_{i+1} from the instance's guess or u0), packed at interval offsets.
+static int firk_upload_stages(bvp_ctx *ctx, bvp_batch *B, int32_t n, const bvp_instance *inst, const double *stages) {
+    const int M = B->M, S = B->firk_s;
+    const long long nint = B->total_nodes - n;
+    std::vector<double> K((size_t)std::max(1LL, nint) * S * M, 0.0);
+    if (stages) {
+        std::memcpy(K.data(), stages, sizeof(double) * (size_t)nint * S * M);
+    } else {
+        long long o = 0;
+        for (int i = 0; i < n; ++i) {
+            const bvp_instance &P = inst[i];
+            for (int k = 0; k < P.N; ++k)
+                for (int r = 0; r < S; ++r)
+                    for (int c = 0; c < M; ++c)
+                        K[((size_t)(o + k) * S + r) * M + c] = P.guess ? P.guess[(size_t)(k + 1) * M + c] : P.u0[c];
+            o += P.N;
+        }
+    }
+    double *dk = nullptr;
+    int rc = upload(ctx, B->arena, dk, K);
+    if (rc) return rc;
+    B->dev.init_K = dk;
+    return 0;
+}
+
+static int firk_component(bvp_ctx *ctx, int32_t method, int op, int32_t n, const bvp_instance *inst,
+                          const double *stages, size_t out0_count, double *out0, double *out1) {
+    (void)hipSetDevice(ctx->device);
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, method, op, n, inst, 0, nullptr, &B, true);
+    if (rc) return rc;
+    if ((rc = firk_upload_stages(ctx, B, n, inst, stages)) == 0 &&
+        (rc = dalloc(ctx, B->arena, B->dev.out0, std::max<size_t>(1, out0_count))) == 0 &&
+        (!out1 || (rc = dalloc(ctx, B->arena, B->dev.out1, n)) == 0)) {
+        hipError_t e = wave_reset_counter(B->dev.counter, ctx->stream);
+        if (e == hipSuccess) e = firk_launch_component(B->dev, B->firk_s, op, B->blocks, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(out0, B->dev.out0, out0_count * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && out1) e = hipMemcpy(out1, B->dev.out1, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "FIRK component: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+    }
+    delete B;
+    return rc;
+}
+
+// doubles of the expanded vectors of a batch: sum_i M (N_i (s + 1) + 1)
+static size_t firk_nx_total(int method, int32_t n, const bvp_instance *inst) {
+    size_t t = 0;
+    const int S = method % 100;
+    for (int i = 0; i < n; ++i) t += (size_t)inst[i].M * ((size_t)inst[i].N * (S + 1) + 1);
+    return t;
+}
+
+int bvp_firk_residual(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, const double *stages,
+                      double *resid) {
+    if (!ctx || !resid || !inst || n <= 0) return BVP_EINVAL;
+    return firk_component(ctx, method, OP_RESIDUAL, n, inst, stages, firk_nx_total(method, n, inst), resid, nullptr);
+}
+
+int bvp_firk_newton_step(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, const double *stages,
+                         double *delta) {
+    if (!ctx || !delta || !inst || n <= 0) return BVP_EINVAL;
+    return firk_component(ctx, method, OP_NEWTON_STEP, n, inst, stages, firk_nx_total(method, n, inst), delta,
+                          nullptr);
+}
+
+int bvp_firk_defect(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, const double *stages,
+                    double *errors, double *defect_norm) {
+    if (!ctx || !errors || !defect_norm || !inst || n <= 0) return BVP_EINVAL;
+    size_t nint = 0;
+    for (int i = 0; i < n; ++i) nint += (size_t)inst[i].N * inst[i].M;
+    return firk_component(ctx, method, OP_DEFECT, n, inst, stages, nint, errors, defect_norm);
+}
+
+int bvp_firk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst, const double *stages,
+                         const int32_t *n_t, const double *t, double *u, double *du) {
+    if (!ctx || !n_t || !u || !inst || n <= 0) return BVP_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    long long nq = 0;
+    for (int i = 0; i < n; ++i) {
+        if (n_t[i] < 0) { set_err(ctx, "instance %d: negative point count", i); return BVP_EINVAL; }
+        nq += n_t[i];
+    }
+    if (nq > 0 && !t) return BVP_EINVAL;
+    bvp_batch *B = nullptr;
+    int rc = run_component(ctx, method, OP_RESIDUAL, n, inst, 0, nullptr, &B, true);
+    if (rc) return rc;
+    if ((rc = firk_upload_stages(ctx, B, n, inst, stages))) {
+        delete B;
+        return rc;
+    }
+    std::vector<int> qi((size_t)nq);
+    for (int i = 0, q = 0; i < n; ++i)
+        for (int k = 0; k < n_t[i]; ++k) qi[q++] = i;
+    std::vector<double> tv(t, t + nq);
+    int *dqi = nullptr;
+    double *dt = nullptr, *du_ = nullptr, *ddu = nullptr;
+    const int M = B->M;
+    if (nq > 0 &&
+        ((rc = upload(ctx, B->arena, dqi, qi)) || (rc = upload(ctx, B->arena, dt, tv)) ||
+         (rc = dalloc(ctx, B->arena, du_, (size_t)nq * M)) || (du && (rc = dalloc(ctx, B->arena, ddu, (size_t)nq * M))))) {
+        delete B;
+        return rc;
+    }
+    if (nq > 0) {
+        hipError_t e = firk_launch_interp(B->dev, B->firk_s, dqi, dt, du_, ddu, nq, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(u, du_, (size_t)nq * M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && du) e = hipMemcpy(du, ddu, (size_t)nq * M * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_err(ctx, "FIRK interpolate: %s", hipGetErrorString(e)); rc = BVP_EHIP; }
+    }
+    delete B;
+    return rc;
+}
+
 int bvp_mirk_mesh_select(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const bvp_instance *inst, int32_t capacity,
                          int32_t *new_N, int32_t *info, double *new_mesh, double *new_y) {
     if (!ctx || !opts || !new_N || !info || !new_mesh || !new_y) return BVP_EINVAL;
@@ -827,7 +1008,10 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     (void)hipSetDevice(ctx->device);
     struct Cand { int variant, blocks, f32, minc; };
     std::vector<Cand> cands;
-    if (B->block) {
+    if (B->firk) {
+        // one kernel per (M, stages): waves per CU only
+        for (int pc = 1; pc <= 4; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc), 0, 1});
+    } else if (B->block) {
         const int occ = std::max(1, block_blocks_per_cu(B->M, B->method, B->opts.precision));
         for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc), 0, 1});
         if (cands.back().blocks != std::min(B->max_slots, ctx->num_cus * occ))
@@ -868,7 +1052,7 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     };
     // phase 1: kernel variant and record precision; phase 2 (wave kernels): the
     // interval-per-lane mapping of the fastest one (at least 2, 4, 8 intervals per lane)
-    for (int phase = 0; phase < (B->block ? 1 : 2); ++phase) {
+    for (int phase = 0; phase < ((B->block || B->firk) ? 1 : 2); ++phase) {
         if (phase == 1) {
             const Cand b0 = cands[best];
             for (int m = 2; m <= kMaxMinChunk; m *= 2) cands.push_back({b0.variant, b0.blocks, b0.f32, m});
@@ -901,7 +1085,8 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     B->drop_graph();
     const int per_cu = (B->blocks + ctx->num_cus - 1) / ctx->num_cus;
     ctx->tuned[std::make_tuple(B->M, B->method, B->opts.precision,
-                               B->block ? 1 : (B->dev.controller != BVP_CONTROL_DEFECT ? 2 : 0), B->family)] =
+                               B->block ? 1 : (B->firk ? 3 : (B->dev.controller != BVP_CONTROL_DEFECT ? 2 : 0)),
+                               B->family)] =
         std::make_tuple(B->variant, per_cu, (int)B->dev.f32rec, (int)B->dev.min_chunk);
     return 0;
 }
@@ -913,7 +1098,8 @@ int bvp_batch_chunking(bvp_batch *B, int32_t *min_chunk) {
 }
 
 int bvp_batch_set_chunking(bvp_batch *B, int32_t min_chunk) {
-    if (!B || min_chunk < 1 || min_chunk > kMaxMinChunk || (B->block && min_chunk != 1)) return BVP_EINVAL;
+    if (!B || min_chunk < 1 || min_chunk > kMaxMinChunk || ((B->block || B->firk) && min_chunk != 1))
+        return BVP_EINVAL;
     B->dev.min_chunk = min_chunk;
     B->drop_graph();
     return 0;
@@ -927,7 +1113,7 @@ int bvp_batch_precision(bvp_batch *B, int32_t *precision) {
 
 int bvp_batch_set_config(bvp_ctx *ctx, bvp_batch *B, int32_t variant, int32_t blocks) {
     if (!ctx || !B) return BVP_EINVAL;
-    const int nv = B->block ? 1 : wave_variants(B->M);
+    const int nv = (B->block || B->firk) ? 1 : wave_variants(B->M);
     if (variant < 0 || variant >= nv || blocks < 1 || blocks > B->max_slots) {
         set_err(ctx, "variant must be in [0, %d) and blocks in [1, %d]", nv, B->max_slots);
         return BVP_EINVAL;

@@ -18,6 +18,18 @@ int params_needed(int family, int M);
 // through an extended-precision intermediate
 void uniform_mesh(double t0, double t1, int N, double *m);
 
+// Tableau of a FIRK method code (BVP_RADAU_IIA(s), BVP_LOBATTO_IIIA/B/C(s)): A in Butcher
+// order, b, c, Q (Q[k][j]: the coefficient of tau^(k+1) in int_0^tau l_j, l_j the
+// Lagrange basis on the nodes), tau* (the defect sample point), order, and whether
+// the method adapts the mesh.  Returns false for an unknown code.
+struct FirkTableau {
+    int s = 0, order = 0;
+    bool adaptive = true;
+    double A[7][7] = {}, b[7] = {}, c[7] = {}, Q[7][7] = {}, tau = 0;
+};
+bool firk_tableau(int code, FirkTableau &T);
+inline bool is_firk(int method) { return method >= 100; }
+
 // supports(M, family): a kernel of the library serves the pair
 typedef bool (*SupportsFn)(int M, int family);
 

@@ -65,6 +65,8 @@ struct bvp_batch {
     int family = -1;     // the batch's single family when all its BCs are two-point, else -1
     int max_slots = 0;   // persistent waves the per-wave scratch was sized for
     bool block = false;  // served by the block kernels (M = 8, 16, 32)
+    bool firk = false;   // a FIRK method (firk_wave_impl.h), firk_s stages
+    int firk_s = 0;
     bool has_u0 = false; // every instance carries u0 (needed by a warm start)
     bvp_opts opts{};
     std::vector<int> N0;

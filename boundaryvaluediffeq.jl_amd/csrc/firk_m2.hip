@@ -1,0 +1,19 @@
+// firk_m2.hip -- the FIRK collocation kernels (expanded form) for state dimension
+// M = 2, one instantiation per stage count of the Radau IIA and Lobatto families.
+#include "firk_wave_impl.h"
+
+namespace bvpdev {
+#define FIRK_INST(S)                                                                                         \
+    template hipError_t firk_launch_solve_t<2, S>(const BatchDev &, int, hipStream_t);                      \
+    template hipError_t firk_launch_component_t<2, S>(const BatchDev &, int, int, hipStream_t);             \
+    template hipError_t firk_launch_interp_t<2, S>(const BatchDev &, const int *, const double *, double *,  \
+                                                     double *, long long, hipStream_t);                       \
+    template hipError_t firk_launch_gather_t<2, S>(const BatchDev &, const long long *, double *, hipStream_t);
+FIRK_INST(1)
+FIRK_INST(2)
+FIRK_INST(3)
+FIRK_INST(4)
+FIRK_INST(5)
+FIRK_INST(7)
+#undef FIRK_INST
+}  // namespace bvpdev

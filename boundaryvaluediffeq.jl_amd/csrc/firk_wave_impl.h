@@ -1,0 +1,925 @@
+// firk_wave_impl.h -- wave-per-instance FIRK collocation kernels (gfx950): Radau IIA
+// and Lobatto IIIA/B/C in the expanded form of lib/BoundaryValueDiffEqFIRK/src
+// ("FIRK/" below; FIRK/firk.jl:290-453, the reference's default nested_nlsolve = false).
+//
+// The unknowns of an instance are the node values y_i and the stage values K_i1..K_is
+// of every interval.  The reference hands the whole expanded Jacobian (banded,
+// FIRK/firk.jl:806-922) to a sparse LU.  Here the stage unknowns of each interval are
+// eliminated inside the interval first (static condensation): with the stage
+// Jacobians J_r = df/dy(Y_r) and W = I - h (A (x) J) (row block r: delta_rj I - h a_rj J_r),
+//   dK = kg + KJ dy_i,   kg = W^-1 G,   KJ = W^-1 [J_1; ...; J_s],
+// and the continuity rows Phi_i = y_{i+1} - y_i - h sum_j b_j K_j become the two-node
+// relation
+//   -(I + h sum_j b_j KJ_j) dy_i + dy_{i+1} = -Phi_i + h sum_j b_j kg_j,
+// which is the block structure of the MIRK path: the same chunk elimination across
+// lanes, LDS tree and root solve (mirk_wave_impl.h) give dy, then dK per interval
+// from the stored kg, KJ.  This is exact linear algebra, so the Newton iterates are
+// those of the expanded system.
+//
+// One wave (64 lanes) per instance, lanes own contiguous chunks of intervals.
+// Template parameters: M state dimension (1..4), S stages (1, 2, 3, 4, 5, 7).
+#pragma once
+
+#include "mirk_wave_impl.h"
+
+namespace bvpdev {
+
+// stage values of instance b: [cap][S][M] (interval, stage, component)
+template <int M, int S>
+__device__ inline double *firk_stages_of(const BatchDev &B, int b) {
+    return as_global(B.stK + (size_t)b * B.cap * S * M);
+}
+
+// ------------------------------------------------------------------ dense solve
+// W X = R in place (R becomes X), Gaussian elimination with partial pivoting (first
+// maximum).  Small systems stay in registers (unrolled, row swaps by selects); larger
+// ones (S M > 8) run as loops over private arrays.  Returns false on a zero pivot.
+template <int NN, int NR>
+__device__ inline bool lu_solve_small(double (&W)[NN][NN], double (&R)[NN][NR]) {
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+        int p = j;
+        double best = fabs(W[j][j]);
+#pragma unroll
+        for (int i = j + 1; i < NN; ++i) {
+            const double v = fabs(W[i][j]);
+            if (v > best) { best = v; p = i; }
+        }
+        if (!(best > 0.0)) return false;
+#pragma unroll
+        for (int i = j + 1; i < NN; ++i) {
+            const bool sw = (i == p);
+#pragma unroll
+            for (int c = j; c < NN; ++c) {
+                const double t = W[j][c];
+                W[j][c] = sw ? W[i][c] : W[j][c];
+                W[i][c] = sw ? t : W[i][c];
+            }
+#pragma unroll
+            for (int c = 0; c < NR; ++c) {
+                const double t = R[j][c];
+                R[j][c] = sw ? R[i][c] : R[j][c];
+                R[i][c] = sw ? t : R[i][c];
+            }
+        }
+        const double inv = 1.0 / W[j][j];
+#pragma unroll
+        for (int i = j + 1; i < NN; ++i) {
+            const double m = W[i][j] * inv;
+#pragma unroll
+            for (int c = j + 1; c < NN; ++c) W[i][c] -= m * W[j][c];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) R[i][c] -= m * R[j][c];
+        }
+    }
+#pragma unroll
+    for (int i = NN - 1; i >= 0; --i) {
+        const double inv = 1.0 / W[i][i];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            double v = R[i][c];
+#pragma unroll
+            for (int k = i + 1; k < NN; ++k) v -= W[i][k] * R[k][c];
+            R[i][c] = v * inv;
+        }
+    }
+    return true;
+}
+
+template <int NN, int NR>
+__device__ inline bool lu_solve_loop(double (&W)[NN][NN], double (&R)[NN][NR]) {
+#pragma unroll 1
+    for (int j = 0; j < NN; ++j) {
+        int p = j;
+        double best = fabs(W[j][j]);
+#pragma unroll 1
+        for (int i = j + 1; i < NN; ++i) {
+            const double v = fabs(W[i][j]);
+            if (v > best) { best = v; p = i; }
+        }
+        if (!(best > 0.0)) return false;
+        if (p != j) {
+#pragma unroll 1
+            for (int c = j; c < NN; ++c) {
+                const double t = W[j][c];
+                W[j][c] = W[p][c];
+                W[p][c] = t;
+            }
+#pragma unroll
+            for (int c = 0; c < NR; ++c) {
+                const double t = R[j][c];
+                R[j][c] = R[p][c];
+                R[p][c] = t;
+            }
+        }
+        const double inv = 1.0 / W[j][j];
+#pragma unroll 1
+        for (int i = j + 1; i < NN; ++i) {
+            const double m = W[i][j] * inv;
+#pragma unroll 4
+            for (int c = j + 1; c < NN; ++c) W[i][c] -= m * W[j][c];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) R[i][c] -= m * R[j][c];
+        }
+    }
+#pragma unroll 1
+    for (int i = NN - 1; i >= 0; --i) {
+        const double inv = 1.0 / W[i][i];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            double v = R[i][c];
+#pragma unroll 1
+            for (int k = i + 1; k < NN; ++k) v -= W[i][k] * R[k][c];
+            R[i][c] = v * inv;
+        }
+    }
+    return true;
+}
+
+template <int NN, int NR>
+__device__ inline bool lu_solve(double (&W)[NN][NN], double (&R)[NN][NR]) {
+    if constexpr (NN <= 8) return lu_solve_small<NN, NR>(W, R);
+    else return lu_solve_loop<NN, NR>(W, R);
+}
+
+// ------------------------------------------------------------------ one interval
+// Stage points Y_r = h (sum_j a_rj K_j) + y_i, stage residuals G_r = f(t_i + c_r h, Y_r)
+// - K_r and the continuity residual Phi = (y_{i+1} - y_i) - h (sum_j b_j K_j), in the
+// order of operations of Phi! (FIRK/collocation.jl:59-93); with JAC the stage
+// Jacobians J_r by forward-mode dual numbers.
+template <int M, int S, bool JAC>
+__device__ inline void firk_interval(const Inst<M> &I, const FirkTab &T, double ti, double h, const double *yi,
+                                     const double *yj, const double (&K)[S][M], double (&G)[S][M], double (&Phi)[M],
+                                     double (&J)[S][M][M]) {
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        double Y[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < S; ++j) a += K[j][c] * T.A[r][j];
+            Y[c] = h * a + yi[c];
+        }
+        const double t = ti + T.c[r] * h;
+        if constexpr (JAC) {
+            Dual<M> Yd[M], Fd[M];
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                Yd[c] = dconst<M>(Y[c]);
+                Yd[c].d[c] = 1.0;
+            }
+            f_eval<M, Dual<M>>(I.fam, I.P, t, Yd, Fd);
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                G[r][a] = Fd[a].v - K[r][a];
+#pragma unroll
+                for (int c = 0; c < M; ++c) J[r][a][c] = Fd[a].d[c];
+            }
+        } else {
+            double F[M];
+            f_eval<M, double>(I.fam, I.P, t, Y, F);
+#pragma unroll
+            for (int a = 0; a < M; ++a) G[r][a] = F[a] - K[r][a];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) a += K[j][c] * T.b[j];
+        Phi[c] = -h * a + (yj[c] - yi[c]);
+    }
+}
+
+// local record of an interval: kg [S M] then KJ [S M][M] (row (r, a), column c)
+template <int M, int S> constexpr int kFirkLoc = S * M * (M + 1);
+
+// Static condensation of interval k: solves W [kg | KJ] = [G | J], stores the record
+// at loc, and forms the two-node relation S_blk dy_i + dy_{i+1} = rhs.
+template <int M, int S>
+__device__ inline bool firk_condense(const FirkTab &T, double h, const double (&G)[S][M], const double (&Phi)[M],
+                                     const double (&J)[S][M][M], double *loc, double (&Sb)[M][M], double (&rhs)[M]) {
+    constexpr int NN = S * M;
+    double W[NN][NN], X[NN][M + 1];
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+#pragma unroll
+                for (int c = 0; c < M; ++c)
+                    W[r * M + a][j * M + c] = ((r == j && a == c) ? 1.0 : 0.0) - h * T.A[r][j] * J[r][a][c];
+            X[r * M + a][0] = G[r][a];
+#pragma unroll
+            for (int c = 0; c < M; ++c) X[r * M + a][1 + c] = J[r][a][c];
+        }
+    const bool ok = lu_solve<NN, M + 1>(W, X);
+#pragma unroll
+    for (int q = 0; q < NN; ++q) {
+        loc[q] = X[q][0];
+#pragma unroll
+        for (int c = 0; c < M; ++c) loc[NN + q * M + c] = X[q][1 + c];
+    }
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+        double g = 0.0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) g += T.b[j] * X[j * M + a][0];
+        rhs[a] = -Phi[a] + h * g;
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            double v = 0.0;
+#pragma unroll
+            for (int j = 0; j < S; ++j) v += T.b[j] * X[j * M + a][1 + c];
+            Sb[a][c] = -((a == c) ? 1.0 : 0.0) - h * v;
+        }
+    }
+    return ok;
+}
+
+template <int M, int S>
+__device__ inline void load_stages(const double *K, int k, double (&Kk)[S][M]) {
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+#pragma unroll
+        for (int c = 0; c < M; ++c) Kk[r][c] = K[((size_t)k * S + r) * M + c];
+}
+
+// Boundary rows of the FIRK path: every row reads a node value (t0 or t1).
+template <int M>
+__device__ inline void firk_bc_rows(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y, int N,
+                                    int lane) {
+    if (lane < M) {
+        const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+        sh.bcnode[lane] = L.node;
+        sh.bcres[lane] = y[(size_t)L.node * M + sel<M>(I.bcc, lane)] - sel<M>(I.bcv, lane);
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ Newton pieces
+// Forward pass at (y, K): residual max (NaN-aware), per-interval records (loc) and the
+// chunk elimination (Back records in scr, the chunk relation in rel).  Returns
+// max|F| over Phi, G and the boundary rows in all lanes (NaN if not finite); *sing is
+// set when a local system was singular.
+template <int M, int S>
+__device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, Shared<M> &sh, const double *mesh,
+                                      const double *y, const double *K, int N, int lane, double *scr, double *loc,
+                                      Rel<M> &rel, bool &sing) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double mx = 0.0;
+    bool bad = false, sg = false;
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
+        load_stages<M, S>(K, k, Kk);
+        firk_interval<M, S, true>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const double a = fabs(Phi[c]);
+            bad |= !(a <= 1.7976931348623157e308);
+            mx = fmax(mx, a);
+        }
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                const double a = fabs(G[r][c]);
+                bad |= !(a <= 1.7976931348623157e308);
+                mx = fmax(mx, a);
+            }
+        double Sb[M][M], rhs[M], Rb[M][M];
+        sg |= !firk_condense<M, S>(T, h, G, Phi, J, loc + (size_t)k * kFirkLoc<M, S>, Sb, rhs);
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+            for (int c = 0; c < M; ++c) Rb[a][c] = (a == c) ? 1.0 : 0.0;
+        if (k == start) {
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) { rel.E[a][c] = Sb[a][c]; rel.F[a][c] = Rb[a][c]; }
+                rel.g[a] = rhs[a];
+            }
+        } else {
+            Rel<M> nr;
+            Back<M> bk;
+            eliminate<M>(rel, Sb, Rb, rhs, nr, bk);
+            put_rec<M, false>(scr, k - start - 1, lane, bk);
+            rel = nr;
+        }
+    }
+    firk_bc_rows<M>(I, sh, mesh, y, N, lane);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double a = fabs(sh.bcres[k]);
+        bad |= !(a <= 1.7976931348623157e308);
+        mx = fmax(mx, a);
+    }
+    const bool anybad = wave_any(bad);
+    sing = wave_any(sg);
+    mx = wave_max(mx);
+    __syncthreads();
+    return anybad ? NAN : mx;
+}
+
+// Tree solve and back-substitution of the condensed system: dy at the nodes, then
+// dK = kg + KJ dy_i per interval.  dy: [N+1][M], dK: [N][S][M].  Returns false when
+// the root system is singular.
+template <int M, int S>
+__device__ inline bool firk_correction(const Inst<M> &I, Shared<M> &sh, int N, int lane, double *scr,
+                                       const double *loc, Rel<M> &rel, double *dy, double *dK) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    if (!tree_solve<M, false>(I, sh, N, lane, P, scr, rel)) return false;
+    if (len > 0) {
+        double ya[M], yn[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { ya[c] = sh.lo[lane][c]; yn[c] = sh.hi[lane][c]; }
+        if (lane == P - 1)
+#pragma unroll
+            for (int c = 0; c < M; ++c) dy[(size_t)N * M + c] = yn[c];
+        for (int k = start + len - 1; k > start; --k) {
+            Back<M> bk;
+            get_rec<M, false>(scr, k - start - 1, lane, bk);
+            double yk[M];
+            back_apply<M>(bk, ya, yn, yk);
+#pragma unroll
+            for (int c = 0; c < M; ++c) { dy[(size_t)k * M + c] = yk[c]; yn[c] = yk[c]; }
+        }
+#pragma unroll
+        for (int c = 0; c < M; ++c) dy[(size_t)start * M + c] = ya[c];
+        // stage corrections of the chunk's intervals (dy of their left nodes: just written by this lane)
+        constexpr int NN = S * M;
+        for (int k = start; k < start + len; ++k) {
+            const double *lk = loc + (size_t)k * kFirkLoc<M, S>;
+            double d[M];
+#pragma unroll
+            for (int c = 0; c < M; ++c) d[c] = dy[(size_t)k * M + c];
+#pragma unroll
+            for (int q = 0; q < NN; ++q) {
+                double v = lk[q];
+#pragma unroll
+                for (int c = 0; c < M; ++c) v += lk[NN + q * M + c] * d[c];
+                dK[(size_t)k * NN + q] = v;
+            }
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// Sum of squares of the residual at (y + alpha dy, K + alpha dK) (the line search's
+// merit, times 2) in all lanes; NaN if not finite.
+template <int M, int S>
+__device__ inline double firk_trial_sumsq(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
+                                          const double *K, const double *dy, const double *dK, double alpha, int N,
+                                          int lane) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double ss = 0.0;
+    bool bad = false;
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            yi[c] = y[(size_t)k * M + c] + alpha * dy[(size_t)k * M + c];
+            yj[c] = y[(size_t)(k + 1) * M + c] + alpha * dy[(size_t)(k + 1) * M + c];
+        }
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                const size_t q = ((size_t)k * S + r) * M + c;
+                Kk[r][c] = K[q] + alpha * dK[q];
+            }
+        firk_interval<M, S, false>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
+#pragma unroll
+        for (int c = 0; c < M; ++c) ss += Phi[c] * Phi[c];
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) ss += G[r][c] * G[r][c];
+    }
+    if (lane < M) {
+        const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+        const int comp = sel<M>(I.bcc, lane);
+        const size_t q = (size_t)L.node * M + comp;
+        const double v = (y[q] + alpha * dy[q]) - sel<M>(I.bcv, lane);
+        ss += v * v;
+    }
+    bad = !(fabs(ss) <= 1.7976931348623157e308);
+    const bool anybad = wave_any(bad);
+    ss = wave_sum(ss);
+    return anybad ? NAN : uniform(ss);
+}
+
+// x += alpha d over the nodes and stages of the lane's chunk (node N by the last lane)
+template <int M, int S>
+__device__ inline void firk_apply(double *y, double *K, const double *dy, const double *dK, double alpha, int N,
+                                  int lane) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    for (int k = start; k < start + len; ++k) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) y[(size_t)k * M + c] += alpha * dy[(size_t)k * M + c];
+#pragma unroll
+        for (int q = 0; q < S * M; ++q) K[(size_t)k * S * M + q] += alpha * dK[(size_t)k * S * M + q];
+    }
+    if (lane == P - 1)
+#pragma unroll
+        for (int c = 0; c < M; ++c) y[(size_t)N * M + c] += alpha * dy[(size_t)N * M + c];
+    __syncthreads();
+}
+
+template <int M, int S>
+__device__ inline void firk_copy_state(double *yd, double *Kd, const double *ys, const double *Ks, int N, int lane) {
+    for (int q = lane; q < (N + 1) * M; q += WAVE) yd[q] = ys[q];
+    for (int q = lane; q < N * S * M; q += WAVE) Kd[q] = Ks[q];
+    __syncthreads();
+}
+
+struct FirkNewtonOut {
+    int ret, iters, stage;
+    double fn;
+};
+
+// The NewtonRaphson stages of the polyalgorithm (Core/default_internal_solve.jl:31-45):
+// plain, then with the BackTracking line search, each from the same start, stopping
+// when max|F| <= tol (checked before every step).  Work areas: dy [cap][M], dK, the
+// start copy y0 [cap][M], K0, loc.
+template <int M, int S>
+__device__ inline FirkNewtonOut firk_newton(const BatchDev &B, const Inst<M> &I, const FirkTab &T, Shared<M> &sh,
+                                            const double *mesh, double *y, double *K, int N, int lane, double *scr,
+                                            double *loc, double *dy, double *dK, double *y0, double *K0) {
+    FirkNewtonOut out{BVP_RET_MAXITERS, 0, -1, NAN};
+    const int stages = B.stages & 3;
+    bool first = true;
+    if ((stages & 3) == 3) firk_copy_state<M, S>(y0, K0, y, K, N, lane);
+    for (int st = 0; st < 2; ++st) {
+        if (!(stages & (1 << st))) continue;
+        if (!first) firk_copy_state<M, S>(y, K, y0, K0, N, lane);
+        first = false;
+        out.stage = st;
+        for (int it = 0;; ++it) {
+            Rel<M> rel;
+            bool sing = false;
+            const double fn = firk_forward<M, S>(I, T, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
+            out.fn = fn;
+            if (!(fabs(fn) <= 1.7976931348623157e308)) { out.ret = BVP_RET_UNSTABLE; break; }
+            if (fn <= B.newton_tol) { out.ret = BVP_RET_SUCCESS; break; }
+            if (it >= B.max_newton) { out.ret = BVP_RET_MAXITERS; break; }
+            if (sing || !firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK)) {
+                out.ret = BVP_RET_FAILURE;
+                break;
+            }
+            out.iters++;
+            double alpha = 1.0;
+            if (st == 1) {
+                const double ss = firk_trial_sumsq<M, S>(I, T, mesh, y, K, dy, dK, 0.0, N, lane);
+                auto phi = [&](double a) { return 0.5 * firk_trial_sumsq<M, S>(I, T, mesh, y, K, dy, dK, a, N, lane); };
+                if (!backtracking(phi, 0.5 * ss, -ss, alpha)) { out.ret = BVP_RET_FAILURE; break; }
+            }
+            firk_apply<M, S>(y, K, dy, dK, alpha, N, lane);
+        }
+        if (out.ret == BVP_RET_SUCCESS) break;
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------ defect, interpolation
+// eval_q (FIRK/adaptivity.jl:581-606): q(tau h) = y_i + sum_k coeff_k (tau h)^(k+1),
+// q' = sum_k (k+1) coeff_k (tau h)^k, coeff_k = (Q K)_k / h^k
+template <int M, int S>
+__device__ inline void firk_eval_q(const FirkTab &T, const double *yi, double tau, double h, const double (&Kk)[S][M],
+                                   double *q, double *qp) {
+    const double th = tau * h;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        double v = 0.0, dv = 0.0, p = 1.0, hk = 1.0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < S; ++j) a += T.Q[k][j] * Kk[j][c];
+            const double coef = a / hk;
+            dv += (k + 1) * coef * p;
+            p *= th;
+            v += coef * p;
+            hk *= h;
+        }
+        q[c] = yi[c] + v;
+        qp[c] = dv;
+    }
+}
+
+// Defect pass (defect_estimate!, FIRK/adaptivity.jl:349-405): per interval the relative
+// defect of the stage polynomial at tau* and 1 - tau* (the sample with the larger
+// max-abs); writes shat_k = (max|d_k| / abstol)^(1/(p+1)) (mesh_selector!,
+// FIRK/adaptivity.jl:247-300) and, when errs is set, the vectors; r1, r2 the max and
+// sum of shat.  Returns the defect norm (NaN-aware) in all lanes.
+template <int M, int S>
+__device__ inline double firk_defect_pass(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
+                                          const double *K, int N, int lane, double abstol, double *shat,
+                                          double *errs, double &r1, double &r2) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double dmx = 0.0, s1 = 0.0, s2 = 0.0;
+    bool bad = false;
+    const double expo = 1.0 / (T.order + 1);
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], Kk[S][M], z[M], zp[M], f[M], d1[M], d2[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) yi[c] = y[(size_t)k * M + c];
+        load_stages<M, S>(K, k, Kk);
+        firk_eval_q<M, S>(T, yi, T.tau, h, Kk, z, zp);
+        f_eval<M, double>(I.fam, I.P, ti + T.tau * h, z, f);
+        double e1 = 0.0, e2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < M; ++c) { d1[c] = (zp[c] - f[c]) / (fabs(f[c]) + 1.0); e1 = fmax(e1, fabs(d1[c])); }
+        firk_eval_q<M, S>(T, yi, 1.0 - T.tau, h, Kk, z, zp);
+        f_eval<M, double>(I.fam, I.P, ti + (1.0 - T.tau) * h, z, f);
+#pragma unroll
+        for (int c = 0; c < M; ++c) { d2[c] = (zp[c] - f[c]) / (fabs(f[c]) + 1.0); e2 = fmax(e2, fabs(d2[c])); }
+        const bool first = e1 > e2;
+        double ei = 0.0;
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const double d = first ? d1[c] : d2[c];
+            bad |= !(fabs(d) <= 1.7976931348623157e308);
+            ei = fmax(ei, fabs(d));
+            if (errs) errs[(size_t)k * M + c] = d;
+        }
+        dmx = fmax(dmx, ei);
+        if (shat) {
+            const double sh = pow(ei / abstol, expo);
+            shat[k] = sh;
+            s1 = fmax(s1, sh);
+            s2 += sh;
+        }
+    }
+    const bool anybad = wave_any(bad);
+    dmx = uniform(wave_max(dmx));
+    r1 = uniform(wave_max(s1));
+    r2 = uniform(wave_sum(s2));
+    __syncthreads();
+    return anybad ? NAN : dmx;
+}
+
+// The quintic through value and slope at 0, h, h/2 (get_S_coeffs / s_constraints_block,
+// FIRK/adaptivity.jl:183-227), solved with partial pivoting as the 6 x 6 system of the
+// reference; bk = (y(0), y(h), y(h/2), y'(0), y'(h), y'(h/2)).
+__device__ inline void firk_quintic(double h, const double (&bk)[6], double (&coef)[6]) {
+    double A[6][6], b[6];
+    const double t[3] = {0.0, h, 0.5 * h};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double p = 1.0, pd = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            A[r][m] = p;
+            A[3 + r][m] = m * pd;
+            pd = p;
+            p *= t[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) b[r] = bk[r];
+    double X[6][1];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) X[r][0] = b[r];
+    lu_solve_small<6, 1>(A, X);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) coef[r] = X[r][0];
+}
+
+// interpolant! of FIRKExpandInterpolation (FIRK/interpolation.jl:229-316), and the node
+// values of interp_eval! (FIRK/adaptivity.jl:8-48): the quintic of interval(mesh, t)
+// through y_i, y_{i+1}, f(y_i), f(y_{i+1}) and the stage polynomial at the midpoint.
+template <int M, int S>
+__device__ inline void firk_interp_at(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
+                                      const double *K, int N, double t, double *u, double *du) {
+    const int j = interval_of(mesh, N, t);
+    const double tj = mesh[j], h = mesh[j + 1] - tj, tau = t - tj;
+    double yi[M], yj[M], fi[M], fj[M], z[M], zp[M], Kk[S][M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)j * M + c]; yj[c] = y[(size_t)(j + 1) * M + c]; }
+    load_stages<M, S>(K, j, Kk);
+    f_eval<M, double>(I.fam, I.P, tj, yi, fi);
+    f_eval<M, double>(I.fam, I.P, mesh[j + 1], yj, fj);
+    firk_eval_q<M, S>(T, yi, 0.5, h, Kk, z, zp);
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        const double bk[6] = {yi[c], yj[c], z[c], fi[c], fj[c], zp[c]};
+        double coef[6];
+        firk_quintic(h, bk, coef);
+        double v = 0.0, dv = 0.0, p = 1.0, pd = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            v += coef[m] * p;
+            dv += m * coef[m] * pd;
+            pd = p;
+            p *= tau;
+        }
+        u[c] = v;
+        if (du) du[c] = dv;
+    }
+}
+
+// extend_y (FIRK/firk.jl:76-83): every stage of interval i set to y_{i+1}
+template <int M, int S>
+__device__ inline void firk_extend(const double *y, double *K, int N, int lane) {
+    for (int q = lane; q < N * S * M; q += WAVE) {
+        const int i = q / (S * M), c = q % M;
+        K[q] = y[(size_t)(i + 1) * M + c];
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ full solve of one instance
+// The outer loop of FIRK/firk.jl:481-525, 572-621: Newton, defect, mesh_selector! and
+// the transfer of the solution (the old quintic interpolant at the new nodes, stages as
+// extend_y sets them for a new problem), or halving with a zero restart when Newton or
+// the defect threshold failed.
+template <int M, int S>
+__device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, int b, Shared<M> &sh, int lane,
+                                           double *scr, double *aux) {
+    Inst<M> I;
+    load_inst<M>(B, b, I);
+    I.minc = 1;
+    Bufs<M> bf = bufs_of<M>(B, b);
+    double *K = firk_stages_of<M, S>(B, b);
+    const size_t cap = (size_t)B.cap;
+    double *dy = aux;                          // [cap][M]
+    double *dK = dy + cap * M;                 // [cap][S][M]
+    double *y0 = dK + cap * S * M;             // [cap][M]
+    double *K0 = y0 + cap * M;                 // [cap][S][M]
+    double *loc = K0 + cap * S * M;            // [cap][kFirkLoc]
+    double *shat = as_global(B.shat + (size_t)b * B.cap);
+    int cur = 0;
+    int N = B.N0[b];
+    {
+        const long long off = B.init_off[b];
+        for (int k = lane; k <= N; k += WAVE) {
+            bf.mesh[0][k] = B.init_mesh[off + k];
+#pragma unroll
+            for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = B.init_y[(off + k) * M + c];
+        }
+        __syncthreads();
+        firk_extend<M, S>(bf.y[0], K, N, lane);
+    }
+    int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
+    int outer = 0, iters = 0, fb = 0;
+    double last_defect = NAN, rn = NAN;
+    bool rn_valid = false;
+    long long cnt_it = 0, cnt_def = 0, cnt_tr = 0;
+    for (;;) {
+        ++outer;
+        const double *mesh = bf.mesh[cur];
+        double *y = bf.y_of(cur);
+        double *mesh_o = bf.mesh[1 - cur], *y_o = bf.y_of(1 - cur);
+        FirkNewtonOut nl = firk_newton<M, S>(B, I, T, sh, mesh, y, K, N, lane, scr, loc, dy, dK, y0, K0);
+        iters += nl.iters;
+        rn = nl.fn;
+        rn_valid = true;
+        cnt_it += (long long)N * nl.iters;
+        if (nl.stage == 1) fb |= 1;
+        nl_ret = nl.ret;
+        double error_norm = 2.0 * B.abstol;
+        info = nl.ret;
+        if (!B.adaptive) break;
+        double r1 = 0.0, r2 = 0.0;
+        if (info == BVP_RET_SUCCESS) {
+            error_norm = firk_defect_pass<M, S>(I, T, mesh, y, K, N, lane, B.abstol, shat, nullptr, r1, r2);
+            cnt_def += N;
+            last_defect = error_norm;
+            info = (error_norm > B.defect_threshold || isnan(error_norm)) ? BVP_RET_FAILURE : BVP_RET_SUCCESS;
+        }
+        if (info == BVP_RET_SUCCESS) {
+            if (error_norm > B.abstol) {
+                const int Nn = mesh_select(mesh, N, shat, r1, r2, B.max_sub, mesh_o, lane, sh);
+                if (Nn < 0) {
+                    info = BVP_RET_FAILURE;
+                } else {
+                    for (int j = lane; j <= Nn; j += WAVE)
+                        firk_interp_at<M, S>(I, T, mesh, y, K, N, mesh_o[j], y_o + (size_t)j * M, nullptr);
+                    __syncthreads();
+                    firk_extend<M, S>(y_o, K, Nn, lane);
+                    cnt_tr += N + Nn;
+                    N = Nn;
+                    cur = 1 - cur;
+                    rn_valid = false;
+                }
+            }
+        } else {
+            if (2 * N > B.max_sub) {
+                info = BVP_RET_FAILURE;
+            } else {
+                half_mesh_into(mesh, N, mesh_o, lane);
+                for (int q = lane; q <= 2 * N * M + M - 1; q += WAVE) y_o[q] = 0.0;
+                for (int q = lane; q < 2 * N * S * M; q += WAVE) K[q] = 0.0;
+                __syncthreads();
+                N = 2 * N;
+                cur = 1 - cur;
+                rn_valid = false;
+                info = BVP_RET_SUCCESS;
+            }
+        }
+        if (!(info == BVP_RET_SUCCESS && error_norm > B.abstol)) break;
+        if (outer >= 64) { info = BVP_RET_FAILURE; break; }
+    }
+    if (!rn_valid) {
+        Rel<M> rel;
+        bool sing = false;
+        rn = firk_forward<M, S>(I, T, sh, bf.mesh[cur], bf.y_of(cur), K, N, lane, scr, loc, rel, sing);
+    }
+    if (lane == 0) {
+        B.err[b] = NAN;
+        B.retcode[b] = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
+        B.N_final[b] = N;
+        B.newton_iters[b] = iters;
+        B.outer_iters[b] = outer;
+        B.fallback[b] = fb;
+        B.cur_buf[b] = cur;
+        B.defect_norm[b] = last_defect;
+        B.resid_norm[b] = rn;
+        B.cnt[(size_t)b * 4 + 0] = cnt_it;
+        B.cnt[(size_t)b * 4 + 1] = cnt_def;
+        B.cnt[(size_t)b * 4 + 2] = cnt_tr;
+        B.cnt[(size_t)b * 4 + 3] = 0;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ kernels
+// Persistent FIRK solve: each wave pulls instances from the queue.
+template <int M, int S>
+__global__ __launch_bounds__(64, 1) void k_firk_solve(BatchDev B) {
+    __shared__ Shared<M> sh;
+    const int lane = threadIdx.x;
+    const FirkTab &T = *reinterpret_cast<const FirkTab *>(B.firk_tab);
+    double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
+    double *aux = as_global(B.aux + (size_t)blockIdx.x * B.aux_per_slot);
+    for (;;) {
+        const int item = next_item<M>(B.counter, sh, lane);
+        if (item >= B.n) break;
+        firk_solve_instance<M, S>(B, T, B.order[item], sh, lane, scr, aux);
+    }
+}
+
+// Component kernels at the packed state (mesh, guess and the packed stages init_K).  Slots of n_x = M (N (S+1) + 1) per instance:
+//   OP_RESIDUAL    out0: the residual in the reference's layout
+//   OP_NEWTON_STEP out0: the correction in the unknown order [y_0, K_0., y_1, ...]
+//   OP_DEFECT      out0: [N][M] defect vectors (interval offset), out1: norm
+template <int M, int S>
+__global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
+    __shared__ Shared<M> sh;
+    const int lane = threadIdx.x;
+    const FirkTab &T = *reinterpret_cast<const FirkTab *>(B.firk_tab);
+    double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
+    double *aux = as_global(B.aux + (size_t)blockIdx.x * B.aux_per_slot);
+    const size_t cap = (size_t)B.cap;
+    for (;;) {
+        const int b = next_item<M>(B.counter, sh, lane);
+        if (b >= B.n) break;
+        Inst<M> I;
+        load_inst<M>(B, b, I);
+        I.minc = 1;
+        const int N = B.N0[b];
+        const long long off = B.init_off[b];
+        const long long ioff = off - b;  // interval offset
+        const double *mesh = as_global(B.init_mesh + off);
+        const double *y = as_global(B.init_y + off * M);
+        double *dy = aux, *dK = dy + cap * M, *loc = dK + cap * S * M;
+        const double *K = as_global(B.init_K + ioff * S * M);
+        const size_t xo = (size_t)M * ((size_t)(S + 1) * ioff + b);  // slot offset of n_x slots
+        if (op == OP_RESIDUAL) {
+            double *out = B.out0 + xo;
+            int nbca = 0;
+            if (I.bc_kind == BVP_BC_TWO_POINT)
+                for (int k = 0; k < M; ++k) nbca += (sel<M>(I.bct, k) == I.t0) ? 1 : 0;
+            else
+                nbca = M;
+            int P, start, len;
+            chunk_of(N, lane, P, start, len);
+            for (int k = start; k < start + len; ++k) {
+                const double ti = mesh[k], h = mesh[k + 1] - ti;
+                double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M];
+#pragma unroll
+                for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
+                load_stages<M, S>(K, k, Kk);
+                firk_interval<M, S, false>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
+                double *o = out + nbca + (size_t)k * (S + 1) * M;
+#pragma unroll
+                for (int c = 0; c < M; ++c) o[c] = Phi[c];
+#pragma unroll
+                for (int r = 0; r < S; ++r)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) o[(size_t)(r + 1) * M + c] = G[r][c];
+            }
+            if (lane == 0) {
+                // BC rows: input order; TWO_POINT splits them into bca (before) and bcb (after)
+                int ia = 0, ib = 0;
+                for (int k = 0; k < M; ++k) {
+                    const BcLoc L = bc_locate<M>(I, k, mesh, N);
+                    const double v = y[(size_t)L.node * M + sel<M>(I.bcc, k)] - sel<M>(I.bcv, k);
+                    if (I.bc_kind == BVP_BC_TWO_POINT && sel<M>(I.bct, k) != I.t0)
+                        out[nbca + (size_t)N * (S + 1) * M + (ib++)] = v;
+                    else
+                        out[ia++] = v;
+                }
+            }
+        } else if (op == OP_NEWTON_STEP) {
+            Rel<M> rel;
+            bool sing = false;
+            firk_forward<M, S>(I, T, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
+            const bool ok = !sing && firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK);
+            double *out = B.out0 + xo;
+            for (int q = lane; q < (N + 1) * M; q += WAVE) {
+                const int i = q / M, c = q % M;
+                out[(size_t)i * (S + 1) * M + c] = ok ? dy[q] : NAN;
+            }
+            for (int q = lane; q < N * S * M; q += WAVE) {
+                const int i = q / (S * M), rc = q % (S * M);
+                out[(size_t)i * (S + 1) * M + M + rc] = ok ? dK[q] : NAN;
+            }
+        } else if (op == OP_DEFECT) {
+            double r1, r2;
+            const double dn = firk_defect_pass<M, S>(I, T, mesh, y, K, N, lane, B.abstol, nullptr,
+                                                     B.out0 + ioff * M, r1, r2);
+            if (lane == 0) B.out1[b] = dn;
+        }
+        __syncthreads();
+    }
+}
+
+// Dense output: one thread per query point (qinst[q] its instance, tq[q] its t).
+template <int M, int S>
+__global__ __launch_bounds__(256) void k_firk_interp(BatchDev B, const int *qinst, const double *tq, double *uo,
+                                                     double *duo, long long nq) {
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    const FirkTab &T = *reinterpret_cast<const FirkTab *>(B.firk_tab);
+    const int b = qinst[q];
+    Inst<M> I;
+    load_inst<M>(B, b, I);
+    const long long off = B.init_off[b];
+    const long long ioff = off - b;
+    const double *mesh = as_global(B.init_mesh + off);
+    const double *y = as_global(B.init_y + off * M);
+    const int N = B.N0[b];
+    const double t = tq[q];
+    const double *K = as_global(B.init_K + ioff * S * M);
+    double u[M], du[M];
+    firk_interp_at<M, S>(I, T, mesh, y, K, N, t, u, du);
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        uo[q * M + c] = u[c];
+        if (duo) duo[q * M + c] = du[c];
+    }
+}
+
+// Final stage values of each instance to out at interval offsets out_off (one block
+// per instance).
+template <int M, int S>
+__global__ __launch_bounds__(256) void k_firk_gather(BatchDev B, const long long *out_off, double *out) {
+    const int b = blockIdx.x;
+    if (b >= B.n) return;
+    const int N = B.N_final[b];
+    const double *K = firk_stages_of<M, S>(B, b);
+    const long long o = out_off[b] * S * M;
+    for (int q = threadIdx.x; q < N * S * M; q += blockDim.x) out[o + q] = K[q];
+}
+
+template <int M, int S>
+hipError_t firk_launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
+    k_firk_solve<M, S><<<blocks, 64, 0, s>>>(B);
+    return hipGetLastError();
+}
+template <int M, int S>
+hipError_t firk_launch_component_t(const BatchDev &B, int op, int blocks, hipStream_t s) {
+    k_firk_component<M, S><<<blocks, 64, 0, s>>>(B, op);
+    return hipGetLastError();
+}
+template <int M, int S>
+hipError_t firk_launch_interp_t(const BatchDev &B, const int *qinst, const double *tq, double *u, double *du,
+                                long long nq, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)((nq + 255) / 256);
+    k_firk_interp<M, S><<<nb, 256, 0, s>>>(B, qinst, tq, u, du, nq);
+    return hipGetLastError();
+}
+template <int M, int S>
+hipError_t firk_launch_gather_t(const BatchDev &B, const long long *out_off, double *out, hipStream_t s) {
+    k_firk_gather<M, S><<<B.n, 256, 0, s>>>(B, out_off, out);
+    return hipGetLastError();
+}
+
+}  // namespace bvpdev

@@ -13,6 +13,14 @@ enum {
     OP_MESH_SELECT = 4,
 };
 
+// Tableau of a FIRK method (Radau IIA, Lobatto IIIA/B/C): A in Butcher order, b, c,
+// Q the coefficients of the stage polynomial (Q[k][j]: tau^(k+1) in int_0^tau l_j),
+// tau* the defect sample point, s stages, order.
+struct FirkTab {
+    double A[7][7], b[7], c[7], Q[7][7], tau;
+    int s, order;
+};
+
 // Everything a kernel needs about a batch; all pointers are device pointers.
 struct BatchDev {
     int n;    // instances
@@ -78,6 +86,13 @@ struct BatchDev {
     double *ge;
     size_t ge_per_slot;
 
+    // FIRK methods (expanded form): the tableau (a FirkTab in device memory), the stage
+    // values of every instance [n][cap][s][M], and for the component kernels the packed
+    // input stages [sum N][s][M] (interval offset = init_off[b] - b)
+    const void *firk_tab;
+    double *stK;
+    const double *init_K;
+
     // options
     int adaptive, max_sub, max_newton, stages, transfer_mode;
     int report_err;           // fill err[] after each solve (one more Newton correction)
@@ -112,6 +127,18 @@ hipError_t wave_launch_interp(const BatchDev &B, int method, const int *qinst, c
                               double *du, long long nq, hipStream_t s);
 bool wave_supports(int M, int family);
 int wave_rel_fields(int M);
+
+// FIRK kernels (firk_wave_impl.h): M = 1..4, S = 1, 2, 3, 4, 5, 7 stages
+bool firk_supports(int M, int S);
+hipError_t firk_launch_solve(const BatchDev &B, int S, int blocks, hipStream_t s);
+hipError_t firk_launch_component(const BatchDev &B, int S, int op, int blocks, hipStream_t s);
+hipError_t firk_launch_interp(const BatchDev &B, int S, const int *qinst, const double *tq, double *u, double *du,
+                              long long nq, hipStream_t s);
+hipError_t firk_launch_gather(const BatchDev &B, int S, const long long *out_off, double *out, hipStream_t s);
+// doubles per slot of the FIRK work area for node capacity cap
+inline size_t firk_aux_doubles(int cap, int M, int S) {
+    return (size_t)cap * (2 * M + 2 * S * M + S * M * (M + 1));
+}
 
 hipError_t launch_copy_probe(const double *src, double *dst, size_t n, hipStream_t s);
 

@@ -806,25 +806,18 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
     });
 }
 
-// Tree up-sweep, root solve (lane 0), tree down-sweep and chunk back-substitution.
-// Writes the correction into `delta` (point-major) when non-null and, when
-// `apply` is set, adds it to y.
-// FUSE (with apply and no delta): while the chunk is walked backwards the corrected
-// values of both ends of each interval are in registers, so the residual of the new
-// state is evaluated there -- *fn_out = max|F| of the corrected state (BC rows
-// included; NaN if not finite), the convergence check of the next Newton iteration
-// without another pass over HBM -- and with FUSE == 2 also the defect estimate,
-// *dn_out = max_i |d_i| (MIRK/adaptivity.jl:416-461; NaN if not finite).
-template <int M, int METHOD, bool F32 = false, int FUSE = 0>
-__device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y, int N,
-                                        int lane, double *scr, size_t tree_base, Rel<M> &rel, double *delta,
-                                        bool apply, double *fn_out = nullptr, double *dn_out = nullptr) {
-    int P, start, len;
-    chunk_of(N, lane, P, start, len, I.minc);
+// Tree up-sweep over the chunk relations (lane l holds the relation of its chunk in
+// `rel`), root solve on lane 0 (the boundary rows, in sh.bcnode / sh.bcres, and the
+// end-to-end relation), tree down-sweep.  Leaves in sh.lo[l] / sh.hi[l] the
+// corrections at the first and last node of lane l's chunk.  Returns false (all
+// lanes) when the root system is singular.  scr holds the chunk records (read by the
+// walk to an interior boundary node), F32 their storage precision.
+template <int M, bool F32>
+__device__ __forceinline__ bool tree_solve(const Inst<M> &I, Shared<M> &sh, int N, int lane, int P, double *scr,
+                                           Rel<M> &rel) {
     int D = 0;
     while ((1 << D) < P) ++D;
     double *tree = sh.tree;  // tree records stay on chip (read back level by level in the down-sweep)
-    (void)tree_base;
     // ---- up-sweep
     if (lane < P) rel_to_lds<M>(sh, lane, rel);
     __syncthreads();
@@ -994,6 +987,26 @@ __device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const d
         }
         __syncthreads();
     }
+    return true;
+}
+
+// Tree up-sweep, root solve (lane 0), tree down-sweep and chunk back-substitution.
+// Writes the correction into `delta` (point-major) when non-null and, when
+// `apply` is set, adds it to y.
+// FUSE (with apply and no delta): while the chunk is walked backwards the corrected
+// values of both ends of each interval are in registers, so the residual of the new
+// state is evaluated there -- *fn_out = max|F| of the corrected state (BC rows
+// included; NaN if not finite), the convergence check of the next Newton iteration
+// without another pass over HBM -- and with FUSE == 2 also the defect estimate,
+// *dn_out = max_i |d_i| (MIRK/adaptivity.jl:416-461; NaN if not finite).
+template <int M, int METHOD, bool F32 = false, int FUSE = 0>
+__device__ inline bool solve_correction(const Inst<M> &I, Shared<M> &sh, const double *mesh, double *y, int N,
+                                        int lane, double *scr, size_t tree_base, Rel<M> &rel, double *delta,
+                                        bool apply, double *fn_out = nullptr, double *dn_out = nullptr) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len, I.minc);
+    (void)tree_base;
+    if (!tree_solve<M, F32>(I, sh, N, lane, P, scr, rel)) return false;
     // ---- chunk back-substitution
     // The corrected state goes to the LDS staging area (the tree records are no
     // longer needed) and then to y in whole lines, when it fits.

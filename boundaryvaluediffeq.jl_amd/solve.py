@@ -142,26 +142,94 @@ class _MIRK:
 
 class MIRK4(_MIRK):
     order = 4
+    code = _abi.MIRK4
 
 
 class MIRK6(_MIRK):
     order = 6
+    code = _abi.MIRK6
+
+
+class _FIRK:
+    """A FIRK collocation method (lib/BoundaryValueDiffEqFIRK/src/algorithms.jl:4-411) in
+    the expanded form, the reference's default (nested_nlsolve = false).  On the GPU
+    the nonlinear solver runs the NewtonRaphson stages of the polyalgorithm (plain, then
+    with BackTracking); nlsolve = None selects both."""
+    code = 0
+    stages = 0
+    order = 0
+
+    def __init__(self, nlsolve=None, jac_alg=None, nested_nlsolve=False, nested_nlsolve_kwargs=None,
+                 defect_threshold=0.1, max_num_subintervals=3000):
+        if nested_nlsolve:
+            raise NotImplementedError("the nested form (nested_nlsolve = true) is not built; "
+                                      "the expanded form is the reference's default")
+        self.nlsolve = nlsolve
+        self.jac_alg = jac_alg
+        self.nested_nlsolve = False
+        self.nested_nlsolve_kwargs = nested_nlsolve_kwargs or {}
+        self.defect_threshold = float(defect_threshold)
+        self.max_num_subintervals = int(max_num_subintervals)
+        self.precision = _abi.PREC_FP64
+        self.refine_iters = 0
+        self.transfer_mode = _abi.TRANSFER_CLEAN
+        self.warm_start = None
+
+    def __repr__(self):
+        return type(self).__name__ + "()"
+
+
+def _firk_class(name, code, order):
+    return type(name, (_FIRK,), {"code": code, "stages": code % 100, "order": order,
+                                 "__doc__": f"{name}: {code % 100} stages, order {order}."})
+
+
+RadauIIa1, RadauIIa2, RadauIIa3, RadauIIa5, RadauIIa7 = (
+    _firk_class(f"RadauIIa{s}", _abi.RADAU_IIA(s), 2 * s - 1) for s in (1, 2, 3, 5, 7))
+LobattoIIIa2, LobattoIIIa3, LobattoIIIa4, LobattoIIIa5 = (
+    _firk_class(f"LobattoIIIa{s}", _abi.LOBATTO_IIIA(s), 2 * s - 2) for s in (2, 3, 4, 5))
+LobattoIIIb2, LobattoIIIb3, LobattoIIIb4, LobattoIIIb5 = (
+    _firk_class(f"LobattoIIIb{s}", _abi.LOBATTO_IIIB(s), 2 * s - 2) for s in (2, 3, 4, 5))
+LobattoIIIc2, LobattoIIIc3, LobattoIIIc4, LobattoIIIc5 = (
+    _firk_class(f"LobattoIIIc{s}", _abi.LOBATTO_IIIC(s), 2 * s - 2) for s in (2, 3, 4, 5))
 
 
 class BVPSolution:
-    def __init__(self, raw, prob, alg):
+    def __init__(self, raw, prob, alg, spec=None, device=0):
         self.prob = prob
         self.alg = alg
         self.t = raw.t
         self.u = [raw.u[i].copy() for i in range(raw.u.shape[0])]
+        self.stages = raw.stages  # FIRK: the stage values (N, s, M) of the solution
         self.retcode = _abi.RETCODE_NAMES.get(raw.retcode, str(raw.retcode))
         self.resid = raw.resid_norm
         self.err_vs_fp64 = raw.err_vs_fp64
         self.stats = {"newton_iters": raw.newton_iters, "outer_iters": raw.outer_iters,
                       "defect_norm": raw.defect_norm, "N": raw.N, "fallback_used": raw.fallback_used}
+        self._spec = spec
+        self._device = device
 
     def __array__(self, dtype=None):
         return np.asarray(self.u, dtype=dtype)
+
+    def __call__(self, t, deriv=0, idxs=None):
+        """Dense output sol(t) (deriv = 0) and sol(t, Val{1}) (deriv = 1) on the GPU: the
+        method's continuous extension (MIRK/interpolation.jl:17-206,
+        FIRK/interpolation.jl:229-316); idxs selects components."""
+        if self._spec is None:
+            raise ValueError("this solution carries no problem description for dense output")
+        scalar = np.ndim(t) == 0
+        tq = np.atleast_1d(np.asarray(t, dtype=np.float64))
+        spec = self._spec.with_state(self.t, np.asarray(self.u, dtype=np.float64).reshape(-1))
+        ctx = _lib.context(self._device)
+        if isinstance(self.alg, _FIRK):
+            ((u, du),) = _lib.firk_interpolate(ctx, self.alg.code, [spec], [tq], [self.stages])
+        else:
+            ((u, du),) = _lib.interpolate(ctx, self.alg.code, [spec], [tq])
+        v = du if deriv == 1 else u
+        if idxs is not None:
+            v = v[:, idxs]
+        return v[0] if scalar else v
 
 
 def successful_retcode(sol):
@@ -184,7 +252,7 @@ class EnsembleSolution(list):
 
 
 def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=None, report_err=False):
-    o = _abi.default_opts(alg.order)
+    o = _abi.default_opts(alg.code)
     o.abstol = float(abstol)
     nk = dict(nlsolve_kwargs or {})
     o.newton_abstol = float(nk.get("abstol", abstol))   # MIRK/mirk.jl:51
@@ -197,12 +265,21 @@ def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=N
     o.transfer_mode = alg.transfer_mode
     o.nlsolve_stages = nlsolve_stages(alg.nlsolve)
     o.report_err = 1 if report_err else 0
+    if isinstance(alg, _FIRK):
+        # the FIRK path keeps its own defect threshold (FIRK/firk.jl:596) and runs the
+        # NewtonRaphson stages
+        if controller is not None and not isinstance(controller, DefectControl):
+            raise ValueError("FIRK methods use DefectControl")
+        o.defect_threshold = alg.defect_threshold
+        if alg.nlsolve is None:
+            o.nlsolve_stages = _abi.STAGE_NEWTON | _abi.STAGE_BACKTRACKING
     return o
 
 
 def solve(prob, alg, *, dt=0.0, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=None,
           trajectories=None, device=0, report_err=False):
-    """Solve a BVProblem / TwoPointBVProblem (or an EnsembleProblem) with MIRK4/MIRK6 on the GPU.
+    """Solve a BVProblem / TwoPointBVProblem (or an EnsembleProblem) with MIRK4/MIRK6 or a
+    FIRK method (RadauIIa*, LobattoIII*) on the GPU.
     report_err: also return, per solution, err_vs_fp64 (the FP64 Newton correction at the result)."""
     opts = make_opts(alg, abstol, adaptive, controller, nlsolve_kwargs, report_err)
     ctx = _lib.context(device)
@@ -212,9 +289,9 @@ def solve(prob, alg, *, dt=0.0, abstol=1e-6, adaptive=True, controller=None, nls
         probs = [prob.prob_func(prob.prob, i + 1, 1) for i in range(trajectories)]
         specs = [extract(p, dt) for p in probs]
         raws = _lib.solve_batch(ctx, opts, specs)
-        return EnsembleSolution(BVPSolution(r, p, alg) for r, p in zip(raws, probs))
+        return EnsembleSolution(BVPSolution(r, p, alg, s, device) for r, p, s in zip(raws, probs, specs))
     if not isinstance(prob, BVProblem):
         raise TypeError("prob must be a BVProblem, TwoPointBVProblem or EnsembleProblem")
     spec = extract(prob, dt)
     raw = _lib.solve_batch(ctx, opts, [spec])[0]
-    return BVPSolution(raw, prob, alg)
+    return BVPSolution(raw, prob, alg, spec, device)

@@ -37,6 +37,25 @@ extern "C" {
 #define BVP_MIRK4 4
 #define BVP_MIRK6 6
 
+/* ---- FIRK methods (lib/BoundaryValueDiffEqFIRK/src/, "FIRK/" below): bvp_opts.method
+ * and the method argument of the bvp_firk_* entry points may name a fully implicit
+ * Runge-Kutta collocation method instead of MIRK4/MIRK6:
+ *   BVP_RADAU_IIA(s)     s = 1, 2, 3, 5, 7   order 2s - 1  (FIRK/radau_tableaus.jl:1-163)
+ *   BVP_LOBATTO_IIIA(s)  s = 2, 3, 4, 5      order 2s - 2  (FIRK/lobatto_tableaus.jl:1-118)
+ *   BVP_LOBATTO_IIIB(s)  s = 2, 3, 4, 5      order 2s - 2  (FIRK/lobatto_tableaus.jl:120-219)
+ *   BVP_LOBATTO_IIIC(s)  s = 2, 3, 4, 5      order 2s - 2  (FIRK/lobatto_tableaus.jl:221-323)
+ * They run in the expanded formulation, the reference's default (nested_nlsolve =
+ * false; FIRK/firk.jl:290-453): the unknowns are the node values y_i and the s stage
+ * values K_i1..K_is of every interval, in the order [y_0, K_01..K_0s, y_1, ..., y_N]
+ * (FIRK/firk.jl:76-83).  RadauIIa1, LobattoIIIb2 and LobattoIIIc2 do not adapt the
+ * mesh (FIRK/algorithms.jl:411): they need adaptive = 0.  On the GPU: M = 1..4, boundary
+ * rows at t0 or t1, DefectControl, FP64, the NewtonRaphson stages of the polyalgorithm
+ * (nlsolve_stages bits 0 and 1). */
+#define BVP_RADAU_IIA(s)    (100 + (s))
+#define BVP_LOBATTO_IIIA(s) (200 + (s))
+#define BVP_LOBATTO_IIIB(s) (300 + (s))
+#define BVP_LOBATTO_IIIC(s) (400 + (s))
+
 /* ---- problem families: the right-hand side f(y, p, t) is compiled into the
  * device code, since a host closure cannot run on the GPU.  State dimensions:
  * M = 1..4 and 6 on the wave-per-instance kernels, M = 8, 16, 32 on the block
@@ -200,6 +219,9 @@ typedef struct {
                                     No counterpart in the reference. */
     double *mesh;                /* out: [N_final+1] */
     double *u;                   /* out: [M*(N_final+1)] point-major */
+    double *stages;              /* out, FIRK methods: [N_final*s*M] the stage values of the
+                                    final state (interval, then stage, then component), or
+                                    NULL; not written for MIRK */
 } bvp_result;
 
 void bvp_opts_default(bvp_opts *opts);
@@ -297,6 +319,42 @@ int bvp_mirk_global_error(bvp_ctx *ctx, const bvp_opts *opts, int32_t n, const b
  * (M = 1..4, 6). */
 int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst,
                          const int32_t *n_t, const double *t, double *u, double *du);
+
+/* ---- FIRK components.  The state of instance i is its (mesh, guess) and its stage
+ * values: `stages` holds N_i*s*M doubles per instance (interval, then stage, then
+ * component; instances back to back), or is NULL for the stages the reference
+ * starts from, K_ij = y_{i+1} (extend_y, FIRK/firk.jl:76-83).  M = 1..4; boundary
+ * rows at t0 or t1.  Slot sizes below use n_x = M*(N*(s+1)+1), the length of the
+ * expanded unknown vector. */
+
+/* Expanded residual in the reference's layout (loss FIRK/firk.jl:1404-1455 with Phi!
+ * FIRK/collocation.jl:59-93): STANDARD [bc; per interval Phi_i, G_i1..G_is],
+ * TWO_POINT [bca; per interval ...; bcb], with
+ *   Phi_i = y_{i+1} - y_i - h_i sum_j b_j K_ij,
+ *   G_ir  = f(t_i + c_r h_i, y_i + h_i sum_j a_rj K_ij) - K_ir.
+ * Slot n_x doubles. */
+int bvp_firk_residual(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst,
+                      const double *stages, double *resid);
+
+/* One Newton correction of the expanded system, J delta = -F, in the unknown order
+ * (slot n_x doubles).  Replaces the LinearSolve call on the banded Jacobian of
+ * FIRK/firk.jl:806-922 (Core/internal_problems.jl:107-169). */
+int bvp_firk_newton_step(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst,
+                         const double *stages, double *delta);
+
+/* Defect estimate (defect_estimate! FIRK/adaptivity.jl:349-405): per-interval defect
+ * vectors from the stage polynomial at tau* and 1 - tau* (slot N*M) and the norm (one
+ * per instance). */
+int bvp_firk_defect(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst,
+                    const double *stages, double *errors, double *defect_norm);
+
+/* Dense output sol(t), sol(t, Val{1}) of FIRKExpandInterpolation
+ * (FIRK/interpolation.jl:229-316): per interval the quintic through y_i, y_{i+1},
+ * f(y_i), f(y_{i+1}) and the stage polynomial's value and slope at the midpoint.
+ * Points packed as for bvp_mirk_interpolate. */
+int bvp_firk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_instance *inst,
+                         const double *stages, const int32_t *n_t, const double *t, double *u,
+                         double *du);
 
 /* ---- batch plans: upload once, solve many times (no counterpart in the
  * reference, whose batches are EnsembleProblem solves run one per thread,

@@ -43,7 +43,7 @@ class _Result(C.Structure):
     _fields_ = [("retcode", C.c_int32), ("N_final", C.c_int32), ("newton_iters", C.c_int32),
                 ("outer_iters", C.c_int32), ("fallback_used", C.c_int32), ("capacity", C.c_int32),
                 ("defect_norm", C.c_double), ("resid_norm", C.c_double), ("err_vs_fp64", C.c_double),
-                ("mesh", _dp), ("u", _dp)]
+                ("mesh", _dp), ("u", _dp), ("stages", _dp)]
 
 
 def _d(a):
@@ -84,6 +84,12 @@ def lib():
         L.ref_dogleg.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
         L.ref_interpolate.argtypes = [C.c_int, C.POINTER(_Instance), C.c_int, _dp, _dp, _dp]
         L.ref_global_error.argtypes = [C.POINTER(Opts), C.POINTER(_Instance), _dp, _dp]
+        L.ref_firk_tableau.argtypes = [C.c_int, C.POINTER(C.c_int), _dp]
+        L.ref_firk_residual.argtypes = [C.c_int, C.POINTER(_Instance), _dp, _dp]
+        L.ref_firk_dense_system.argtypes = [C.c_int, C.POINTER(_Instance), _dp, _dp, _dp]
+        L.ref_firk_newton_step.argtypes = [C.c_int, C.POINTER(_Instance), _dp, _dp]
+        L.ref_firk_defect.argtypes = [C.c_int, C.POINTER(_Instance), _dp, _dp, _dp]
+        L.ref_firk_interpolate.argtypes = [C.c_int, C.POINTER(_Instance), _dp, C.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -280,6 +286,84 @@ def dogleg(J, F, radius):
     return d, float(pred[0])
 
 
+# ---- FIRK (expanded form).  Method codes: 100 + s (Radau IIA), 200 + s / 300 + s /
+# 400 + s (Lobatto IIIA / IIIB / IIIC).  `stages` is an (N, s, M) array or None for
+# the reference's initial stages K_ij = y_{i+1}.
+
+def firk_tableau(code):
+    ints = (C.c_int * 3)()
+    dbl = np.zeros(7 * 7 * 2 + 15)
+    _check(lib().ref_firk_tableau(code, ints, _d(dbl)), "firk_tableau")
+    s = ints[0]
+    A = dbl[:49].reshape(7, 7)[:s, :s].copy()
+    b = dbl[49:56][:s].copy()
+    c = dbl[56:63][:s].copy()
+    Q = dbl[63:112].reshape(7, 7)[:s, :s].copy()
+    return {"s": s, "order": ints[1], "adaptive": bool(ints[2]), "A": A, "b": b, "c": c, "Q": Q,
+            "tau": float(dbl[112])}
+
+
+def _stg(stages):
+    return None if stages is None else np.ascontiguousarray(stages, dtype=np.float64).reshape(-1)
+
+
+def firk_nx(code, spec):
+    s = code % 100
+    return spec.M * (spec.N * (s + 1) + 1)
+
+
+def firk_residual(code, spec, stages=None):
+    P = _Pinned(spec)
+    st = _stg(stages)
+    out = np.zeros(firk_nx(code, spec))
+    _check(lib().ref_firk_residual(code, C.byref(P.inst), _d(st), _d(out)), "firk_residual")
+    return out
+
+
+def firk_dense_system(code, spec, stages=None):
+    P = _Pinned(spec)
+    st = _stg(stages)
+    n = firk_nx(code, spec)
+    J, F = np.zeros((n, n)), np.zeros(n)
+    _check(lib().ref_firk_dense_system(code, C.byref(P.inst), _d(st), _d(J), _d(F)), "firk_dense_system")
+    return J, F
+
+
+def firk_newton_step(code, spec, stages=None):
+    P = _Pinned(spec)
+    st = _stg(stages)
+    out = np.zeros(firk_nx(code, spec))
+    _check(lib().ref_firk_newton_step(code, C.byref(P.inst), _d(st), _d(out)), "firk_newton_step")
+    return out
+
+
+def firk_defect(code, spec, stages=None):
+    P = _Pinned(spec)
+    st = _stg(stages)
+    err, dn = np.zeros(spec.N * spec.M), np.zeros(1)
+    _check(lib().ref_firk_defect(code, C.byref(P.inst), _d(st), _d(err), _d(dn)), "firk_defect")
+    return err.reshape(spec.N, spec.M), float(dn[0])
+
+
+def firk_interpolate(code, spec, t, stages=None):
+    P = _Pinned(spec)
+    st = _stg(stages)
+    t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+    u, du = np.zeros(max(1, len(t)) * spec.M), np.zeros(max(1, len(t)) * spec.M)
+    _check(lib().ref_firk_interpolate(code, C.byref(P.inst), _d(st), len(t), _d(t), _d(u), _d(du)),
+           "firk_interpolate")
+    return u[:len(t) * spec.M].reshape(-1, spec.M), du[:len(t) * spec.M].reshape(-1, spec.M)
+
+
+def firk_unpack(code, spec, x):
+    """Split an expanded vector [y_0, K_0., y_1, ...] into nodes (N+1, M) and stages (N, s, M)."""
+    s, M, N = code % 100, spec.M, spec.N
+    x = np.asarray(x).reshape(-1)
+    y = np.stack([x[i * (s + 1) * M:i * (s + 1) * M + M] for i in range(N + 1)])
+    K = np.stack([x[i * (s + 1) * M + M:(i + 1) * (s + 1) * M].reshape(s, M) for i in range(N)])
+    return y, K
+
+
 class Solution:
     def __init__(self, res, M, mesh, u):
         self.retcode = res.retcode
@@ -292,17 +376,25 @@ class Solution:
         self.err_vs_fp64 = res.err_vs_fp64
         self.t = mesh[:self.N + 1].copy()
         self.u = u[:M * (self.N + 1)].reshape(self.N + 1, M).copy()
+        self.stages = None
 
 
 def solve(o, spec, capacity=None):
     P = _Pinned(spec)
     cap = capacity or (o.max_num_subintervals + 1)
     mesh, u = np.zeros(cap), np.zeros(cap * spec.M)
+    s = o.method % 100 if o.method >= 100 else 0
+    st = np.zeros(max(1, cap * s * spec.M))
     r = _Result()
     r.capacity = cap
     r.mesh, r.u = _d(mesh), _d(u)
+    if s:
+        r.stages = _d(st)
     _check(lib().ref_solve(C.byref(o), C.byref(P.inst), C.byref(r)), "solve")
-    return Solution(r, spec.M, mesh, u)
+    sol = Solution(r, spec.M, mesh, u)
+    if s:
+        sol.stages = st[:sol.N * s * spec.M].reshape(sol.N, s, spec.M).copy()
+    return sol
 
 
 def solve_batch(o, specs, nthreads=0, capacity=None, keep_solutions=True):

@@ -1226,6 +1226,703 @@ int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
     return 0;
 }
 
+// ======================================================================
+// FIRK collocation in the expanded form (lib/BoundaryValueDiffEqFIRK/src/,
+// FIRK/ below; the reference's default nested_nlsolve = false):
+//   tableaus ............ FIRK/radau_tableaus.jl:1-163, FIRK/lobatto_tableaus.jl:1-323
+//                         (derived here from the nodes, firk_tableau below)
+//   unknowns ............ FIRK/firk.jl:76-92 (extend_y / shrink_y)
+//   Phi! (expanded) ..... FIRK/collocation.jl:59-93
+//   loss / layout ....... FIRK/firk.jl:1404-1455
+//   Jacobian ............ exact (the reference differentiates the loss with sparse
+//                         ForwardDiff on the banded prototype, FIRK/firk.jl:806-922)
+//   linear solve ........ banded Gaussian elimination with partial pivoting
+//   Newton .............. the polyalgorithm of the MIRK path (newton above)
+//   outer loop .......... FIRK/firk.jl:481-525, 572-621
+//   defect .............. FIRK/adaptivity.jl:349-405, 581-606 (eval_q)
+//   mesh_selector! ...... FIRK/adaptivity.jl:247-300 (the MIRK rules, rho = 1)
+//   interpolation ....... FIRK/adaptivity.jl:8-48, 183-227; FIRK/interpolation.jl:229-316
+// A STANDARD boundary row must sit at t0 or t1 here (EvalSol's end-point shortcut,
+// FIRK/interpolation.jl:331-332); the interior form of FIRK/interpolation.jl:333-368
+// is not restated.
+// The solution transfer after a mesh redistribution restates what interp_eval!
+// computes for each new node (the quintic interpolant of the old solution,
+// FIRK/adaptivity.jl:8-48) and places the nodes as extend_y does for a new problem
+// (stages K_ij = y_{i+1}, FIRK/firk.jl:76-83).  The reference writes the new node
+// values into the first N+1 slots of its expanded state and leaves the remaining
+// slots of the resized buffer as they were or uninitialised (FIRK/firk.jl:600-606
+// with FIRK/utils.jl:1-8), and it scales h by the mesh-growth factor there; that
+// state is not reproducible, so the restart differs from the reference's.  The mesh
+// sequence and the converged solutions do not depend on the restart as long as Newton
+// converges.
+
+struct FTab {
+    int code = 0, kind = 0, s = 0, order = 0;
+    bool adaptive = true;
+    double A[7][7] = {}, b[7] = {}, c[7] = {}, Q[7][7] = {}, tau = 0;
+};
+
+using ld = long double;
+
+// P_n(x) and P_n'(x) (|x| < 1) by the three-term recurrence
+void legendre(int n, ld x, ld &p, ld &dp) {
+    ld p0 = 1, p1 = x;
+    if (n == 0) { p = 1; dp = 0; return; }
+    for (int k = 2; k <= n; ++k) {
+        const ld p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    p = p1;
+    dp = n * (x * p1 - p0) / (x * x - 1);
+}
+
+// zeros of g on (-1, 1) by a sign scan and bisection
+template <class G>
+std::vector<ld> zeros_in(G &&g) {
+    std::vector<ld> z;
+    const int NG = 20000;
+    ld xa = -1 + 2.0L / NG, ga = g(xa);
+    for (int k = 2; k < NG; ++k) {
+        const ld xb = -1 + 2.0L * k / NG, gb = g(xb);
+        if (ga == 0) z.push_back(xa);
+        else if ((ga < 0) != (gb < 0) && gb != 0) {
+            ld lo = xa, hi = xb, glo = ga;
+            for (int it = 0; it < 200; ++it) {
+                const ld mid = (lo + hi) / 2, gm = g(mid);
+                if ((gm < 0) == (glo < 0)) { lo = mid; glo = gm; } else hi = mid;
+            }
+            z.push_back((lo + hi) / 2);
+        }
+        xa = xb;
+        ga = gb;
+    }
+    return z;
+}
+
+// inverse of an n x n matrix (Gauss-Jordan, partial pivoting)
+void inverse(int n, std::vector<ld> a, std::vector<ld> &inv) {
+    inv.assign((size_t)n * n, 0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1;
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        for (int i = j + 1; i < n; ++i)
+            if (std::fabs(a[(size_t)i * n + j]) > std::fabs(a[(size_t)p * n + j])) p = i;
+        for (int c = 0; c < n; ++c) {
+            std::swap(a[(size_t)j * n + c], a[(size_t)p * n + c]);
+            std::swap(inv[(size_t)j * n + c], inv[(size_t)p * n + c]);
+        }
+        const ld d = a[(size_t)j * n + j];
+        for (int c = 0; c < n; ++c) { a[(size_t)j * n + c] /= d; inv[(size_t)j * n + c] /= d; }
+        for (int i = 0; i < n; ++i) {
+            if (i == j) continue;
+            const ld m = a[(size_t)i * n + j];
+            for (int c = 0; c < n; ++c) {
+                a[(size_t)i * n + c] -= m * a[(size_t)j * n + c];
+                inv[(size_t)i * n + c] -= m * inv[(size_t)j * n + c];
+            }
+        }
+    }
+}
+
+// The tableau of a FIRK method code.  The nodes are those of the methods (Radau IIA:
+// the zeros of P_s(2t-1) - P_{s-1}(2t-1); Lobatto: 0, 1 and the zeros of P'_{s-1}(2t-1));
+// with l_j the Lagrange basis on the nodes,
+//   collocation  A_ij = int_0^{c_i} l_j,  b_j = int_0^1 l_j      (Radau IIA, Lobatto IIIA)
+//   Lobatto IIIB A_ij = b_j (1 - A^IIIA_ji / b_i)
+//   Lobatto IIIC A_i1 = b_1 and sum_j A_ij c_j^(k-1) = c_i^k / k for k = 1..s-1
+// and q_coeff Q_kj = the coefficient of tau^(k+1) in int_0^tau l_j (the stage polynomial
+// of the defect estimate and the interpolant, "bvp5c" in the reference's comments).
+// tau* is the reference's own sample point (data of the tables).
+bool firk_tableau(int code, FTab &T) {
+    T = FTab();
+    T.code = code;
+    T.kind = code / 100;
+    T.s = code % 100;
+    const int s = T.s;
+    if (T.kind == 1) {
+        if (s != 1 && s != 2 && s != 3 && s != 5 && s != 7) return false;
+        T.order = 2 * s - 1;
+        T.adaptive = s != 1;
+    } else if (T.kind >= 2 && T.kind <= 4) {
+        if (s < 2 || s > 5) return false;
+        T.order = 2 * s - 2;
+        T.adaptive = !(s == 2 && (T.kind == 3 || T.kind == 4));
+    } else {
+        return false;
+    }
+    std::vector<ld> c;
+    if (T.kind == 1) {
+        c = zeros_in([&](ld x) {
+            ld p, dp, q, dq;
+            legendre(s, x, p, dp);
+            legendre(s - 1, x, q, dq);
+            return p - q;
+        });
+        c.push_back(1);
+    } else {
+        c.push_back(-1);
+        if (s > 2) {
+            std::vector<ld> z = zeros_in([&](ld x) {
+                ld p, dp;
+                legendre(s - 1, x, p, dp);
+                return dp;
+            });
+            c.insert(c.end(), z.begin(), z.end());
+        }
+        c.push_back(1);
+    }
+    if ((int)c.size() != s) return false;
+    for (ld &v : c) v = (v + 1) / 2;
+    std::vector<ld> Cp((size_t)s * s), L;
+    for (int i = 0; i < s; ++i)
+        for (int m = 0; m < s; ++m) Cp[(size_t)i * s + m] = std::pow(c[i], (ld)m);
+    inverse(s, Cp, L);  // L[m][j]: coefficient of t^m in l_j
+    ld Acol[7][7], bcol[7];
+    for (int j = 0; j < s; ++j) {
+        bcol[j] = 0;
+        for (int m = 0; m < s; ++m) bcol[j] += L[(size_t)m * s + j] / (m + 1);
+        for (int i = 0; i < s; ++i) {
+            ld a = 0;
+            for (int m = 0; m < s; ++m) a += L[(size_t)m * s + j] * std::pow(c[i], (ld)(m + 1)) / (m + 1);
+            Acol[i][j] = a;
+        }
+    }
+    ld A[7][7];
+    for (int i = 0; i < s; ++i)
+        for (int j = 0; j < s; ++j) {
+            if (T.kind <= 2) A[i][j] = Acol[i][j];
+            else if (T.kind == 3) A[i][j] = bcol[j] * (1 - Acol[j][i] / bcol[i]);
+        }
+    if (T.kind == 4) {
+        const int n = s - 1;
+        std::vector<ld> V((size_t)n * n), Vi;
+        for (int k = 0; k < n; ++k)
+            for (int j = 1; j < s; ++j) V[(size_t)k * n + (j - 1)] = std::pow(c[j], (ld)k);
+        inverse(n, V, Vi);
+        for (int i = 0; i < s; ++i) {
+            A[i][0] = bcol[0];
+            for (int j = 1; j < s; ++j) {
+                ld a = 0;
+                for (int k = 0; k < n; ++k) {
+                    const ld rhs = std::pow(c[i], (ld)(k + 1)) / (k + 1) - (k == 0 ? bcol[0] : 0);
+                    a += Vi[(size_t)(j - 1) * n + k] * rhs;
+                }
+                A[i][j] = a;
+            }
+        }
+    }
+    for (int i = 0; i < s; ++i) {
+        T.c[i] = (double)c[i];
+        T.b[i] = (double)bcol[i];
+        for (int j = 0; j < s; ++j) {
+            T.A[i][j] = (double)A[i][j];
+            T.Q[i][j] = (double)(L[(size_t)i * s + j] / (i + 1));
+        }
+    }
+    // tau* (FIRK/radau_tableaus.jl, FIRK/lobatto_tableaus.jl: tau_star of each method)
+    if (T.kind == 1) T.tau = 0.0;
+    else if (s == 2 || s == 4) T.tau = 0.5;
+    else if (s == 3) T.tau = T.kind == 4 ? 0.7886751345948129 : 0.21132486540518713;
+    else T.tau = T.kind == 4 ? 0.6699905217924309 : 0.33000947820757126;
+    return true;
+}
+
+struct FCtx {
+    const bvp_instance &P;
+    FTab T;
+    int M;
+    FCtx(const bvp_instance &p, const FTab &t) : P(p), T(t), M(p.M) {}
+    int s() const { return T.s; }
+    size_t yoff(int i) const { return (size_t)i * (T.s + 1) * M; }
+    size_t koff(int i, int j) const { return ((size_t)i * (T.s + 1) + 1 + j) * M; }
+    size_t nx(int N) const { return (size_t)M * ((size_t)N * (T.s + 1) + 1); }
+};
+
+// extend_y (FIRK/firk.jl:76-83): nodes into the expanded order, every stage of
+// interval i set to y_{i+1}; or the given stages
+void firk_extend(const FCtx &C, int N, const double *y, const double *stages, std::vector<double> &x) {
+    const int M = C.M, s = C.s();
+    x.assign(C.nx(N), 0.0);
+    for (int i = 0; i <= N; ++i)
+        for (int c = 0; c < M; ++c) x[C.yoff(i) + c] = y[(size_t)i * M + c];
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < s; ++j)
+            for (int c = 0; c < M; ++c)
+                x[C.koff(i, j) + c] = stages ? stages[((size_t)i * s + j) * M + c] : y[(size_t)(i + 1) * M + c];
+}
+
+struct FEval {
+    std::vector<double> col;  // per interval (s+1) M: Phi_i, then G_i1..G_is
+    std::vector<double> Y;    // per interval s M: stage points y_i + h sum_j a_rj K_j
+    std::vector<double> bc;   // per BC row, input order
+};
+
+// Phi! of the expanded form (FIRK/collocation.jl:59-93), in the reference's order
+// of operations: Y_r = y_i + h (K a_r), G_r = f(Y_r) - K_r, Phi = (y_{i+1} - y_i) - h (K b)
+void firk_evaluate(const FCtx &C, int N, const double *mesh, const double *x, const std::vector<BcRow> &rows,
+                   FEval &E) {
+    const int M = C.M, s = C.s();
+    E.col.assign((size_t)N * (s + 1) * M, 0.0);
+    E.Y.assign((size_t)N * s * M, 0.0);
+    std::vector<double> f(M);
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        const double *yi = x + C.yoff(i), *yj = x + C.yoff(i + 1);
+        double *out = &E.col[(size_t)i * (s + 1) * M];
+        for (int r = 0; r < s; ++r) {
+            double *Yr = &E.Y[((size_t)i * s + r) * M];
+            for (int c = 0; c < M; ++c) {
+                double a = 0;
+                for (int j = 0; j < s; ++j) a += x[C.koff(i, j) + c] * C.T.A[r][j];
+                Yr[c] = h * a + yi[c];
+            }
+            f_eval(C.P, mesh[i] + C.T.c[r] * h, Yr, f.data());
+            for (int c = 0; c < M; ++c) out[(size_t)(r + 1) * M + c] = f[c] - x[C.koff(i, r) + c];
+        }
+        for (int c = 0; c < M; ++c) {
+            double a = 0;
+            for (int j = 0; j < s; ++j) a += x[C.koff(i, j) + c] * C.T.b[j];
+            out[c] = -h * a + (yj[c] - yi[c]);
+        }
+    }
+    E.bc.assign(rows.size(), 0.0);
+    for (size_t k = 0; k < rows.size(); ++k) E.bc[k] = x[C.yoff(rows[k].node) + rows[k].comp] - C.P.bc_val[k];
+}
+
+double firk_max_abs(const FEval &E) {
+    double m = 0;
+    for (double v : E.col) { if (std::isnan(v)) return NAN; m = std::max(m, std::fabs(v)); }
+    for (double v : E.bc) { if (std::isnan(v)) return NAN; m = std::max(m, std::fabs(v)); }
+    return m;
+}
+double firk_sumsq(const FEval &E) {
+    double s = 0;
+    for (double v : E.col) s += v * v;
+    for (double v : E.bc) s += v * v;
+    return s;
+}
+
+// reference layout: STANDARD [bc; collocation], TWO_POINT [bca; collocation; bcb]
+void firk_flatten(const FCtx &C, const std::vector<BcRow> &rows, const FEval &E, double *out) {
+    size_t o = 0;
+    const bool tp = C.P.bc_kind == BVP_BC_TWO_POINT;
+    for (size_t k = 0; k < rows.size(); ++k) if (!tp || !rows[k].side_b) out[o++] = E.bc[k];
+    for (double v : E.col) out[o++] = v;
+    if (tp) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) out[o++] = E.bc[k];
+}
+
+// The Newton matrix as (row, col, value) entries with rows ordered so that it is
+// banded (BC rows on node 0, then per interval Phi_i and G_i1..G_is, then BC rows on
+// node N); rhs receives -F in the same order.  Row order does not change delta.
+struct Trip { int r, c; double v; };
+void firk_jacobian(const FCtx &C, int N, const double *mesh, const std::vector<BcRow> &rows, const FEval &E,
+                   std::vector<Trip> &J, std::vector<double> &rhs) {
+    const int M = C.M, s = C.s();
+    J.clear();
+    rhs.clear();
+    int r = 0;
+    auto bc_rows = [&](int node) {
+        for (size_t k = 0; k < rows.size(); ++k)
+            if (rows[k].node == node) {
+                J.push_back({r, (int)C.yoff(node) + rows[k].comp, 1.0});
+                rhs.push_back(-E.bc[k]);
+                ++r;
+            }
+    };
+    bc_rows(0);
+    std::vector<double> Jr((size_t)M * M);
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        const double *col = &E.col[(size_t)i * (s + 1) * M];
+        for (int a = 0; a < M; ++a) {  // Phi_i
+            J.push_back({r, (int)C.yoff(i) + a, -1.0});
+            for (int j = 0; j < s; ++j) J.push_back({r, (int)C.koff(i, j) + a, -h * C.T.b[j]});
+            J.push_back({r, (int)C.yoff(i + 1) + a, 1.0});
+            rhs.push_back(-col[a]);
+            ++r;
+        }
+        for (int q = 0; q < s; ++q) {  // G_iq
+            J_eval(C.P, mesh[i] + C.T.c[q] * h, &E.Y[((size_t)i * s + q) * M], Jr.data());
+            for (int a = 0; a < M; ++a) {
+                for (int c = 0; c < M; ++c) J.push_back({r, (int)C.yoff(i) + c, Jr[(size_t)a * M + c]});
+                for (int j = 0; j < s; ++j)
+                    for (int c = 0; c < M; ++c) {
+                        const double v = h * C.T.A[q][j] * Jr[(size_t)a * M + c] - ((j == q && c == a) ? 1.0 : 0.0);
+                        J.push_back({r, (int)C.koff(i, j) + c, v});
+                    }
+                rhs.push_back(-col[(size_t)(q + 1) * M + a]);
+                ++r;
+            }
+        }
+    }
+    if (N > 0) bc_rows(N);
+}
+
+// banded solve of the triplet system; delta in the unknown order
+int firk_newton_dir(const FCtx &C, int N, const double *mesh, const std::vector<BcRow> &rows, const FEval &E,
+                    double *delta) {
+    std::vector<Trip> J;
+    std::vector<double> rhs;
+    firk_jacobian(C, N, mesh, rows, E, J, rhs);
+    const int n = (int)C.nx(N);
+    if ((int)rhs.size() != n) return 1;
+    int kl = 0, ku = 0;
+    for (const Trip &t : J) { kl = std::max(kl, t.r - t.c); ku = std::max(ku, t.c - t.r); }
+    Band B;
+    B.init(n, kl, ku);
+    for (const Trip &t : J) B.at(t.r, t.c) += t.v;
+    if (B.factor()) return 1;
+    B.solve(rhs.data());
+    std::memcpy(delta, rhs.data(), sizeof(double) * n);
+    return 0;
+}
+
+// The expanded system as the nonlinear solver sees it: x -> F(x), the Newton
+// direction and products with J and J' (rows in the banded order).
+struct FirkSys {
+    const FCtx &C;
+    int N;
+    const double *mesh;
+    const std::vector<BcRow> &rows;
+    FEval E;
+    std::vector<Trip> J;
+    std::vector<double> rhs;
+    double eval(const double *x) {
+        firk_evaluate(C, N, mesh, x, rows, E);
+        return firk_max_abs(E);
+    }
+    double sumsq() const { return firk_sumsq(E); }
+    int dir(double *d) { return firk_newton_dir(C, N, mesh, rows, E, d); }
+    // J, -F at the last evaluation (for the trust region)
+    void linearise() { firk_jacobian(C, N, mesh, rows, E, J, rhs); }
+    void jmul(const double *v, double *out) const {
+        std::fill(out, out + rhs.size(), 0.0);
+        for (const Trip &t : J) out[t.r] += t.v * v[t.c];
+    }
+    void jtmul(const double *w, double *out, size_t n) const {
+        std::fill(out, out + n, 0.0);
+        for (const Trip &t : J) out[t.c] += t.v * w[t.r];
+    }
+};
+
+int firk_newton_stage(FirkSys &S, std::vector<double> &x, double tol, int maxiters, bool linesearch, NlOut &out) {
+    const size_t n = x.size();
+    std::vector<double> delta(n), xt(n);
+    for (int it = 0;; ++it) {
+        out.fnorm = S.eval(x.data());
+        if (!std::isfinite(out.fnorm)) return BVP_RET_UNSTABLE;
+        if (out.fnorm <= tol) return BVP_RET_SUCCESS;
+        if (it >= maxiters) return BVP_RET_MAXITERS;
+        if (S.dir(delta.data())) return BVP_RET_FAILURE;
+        out.iters++;
+        double alpha = 1.0;
+        if (linesearch) {
+            const double ss = S.sumsq();
+            auto phi = [&](double a) {
+                for (size_t i = 0; i < n; ++i) xt[i] = x[i] + a * delta[i];
+                S.eval(xt.data());
+                return 0.5 * S.sumsq();
+            };
+            double pa;
+            if (backtracking(phi, 0.5 * ss, -ss, &alpha, &pa)) return BVP_RET_FAILURE;
+        }
+        for (size_t i = 0; i < n; ++i) x[i] += alpha * delta[i];
+    }
+}
+
+// TrustRegion stage on the expanded system (the MIRK restatement's rules above)
+int firk_trust_region_stage(FirkSys &S, std::vector<double> &x, double tol, int maxiters, NlOut &out) {
+    const size_t n = x.size();
+    std::vector<double> dn(n), g(n), d(n), xt(n), F, Jg;
+    double fy = S.eval(x.data()), ss = S.sumsq();
+    double xmax = -INFINITY, xmin = INFINITY;
+    for (double v : x) { xmax = std::max(xmax, v); xmin = std::min(xmin, v); }
+    double radius = std::max(std::sqrt(ss), xmax - xmin) / 11;
+    int shrinks = 0;
+    bool fresh = true;
+    double l2 = 0, jg2 = 0, dn2 = 0;
+    for (int it = 0;; ++it) {
+        out.fnorm = fy;
+        if (!std::isfinite(out.fnorm)) return BVP_RET_UNSTABLE;
+        if (out.fnorm <= tol) return BVP_RET_SUCCESS;
+        if (it >= maxiters) return BVP_RET_MAXITERS;
+        if (fresh) {
+            S.eval(x.data());
+            if (S.dir(dn.data())) return BVP_RET_FAILURE;
+            dn2 = 0;
+            for (double v : dn) dn2 += v * v;
+            S.linearise();
+            F.assign(S.rhs.size(), 0.0);
+            for (size_t i = 0; i < F.size(); ++i) F[i] = -S.rhs[i];
+            S.jtmul(F.data(), g.data(), n);
+            l2 = 0;
+            for (double v : g) l2 += v * v;
+            Jg.assign(S.rhs.size(), 0.0);
+            S.jmul(g.data(), Jg.data());
+            jg2 = 0;
+            for (double v : Jg) jg2 += v * v;
+            fresh = false;
+        }
+        out.iters++;
+        double pred;
+        if (std::sqrt(dn2) <= radius) {
+            d = dn;
+            pred = ss;
+        } else {
+            pred = dogleg((int)n, dn.data(), g.data(), l2, jg2, ss, radius, d.data());
+        }
+        for (size_t i = 0; i < n; ++i) xt[i] = x[i] + d[i];
+        const double ft = S.eval(xt.data());
+        const double sst = S.sumsq();
+        const double rho = (ss - sst) / pred;
+        if (rho > 1e-4) {
+            x = xt;
+            ss = sst;
+            fy = ft;
+            fresh = true;
+        }
+        if (rho < 0.25) {
+            radius *= 0.25;
+            if (++shrinks > 32) { out.fnorm = fy; return BVP_RET_FAILURE; }
+        } else {
+            shrinks = 0;
+            if (rho > 0.75) radius *= 2;
+        }
+    }
+}
+
+NlOut firk_newton(FirkSys &S, std::vector<double> &x, double tol, int maxiters, int stages) {
+    const std::vector<double> x0 = x;
+    NlOut out{BVP_RET_MAXITERS, 0, 0, -1};
+    bool first = true;
+    for (int st = 0; st < 3; ++st) {
+        if (!(stages & (1 << st))) continue;
+        if (!first) x = x0;
+        first = false;
+        out.stage = st;
+        out.ret = st < 2 ? firk_newton_stage(S, x, tol, maxiters, st == 1, out)
+                         : firk_trust_region_stage(S, x, tol, maxiters, out);
+        if (out.ret == BVP_RET_SUCCESS) break;
+    }
+    return out;
+}
+
+// eval_q (FIRK/adaptivity.jl:581-606): the stage polynomial q(tau h) = y_i +
+// sum_k coeff_k (tau h)^(k+1) and q', coeff_k = (Q K)_k / h^k
+void firk_eval_q(const FCtx &C, const double *yi, double tau, double h, const double *x, int i, double *q,
+                 double *qp) {
+    const int M = C.M, s = C.s();
+    for (int c = 0; c < M; ++c) {
+        double coef[7];
+        for (int k = 0; k < s; ++k) {
+            double a = 0;
+            for (int j = 0; j < s; ++j) a += C.T.Q[k][j] * x[C.koff(i, j) + c];
+            coef[k] = a / std::pow(h, (double)k);
+        }
+        const double th = tau * h;
+        double v = 0, dv = 0, p = 1;
+        for (int k = 0; k < s; ++k) {
+            dv += (k + 1) * coef[k] * p;
+            p *= th;
+            v += coef[k] * p;
+        }
+        q[c] = yi[c] + v;
+        qp[c] = dv;
+    }
+}
+
+// defect_estimate! (FIRK/adaptivity.jl:349-405)
+double firk_defect(const FCtx &C, int N, const double *mesh, const double *x, std::vector<double> &errors) {
+    const int M = C.M;
+    std::vector<double> z(M), zp(M), f(M), d1(M), d2(M);
+    errors.assign((size_t)N * M, 0.0);
+    double dn = 0;
+    const double tau = C.T.tau;
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        const double *yi = x + C.yoff(i);
+        firk_eval_q(C, yi, tau, h, x, i, z.data(), zp.data());
+        f_eval(C.P, mesh[i] + tau * h, z.data(), f.data());
+        double e1 = 0;
+        for (int c = 0; c < M; ++c) { d1[c] = (zp[c] - f[c]) / (std::fabs(f[c]) + 1.0); e1 = std::max(e1, std::fabs(d1[c])); }
+        firk_eval_q(C, yi, 1.0 - tau, h, x, i, z.data(), zp.data());
+        f_eval(C.P, mesh[i] + (1.0 - tau) * h, z.data(), f.data());
+        double e2 = 0;
+        for (int c = 0; c < M; ++c) { d2[c] = (zp[c] - f[c]) / (std::fabs(f[c]) + 1.0); e2 = std::max(e2, std::fabs(d2[c])); }
+        const double *d = e1 > e2 ? d1.data() : d2.data();
+        double ei = 0;
+        for (int c = 0; c < M; ++c) { errors[(size_t)i * M + c] = d[c]; ei = std::max(ei, std::fabs(d[c])); }
+        if (!(ei <= dn)) dn = ei;
+    }
+    return dn;
+}
+
+// 6 x 6 solve of the quintic through value and slope at 0, h, h/2
+// (get_S_coeffs / s_constraints_block, FIRK/adaptivity.jl:183-227)
+void quintic(double h, const double *bk, double *coef) {
+    const double t[3] = {0.0, h, h / 2};
+    double A[6][6], b[6];
+    for (int r = 0; r < 3; ++r)
+        for (int m = 0; m < 6; ++m) {
+            A[r][m] = std::pow(t[r], (double)m);
+            A[3 + r][m] = m == 0 ? 0.0 : m * std::pow(t[r], (double)(m - 1));
+        }
+    // rows: value 0, value h, value h/2, slope 0, slope h, slope h/2
+    for (int r = 0; r < 6; ++r) b[r] = bk[r];
+    for (int j = 0; j < 6; ++j) {
+        int p = j;
+        for (int i = j + 1; i < 6; ++i) if (std::fabs(A[i][j]) > std::fabs(A[p][j])) p = i;
+        for (int c = 0; c < 6; ++c) std::swap(A[j][c], A[p][c]);
+        std::swap(b[j], b[p]);
+        for (int i = j + 1; i < 6; ++i) {
+            const double m = A[i][j] / A[j][j];
+            for (int c = j; c < 6; ++c) A[i][c] -= m * A[j][c];
+            b[i] -= m * b[j];
+        }
+    }
+    for (int i = 5; i >= 0; --i) {
+        double v = b[i];
+        for (int c = i + 1; c < 6; ++c) v -= A[i][c] * coef[c];
+        coef[i] = v / A[i][i];
+    }
+}
+
+// interpolant! of FIRKExpandInterpolation (FIRK/interpolation.jl:229-316), also
+// interp_eval! of the transfer (FIRK/adaptivity.jl:8-48) without its h scaling
+void firk_interp(const FCtx &C, int N, const double *mesh, const double *x, double t, double *u, double *du) {
+    const int M = C.M;
+    const int j = interval_of(mesh, N, t);
+    const double h = mesh[j + 1] - mesh[j], tau = t - mesh[j];
+    const double *yi = x + C.yoff(j), *yj = x + C.yoff(j + 1);
+    std::vector<double> fi(M), fj(M), z(M), zp(M);
+    f_eval(C.P, mesh[j], yi, fi.data());
+    f_eval(C.P, mesh[j + 1], yj, fj.data());
+    firk_eval_q(C, yi, 0.5, h, x, j, z.data(), zp.data());
+    for (int c = 0; c < M; ++c) {
+        const double bk[6] = {yi[c], yj[c], z[c], fi[c], fj[c], zp[c]};
+        double coef[6];
+        quintic(h, bk, coef);
+        double v = 0, dv = 0;
+        for (int m = 0; m < 6; ++m) {
+            v += coef[m] * std::pow(tau, (double)m);
+            if (m > 0) dv += m * coef[m] * std::pow(tau, (double)(m - 1));
+        }
+        u[c] = v;
+        if (du) du[c] = dv;
+    }
+}
+
+int firk_check(const bvp_instance &P, int code, FTab &T) {
+    if (!firk_tableau(code, T)) return BVP_EINVAL;
+    if (check_instance(P)) return BVP_EINVAL;
+    for (int k = 0; k < P.n_bc; ++k)
+        if (P.bc_t[k] != P.t0 && P.bc_t[k] != P.t1) return BVP_ENOTSUP;
+    return 0;
+}
+
+// state of an instance: mesh, nodes (guess/u0) and stages in the expanded order
+int firk_state(const FCtx &C, const double *stages, std::vector<double> &mesh, std::vector<double> &x,
+               std::vector<BcRow> &rows) {
+    std::vector<double> y;
+    int rc = instance_mesh_and_guess(C.P, mesh, y);
+    if (rc) return rc;
+    if ((rc = resolve_bc(C.P, C.P.N, mesh.data(), rows))) return rc;
+    firk_extend(C, C.P.N, y.data(), stages, x);
+    return 0;
+}
+
+int firk_solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
+    FTab T;
+    int rc = firk_check(P, o.method, T);
+    if (rc) return rc;
+    if (o.adaptive && !T.adaptive) return BVP_ENOTSUP;
+    if (o.controller != BVP_CONTROL_DEFECT) return BVP_ENOTSUP;
+    FCtx C(P, T);
+    std::vector<double> mesh, x;
+    std::vector<BcRow> rows;
+    if ((rc = firk_state(C, nullptr, mesh, x, rows))) return rc;
+    const double tol = o.newton_abstol > 0 ? o.newton_abstol : o.abstol;
+    const int M = P.M, s = T.s;
+    int N = P.N;
+    int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
+    double defect_norm = 2 * o.abstol, last_defect = NAN;
+    int outer = 0, newton_iters = 0, fb = 0;
+    std::vector<double> errors, new_mesh;
+    const int MAX_OUTER = 64;
+    for (;;) {
+        ++outer;
+        if (resolve_bc(P, N, mesh.data(), rows)) return BVP_EINVAL;
+        FirkSys S{C, N, mesh.data(), rows, {}, {}, {}};
+        NlOut nl = firk_newton(S, x, tol, o.max_newton_iters, o.nlsolve_stages);
+        newton_iters += nl.iters;
+        if (nl.stage == 1) fb |= 1;
+        if (nl.stage == 2) fb |= 16;
+        nl_ret = nl.ret;
+        defect_norm = 2 * o.abstol;
+        info = nl.ret;
+        if (!o.adaptive) break;
+        if (info == BVP_RET_SUCCESS) {
+            defect_norm = firk_defect(C, N, mesh.data(), x.data(), errors);
+            last_defect = defect_norm;
+            if (defect_norm > o.defect_threshold || std::isnan(defect_norm)) info = BVP_RET_FAILURE;
+        }
+        if (info == BVP_RET_SUCCESS) {
+            if (defect_norm > o.abstol) {
+                info = mesh_select(T.order, o.abstol, o.max_num_subintervals, mesh, errors, M, new_mesh);
+                if (info == BVP_RET_SUCCESS) {
+                    const int Nn = (int)new_mesh.size() - 1;
+                    std::vector<double> yn((size_t)(Nn + 1) * M);
+                    for (int k = 0; k <= Nn; ++k)
+                        firk_interp(C, N, mesh.data(), x.data(), new_mesh[k], &yn[(size_t)k * M], nullptr);
+                    mesh = new_mesh;
+                    N = Nn;
+                    firk_extend(C, N, yn.data(), nullptr, x);
+                }
+            }
+        } else {
+            if (2 * N > o.max_num_subintervals) {
+                info = BVP_RET_FAILURE;
+            } else {
+                std::vector<double> hm;
+                half_mesh(mesh, hm);
+                mesh = hm;
+                N = (int)mesh.size() - 1;
+                x.assign(C.nx(N), 0.0);
+                info = BVP_RET_SUCCESS;
+            }
+        }
+        if (!(info == BVP_RET_SUCCESS && defect_norm > o.abstol)) break;
+        if (outer >= MAX_OUTER) { info = BVP_RET_FAILURE; break; }
+    }
+    res.retcode = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
+    res.N_final = N;
+    res.newton_iters = newton_iters;
+    res.outer_iters = outer;
+    res.fallback_used = fb;
+    res.defect_norm = last_defect;
+    res.err_vs_fp64 = NAN;
+    if (resolve_bc(P, N, mesh.data(), rows) == 0) {
+        FEval E;
+        firk_evaluate(C, N, mesh.data(), x.data(), rows, E);
+        res.resid_norm = firk_max_abs(E);
+    }
+    if (res.capacity < N + 1) return BVP_EINVAL;
+    if (res.mesh) std::memcpy(res.mesh, mesh.data(), sizeof(double) * (N + 1));
+    if (res.u)
+        for (int i = 0; i <= N; ++i)
+            for (int c = 0; c < M; ++c) res.u[(size_t)i * M + c] = x[C.yoff(i) + c];
+    if (res.stages)
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < s; ++j)
+                for (int c = 0; c < M; ++c) res.stages[((size_t)i * s + j) * M + c] = x[C.koff(i, j) + c];
+    return 0;
+}
+
+int solve_any(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
+    return o.method >= 100 ? firk_solve_one(P, o, res) : solve_one(P, o, res);
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1507,7 +2204,118 @@ int ref_mesh_select_errors(int order, double abstol, int max_sub, int N, const d
     return (int)nm.size() - 1;
 }
 
-int ref_solve(const bvp_opts *o, const bvp_instance *P, bvp_result *res) { return solve_one(*P, *o, *res); }
+// ---- FIRK (expanded form).  `stages` is [N*s*M] (interval, stage, component) or
+// NULL for extend_y's stages.
+// tableau export: ints = [s, order, adaptive], dbl = A[49] (row-major, Butcher
+// order), b[7], c[7], Q[49], tau
+int ref_firk_tableau(int code, int *ints, double *dbl) {
+    FTab T;
+    if (!firk_tableau(code, T)) return BVP_EINVAL;
+    ints[0] = T.s; ints[1] = T.order; ints[2] = T.adaptive ? 1 : 0;
+    int o = 0;
+    for (int i = 0; i < 7; ++i) for (int j = 0; j < 7; ++j) dbl[o++] = T.A[i][j];
+    for (int i = 0; i < 7; ++i) dbl[o++] = T.b[i];
+    for (int i = 0; i < 7; ++i) dbl[o++] = T.c[i];
+    for (int i = 0; i < 7; ++i) for (int j = 0; j < 7; ++j) dbl[o++] = T.Q[i][j];
+    dbl[o++] = T.tau;
+    return 0;
+}
+
+int ref_firk_residual(int code, const bvp_instance *P, const double *stages, double *resid) {
+    FTab T;
+    int rc = firk_check(*P, code, T);
+    if (rc) return rc;
+    FCtx C(*P, T);
+    std::vector<double> mesh, x;
+    std::vector<BcRow> rows;
+    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+    FEval E;
+    firk_evaluate(C, P->N, mesh.data(), x.data(), rows, E);
+    firk_flatten(C, rows, E, resid);
+    return 0;
+}
+
+// Dense expanded Jacobian (reference row layout, n_x x n_x row-major) and residual.
+int ref_firk_dense_system(int code, const bvp_instance *P, const double *stages, double *J, double *F) {
+    FTab T;
+    int rc = firk_check(*P, code, T);
+    if (rc) return rc;
+    FCtx C(*P, T);
+    std::vector<double> mesh, x;
+    std::vector<BcRow> rows;
+    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+    FEval E;
+    firk_evaluate(C, P->N, mesh.data(), x.data(), rows, E);
+    firk_flatten(C, rows, E, F);
+    std::vector<Trip> Jt;
+    std::vector<double> rhs;
+    firk_jacobian(C, P->N, mesh.data(), rows, E, Jt, rhs);
+    // banded row order -> reference row order
+    const int n = (int)C.nx(P->N), M = C.M, s = T.s, N = P->N;
+    const bool tp = P->bc_kind == BVP_BC_TWO_POINT;
+    std::vector<int> nb0, nbN;  // BC rows (input index) on node 0 / node N, banded order
+    for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == 0) nb0.push_back((int)k);
+    for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == N) nbN.push_back((int)k);
+    std::vector<int> bc_ref(rows.size());  // input index -> reference row
+    {
+        int r = 0;
+        for (size_t k = 0; k < rows.size(); ++k) if (!tp || !rows[k].side_b) bc_ref[k] = r++;
+        r += N * (s + 1) * M;
+        if (tp) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) bc_ref[k] = r++;
+    }
+    const int nbc_top = tp ? (int)std::count_if(rows.begin(), rows.end(), [](const BcRow &b) { return !b.side_b; })
+                           : (int)rows.size();
+    std::vector<int> map(n);
+    int r = 0;
+    for (int k : nb0) map[r++] = bc_ref[k];
+    for (int i = 0; i < N * (s + 1) * M; ++i) map[r++] = nbc_top + i;
+    for (int k : nbN) map[r++] = bc_ref[k];
+    std::memset(J, 0, sizeof(double) * (size_t)n * n);
+    for (const Trip &t : Jt) J[(size_t)map[t.r] * n + t.c] += t.v;
+    return 0;
+}
+
+int ref_firk_newton_step(int code, const bvp_instance *P, const double *stages, double *delta) {
+    FTab T;
+    int rc = firk_check(*P, code, T);
+    if (rc) return rc;
+    FCtx C(*P, T);
+    std::vector<double> mesh, x;
+    std::vector<BcRow> rows;
+    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+    FEval E;
+    firk_evaluate(C, P->N, mesh.data(), x.data(), rows, E);
+    return firk_newton_dir(C, P->N, mesh.data(), rows, E, delta) ? 1 : 0;
+}
+
+int ref_firk_defect(int code, const bvp_instance *P, const double *stages, double *errors, double *defect_norm) {
+    FTab T;
+    int rc = firk_check(*P, code, T);
+    if (rc) return rc;
+    FCtx C(*P, T);
+    std::vector<double> mesh, x, err;
+    std::vector<BcRow> rows;
+    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+    *defect_norm = firk_defect(C, P->N, mesh.data(), x.data(), err);
+    std::memcpy(errors, err.data(), sizeof(double) * err.size());
+    return 0;
+}
+
+int ref_firk_interpolate(int code, const bvp_instance *P, const double *stages, int nt, const double *t, double *u,
+                         double *du) {
+    FTab T;
+    int rc = firk_check(*P, code, T);
+    if (rc) return rc;
+    FCtx C(*P, T);
+    std::vector<double> mesh, x;
+    std::vector<BcRow> rows;
+    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+    for (int q = 0; q < nt; ++q)
+        firk_interp(C, P->N, mesh.data(), x.data(), t[q], u + (size_t)q * C.M, du ? du + (size_t)q * C.M : nullptr);
+    return 0;
+}
+
+int ref_solve(const bvp_opts *o, const bvp_instance *P, bvp_result *res) { return solve_any(*P, *o, *res); }
 
 // EnsembleThreads analogue: one instance per thread, dynamic schedule.
 int ref_solve_batch(const bvp_opts *o, int n, const bvp_instance *inst, bvp_result *res, int nthreads) {
@@ -1517,7 +2325,7 @@ int ref_solve_batch(const bvp_opts *o, int n, const bvp_instance *inst, bvp_resu
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
 #endif
     for (int i = 0; i < n; ++i) {
-        int rc = solve_one(inst[i], *o, res[i]);
+        int rc = solve_any(inst[i], *o, res[i]);
         if (rc) bad |= 1;
     }
     return bad ? BVP_EINVAL : 0;

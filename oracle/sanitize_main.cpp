@@ -24,6 +24,13 @@ int ref_mesh_select(const bvp_opts *o, const bvp_instance *P, int capacity, int3
 int ref_backtracking_poly(const double *coef, int ncoef, double *alpha, double *trace, int max_trace,
                           int *n_trace);
 int ref_dogleg(int n, const double *J, const double *F, double radius, double *d, double *pred);
+int ref_firk_tableau(int code, int *ints, double *dbl);
+int ref_firk_residual(int code, const bvp_instance *P, const double *stages, double *resid);
+int ref_firk_dense_system(int code, const bvp_instance *P, const double *stages, double *J, double *F);
+int ref_firk_newton_step(int code, const bvp_instance *P, const double *stages, double *delta);
+int ref_firk_defect(int code, const bvp_instance *P, const double *stages, double *errors, double *defect_norm);
+int ref_firk_interpolate(int code, const bvp_instance *P, const double *stages, int nt, const double *t, double *u,
+                         double *du);
 }
 
 namespace {
@@ -120,6 +127,52 @@ void exercise(Problem &q, const char *name) {
     }
 }
 
+// the FIRK entry points (expanded form) for every method on an end-point-BC problem
+void exercise_firk(Problem &q, const char *name) {
+    const bvp_instance &P = q.inst;
+    const int M = P.M, N = P.N;
+    const int codes[] = {101, 102, 103, 105, 107, 202, 203, 204, 205, 302, 303, 304, 305, 402, 403, 404, 405};
+    for (int code : codes) {
+        int ints[3];
+        double tab[113];
+        expect(ref_firk_tableau(code, ints, tab) == 0, name);
+        const int s = ints[0];
+        const size_t nx = (size_t)M * ((size_t)N * (s + 1) + 1);
+        std::vector<double> K((size_t)N * s * M, 0.1), F(nx), J(nx * nx), d(nx), err((size_t)N * M);
+        expect(ref_firk_residual(code, &P, nullptr, F.data()) == 0, name);
+        expect(ref_firk_residual(code, &P, K.data(), F.data()) == 0, name);
+        expect(ref_firk_dense_system(code, &P, K.data(), J.data(), F.data()) == 0, name);
+        ref_firk_newton_step(code, &P, K.data(), d.data());  // may be singular at this state
+        double dn = 0;
+        expect(ref_firk_defect(code, &P, K.data(), err.data(), &dn) == 0, name);
+        const double tq[3] = {P.t0 - 0.1, 0.5 * (P.t0 + P.t1), P.t1 + 0.1};
+        std::vector<double> u(3 * M), du(3 * M);
+        expect(ref_firk_interpolate(code, &P, K.data(), 3, tq, u.data(), du.data()) == 0, name);
+        bvp_opts o;
+        ref_opts_default(&o);
+        o.method = code;
+        o.abstol = 1e-5;
+        o.max_num_subintervals = 200;
+        o.adaptive = ints[2];
+        const int cap = 201;
+        for (int stages : {1, 3, 4, 7}) {
+            o.nlsolve_stages = stages;
+            o.max_newton_iters = 30;
+            std::vector<double> rm(cap), ru((size_t)cap * M), rs((size_t)cap * s * M);
+            bvp_result res{};
+            res.capacity = cap;
+            res.mesh = rm.data();
+            res.u = ru.data();
+            res.stages = rs.data();
+            expect(ref_solve(&o, &P, &res) == 0, name);
+        }
+    }
+    int ints[3];
+    double tab[113];
+    expect(ref_firk_tableau(106, ints, tab) == BVP_EINVAL, "no RadauIIa6");
+    expect(ref_firk_tableau(206, ints, tab) == BVP_EINVAL, "no LobattoIIIa6");
+}
+
 }  // namespace
 
 int main() {
@@ -139,6 +192,14 @@ int main() {
                       {-1, 0, 0, 1, 0, 0}, 10));
     const char *names[] = {"bratu", "pendulum", "singpert", "turning", "scalar_tu", "linear_tanh3", "swirl"};
     for (size_t i = 0; i < ps.size(); ++i) exercise(ps[i], names[i]);
+    exercise_firk(ps[0], "firk bratu");
+    Problem tu0 = make(BVP_FAMILY_SCALAR_TU, 1, 0, 1, {}, BVP_BC_STANDARD, {0.0}, {0}, {1.0}, 9);
+    exercise_firk(tu0, "firk scalar_tu");
+    {
+        std::vector<double> F(64);
+        expect(ref_firk_residual(103, &ps[4].inst, nullptr, F.data()) == BVP_ENOTSUP, "firk interior row");
+    }
+    exercise_firk(ps[5], "firk linear_tanh3");
 
     // invalid inputs: error codes, no out-of-bounds access
     Problem bad = ps[0];

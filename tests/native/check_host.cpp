@@ -196,6 +196,82 @@ int main() {
     }
     std::string err;
     expect(bvphost::validate_instances(0, nullptr, M, supports, err) == BVP_EINVAL, "empty batch");
+    // ---- FIRK tableaus: order conditions B(p), collocation C(s) for Radau IIA / Lobatto
+    // IIIA, and entries the reference tabulates (FIRK/radau_tableaus.jl:40-85,
+    // FIRK/lobatto_tableaus.jl:146-166, 246-266, 293-323)
+    {
+        const int codes[] = {101, 102, 103, 105, 107, 202, 203, 204, 205, 302, 303, 304, 305, 402, 403, 404, 405};
+        for (int code : codes) {
+            bvphost::FirkTableau T;
+            expect(bvphost::firk_tableau(code, T), "firk tableau exists");
+            const int s = T.s;
+            expect(s == code % 100, "stages");
+            for (int k = 1; k <= T.order; ++k) {
+                double v = 0;
+                for (int j = 0; j < s; ++j) v += T.b[j] * std::pow(T.c[j], k - 1);
+                expect(std::fabs(v - 1.0 / k) < 1e-13, "B(p)");
+            }
+            if (code / 100 <= 2)
+                for (int k = 1; k <= s; ++k)
+                    for (int i = 0; i < s; ++i) {
+                        double v = 0;
+                        for (int j = 0; j < s; ++j) v += T.A[i][j] * std::pow(T.c[j], k - 1);
+                        expect(std::fabs(v - std::pow(T.c[i], k) / k) < 1e-13, "C(s)");
+                    }
+            // q' interpolates the stages: sum_k (k+1) Q[k][j] c_i^k = delta_ij
+            for (int i = 0; i < s; ++i)
+                for (int j = 0; j < s; ++j) {
+                    double v = 0;
+                    for (int k = 0; k < s; ++k) v += (k + 1) * T.Q[k][j] * std::pow(T.c[i], k);
+                    expect(std::fabs(v - (i == j ? 1.0 : 0.0)) < 1e-11, "Q");
+                }
+        }
+        bvphost::FirkTableau T;
+        expect(!bvphost::firk_tableau(104, T) && !bvphost::firk_tableau(206, T) && !bvphost::firk_tableau(4, T),
+               "unknown FIRK codes");
+        const double r6 = std::sqrt(6.0), r21 = std::sqrt(21.0);
+        bvphost::firk_tableau(103, T);
+        expect(std::fabs(T.A[0][1] - (37.0 / 225 - 169 * r6 / 1800)) < 1e-14, "RadauIIa3 a12");
+        expect(std::fabs(T.Q[1][1] - 3.320280554681775) < 1e-13, "RadauIIa3 q22");
+        bvphost::firk_tableau(303, T);
+        expect(std::fabs(T.A[2][1] - 5.0 / 6) < 1e-15 && std::fabs(T.A[0][2]) < 1e-15, "LobattoIIIb3");
+        bvphost::firk_tableau(403, T);
+        expect(std::fabs(T.A[1][2] + 1.0 / 12) < 1e-15 && T.tau == 0.7886751345948129, "LobattoIIIc3");
+        bvphost::firk_tableau(405, T);
+        expect(std::fabs(T.A[1][2] - (47 - 15 * r21) / 315) < 1e-14, "LobattoIIIc5 a23");
+        expect(!T.adaptive == false, "LobattoIIIc5 adapts");
+        bvphost::firk_tableau(402, T);
+        expect(!T.adaptive, "LobattoIIIc2 does not adapt");
+    }
+    // ---- FIRK options
+    {
+        std::vector<Inst> v = {bratu(1.0, 7)};
+        v[0].fix();
+        std::vector<bvp_instance> a = {v[0].p};
+        bvp_opts o{};
+        o.abstol = 1e-6;
+        o.max_num_subintervals = 3000;
+        o.max_newton_iters = 1000;
+        o.adaptive = 1;
+        std::string err;
+        o.method = 103;
+        o.nlsolve_stages = 3;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_OK, "radau5 options");
+        o.nlsolve_stages = 7;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_ENOTSUP, "firk trust region");
+        o.nlsolve_stages = 3;
+        o.method = 101;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_ENOTSUP, "radau1 adaptive");
+        o.adaptive = 0;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_OK, "radau1 fixed mesh");
+        o.method = 106;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_EINVAL, "no radau6");
+        o.method = 204;
+        o.controller = BVP_CONTROL_GLOBAL;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_ENOTSUP, "firk global error");
+        o.controller = BVP_CONTROL_DEFECT;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 8, true, err) == BVP_ENOTSUP, "firk on the block kernels");
+    }
     std::printf("%s: %d failures\n", failures ? "FAIL" : "OK", failures);
     return failures ? 1 : 0;
 }

@@ -66,6 +66,25 @@ def test_constants_match_header():
         assert d[k] == v, k
 
 
+def test_firk_method_codes_match_header():
+    """BVP_RADAU_IIA(s) and BVP_LOBATTO_IIIA/B/C(s) are function-like macros of the header;
+    the Python mirror and the algorithm classes give the same codes."""
+    text = (ROOT / "include" / "bvp_mirk.h").read_text()
+    pairs = {"BVP_RADAU_IIA": _abi.RADAU_IIA, "BVP_LOBATTO_IIIA": _abi.LOBATTO_IIIA,
+             "BVP_LOBATTO_IIIB": _abi.LOBATTO_IIIB, "BVP_LOBATTO_IIIC": _abi.LOBATTO_IIIC}
+    for name, fn in pairs.items():
+        m = re.search(r"#define " + name + r"\(s\)\s+\((\d+) \+ \(s\)\)", text)
+        assert m, name
+        for st in (1, 2, 3, 4, 5, 7):
+            assert fn(st) == int(m.group(1)) + st
+    assert bvp.RadauIIa5().code == _abi.RADAU_IIA(5) and bvp.LobattoIIIc4().code == _abi.LOBATTO_IIIC(4)
+    assert bvp.RadauIIa5.order == 9 and bvp.LobattoIIIb3.order == 4 and bvp.RadauIIa7().stages == 7
+    o = bvp.make_opts(bvp.LobattoIIIa4(defect_threshold=0.2), abstol=1e-7)
+    assert o.method == 204 and o.nlsolve_stages == 3 and o.defect_threshold == 0.2 and o.abstol == 1e-7
+    with pytest.raises(NotImplementedError):
+        bvp.RadauIIa3(nested_nlsolve=True)
+
+
 def test_struct_layouts(tmp_path):
     """ctypes mirrors (product and oracle) have the C compiler's sizes and offsets."""
     import subprocess
