@@ -385,6 +385,77 @@ def bench_singpert(args, _abi, _lib, synthetic, learned):
     print(json.dumps(out), flush=True)
 
 
+def bench_firk(args, _abi, _lib, synthetic, learned):
+    """FIRK collocation on the headline's 16,384 Bratu instances (the expanded form of
+    lib/BoundaryValueDiffEqFIRK; RadauIIa3 unless --firk-method), abstol 1e-8, the
+    headline's Newton tolerance, constant zero guess, FP64, one persistent launch; the
+    oracle's FIRK restatement on the host cores as the CPU baseline and the checker."""
+    import numpy as np
+    n = args.instances
+    code = args.firk_method
+    s_ = code % 100
+    abstol = 1e-8
+    specs, lams, Ns = synthetic.bratu_batch(n)
+    ctx = _lib.context(0)
+    opts = _abi.default_opts(code)
+    opts.abstol = abstol
+    opts.newton_abstol = args.newton_abstol
+    opts.nlsolve_stages = _abi.STAGE_NEWTON | _abi.STAGE_BACKTRACKING
+    batch = _lib.Batch(ctx, opts, specs)
+    batch.set_graph(not args.no_graph)
+    tune = None
+    if not args.no_tune:
+        cand_ms, (variant, blocks) = batch.autotune(repeats=1)
+        tune = {"candidates_ms": cand_ms, "persistent_waves": blocks}
+        log(f"[bench] autotune: {['%.2f' % x for x in cand_ms]} ms -> {blocks} waves")
+    t = timed_runs(batch, args.steps, args.warmup, f"firk {code}")
+    res = batch.results()
+    cnt = batch.counters()
+    batch.close()
+    ok = n_solved(res, abstol)
+    value = ok / (t["solve_ms"] * 1e-3)
+    out = {"workload": f"bratu{n // 1024}k, FIRK method {code} ({s_} stages, expanded form): {n} Bratu BVPs, "
+                       f"lambda~U(0.1,3.4), N~U{{64..1024}}, abstol {abstol}, Newton tolerance {args.newton_abstol}, "
+                       "constant zero guess, FP64",
+           "method": code, "value": value, "unit": "solves/s (device solve time)", "solve_ms": t["solve_ms"],
+           "wall_ms_per_step": t["wall"] * 1e3, "solved": ok, **learned.summary(res),
+           "final_N_mean": float(sum(r.N for r in res) / n), "counters": cnt, "autotune": tune}
+    log(f"[bench] firk {code}: {ok}/{n} solved; {value:.1f} solves/s; solve {t['solve_ms']:.3f} ms")
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as O
+        threads = args.cpu_threads or os.cpu_count() or 1
+        oo = O.opts(code, abstol=abstol, newton_abstol=args.newton_abstol, nlsolve_stages=3)
+        tc = time.perf_counter()
+        rs = O.solve_batch(oo, specs, nthreads=threads, keep_solutions=not args.no_accuracy)
+        tcpu = time.perf_counter() - tc
+        okc = sum(1 for r in rs if (r if isinstance(r, int) else r.retcode) == 0)
+        out["cpu_baseline"] = {"value": okc / tcpu, "unit": "solves/s", "cores": threads, "cpu_model": cpu_model(),
+                               "kind": "port", "sample": f"the same {n} instances, {tcpu:.1f} s wall"}
+        log(f"[bench] oracle: {okc / tcpu:.1f} solves/s on {threads} threads")
+        if not args.no_accuracy:
+            emax, same, other, rdiff = 0.0, 0, 0, 0
+            for sp, r, g in zip(specs, rs, res):
+                if r.retcode != g.retcode:
+                    rdiff += 1
+                if r.retcode != 0 or g.retcode != 0:
+                    continue
+                if r.N == g.N and np.max(np.abs(r.t - g.t)) <= 1e-12:
+                    e = float(np.max(np.abs(g.u - r.u) / np.maximum(1.0, np.abs(r.u))))
+                    same += 1
+                else:
+                    st = sp.with_state(g.t, g.u.reshape(-1))
+                    d = O.firk_newton_step(code, st, g.stages)
+                    e = float(np.max(np.abs(d)) / max(1.0, float(np.max(np.abs(g.u)))))
+                    other += 1
+                emax = max(emax, e)
+            out["accuracy"] = {"max_rel_err_vs_oracle": emax, "compared_same_mesh": same,
+                               "compared_on_gpu_mesh": other, "retcode_mismatches": rdiff, "tolerance": 1e-9,
+                               "within_tolerance": emax <= 1e-9}
+            log(f"[bench] accuracy vs oracle: {out['accuracy']}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,7 +479,9 @@ def main():
     ap.add_argument("--load-net", default="", help="use the network in this .npz instead of training one")
     ap.add_argument("--anchors", type=int, default=4,
                     help="network evaluations per instance (Hermite-interpolated to the nodes); 0: every node")
-    ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh", "learned", "singpert"],
+    ap.add_argument("--firk-method", type=int, default=103,
+                    help="FIRK method code of --workload firk (100 + s Radau IIA, 200/300/400 + s Lobatto IIIA/B/C)")
+    ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh", "learned", "singpert", "firk"],
                     help="bratu: 16,384 Bratu BVPs (MIRK4, FP64, learned warm start); lintanh: 2,048 n=32 "
                          "linear+tanh BVPs (MIRK6, FP32 factors + FP64 refinement); learned: warm start on "
                          "16,384 unseen Bratu (a, b) BVPs")
@@ -446,6 +519,8 @@ def main():
         return bench_learned(args, _abi, _lib, synthetic, learned)
     if args.workload == "singpert":
         return bench_singpert(args, _abi, _lib, synthetic, learned)
+    if args.workload == "firk":
+        return bench_firk(args, _abi, _lib, synthetic, learned)
 
     n = args.instances
     abstol = 1e-8

@@ -212,12 +212,16 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     int variant = 0, per_cu = 0, tuned_f32 = f32, tuned_minc = 1;
     if (ctx->tuned.count(key)) std::tie(variant, per_cu, tuned_f32, tuned_minc) = ctx->tuned[key];
     if (ctx->blocks_per_cu > 0) per_cu = ctx->blocks_per_cu;
-    // the FIRK kernels hold one wave per SIMD (register allocation of 512)
+    if (firk && (variant < 0 || variant >= kFirkVariants)) variant = 0;
     if (per_cu < 1)
         per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision)
-                     : (firk ? 4 : wave_blocks_per_cu(M, o->method, variant, ufam, f32));
+                     : (firk ? firk_blocks_per_cu(M, ftab.s, variant)
+                             : wave_blocks_per_cu(M, o->method, variant, ufam, f32));
     if (per_cu < 1) per_cu = 1;
-    int slots = std::min(n, ctx->num_cus * (firk ? 4 : per_cu));
+    int slots = std::min(n, ctx->num_cus * per_cu);
+    if (firk)
+        for (int v = 0; v < kFirkVariants; ++v)
+            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, firk_blocks_per_cu(M, ftab.s, v))));
     if (!blk && !firk)
         for (int v = 0; v < wave_variants(M); ++v)
             slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v, ufam, f32))));
@@ -345,7 +349,7 @@ static int enqueue_solve(bvp_ctx *ctx, bvp_batch *B) {
         D.warm = 1;
     }
     if (B->firk)
-        HIP_TRY(ctx, firk_launch_solve(D, B->firk_s, B->blocks, ctx->stream));
+        HIP_TRY(ctx, firk_launch_solve(D, B->firk_s, B->variant, B->blocks, ctx->stream));
     else if (B->block)
         HIP_TRY(ctx, block_launch_solve(D, B->method, B->opts.precision, B->blocks, ctx->stream));
     else if (D.controller != BVP_CONTROL_DEFECT)
@@ -1009,8 +1013,13 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     struct Cand { int variant, blocks, f32, minc; };
     std::vector<Cand> cands;
     if (B->firk) {
-        // one kernel per (M, stages): waves per CU only
-        for (int pc = 1; pc <= 4; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc), 0, 1});
+        // the two register variants, each at 1, 2, 4, ... waves per CU up to its occupancy
+        for (int v = 0; v < kFirkVariants; ++v) {
+            const int occ = std::max(1, firk_blocks_per_cu(B->M, B->firk_s, v));
+            for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({v, std::min(B->max_slots, ctx->num_cus * pc), 0, 1});
+            if (cands.back().blocks != std::min(B->max_slots, ctx->num_cus * occ))
+                cands.push_back({v, std::min(B->max_slots, ctx->num_cus * occ), 0, 1});
+        }
     } else if (B->block) {
         const int occ = std::max(1, block_blocks_per_cu(B->M, B->method, B->opts.precision));
         for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({0, std::min(B->max_slots, ctx->num_cus * pc), 0, 1});
@@ -1113,7 +1122,7 @@ int bvp_batch_precision(bvp_batch *B, int32_t *precision) {
 
 int bvp_batch_set_config(bvp_ctx *ctx, bvp_batch *B, int32_t variant, int32_t blocks) {
     if (!ctx || !B) return BVP_EINVAL;
-    const int nv = (B->block || B->firk) ? 1 : wave_variants(B->M);
+    const int nv = B->block ? 1 : (B->firk ? kFirkVariants : wave_variants(B->M));
     if (variant < 0 || variant >= nv || blocks < 1 || blocks > B->max_slots) {
         set_err(ctx, "variant must be in [0, %d) and blocks in [1, %d]", nv, B->max_slots);
         return BVP_EINVAL;

@@ -4,11 +4,14 @@
 
 namespace bvpdev {
 #define FIRK_INST(S)                                                                                         \
-    template hipError_t firk_launch_solve_t<4, S>(const BatchDev &, int, hipStream_t);                      \
+    template hipError_t firk_launch_solve_t<4, S, 1>(const BatchDev &, int, hipStream_t);                   \
+    template hipError_t firk_launch_solve_t<4, S, 2>(const BatchDev &, int, hipStream_t);                   \
     template hipError_t firk_launch_component_t<4, S>(const BatchDev &, int, int, hipStream_t);             \
     template hipError_t firk_launch_interp_t<4, S>(const BatchDev &, const int *, const double *, double *,  \
                                                      double *, long long, hipStream_t);                       \
-    template hipError_t firk_launch_gather_t<4, S>(const BatchDev &, const long long *, double *, hipStream_t);
+    template hipError_t firk_launch_gather_t<4, S>(const BatchDev &, const long long *, double *, hipStream_t); \
+    template int firk_occupancy_t<4, S, 1>();                                                              \
+    template int firk_occupancy_t<4, S, 2>();
 FIRK_INST(1)
 FIRK_INST(2)
 FIRK_INST(3)

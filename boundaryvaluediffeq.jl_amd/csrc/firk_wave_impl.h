@@ -761,7 +761,7 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
 // ------------------------------------------------------------------ kernels
 // Persistent FIRK solve: each wave pulls instances from the queue.
 template <int M, int S>
-__global__ __launch_bounds__(64, 1) void k_firk_solve(BatchDev B) {
+__device__ __forceinline__ void firk_solve_queue(const BatchDev &B) {
     __shared__ Shared<M> sh;
     const int lane = threadIdx.x;
     const FirkTab &T = *reinterpret_cast<const FirkTab *>(B.firk_tab);
@@ -772,6 +772,17 @@ __global__ __launch_bounds__(64, 1) void k_firk_solve(BatchDev B) {
         if (item >= B.n) break;
         firk_solve_instance<M, S>(B, T, B.order[item], sh, lane, scr, aux);
     }
+}
+
+// Two register-allocation variants the autotuner chooses between: one wave per SIMD
+// (few spills) and two waves per SIMD (twice the waves to hide latency, more spills).
+template <int M, int S>
+__global__ __launch_bounds__(64, 1) void k_firk_solve(BatchDev B) {
+    firk_solve_queue<M, S>(B);
+}
+template <int M, int S>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_firk_solve_w2(BatchDev B) {
+    firk_solve_queue<M, S>(B);
 }
 
 // Component kernels at the packed state (mesh, guess and the packed stages init_K).  Slots of n_x = M (N (S+1) + 1) per instance:
@@ -898,9 +909,10 @@ __global__ __launch_bounds__(256) void k_firk_gather(BatchDev B, const long long
     for (int q = threadIdx.x; q < N * S * M; q += blockDim.x) out[o + q] = K[q];
 }
 
-template <int M, int S>
+template <int M, int S, int WPE>
 hipError_t firk_launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
-    k_firk_solve<M, S><<<blocks, 64, 0, s>>>(B);
+    if constexpr (WPE == 2) k_firk_solve_w2<M, S><<<blocks, 64, 0, s>>>(B);
+    else k_firk_solve<M, S><<<blocks, 64, 0, s>>>(B);
     return hipGetLastError();
 }
 template <int M, int S>
@@ -915,6 +927,16 @@ hipError_t firk_launch_interp_t(const BatchDev &B, const int *qinst, const doubl
     const unsigned nb = (unsigned)((nq + 255) / 256);
     k_firk_interp<M, S><<<nb, 256, 0, s>>>(B, qinst, tq, u, du, nq);
     return hipGetLastError();
+}
+// resident FIRK solve waves per CU (registers and LDS) of variant WPE
+template <int M, int S, int WPE>
+int firk_occupancy_t() {
+    int nb = 0;
+    hipError_t e;
+    if constexpr (WPE == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S>, 64, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S>, 64, 0);
+    if (e != hipSuccess) return 4 * WPE;
+    return nb;
 }
 template <int M, int S>
 hipError_t firk_launch_gather_t(const BatchDev &B, const long long *out_off, double *out, hipStream_t s) {

@@ -130,11 +130,14 @@ int wave_rel_fields(int M);
 
 // FIRK kernels (firk_wave_impl.h): M = 1..4, S = 1, 2, 3, 4, 5, 7 stages
 bool firk_supports(int M, int S);
-hipError_t firk_launch_solve(const BatchDev &B, int S, int blocks, hipStream_t s);
+// variant v: register allocation held to v + 1 waves per SIMD (v = 0, 1)
+constexpr int kFirkVariants = 2;
+hipError_t firk_launch_solve(const BatchDev &B, int S, int variant, int blocks, hipStream_t s);
 hipError_t firk_launch_component(const BatchDev &B, int S, int op, int blocks, hipStream_t s);
 hipError_t firk_launch_interp(const BatchDev &B, int S, const int *qinst, const double *tq, double *u, double *du,
                               long long nq, hipStream_t s);
 hipError_t firk_launch_gather(const BatchDev &B, int S, const long long *out_off, double *out, hipStream_t s);
+int firk_blocks_per_cu(int M, int S, int variant);  // resident solve waves per CU
 // doubles per slot of the FIRK work area for node capacity cap
 inline size_t firk_aux_doubles(int cap, int M, int S) {
     return (size_t)cap * (2 * M + 2 * S * M + S * M * (M + 1));
