@@ -8,6 +8,9 @@ MIRK4 = 4
 MIRK6 = 6
 
 
+FIRK_NESTED = 1000
+
+
 def RADAU_IIA(s):
     return 100 + s
 
@@ -30,6 +33,10 @@ def is_firk(method):
 
 def firk_stages(method):
     return method % 100
+
+
+def is_firk_nested(method):
+    return method >= FIRK_NESTED
 
 FAMILY_LINEAR_TANH = 0
 FAMILY_PENDULUM = 1
@@ -122,6 +129,7 @@ class Opts(C.Structure):
         ("global_method", C.c_int32),
         ("hybrid_de", C.c_double),
         ("hybrid_ge", C.c_double),
+        ("nest_abstol", C.c_double),
     ]
 
 
@@ -178,6 +186,7 @@ def default_opts(method=MIRK4):
     o.global_method = GLOBAL_HO
     o.hybrid_de = 1.0
     o.hybrid_ge = 1.0
+    o.nest_abstol = 0.0
     return o
 
 

@@ -277,6 +277,8 @@ def _firk_stages_arg(method, specs, stages):
 
 
 def _nx(method, spec):
+    if _abi.is_firk_nested(method):
+        return spec.M * (spec.N + 1)
     return spec.M * (spec.N * (_abi.firk_stages(method) + 1) + 1)
 
 

@@ -159,6 +159,18 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     std::vector<double> &t0 = pk.t0, &t1 = pk.t1, &params = pk.params, &bct = pk.bct, &bcv = pk.bcv,
                         &imesh = pk.mesh, &iy = pk.y, &u0c = pk.u0c;
     std::vector<long long> &ioff = pk.off;
+    const bool nested = firk && o->method >= BVP_FIRK_NESTED;
+    if (nested)
+        // the inner solves' start k0 (__K0_on_u0, FIRK/utils.jl:41-83): u0, else the mean of
+        // the guess's nodes
+        for (int i = 0; i < n; ++i) {
+            if (inst[i].u0) continue;
+            double *k0 = &u0c[(size_t)i * M];
+            const double *yi = &iy[(size_t)ioff[i] * M];
+            for (int k = 0; k <= N0[i]; ++k)
+                for (int c = 0; c < M; ++c) k0[c] += yi[(size_t)k * M + c];
+            for (int c = 0; c < M; ++c) k0[c] /= (N0[i] + 1);
+        }
 
     BatchDev &D = B->dev;
     std::memset(&D, 0, sizeof D);
@@ -215,13 +227,13 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
     if (firk && (variant < 0 || variant >= kFirkVariants)) variant = 0;
     if (per_cu < 1)
         per_cu = blk ? block_blocks_per_cu(M, o->method, o->precision)
-                     : (firk ? firk_blocks_per_cu(M, ftab.s, variant)
+                     : (firk ? firk_blocks_per_cu(M, ftab.s, variant, nested)
                              : wave_blocks_per_cu(M, o->method, variant, ufam, f32));
     if (per_cu < 1) per_cu = 1;
     int slots = std::min(n, ctx->num_cus * per_cu);
     if (firk)
         for (int v = 0; v < kFirkVariants; ++v)
-            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, firk_blocks_per_cu(M, ftab.s, v))));
+            slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, firk_blocks_per_cu(M, ftab.s, v, nested))));
     if (!blk && !firk)
         for (int v = 0; v < wave_variants(M); ++v)
             slots = std::max(slots, std::min(n, ctx->num_cus * std::max(1, wave_blocks_per_cu(M, o->method, v, ufam, f32))));
@@ -295,6 +307,8 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
             return rc;
         }
         D.firk_tab = dtab;
+        D.firk_nested = nested;
+        D.nest_tol = o->nest_abstol > 0 ? o->nest_abstol : 3.000213634488528e-13;  // eps^(4/5)
     }
     // options
     D.adaptive = o->adaptive;
@@ -820,6 +834,7 @@ int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_inst
 // The stage values of the component kernels: the caller's, or extend_y's
 // (K_ij = y_{i+1} from the instance's guess or u0), packed at interval offsets.
 static int firk_upload_stages(bvp_ctx *ctx, bvp_batch *B, int32_t n, const bvp_instance *inst, const double *stages) {
+    if (B->dev.firk_nested) return 0;  // the nested form solves its stages from k0
     const int M = B->M, S = B->firk_s;
     const long long nint = B->total_nodes - n;
     std::vector<double> K((size_t)std::max(1LL, nint) * S * M, 0.0);
@@ -863,10 +878,11 @@ static int firk_component(bvp_ctx *ctx, int32_t method, int op, int32_t n, const
     return rc;
 }
 
-// doubles of the expanded vectors of a batch: sum_i M (N_i (s + 1) + 1)
+// doubles of the unknown vectors of a batch: sum_i M (N_i (s + 1) + 1) (expanded) or
+// sum_i M (N_i + 1) (nested)
 static size_t firk_nx_total(int method, int32_t n, const bvp_instance *inst) {
     size_t t = 0;
-    const int S = method % 100;
+    const int S = method >= BVP_FIRK_NESTED ? 0 : method % 100;
     for (int i = 0; i < n; ++i) t += (size_t)inst[i].M * ((size_t)inst[i].N * (S + 1) + 1);
     return t;
 }
@@ -1015,7 +1031,7 @@ int bvp_batch_autotune(bvp_ctx *ctx, bvp_batch *B, int32_t repeats, float *cand_
     if (B->firk) {
         // the two register variants, each at 1, 2, 4, ... waves per CU up to its occupancy
         for (int v = 0; v < kFirkVariants; ++v) {
-            const int occ = std::max(1, firk_blocks_per_cu(B->M, B->firk_s, v));
+            const int occ = std::max(1, firk_blocks_per_cu(B->M, B->firk_s, v, B->dev.firk_nested));
             for (int pc = 1; pc <= occ; pc *= 2) cands.push_back({v, std::min(B->max_slots, ctx->num_cus * pc), 0, 1});
             if (cands.back().blocks != std::min(B->max_slots, ctx->num_cus * occ))
                 cands.push_back({v, std::min(B->max_slots, ctx->num_cus * occ), 0, 1});

@@ -215,7 +215,9 @@ void solve_ld(int n, ld *V, ld *r) {
 // same methods (FIRK/radau_tableaus.jl, FIRK/lobatto_tableaus.jl), tau* included.
 bool firk_tableau(int code, FirkTableau &T) {
     T = FirkTableau();
-    const int kind = code / 100, s = code % 100;
+    if (code >= 2 * BVP_FIRK_NESTED) return false;
+    const int base = code % BVP_FIRK_NESTED;  // the nested flag selects the form, not the tableau
+    const int kind = base / 100, s = base % 100;
     if (kind == 1) {
         if (s != 1 && s != 2 && s != 3 && s != 5 && s != 7) return false;
         T.order = 2 * s - 1;

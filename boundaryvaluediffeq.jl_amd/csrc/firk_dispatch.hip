@@ -10,8 +10,8 @@ namespace bvpdev {
                                                            double *, double *, long long, hipStream_t);       \
     extern template hipError_t firk_launch_gather_t<M, S>(const BatchDev &, const long long *, double *,      \
                                                            hipStream_t);                                      \
-    extern template int firk_occupancy_t<M, S, 1>();                                                         \
-    extern template int firk_occupancy_t<M, S, 2>();
+    extern template int firk_occupancy_t<M, S, 1>(int);                                                         \
+    extern template int firk_occupancy_t<M, S, 2>(int);
 #define FIRK_DECL_M(M) FIRK_DECL(M, 1) FIRK_DECL(M, 2) FIRK_DECL(M, 3) FIRK_DECL(M, 4) FIRK_DECL(M, 5) FIRK_DECL(M, 7)
 FIRK_DECL_M(1)
 FIRK_DECL_M(2)
@@ -59,9 +59,10 @@ hipError_t firk_launch_interp(const BatchDev &B, int S, const int *qinst, const 
 hipError_t firk_launch_gather(const BatchDev &B, int S, const long long *out_off, double *out, hipStream_t s) {
     FIRK_SWITCH((firk_launch_gather_t<MM, SS>(B, out_off, out, s)))
 }
-int firk_blocks_per_cu(int M, int S, int variant) {
-#define FIRK_OCC(MM, SS) \
-    case MM * 16 + SS: return variant == 1 ? firk_occupancy_t<MM, SS, 2>() : firk_occupancy_t<MM, SS, 1>();
+int firk_blocks_per_cu(int M, int S, int variant, int nested) {
+#define FIRK_OCC(MM, SS)                                                                   \
+    case MM * 16 + SS:                                                                     \
+        return variant == 1 ? firk_occupancy_t<MM, SS, 2>(nested) : firk_occupancy_t<MM, SS, 1>(nested);
 #define FIRK_OCC_M(MM) FIRK_OCC(MM, 1) FIRK_OCC(MM, 2) FIRK_OCC(MM, 3) FIRK_OCC(MM, 4) FIRK_OCC(MM, 5) FIRK_OCC(MM, 7)
     switch (M * 16 + S) {
         FIRK_OCC_M(1) FIRK_OCC_M(2) FIRK_OCC_M(3) FIRK_OCC_M(4)

@@ -1,4 +1,4 @@
-// firk_m2.hip -- the FIRK collocation kernels (expanded form) for state dimension
+// firk_m2.hip -- the FIRK collocation kernels (expanded and nested forms) for state dimension
 // M = 2, one instantiation per stage count of the Radau IIA and Lobatto families.
 #include "firk_wave_impl.h"
 
@@ -10,8 +10,8 @@ namespace bvpdev {
     template hipError_t firk_launch_interp_t<2, S>(const BatchDev &, const int *, const double *, double *,  \
                                                      double *, long long, hipStream_t);                       \
     template hipError_t firk_launch_gather_t<2, S>(const BatchDev &, const long long *, double *, hipStream_t); \
-    template int firk_occupancy_t<2, S, 1>();                                                              \
-    template int firk_occupancy_t<2, S, 2>();
+    template int firk_occupancy_t<2, S, 1>(int);                                                              \
+    template int firk_occupancy_t<2, S, 2>(int);
 FIRK_INST(1)
 FIRK_INST(2)
 FIRK_INST(3)

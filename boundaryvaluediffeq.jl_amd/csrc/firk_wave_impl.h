@@ -197,9 +197,12 @@ template <int M, int S> constexpr int kFirkLoc = S * M * (M + 1);
 
 // Static condensation of interval k: solves W [kg | KJ] = [G | J], stores the record
 // at loc, and forms the two-node relation S_blk dy_i + dy_{i+1} = rhs.
+// nested: the stages are solved (G ~ 0 to the inner tolerance) and the relation is
+// the Newton step of the nested residual, rhs = -Phi.
 template <int M, int S>
 __device__ inline bool firk_condense(const FirkTab &T, double h, const double (&G)[S][M], const double (&Phi)[M],
-                                     const double (&J)[S][M][M], double *loc, double (&Sb)[M][M], double (&rhs)[M]) {
+                                     const double (&J)[S][M][M], double *loc, double (&Sb)[M][M], double (&rhs)[M],
+                                     bool nested = false) {
     constexpr int NN = S * M;
     double W[NN][NN], X[NN][M + 1];
 #pragma unroll
@@ -227,7 +230,7 @@ __device__ inline bool firk_condense(const FirkTab &T, double h, const double (&
         double g = 0.0;
 #pragma unroll
         for (int j = 0; j < S; ++j) g += T.b[j] * X[j * M + a][0];
-        rhs[a] = -Phi[a] + h * g;
+        rhs[a] = nested ? -Phi[a] : -Phi[a] + h * g;
 #pragma unroll
         for (int c = 0; c < M; ++c) {
             double v = 0.0;
@@ -247,6 +250,77 @@ __device__ inline void load_stages(const double *K, int k, double (&Kk)[S][M]) {
         for (int c = 0; c < M; ++c) Kk[r][c] = K[((size_t)k * S + r) * M + c];
 }
 
+// The nested form (nested_nlsolve = true, FIRK/firk.jl:114-288): the unknowns are the
+// node values; each evaluation solves an interval's stage equations G(K) = 0 for K
+// from the fixed start k0 by NewtonRaphson (NonlinearSolve's, restated: K <- K + W^-1 G
+// until max|G| <= tol, checked before every step, NaN stops, at most 1000 steps) and
+// the residual is Phi_i.  Its Jacobian enters the outer one by the implicit function
+// theorem, dK/dy_i = W^-1 [J_1; ...; J_s] -- the KJ of the condensation.
+template <int M> struct FirkNest {
+    bool on;
+    double tol;
+    double k0[M];
+};
+
+template <int M>
+__device__ inline FirkNest<M> firk_nest_of(const BatchDev &B, int b) {
+    FirkNest<M> n;
+    n.on = B.firk_nested != 0;
+    n.tol = B.nest_tol;
+#pragma unroll
+    for (int c = 0; c < M; ++c) n.k0[c] = n.on ? B.u0c[(size_t)b * M + c] : 0.0;
+    return n;
+}
+
+template <int M, int S>
+__device__ inline void firk_inner(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest, double ti, double h,
+                                  const double *yi, double (&K)[S][M]) {
+    constexpr int NN = S * M;
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+#pragma unroll
+        for (int c = 0; c < M; ++c) K[r][c] = nest.k0[c];
+    for (int it = 0; it < 1000; ++it) {
+        double G[S][M], Phi[M], J[S][M][M];
+        firk_interval<M, S, true>(I, T, ti, h, yi, yi, K, G, Phi, J);
+        double mx = 0.0;
+        bool bad = false;
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                bad |= isnan(G[r][c]);
+                mx = fmax(mx, fabs(G[r][c]));
+            }
+        if (bad || !(mx > nest.tol)) break;
+        double W[NN][NN], X[NN][1];
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+#pragma unroll
+                    for (int c = 0; c < M; ++c)
+                        W[r * M + a][j * M + c] = ((r == j && a == c) ? 1.0 : 0.0) - h * T.A[r][j] * J[r][a][c];
+                X[r * M + a][0] = G[r][a];
+            }
+        if (!lu_solve<NN, 1>(W, X)) break;
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) K[r][c] += X[r * M + c][0];
+    }
+}
+
+template <int M, int S>
+__device__ inline void store_stages(double *K, int k, const double (&Kk)[S][M]) {
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+#pragma unroll
+        for (int c = 0; c < M; ++c) K[((size_t)k * S + r) * M + c] = Kk[r][c];
+}
+
 // Boundary rows of the FIRK path: every row reads a node value (t0 or t1).
 template <int M>
 __device__ inline void firk_bc_rows(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y, int N,
@@ -263,11 +337,12 @@ __device__ inline void firk_bc_rows(const Inst<M> &I, Shared<M> &sh, const doubl
 // Forward pass at (y, K): residual max (NaN-aware), per-interval records (loc) and the
 // chunk elimination (Back records in scr, the chunk relation in rel).  Returns
 // max|F| over Phi, G and the boundary rows in all lanes (NaN if not finite); *sing is
-// set when a local system was singular.
+// set when a local system was singular.  Nested form: K is written, not read (each
+// interval's stages solved from k0), and max|F| is over Phi and the boundary rows.
 template <int M, int S>
-__device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, Shared<M> &sh, const double *mesh,
-                                      const double *y, const double *K, int N, int lane, double *scr, double *loc,
-                                      Rel<M> &rel, bool &sing) {
+__device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest, Shared<M> &sh,
+                                      const double *mesh, const double *y, double *K, int N, int lane, double *scr,
+                                      double *loc, Rel<M> &rel, bool &sing) {
     int P, start, len;
     chunk_of(N, lane, P, start, len);
     double mx = 0.0;
@@ -277,7 +352,12 @@ __device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, Shared
         double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M];
 #pragma unroll
         for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
-        load_stages<M, S>(K, k, Kk);
+        if (nest.on) {
+            firk_inner<M, S>(I, T, nest, ti, h, yi, Kk);
+            store_stages<M, S>(K, k, Kk);
+        } else {
+            load_stages<M, S>(K, k, Kk);
+        }
         firk_interval<M, S, true>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
 #pragma unroll
         for (int c = 0; c < M; ++c) {
@@ -285,16 +365,18 @@ __device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, Shared
             bad |= !(a <= 1.7976931348623157e308);
             mx = fmax(mx, a);
         }
+        if (!nest.on) {
 #pragma unroll
-        for (int r = 0; r < S; ++r)
+            for (int r = 0; r < S; ++r)
 #pragma unroll
-            for (int c = 0; c < M; ++c) {
-                const double a = fabs(G[r][c]);
-                bad |= !(a <= 1.7976931348623157e308);
-                mx = fmax(mx, a);
-            }
+                for (int c = 0; c < M; ++c) {
+                    const double a = fabs(G[r][c]);
+                    bad |= !(a <= 1.7976931348623157e308);
+                    mx = fmax(mx, a);
+                }
+        }
         double Sb[M][M], rhs[M], Rb[M][M];
-        sg |= !firk_condense<M, S>(T, h, G, Phi, J, loc + (size_t)k * kFirkLoc<M, S>, Sb, rhs);
+        sg |= !firk_condense<M, S>(T, h, G, Phi, J, loc + (size_t)k * kFirkLoc<M, S>, Sb, rhs, nest.on);
 #pragma unroll
         for (int a = 0; a < M; ++a)
 #pragma unroll
@@ -333,7 +415,7 @@ __device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, Shared
 // the root system is singular.
 template <int M, int S>
 __device__ inline bool firk_correction(const Inst<M> &I, Shared<M> &sh, int N, int lane, double *scr,
-                                       const double *loc, Rel<M> &rel, double *dy, double *dK) {
+                                       const double *loc, Rel<M> &rel, double *dy, double *dK, bool nested = false) {
     int P, start, len;
     chunk_of(N, lane, P, start, len);
     if (!tree_solve<M, false>(I, sh, N, lane, P, scr, rel)) return false;
@@ -354,9 +436,10 @@ __device__ inline bool firk_correction(const Inst<M> &I, Shared<M> &sh, int N, i
         }
 #pragma unroll
         for (int c = 0; c < M; ++c) dy[(size_t)start * M + c] = ya[c];
-        // stage corrections of the chunk's intervals (dy of their left nodes: just written by this lane)
+        // stage corrections of the chunk's intervals (dy of their left nodes: just written by
+        // this lane); the nested form has no stage unknowns
         constexpr int NN = S * M;
-        for (int k = start; k < start + len; ++k) {
+        for (int k = start; k < start + (nested ? 0 : len); ++k) {
             const double *lk = loc + (size_t)k * kFirkLoc<M, S>;
             double d[M];
 #pragma unroll
@@ -377,9 +460,9 @@ __device__ inline bool firk_correction(const Inst<M> &I, Shared<M> &sh, int N, i
 // Sum of squares of the residual at (y + alpha dy, K + alpha dK) (the line search's
 // merit, times 2) in all lanes; NaN if not finite.
 template <int M, int S>
-__device__ inline double firk_trial_sumsq(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
-                                          const double *K, const double *dy, const double *dK, double alpha, int N,
-                                          int lane) {
+__device__ inline double firk_trial_sumsq(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest,
+                                          const double *mesh, const double *y, const double *K, const double *dy,
+                                          const double *dK, double alpha, int N, int lane) {
     int P, start, len;
     chunk_of(N, lane, P, start, len);
     double ss = 0.0;
@@ -392,20 +475,26 @@ __device__ inline double firk_trial_sumsq(const Inst<M> &I, const FirkTab &T, co
             yi[c] = y[(size_t)k * M + c] + alpha * dy[(size_t)k * M + c];
             yj[c] = y[(size_t)(k + 1) * M + c] + alpha * dy[(size_t)(k + 1) * M + c];
         }
+        if (nest.on) {
+            firk_inner<M, S>(I, T, nest, ti, h, yi, Kk);
+        } else {
 #pragma unroll
-        for (int r = 0; r < S; ++r)
+            for (int r = 0; r < S; ++r)
 #pragma unroll
-            for (int c = 0; c < M; ++c) {
-                const size_t q = ((size_t)k * S + r) * M + c;
-                Kk[r][c] = K[q] + alpha * dK[q];
-            }
+                for (int c = 0; c < M; ++c) {
+                    const size_t q = ((size_t)k * S + r) * M + c;
+                    Kk[r][c] = K[q] + alpha * dK[q];
+                }
+        }
         firk_interval<M, S, false>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
 #pragma unroll
         for (int c = 0; c < M; ++c) ss += Phi[c] * Phi[c];
+        if (!nest.on) {
 #pragma unroll
-        for (int r = 0; r < S; ++r)
+            for (int r = 0; r < S; ++r)
 #pragma unroll
-            for (int c = 0; c < M; ++c) ss += G[r][c] * G[r][c];
+                for (int c = 0; c < M; ++c) ss += G[r][c] * G[r][c];
+        }
     }
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
@@ -423,14 +512,16 @@ __device__ inline double firk_trial_sumsq(const Inst<M> &I, const FirkTab &T, co
 // x += alpha d over the nodes and stages of the lane's chunk (node N by the last lane)
 template <int M, int S>
 __device__ inline void firk_apply(double *y, double *K, const double *dy, const double *dK, double alpha, int N,
-                                  int lane) {
+                                  int lane, bool nested) {
     int P, start, len;
     chunk_of(N, lane, P, start, len);
     for (int k = start; k < start + len; ++k) {
 #pragma unroll
         for (int c = 0; c < M; ++c) y[(size_t)k * M + c] += alpha * dy[(size_t)k * M + c];
+        if (!nested) {
 #pragma unroll
-        for (int q = 0; q < S * M; ++q) K[(size_t)k * S * M + q] += alpha * dK[(size_t)k * S * M + q];
+            for (int q = 0; q < S * M; ++q) K[(size_t)k * S * M + q] += alpha * dK[(size_t)k * S * M + q];
+        }
     }
     if (lane == P - 1)
 #pragma unroll
@@ -455,9 +546,10 @@ struct FirkNewtonOut {
 // when max|F| <= tol (checked before every step).  Work areas: dy [cap][M], dK, the
 // start copy y0 [cap][M], K0, loc.
 template <int M, int S>
-__device__ inline FirkNewtonOut firk_newton(const BatchDev &B, const Inst<M> &I, const FirkTab &T, Shared<M> &sh,
-                                            const double *mesh, double *y, double *K, int N, int lane, double *scr,
-                                            double *loc, double *dy, double *dK, double *y0, double *K0) {
+__device__ inline FirkNewtonOut firk_newton(const BatchDev &B, const Inst<M> &I, const FirkTab &T,
+                                            const FirkNest<M> &nest, Shared<M> &sh, const double *mesh, double *y,
+                                            double *K, int N, int lane, double *scr, double *loc, double *dy,
+                                            double *dK, double *y0, double *K0) {
     FirkNewtonOut out{BVP_RET_MAXITERS, 0, -1, NAN};
     const int stages = B.stages & 3;
     bool first = true;
@@ -470,23 +562,25 @@ __device__ inline FirkNewtonOut firk_newton(const BatchDev &B, const Inst<M> &I,
         for (int it = 0;; ++it) {
             Rel<M> rel;
             bool sing = false;
-            const double fn = firk_forward<M, S>(I, T, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
+            const double fn = firk_forward<M, S>(I, T, nest, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
             out.fn = fn;
             if (!(fabs(fn) <= 1.7976931348623157e308)) { out.ret = BVP_RET_UNSTABLE; break; }
             if (fn <= B.newton_tol) { out.ret = BVP_RET_SUCCESS; break; }
             if (it >= B.max_newton) { out.ret = BVP_RET_MAXITERS; break; }
-            if (sing || !firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK)) {
+            if (sing || !firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK, nest.on)) {
                 out.ret = BVP_RET_FAILURE;
                 break;
             }
             out.iters++;
             double alpha = 1.0;
             if (st == 1) {
-                const double ss = firk_trial_sumsq<M, S>(I, T, mesh, y, K, dy, dK, 0.0, N, lane);
-                auto phi = [&](double a) { return 0.5 * firk_trial_sumsq<M, S>(I, T, mesh, y, K, dy, dK, a, N, lane); };
+                const double ss = firk_trial_sumsq<M, S>(I, T, nest, mesh, y, K, dy, dK, 0.0, N, lane);
+                auto phi = [&](double a) {
+                    return 0.5 * firk_trial_sumsq<M, S>(I, T, nest, mesh, y, K, dy, dK, a, N, lane);
+                };
                 if (!backtracking(phi, 0.5 * ss, -ss, alpha)) { out.ret = BVP_RET_FAILURE; break; }
             }
-            firk_apply<M, S>(y, K, dy, dK, alpha, N, lane);
+            firk_apply<M, S>(y, K, dy, dK, alpha, N, lane, nest.on);
         }
         if (out.ret == BVP_RET_SUCCESS) break;
     }
@@ -600,18 +694,24 @@ __device__ inline void firk_quintic(double h, const double (&bk)[6], double (&co
     for (int r = 0; r < 6; ++r) coef[r] = X[r][0];
 }
 
-// interpolant! of FIRKExpandInterpolation (FIRK/interpolation.jl:229-316), and the node
-// values of interp_eval! (FIRK/adaptivity.jl:8-48): the quintic of interval(mesh, t)
-// through y_i, y_{i+1}, f(y_i), f(y_{i+1}) and the stage polynomial at the midpoint.
+// interpolant! of FIRKExpandInterpolation / FIRKNestedInterpolation
+// (FIRK/interpolation.jl:127-316), and the node values of interp_eval!
+// (FIRK/adaptivity.jl:8-135): the quintic of interval(mesh, t) through y_i, y_{i+1},
+// f(y_i), f(y_{i+1}) and the stage polynomial at the midpoint.  hscale multiplies the
+// interval's step in the stage polynomial and the quintic (the nested form's transfer
+// passes the reference's growth factor, FIRK/adaptivity.jl:98-104); with nest.on the
+// stages come from the inner solve (at the unscaled step), else from K.
 template <int M, int S>
-__device__ inline void firk_interp_at(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
-                                      const double *K, int N, double t, double *u, double *du) {
+__device__ inline void firk_interp_at(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest, const double *mesh,
+                                      const double *y, const double *K, int N, double t, double *u, double *du,
+                                      double hscale = 1.0) {
     const int j = interval_of(mesh, N, t);
-    const double tj = mesh[j], h = mesh[j + 1] - tj, tau = t - tj;
+    const double tj = mesh[j], h0 = mesh[j + 1] - tj, tau = t - tj, h = h0 * hscale;
     double yi[M], yj[M], fi[M], fj[M], z[M], zp[M], Kk[S][M];
 #pragma unroll
     for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)j * M + c]; yj[c] = y[(size_t)(j + 1) * M + c]; }
-    load_stages<M, S>(K, j, Kk);
+    if (nest.on && !K) firk_inner<M, S>(I, T, nest, tj, h0, yi, Kk);
+    else load_stages<M, S>(K, j, Kk);
     f_eval<M, double>(I.fam, I.P, tj, yi, fi);
     f_eval<M, double>(I.fam, I.P, mesh[j + 1], yj, fj);
     firk_eval_q<M, S>(T, yi, 0.5, h, Kk, z, zp);
@@ -648,7 +748,7 @@ __device__ inline void firk_extend(const double *y, double *K, int N, int lane) 
 // the transfer of the solution (the old quintic interpolant at the new nodes, stages as
 // extend_y sets them for a new problem), or halving with a zero restart when Newton or
 // the defect threshold failed.
-template <int M, int S>
+template <int M, int S, bool NEST>
 __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, int b, Shared<M> &sh, int lane,
                                            double *scr, double *aux) {
     Inst<M> I;
@@ -656,6 +756,8 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
     I.minc = 1;
     Bufs<M> bf = bufs_of<M>(B, b);
     double *K = firk_stages_of<M, S>(B, b);
+    FirkNest<M> nest = firk_nest_of<M>(B, b);
+    nest.on = NEST;  // a compile-time form: the expanded kernels carry no inner-solve code
     const size_t cap = (size_t)B.cap;
     double *dy = aux;                          // [cap][M]
     double *dK = dy + cap * M;                 // [cap][S][M]
@@ -673,7 +775,7 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
             for (int c = 0; c < M; ++c) bf.y[0][(size_t)k * M + c] = B.init_y[(off + k) * M + c];
         }
         __syncthreads();
-        firk_extend<M, S>(bf.y[0], K, N, lane);
+        if (!nest.on) firk_extend<M, S>(bf.y[0], K, N, lane);
     }
     int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
     int outer = 0, iters = 0, fb = 0;
@@ -685,7 +787,7 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
         const double *mesh = bf.mesh[cur];
         double *y = bf.y_of(cur);
         double *mesh_o = bf.mesh[1 - cur], *y_o = bf.y_of(1 - cur);
-        FirkNewtonOut nl = firk_newton<M, S>(B, I, T, sh, mesh, y, K, N, lane, scr, loc, dy, dK, y0, K0);
+        FirkNewtonOut nl = firk_newton<M, S>(B, I, T, nest, sh, mesh, y, K, N, lane, scr, loc, dy, dK, y0, K0);
         iters += nl.iters;
         rn = nl.fn;
         rn_valid = true;
@@ -708,10 +810,14 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
                 if (Nn < 0) {
                     info = BVP_RET_FAILURE;
                 } else {
+                    // the nested form restates interp_eval! with its step scaled by the growth
+                    // of the node count (FIRK/adaptivity.jl:98-104); the expanded form's
+                    // restart is the interpolated nodes with extend_y's stages
+                    const double hs = (nest.on && Nn > N) ? (double)Nn / (double)N : 1.0;
                     for (int j = lane; j <= Nn; j += WAVE)
-                        firk_interp_at<M, S>(I, T, mesh, y, K, N, mesh_o[j], y_o + (size_t)j * M, nullptr);
+                        firk_interp_at<M, S>(I, T, nest, mesh, y, K, N, mesh_o[j], y_o + (size_t)j * M, nullptr, hs);
                     __syncthreads();
-                    firk_extend<M, S>(y_o, K, Nn, lane);
+                    if (!nest.on) firk_extend<M, S>(y_o, K, Nn, lane);
                     cnt_tr += N + Nn;
                     N = Nn;
                     cur = 1 - cur;
@@ -738,7 +844,7 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
     if (!rn_valid) {
         Rel<M> rel;
         bool sing = false;
-        rn = firk_forward<M, S>(I, T, sh, bf.mesh[cur], bf.y_of(cur), K, N, lane, scr, loc, rel, sing);
+        rn = firk_forward<M, S>(I, T, nest, sh, bf.mesh[cur], bf.y_of(cur), K, N, lane, scr, loc, rel, sing);
     }
     if (lane == 0) {
         B.err[b] = NAN;
@@ -759,8 +865,8 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
 }
 
 // ------------------------------------------------------------------ kernels
-// Persistent FIRK solve: each wave pulls instances from the queue.
-template <int M, int S>
+// Persistent FIRK solve: each wave pulls instances from the queue.  NEST: the nested form.
+template <int M, int S, bool NEST>
 __device__ __forceinline__ void firk_solve_queue(const BatchDev &B) {
     __shared__ Shared<M> sh;
     const int lane = threadIdx.x;
@@ -770,24 +876,27 @@ __device__ __forceinline__ void firk_solve_queue(const BatchDev &B) {
     for (;;) {
         const int item = next_item<M>(B.counter, sh, lane);
         if (item >= B.n) break;
-        firk_solve_instance<M, S>(B, T, B.order[item], sh, lane, scr, aux);
+        firk_solve_instance<M, S, NEST>(B, T, B.order[item], sh, lane, scr, aux);
     }
 }
 
 // Two register-allocation variants the autotuner chooses between: one wave per SIMD
 // (few spills) and two waves per SIMD (twice the waves to hide latency, more spills).
-template <int M, int S>
+template <int M, int S, bool NEST>
 __global__ __launch_bounds__(64, 1) void k_firk_solve(BatchDev B) {
-    firk_solve_queue<M, S>(B);
+    firk_solve_queue<M, S, NEST>(B);
 }
-template <int M, int S>
+template <int M, int S, bool NEST>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_firk_solve_w2(BatchDev B) {
-    firk_solve_queue<M, S>(B);
+    firk_solve_queue<M, S, NEST>(B);
 }
 
-// Component kernels at the packed state (mesh, guess and the packed stages init_K).  Slots of n_x = M (N (S+1) + 1) per instance:
+// Component kernels at the packed state (mesh, guess and the packed stages init_K; the
+// nested form solves the stages from k0 instead).  Slots of n_x per instance, n_x =
+// M (N (S+1) + 1) (expanded) or M (N + 1) (nested):
 //   OP_RESIDUAL    out0: the residual in the reference's layout
-//   OP_NEWTON_STEP out0: the correction in the unknown order [y_0, K_0., y_1, ...]
+//   OP_NEWTON_STEP out0: the correction in the unknown order ([y_0, K_0., y_1, ...] or the
+//                  node values)
 //   OP_DEFECT      out0: [N][M] defect vectors (interval offset), out1: norm
 template <int M, int S>
 __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
@@ -809,8 +918,12 @@ __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
         const double *mesh = as_global(B.init_mesh + off);
         const double *y = as_global(B.init_y + off * M);
         double *dy = aux, *dK = dy + cap * M, *loc = dK + cap * S * M;
-        const double *K = as_global(B.init_K + ioff * S * M);
-        const size_t xo = (size_t)M * ((size_t)(S + 1) * ioff + b);  // slot offset of n_x slots
+        const FirkNest<M> nest = firk_nest_of<M>(B, b);
+        // the stage state: the packed input, or (nested) the inner solves' stages in the
+        // slot's dK area, which the nested correction does not use
+        double *K = nest.on ? dK : const_cast<double *>(as_global(B.init_K + ioff * S * M));
+        const size_t xo = nest.on ? (size_t)M * off : (size_t)M * ((size_t)(S + 1) * ioff + b);  // slot offset
+        const int rowlen = nest.on ? M : (S + 1) * M;  // residual entries per interval
         if (op == OP_RESIDUAL) {
             double *out = B.out0 + xo;
             int nbca = 0;
@@ -825,15 +938,18 @@ __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
                 double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M];
 #pragma unroll
                 for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
-                load_stages<M, S>(K, k, Kk);
+                if (nest.on) firk_inner<M, S>(I, T, nest, ti, h, yi, Kk);
+                else load_stages<M, S>(K, k, Kk);
                 firk_interval<M, S, false>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
-                double *o = out + nbca + (size_t)k * (S + 1) * M;
+                double *o = out + nbca + (size_t)k * rowlen;
 #pragma unroll
                 for (int c = 0; c < M; ++c) o[c] = Phi[c];
+                if (!nest.on) {
 #pragma unroll
-                for (int r = 0; r < S; ++r)
+                    for (int r = 0; r < S; ++r)
 #pragma unroll
-                    for (int c = 0; c < M; ++c) o[(size_t)(r + 1) * M + c] = G[r][c];
+                        for (int c = 0; c < M; ++c) o[(size_t)(r + 1) * M + c] = G[r][c];
+                }
             }
             if (lane == 0) {
                 // BC rows: input order; TWO_POINT splits them into bca (before) and bcb (after)
@@ -842,7 +958,7 @@ __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
                     const BcLoc L = bc_locate<M>(I, k, mesh, N);
                     const double v = y[(size_t)L.node * M + sel<M>(I.bcc, k)] - sel<M>(I.bcv, k);
                     if (I.bc_kind == BVP_BC_TWO_POINT && sel<M>(I.bct, k) != I.t0)
-                        out[nbca + (size_t)N * (S + 1) * M + (ib++)] = v;
+                        out[nbca + (size_t)N * rowlen + (ib++)] = v;
                     else
                         out[ia++] = v;
                 }
@@ -850,18 +966,30 @@ __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
         } else if (op == OP_NEWTON_STEP) {
             Rel<M> rel;
             bool sing = false;
-            firk_forward<M, S>(I, T, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
-            const bool ok = !sing && firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK);
+            firk_forward<M, S>(I, T, nest, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
+            const bool ok = !sing && firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK, nest.on);
             double *out = B.out0 + xo;
             for (int q = lane; q < (N + 1) * M; q += WAVE) {
                 const int i = q / M, c = q % M;
-                out[(size_t)i * (S + 1) * M + c] = ok ? dy[q] : NAN;
+                out[(size_t)i * rowlen + c] = ok ? dy[q] : NAN;
             }
-            for (int q = lane; q < N * S * M; q += WAVE) {
+            for (int q = lane; q < (nest.on ? 0 : N * S * M); q += WAVE) {
                 const int i = q / (S * M), rc = q % (S * M);
                 out[(size_t)i * (S + 1) * M + M + rc] = ok ? dK[q] : NAN;
             }
         } else if (op == OP_DEFECT) {
+            if (nest.on) {
+                // the stages from the inner solves, as defect_estimate! solves them
+                // (FIRK/adaptivity.jl:465-517)
+                for (int k = lane; k < N; k += WAVE) {
+                    double yi[M], Kk[S][M];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) yi[c] = y[(size_t)k * M + c];
+                    firk_inner<M, S>(I, T, nest, mesh[k], mesh[k + 1] - mesh[k], yi, Kk);
+                    store_stages<M, S>(K, k, Kk);
+                }
+                __syncthreads();
+            }
             double r1, r2;
             const double dn = firk_defect_pass<M, S>(I, T, mesh, y, K, N, lane, B.abstol, nullptr,
                                                      B.out0 + ioff * M, r1, r2);
@@ -887,9 +1015,10 @@ __global__ __launch_bounds__(256) void k_firk_interp(BatchDev B, const int *qins
     const double *y = as_global(B.init_y + off * M);
     const int N = B.N0[b];
     const double t = tq[q];
-    const double *K = as_global(B.init_K + ioff * S * M);
+    const FirkNest<M> nest = firk_nest_of<M>(B, b);
+    const double *K = nest.on ? nullptr : as_global(B.init_K + ioff * S * M);
     double u[M], du[M];
-    firk_interp_at<M, S>(I, T, mesh, y, K, N, t, u, du);
+    firk_interp_at<M, S>(I, T, nest, mesh, y, K, N, t, u, du);
 #pragma unroll
     for (int c = 0; c < M; ++c) {
         uo[q * M + c] = u[c];
@@ -911,8 +1040,13 @@ __global__ __launch_bounds__(256) void k_firk_gather(BatchDev B, const long long
 
 template <int M, int S, int WPE>
 hipError_t firk_launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
-    if constexpr (WPE == 2) k_firk_solve_w2<M, S><<<blocks, 64, 0, s>>>(B);
-    else k_firk_solve<M, S><<<blocks, 64, 0, s>>>(B);
+    if (B.firk_nested) {
+        if constexpr (WPE == 2) k_firk_solve_w2<M, S, true><<<blocks, 64, 0, s>>>(B);
+        else k_firk_solve<M, S, true><<<blocks, 64, 0, s>>>(B);
+    } else {
+        if constexpr (WPE == 2) k_firk_solve_w2<M, S, false><<<blocks, 64, 0, s>>>(B);
+        else k_firk_solve<M, S, false><<<blocks, 64, 0, s>>>(B);
+    }
     return hipGetLastError();
 }
 template <int M, int S>
@@ -930,11 +1064,18 @@ hipError_t firk_launch_interp_t(const BatchDev &B, const int *qinst, const doubl
 }
 // resident FIRK solve waves per CU (registers and LDS) of variant WPE
 template <int M, int S, int WPE>
-int firk_occupancy_t() {
+int firk_occupancy_t(int nested) {
     int nb = 0;
     hipError_t e;
-    if constexpr (WPE == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S>, 64, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S>, 64, 0);
+    if (nested) {
+        if constexpr (WPE == 2)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S, true>, 64, 0);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S, true>, 64, 0);
+    } else {
+        if constexpr (WPE == 2)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S, false>, 64, 0);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S, false>, 64, 0);
+    }
     if (e != hipSuccess) return 4 * WPE;
     return nb;
 }

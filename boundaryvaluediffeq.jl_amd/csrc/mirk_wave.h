@@ -92,6 +92,8 @@ struct BatchDev {
     const void *firk_tab;
     double *stK;
     const double *init_K;
+    int firk_nested;          // the nested form: u0c holds each instance's inner-solve start k0
+    double nest_tol;          // its inner tolerance
 
     // options
     int adaptive, max_sub, max_newton, stages, transfer_mode;
@@ -137,7 +139,7 @@ hipError_t firk_launch_component(const BatchDev &B, int S, int op, int blocks, h
 hipError_t firk_launch_interp(const BatchDev &B, int S, const int *qinst, const double *tq, double *u, double *du,
                               long long nq, hipStream_t s);
 hipError_t firk_launch_gather(const BatchDev &B, int S, const long long *out_off, double *out, hipStream_t s);
-int firk_blocks_per_cu(int M, int S, int variant);  // resident solve waves per CU
+int firk_blocks_per_cu(int M, int S, int variant, int nested);  // resident solve waves per CU
 // doubles per slot of the FIRK work area for node capacity cap
 inline size_t firk_aux_doubles(int cap, int M, int S) {
     return (size_t)cap * (2 * M + 2 * S * M + S * M * (M + 1));

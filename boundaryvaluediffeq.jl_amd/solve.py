@@ -151,23 +151,29 @@ class MIRK6(_MIRK):
 
 
 class _FIRK:
-    """A FIRK collocation method (lib/BoundaryValueDiffEqFIRK/src/algorithms.jl:4-411) in
-    the expanded form, the reference's default (nested_nlsolve = false).  On the GPU
-    the nonlinear solver runs the NewtonRaphson stages of the polyalgorithm (plain, then
-    with BackTracking); nlsolve = None selects both."""
+    """A FIRK collocation method (lib/BoundaryValueDiffEqFIRK/src/algorithms.jl:4-411).
+    nested_nlsolve = false (the reference's default) solves the expanded system of node
+    values and stages; nested_nlsolve = true solves for the node values only, each
+    interval's stages from an inner NewtonRaphson (lib/BoundaryValueDiffEqFIRK/src/
+    collocation.jl:118-205), whose tolerance nested_nlsolve_kwargs["abstol"] sets
+    (default eps^(4/5)).  On the GPU the outer nonlinear solver runs the NewtonRaphson
+    stages of the polyalgorithm (plain, then with BackTracking); nlsolve = None selects
+    both."""
     code = 0
     stages = 0
     order = 0
 
     def __init__(self, nlsolve=None, jac_alg=None, nested_nlsolve=False, nested_nlsolve_kwargs=None,
                  defect_threshold=0.1, max_num_subintervals=3000):
-        if nested_nlsolve:
-            raise NotImplementedError("the nested form (nested_nlsolve = true) is not built; "
-                                      "the expanded form is the reference's default")
         self.nlsolve = nlsolve
         self.jac_alg = jac_alg
-        self.nested_nlsolve = False
-        self.nested_nlsolve_kwargs = nested_nlsolve_kwargs or {}
+        self.nested_nlsolve = bool(nested_nlsolve)
+        self.nested_nlsolve_kwargs = dict(nested_nlsolve_kwargs or {})
+        unknown = set(self.nested_nlsolve_kwargs) - {"abstol"}
+        if unknown:
+            raise ValueError(f"nested_nlsolve_kwargs: only abstol is supported, got {sorted(unknown)}")
+        if self.nested_nlsolve:
+            self.code = type(self).code + _abi.FIRK_NESTED
         self.defect_threshold = float(defect_threshold)
         self.max_num_subintervals = int(max_num_subintervals)
         self.precision = _abi.PREC_FP64
@@ -176,7 +182,7 @@ class _FIRK:
         self.warm_start = None
 
     def __repr__(self):
-        return type(self).__name__ + "()"
+        return type(self).__name__ + ("(nested_nlsolve=True)" if self.nested_nlsolve else "()")
 
 
 def _firk_class(name, code, order):
@@ -271,6 +277,7 @@ def make_opts(alg, abstol=1e-6, adaptive=True, controller=None, nlsolve_kwargs=N
         if controller is not None and not isinstance(controller, DefectControl):
             raise ValueError("FIRK methods use DefectControl")
         o.defect_threshold = alg.defect_threshold
+        o.nest_abstol = float(alg.nested_nlsolve_kwargs.get("abstol", 0.0))
         if alg.nlsolve is None:
             o.nlsolve_stages = _abi.STAGE_NEWTON | _abi.STAGE_BACKTRACKING
     return o

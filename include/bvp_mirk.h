@@ -47,10 +47,15 @@ extern "C" {
  * They run in the expanded formulation, the reference's default (nested_nlsolve =
  * false; FIRK/firk.jl:290-453): the unknowns are the node values y_i and the s stage
  * values K_i1..K_is of every interval, in the order [y_0, K_01..K_0s, y_1, ..., y_N]
- * (FIRK/firk.jl:76-83).  RadauIIa1, LobattoIIIb2 and LobattoIIIc2 do not adapt the
+ * (FIRK/firk.jl:76-83).  code + BVP_FIRK_NESTED selects the nested form (nested_nlsolve =
+ * true, FIRK/firk.jl:114-288): the unknowns are the node values, and every evaluation
+ * solves each interval's stage equations from the fixed start K0 (u0 per stage, or the
+ * mean of the guess, FIRK/utils.jl:41-83) by NewtonRaphson to bvp_opts.nest_abstol.
+ * RadauIIa1, LobattoIIIb2 and LobattoIIIc2 do not adapt the
  * mesh (FIRK/algorithms.jl:411): they need adaptive = 0.  On the GPU: M = 1..4, boundary
  * rows at t0 or t1, DefectControl, FP64, the NewtonRaphson stages of the polyalgorithm
  * (nlsolve_stages bits 0 and 1). */
+#define BVP_FIRK_NESTED     1000  /* added to a FIRK code: the nested form (below) */
 #define BVP_RADAU_IIA(s)    (100 + (s))
 #define BVP_LOBATTO_IIIA(s) (200 + (s))
 #define BVP_LOBATTO_IIIB(s) (300 + (s))
@@ -196,6 +201,9 @@ typedef struct {
                                      kernels with FP64 records (one kernel variant). */
     int32_t global_method;        /* BVP_GLOBAL_*; default HO  Core/calc_errors.jl:70 */
     double  hybrid_de, hybrid_ge; /* HybridErrorControl weights; 1, 1  Core/calc_errors.jl:100 */
+    double  nest_abstol;          /* FIRK nested form: the inner solves stop at
+                                     max|G| <= nest_abstol (nested_nlsolve_kwargs.abstol);
+                                     <= 0: eps^(4/5), NonlinearSolve's default; default 0 */
 } bvp_opts;
 
 /* Per-instance result.  mesh/u are caller-allocated with room for
@@ -325,7 +333,10 @@ int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_inst
  * component; instances back to back), or is NULL for the stages the reference
  * starts from, K_ij = y_{i+1} (extend_y, FIRK/firk.jl:76-83).  M = 1..4; boundary
  * rows at t0 or t1.  Slot sizes below use n_x = M*(N*(s+1)+1), the length of the
- * expanded unknown vector. */
+ * expanded unknown vector.  With a nested code (+ BVP_FIRK_NESTED) `stages` is not
+ * read (the inner solves give them, from K0, at the default inner tolerance), and
+ * residual and correction have the node layout, n_x = M*(N+1): [bc; Phi] or
+ * [bca; Phi; bcb] (FIRK/collocation.jl:140-428). */
 
 /* Expanded residual in the reference's layout (loss FIRK/firk.jl:1404-1455 with Phi!
  * FIRK/collocation.jl:59-93): STANDARD [bc; per interval Phi_i, G_i1..G_is],

@@ -36,7 +36,7 @@ class Opts(C.Structure):
                 ("nlsolve_stages", C.c_int32), ("transfer_mode", C.c_int32),
                 ("precision", C.c_int32), ("refine_iters", C.c_int32), ("warm_start", C.c_int32),
                 ("report_err", C.c_int32), ("controller", C.c_int32), ("global_method", C.c_int32),
-                ("hybrid_de", C.c_double), ("hybrid_ge", C.c_double)]
+                ("hybrid_de", C.c_double), ("hybrid_ge", C.c_double), ("nest_abstol", C.c_double)]
 
 
 class _Result(C.Structure):
@@ -308,6 +308,10 @@ def _stg(stages):
 
 
 def firk_nx(code, spec):
+    """Length of the state vector of a FIRK method: the expanded form's M (N (s+1) + 1),
+    the nested form's (code + 1000) M (N + 1)."""
+    if code >= 1000:
+        return spec.M * (spec.N + 1)
     s = code % 100
     return spec.M * (spec.N * (s + 1) + 1)
 
@@ -383,7 +387,7 @@ def solve(o, spec, capacity=None):
     P = _Pinned(spec)
     cap = capacity or (o.max_num_subintervals + 1)
     mesh, u = np.zeros(cap), np.zeros(cap * spec.M)
-    s = o.method % 100 if o.method >= 100 else 0
+    s = o.method % 100 if o.method >= 100 else 0  # stages (FIRK, either form)
     st = np.zeros(max(1, cap * s * spec.M))
     r = _Result()
     r.capacity = cap

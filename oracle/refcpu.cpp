@@ -1337,6 +1337,7 @@ void inverse(int n, std::vector<ld> a, std::vector<ld> &inv) {
 bool firk_tableau(int code, FTab &T) {
     T = FTab();
     T.code = code;
+    code %= 1000;  // BVP_FIRK_NESTED adds 1000
     T.kind = code / 100;
     T.s = code % 100;
     const int s = T.s;
@@ -1432,6 +1433,7 @@ struct FCtx {
     const bvp_instance &P;
     FTab T;
     int M;
+    double itol = 3.000213634488528e-13;  // nested form: the inner solves' tolerance, eps(Float64)^(4/5)
     FCtx(const bvp_instance &p, const FTab &t) : P(p), T(t), M(p.M) {}
     int s() const { return T.s; }
     size_t yoff(int i) const { return (size_t)i * (T.s + 1) * M; }
@@ -1919,7 +1921,398 @@ int firk_solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
     return 0;
 }
 
+// ---------------------------------------------------------------- nested form
+// nested_nlsolve = true (FIRK/firk.jl:114-288, FIRK/collocation.jl:140-428): the
+// unknowns are the node values only; every evaluation solves each interval's stage
+// equations G(K) = 0 for K from the fixed start K0 (__K0_on_u0, FIRK/utils.jl:41-83:
+// u0 repeated per stage, or the mean of a vector guess) and the residual is
+// Phi_i = y_{i+1} - y_i - h sum_j b_j K_j(y_i).  The inner solve restates
+// NonlinearSolve's NewtonRaphson at its default tolerance for Float64,
+// max|G| <= eps^(4/5) (or nested_nlsolve_kwargs.abstol, bvp_opts.nest_abstol), at most
+// 1000 steps; its Jacobian enters the outer one by the
+// implicit function theorem, dK/dy_i = W^-1 [J_1; ...; J_s] (what ForwardDiff through
+// the nested solve computes).
+bool firk_is_nested(int code) { return code >= 1000; }
+
+// K0 of an instance (M values)
+void firk_k0(const bvp_instance &P, std::vector<double> &k0) {
+    const int M = P.M;
+    k0.assign(M, 0.0);
+    if (P.u0) {
+        for (int c = 0; c < M; ++c) k0[c] = P.u0[c];
+    } else if (P.guess) {
+        for (int k = 0; k <= P.N; ++k)
+            for (int c = 0; c < M; ++c) k0[c] += P.guess[(size_t)k * M + c];
+        for (int c = 0; c < M; ++c) k0[c] /= (P.N + 1);
+    }
+}
+
+// W = I - h (A (x) J) blockwise at the stage points of K (s M x s M, row-major), the
+// stage residuals G and the stacked stage Jacobians Jh (s M x M)
+void firk_stage_system(const FCtx &C, double ti, double h, const double *yi, const double *K, std::vector<double> &W,
+                       std::vector<double> &G, std::vector<double> &Jh) {
+    const int M = C.M, s = C.s(), n = s * M;
+    W.assign((size_t)n * n, 0.0);
+    G.assign(n, 0.0);
+    Jh.assign((size_t)n * M, 0.0);
+    std::vector<double> Y(M), f(M), J((size_t)M * M);
+    for (int r = 0; r < s; ++r) {
+        for (int c = 0; c < M; ++c) {
+            double a = 0;
+            for (int j = 0; j < s; ++j) a += K[j * M + c] * C.T.A[r][j];
+            Y[c] = h * a + yi[c];
+        }
+        const double t = ti + C.T.c[r] * h;
+        f_eval(C.P, t, Y.data(), f.data());
+        J_eval(C.P, t, Y.data(), J.data());
+        for (int a = 0; a < M; ++a) {
+            G[r * M + a] = f[a] - K[r * M + a];
+            for (int c = 0; c < M; ++c) {
+                Jh[(size_t)(r * M + a) * M + c] = J[(size_t)a * M + c];
+                for (int j = 0; j < s; ++j)
+                    W[(size_t)(r * M + a) * n + j * M + c] =
+                        ((r == j && a == c) ? 1.0 : 0.0) - h * C.T.A[r][j] * J[(size_t)a * M + c];
+            }
+        }
+    }
+}
+
+// dense solve W X = B (n x n, nr right-hand sides, row-major), partial pivoting
+bool dense_lu_solve(int n, int nr, std::vector<double> W, std::vector<double> &B) {
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        for (int i = j + 1; i < n; ++i) if (std::fabs(W[(size_t)i * n + j]) > std::fabs(W[(size_t)p * n + j])) p = i;
+        if (W[(size_t)p * n + j] == 0.0) return false;
+        if (p != j) {
+            for (int c = 0; c < n; ++c) std::swap(W[(size_t)j * n + c], W[(size_t)p * n + c]);
+            for (int c = 0; c < nr; ++c) std::swap(B[(size_t)j * nr + c], B[(size_t)p * nr + c]);
+        }
+        for (int i = j + 1; i < n; ++i) {
+            const double m = W[(size_t)i * n + j] / W[(size_t)j * n + j];
+            for (int c = j + 1; c < n; ++c) W[(size_t)i * n + c] -= m * W[(size_t)j * n + c];
+            for (int c = 0; c < nr; ++c) B[(size_t)i * nr + c] -= m * B[(size_t)j * nr + c];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i)
+        for (int c = 0; c < nr; ++c) {
+            double v = B[(size_t)i * nr + c];
+            for (int k = i + 1; k < n; ++k) v -= W[(size_t)i * n + k] * B[(size_t)k * nr + c];
+            B[(size_t)i * nr + c] = v / W[(size_t)i * n + i];
+        }
+    return true;
+}
+
+// The interval's stage values K (s M) from K0: NewtonRaphson, K <- K + W^-1 G until
+// max|G| <= C.itol (checked before every step; NaN stops), at most 1000 steps or a
+// singular W.
+void firk_inner_solve(const FCtx &C, double ti, double h, const double *yi, const std::vector<double> &k0,
+                      double *K) {
+    const int M = C.M, s = C.s(), n = s * M;
+    for (int r = 0; r < s; ++r)
+        for (int c = 0; c < M; ++c) K[r * M + c] = k0[c];
+    std::vector<double> W, G, Jh;
+    for (int it = 0; it < 1000; ++it) {
+        firk_stage_system(C, ti, h, yi, K, W, G, Jh);
+        double mx = 0;
+        bool bad = false;
+        for (double v : G) { bad |= std::isnan(v); mx = std::max(mx, std::fabs(v)); }
+        if (bad || !(mx > C.itol)) return;
+        if (!dense_lu_solve(n, 1, W, G)) return;
+        for (int q = 0; q < n; ++q) K[q] += G[q];
+    }
+}
+
+struct FNEval {
+    std::vector<double> phi;  // [N][M]
+    std::vector<double> K;    // [N][s][M] converged stages
+    std::vector<double> bc;
+};
+
+void firk_nested_evaluate(const FCtx &C, int N, const double *mesh, const double *y, const std::vector<double> &k0,
+                          const std::vector<BcRow> &rows, FNEval &E) {
+    const int M = C.M, s = C.s();
+    E.phi.assign((size_t)N * M, 0.0);
+    E.K.assign((size_t)N * s * M, 0.0);
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        double *K = &E.K[(size_t)i * s * M];
+        firk_inner_solve(C, mesh[i], h, y + (size_t)i * M, k0, K);
+        for (int c = 0; c < M; ++c) {
+            double a = 0;
+            for (int j = 0; j < s; ++j) a += K[j * M + c] * C.T.b[j];
+            E.phi[(size_t)i * M + c] = -h * a + (y[(size_t)(i + 1) * M + c] - y[(size_t)i * M + c]);
+        }
+    }
+    E.bc.assign(rows.size(), 0.0);
+    for (size_t k = 0; k < rows.size(); ++k) E.bc[k] = y[(size_t)rows[k].node * M + rows[k].comp] - C.P.bc_val[k];
+}
+
+double firk_nested_max_abs(const FNEval &E) {
+    double m = 0;
+    for (double v : E.phi) { if (std::isnan(v)) return NAN; m = std::max(m, std::fabs(v)); }
+    for (double v : E.bc) { if (std::isnan(v)) return NAN; m = std::max(m, std::fabs(v)); }
+    return m;
+}
+double firk_nested_sumsq(const FNEval &E) {
+    double s = 0;
+    for (double v : E.phi) s += v * v;
+    for (double v : E.bc) s += v * v;
+    return s;
+}
+
+// the Newton matrix of the nested residual (rows: BC rows on node 0, Phi_i, BC rows on
+// node N) with S_i = -(I + h sum_j b_j KJ_j), KJ = W^-1 [J_r] at the converged stages
+void firk_nested_jacobian(const FCtx &C, int N, const double *mesh, const double *y, const std::vector<BcRow> &rows,
+                          const FNEval &E, std::vector<Trip> &J, std::vector<double> &rhs) {
+    const int M = C.M, s = C.s(), n = s * M;
+    J.clear();
+    rhs.clear();
+    int r = 0;
+    auto bc_rows = [&](int node) {
+        for (size_t k = 0; k < rows.size(); ++k)
+            if (rows[k].node == node) {
+                J.push_back({r, node * M + rows[k].comp, 1.0});
+                rhs.push_back(-E.bc[k]);
+                ++r;
+            }
+    };
+    bc_rows(0);
+    std::vector<double> W, G, Jh;
+    for (int i = 0; i < N; ++i) {
+        const double h = mesh[i + 1] - mesh[i];
+        firk_stage_system(C, mesh[i], h, y + (size_t)i * M, &E.K[(size_t)i * s * M], W, G, Jh);
+        dense_lu_solve(n, M, W, Jh);  // Jh <- KJ
+        for (int a = 0; a < M; ++a) {
+            for (int c = 0; c < M; ++c) {
+                double v = 0;
+                for (int j = 0; j < s; ++j) v += C.T.b[j] * Jh[(size_t)(j * M + a) * M + c];
+                J.push_back({r, i * M + c, -((a == c) ? 1.0 : 0.0) - h * v});
+            }
+            J.push_back({r, (i + 1) * M + a, 1.0});
+            rhs.push_back(-E.phi[(size_t)i * M + a]);
+            ++r;
+        }
+    }
+    if (N > 0) bc_rows(N);
+}
+
+int firk_nested_dir(const FCtx &C, int N, const double *mesh, const double *y, const std::vector<BcRow> &rows,
+                    const FNEval &E, double *delta) {
+    std::vector<Trip> J;
+    std::vector<double> rhs;
+    firk_nested_jacobian(C, N, mesh, y, rows, E, J, rhs);
+    const int n = C.M * (N + 1);
+    if ((int)rhs.size() != n) return 1;
+    int kl = 0, ku = 0;
+    for (const Trip &t : J) { kl = std::max(kl, t.r - t.c); ku = std::max(ku, t.c - t.r); }
+    Band B;
+    B.init(n, kl, ku);
+    for (const Trip &t : J) B.at(t.r, t.c) += t.v;
+    if (B.factor()) return 1;
+    B.solve(rhs.data());
+    std::memcpy(delta, rhs.data(), sizeof(double) * n);
+    return 0;
+}
+
+// the NewtonRaphson stages on the nested residual (the outer solve)
+int firk_nested_newton_stage(const FCtx &C, int N, const double *mesh, std::vector<double> &y,
+                             const std::vector<double> &k0, const std::vector<BcRow> &rows, double tol, int maxiters,
+                             bool linesearch, NlOut &out, FNEval &E) {
+    const size_t n = y.size();
+    std::vector<double> delta(n), yt(n);
+    FNEval Et;
+    for (int it = 0;; ++it) {
+        firk_nested_evaluate(C, N, mesh, y.data(), k0, rows, E);
+        out.fnorm = firk_nested_max_abs(E);
+        if (!std::isfinite(out.fnorm)) return BVP_RET_UNSTABLE;
+        if (out.fnorm <= tol) return BVP_RET_SUCCESS;
+        if (it >= maxiters) return BVP_RET_MAXITERS;
+        if (firk_nested_dir(C, N, mesh, y.data(), rows, E, delta.data())) return BVP_RET_FAILURE;
+        out.iters++;
+        double alpha = 1.0;
+        if (linesearch) {
+            const double ss = firk_nested_sumsq(E);
+            auto phi = [&](double a) {
+                for (size_t i = 0; i < n; ++i) yt[i] = y[i] + a * delta[i];
+                firk_nested_evaluate(C, N, mesh, yt.data(), k0, rows, Et);
+                return 0.5 * firk_nested_sumsq(Et);
+            };
+            double pa;
+            if (backtracking(phi, 0.5 * ss, -ss, &alpha, &pa)) return BVP_RET_FAILURE;
+        }
+        for (size_t i = 0; i < n; ++i) y[i] += alpha * delta[i];
+    }
+}
+
+// the quintic interpolant of interval j at t with stages K (s M) and step hq (the
+// transfer passes the reference's scaled step, FIRK/adaptivity.jl:98-104)
+void firk_quintic_at(const FCtx &C, const double *mesh, const double *y, int j, const double *K, double hq, double t,
+                     double *u, double *du) {
+    const int M = C.M, s = C.s();
+    const double tau = t - mesh[j];
+    const double *yi = y + (size_t)j * M, *yj = y + (size_t)(j + 1) * M;
+    std::vector<double> fi(M), fj(M), z(M), zp(M);
+    f_eval(C.P, mesh[j], yi, fi.data());
+    f_eval(C.P, mesh[j + 1], yj, fj.data());
+    for (int c = 0; c < M; ++c) {
+        double coef[7];
+        for (int k = 0; k < s; ++k) {
+            double a = 0;
+            for (int jj = 0; jj < s; ++jj) a += C.T.Q[k][jj] * K[jj * M + c];
+            coef[k] = a / std::pow(hq, (double)k);
+        }
+        const double th = 0.5 * hq;
+        double v = 0, dv = 0, p = 1;
+        for (int k = 0; k < s; ++k) {
+            dv += (k + 1) * coef[k] * p;
+            p *= th;
+            v += coef[k] * p;
+        }
+        z[c] = yi[c] + v;
+        zp[c] = dv;
+    }
+    for (int c = 0; c < M; ++c) {
+        const double bk[6] = {yi[c], yj[c], z[c], fi[c], fj[c], zp[c]};
+        double cf[6];
+        quintic(hq, bk, cf);
+        double v = 0, dv = 0;
+        for (int m = 0; m < 6; ++m) {
+            v += cf[m] * std::pow(tau, (double)m);
+            if (m > 0) dv += m * cf[m] * std::pow(tau, (double)(m - 1));
+        }
+        u[c] = v;
+        if (du) du[c] = dv;
+    }
+}
+
+int firk_nested_solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
+    FTab T;
+    int rc = firk_check(P, o.method, T);
+    if (rc) return rc;
+    if (o.adaptive && !T.adaptive) return BVP_ENOTSUP;
+    if (o.controller != BVP_CONTROL_DEFECT) return BVP_ENOTSUP;
+    FCtx C(P, T);
+    if (o.nest_abstol > 0) C.itol = o.nest_abstol;
+    std::vector<double> mesh, y, k0;
+    if ((rc = instance_mesh_and_guess(P, mesh, y))) return rc;
+    firk_k0(P, k0);
+    const double tol = o.newton_abstol > 0 ? o.newton_abstol : o.abstol;
+    const int M = P.M, s = T.s;
+    int N = P.N;
+    int info = BVP_RET_SUCCESS, nl_ret = BVP_RET_SUCCESS;
+    double defect_norm = 2 * o.abstol, last_defect = NAN;
+    int outer = 0, newton_iters = 0, fb = 0;
+    std::vector<double> errors, new_mesh, x;
+    std::vector<BcRow> rows;
+    FNEval E;
+    const int MAX_OUTER = 64;
+    bool eval_valid = false;
+    for (;;) {
+        ++outer;
+        if (resolve_bc(P, N, mesh.data(), rows)) return BVP_EINVAL;
+        const std::vector<double> y0 = y;
+        NlOut nl{BVP_RET_MAXITERS, 0, 0, -1};
+        bool first = true;
+        for (int st = 0; st < 2; ++st) {
+            if (!(o.nlsolve_stages & (1 << st))) continue;
+            if (!first) y = y0;
+            first = false;
+            nl.stage = st;
+            nl.ret = firk_nested_newton_stage(C, N, mesh.data(), y, k0, rows, tol, o.max_newton_iters, st == 1, nl, E);
+            if (nl.ret == BVP_RET_SUCCESS) break;
+        }
+        eval_valid = true;
+        newton_iters += nl.iters;
+        if (nl.stage == 1) fb |= 1;
+        nl_ret = nl.ret;
+        defect_norm = 2 * o.abstol;
+        info = nl.ret;
+        if (!o.adaptive) break;
+        firk_extend(C, N, y.data(), E.K.data(), x);  // nodes and the converged stages
+        if (info == BVP_RET_SUCCESS) {
+            defect_norm = firk_defect(C, N, mesh.data(), x.data(), errors);
+            last_defect = defect_norm;
+            if (defect_norm > o.defect_threshold || std::isnan(defect_norm)) info = BVP_RET_FAILURE;
+        }
+        if (info == BVP_RET_SUCCESS) {
+            if (defect_norm > o.abstol) {
+                info = mesh_select(T.order, o.abstol, o.max_num_subintervals, mesh, errors, M, new_mesh);
+                if (info == BVP_RET_SUCCESS) {
+                    // interp_eval! (FIRK/adaptivity.jl:90-135): the old interpolant with h scaled
+                    // by the growth factor of the node count when the mesh grows
+                    const int Nn = (int)new_mesh.size() - 1;
+                    const double lf = (double)Nn / (double)N;
+                    std::vector<double> yn((size_t)(Nn + 1) * M);
+                    for (int k = 0; k <= Nn; ++k) {
+                        const int j = interval_of(mesh.data(), N, new_mesh[k]);
+                        const double hq = (mesh[j + 1] - mesh[j]) * (lf > 1 ? lf : 1.0);
+                        firk_quintic_at(C, mesh.data(), y.data(), j, &E.K[(size_t)j * s * M], hq, new_mesh[k],
+                                        &yn[(size_t)k * M], nullptr);
+                    }
+                    mesh = new_mesh;
+                    N = Nn;
+                    y = yn;
+                    eval_valid = false;
+                }
+            }
+        } else {
+            if (2 * N > o.max_num_subintervals) {
+                info = BVP_RET_FAILURE;
+            } else {
+                std::vector<double> hm;
+                half_mesh(mesh, hm);
+                mesh = hm;
+                N = (int)mesh.size() - 1;
+                y.assign((size_t)(N + 1) * M, 0.0);
+                eval_valid = false;
+                info = BVP_RET_SUCCESS;
+            }
+        }
+        if (!(info == BVP_RET_SUCCESS && defect_norm > o.abstol)) break;
+        if (outer >= MAX_OUTER) { info = BVP_RET_FAILURE; break; }
+    }
+    res.retcode = (nl_ret == BVP_RET_SUCCESS) ? info : nl_ret;
+    res.N_final = N;
+    res.newton_iters = newton_iters;
+    res.outer_iters = outer;
+    res.fallback_used = fb;
+    res.defect_norm = last_defect;
+    res.err_vs_fp64 = NAN;
+    if (resolve_bc(P, N, mesh.data(), rows) == 0) {
+        if (!eval_valid) firk_nested_evaluate(C, N, mesh.data(), y.data(), k0, rows, E);
+        res.resid_norm = firk_nested_max_abs(E);
+    }
+    if (res.capacity < N + 1) return BVP_EINVAL;
+    if (res.mesh) std::memcpy(res.mesh, mesh.data(), sizeof(double) * (N + 1));
+    if (res.u) std::memcpy(res.u, y.data(), sizeof(double) * (size_t)M * (N + 1));
+    if (res.stages) std::memcpy(res.stages, E.K.data(), sizeof(double) * (size_t)N * s * M);
+    return 0;
+}
+
+// [bc; Phi] (STANDARD) or [bca; Phi; bcb] (TWO_POINT) of node-layout residuals
+void flatten_residual_phi(const bvp_instance &P, const std::vector<BcRow> &rows, const std::vector<double> &phi,
+                          const std::vector<double> &bc, double *out) {
+    size_t o = 0;
+    const bool tp = P.bc_kind == BVP_BC_TWO_POINT;
+    for (size_t k = 0; k < rows.size(); ++k) if (!tp || !rows[k].side_b) out[o++] = bc[k];
+    for (double v : phi) out[o++] = v;
+    if (tp) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) out[o++] = bc[k];
+}
+
+// nested codes (BVP_FIRK_NESTED): the state is (mesh, guess); stages are ignored (the
+// inner solves compute them) and the vectors have the node layout M (N + 1)
+int firk_nested_state(const FCtx &C, std::vector<double> &mesh, std::vector<double> &y, std::vector<double> &k0,
+                      std::vector<BcRow> &rows, FNEval &E) {
+    int rc = instance_mesh_and_guess(C.P, mesh, y);
+    if (rc) return rc;
+    if ((rc = resolve_bc(C.P, C.P.N, mesh.data(), rows))) return rc;
+    firk_k0(C.P, k0);
+    firk_nested_evaluate(C, C.P.N, mesh.data(), y.data(), k0, rows, E);
+    return 0;
+}
+
 int solve_any(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
+    if (o.method >= 1000) return firk_nested_solve_one(P, o, res);
     return o.method >= 100 ? firk_solve_one(P, o, res) : solve_one(P, o, res);
 }
 
@@ -1948,6 +2341,7 @@ void ref_opts_default(bvp_opts *o) {
     o->global_method = BVP_GLOBAL_HO;
     o->hybrid_de = 1.0;
     o->hybrid_ge = 1.0;
+    o->nest_abstol = 0.0;
 }
 
 // The global error estimate of opts.global_method at (mesh, guess): per-interval
@@ -2226,6 +2620,14 @@ int ref_firk_residual(int code, const bvp_instance *P, const double *stages, dou
     int rc = firk_check(*P, code, T);
     if (rc) return rc;
     FCtx C(*P, T);
+    if (firk_is_nested(code)) {
+        std::vector<double> mesh, y, k0;
+        std::vector<BcRow> rows;
+        FNEval E;
+        if ((rc = firk_nested_state(C, mesh, y, k0, rows, E))) return rc;
+        flatten_residual_phi(*P, rows, E.phi, E.bc, resid);
+        return 0;
+    }
     std::vector<double> mesh, x;
     std::vector<BcRow> rows;
     if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
@@ -2241,6 +2643,33 @@ int ref_firk_dense_system(int code, const bvp_instance *P, const double *stages,
     int rc = firk_check(*P, code, T);
     if (rc) return rc;
     FCtx C(*P, T);
+    if (firk_is_nested(code)) {
+        // node layout: J is M (N+1) square, rows in the reference's order
+        std::vector<double> mesh, y, k0;
+        std::vector<BcRow> rows;
+        FNEval E;
+        if ((rc = firk_nested_state(C, mesh, y, k0, rows, E))) return rc;
+        flatten_residual_phi(*P, rows, E.phi, E.bc, F);
+        std::vector<Trip> Jt;
+        std::vector<double> rhs;
+        firk_nested_jacobian(C, P->N, mesh.data(), y.data(), rows, E, Jt, rhs);
+        const int M = C.M, N = P->N, n = M * (N + 1);
+        const bool tp = P->bc_kind == BVP_BC_TWO_POINT;
+        std::vector<int> bc_ref(rows.size());
+        int r = 0;
+        for (size_t k = 0; k < rows.size(); ++k) if (!tp || !rows[k].side_b) bc_ref[k] = r++;
+        const int top = r;
+        r += N * M;
+        if (tp) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) bc_ref[k] = r++;
+        std::vector<int> map(n);
+        r = 0;
+        for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == 0) map[r++] = bc_ref[k];
+        for (int i = 0; i < N * M; ++i) map[r++] = top + i;
+        for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == N) map[r++] = bc_ref[k];
+        std::memset(J, 0, sizeof(double) * (size_t)n * n);
+        for (const Trip &t : Jt) J[(size_t)map[t.r] * n + t.c] += t.v;
+        return 0;
+    }
     std::vector<double> mesh, x;
     std::vector<BcRow> rows;
     if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
@@ -2280,6 +2709,13 @@ int ref_firk_newton_step(int code, const bvp_instance *P, const double *stages, 
     int rc = firk_check(*P, code, T);
     if (rc) return rc;
     FCtx C(*P, T);
+    if (firk_is_nested(code)) {
+        std::vector<double> mesh, y, k0;
+        std::vector<BcRow> rows;
+        FNEval E;
+        if ((rc = firk_nested_state(C, mesh, y, k0, rows, E))) return rc;
+        return firk_nested_dir(C, P->N, mesh.data(), y.data(), rows, E, delta) ? 1 : 0;
+    }
     std::vector<double> mesh, x;
     std::vector<BcRow> rows;
     if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
@@ -2295,7 +2731,14 @@ int ref_firk_defect(int code, const bvp_instance *P, const double *stages, doubl
     FCtx C(*P, T);
     std::vector<double> mesh, x, err;
     std::vector<BcRow> rows;
-    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+    if (firk_is_nested(code)) {
+        std::vector<double> y, k0;
+        FNEval E;
+        if ((rc = firk_nested_state(C, mesh, y, k0, rows, E))) return rc;
+        firk_extend(C, P->N, y.data(), E.K.data(), x);
+    } else if ((rc = firk_state(C, stages, mesh, x, rows))) {
+        return rc;
+    }
     *defect_norm = firk_defect(C, P->N, mesh.data(), x.data(), err);
     std::memcpy(errors, err.data(), sizeof(double) * err.size());
     return 0;
@@ -2309,7 +2752,16 @@ int ref_firk_interpolate(int code, const bvp_instance *P, const double *stages, 
     FCtx C(*P, T);
     std::vector<double> mesh, x;
     std::vector<BcRow> rows;
-    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+    if (firk_is_nested(code)) {
+        // interpolant! of FIRKNestedInterpolation (FIRK/interpolation.jl:127-181): the stages
+        // of the point's interval from the inner solve
+        std::vector<double> y, k0;
+        FNEval E;
+        if ((rc = firk_nested_state(C, mesh, y, k0, rows, E))) return rc;
+        firk_extend(C, P->N, y.data(), E.K.data(), x);
+    } else if ((rc = firk_state(C, stages, mesh, x, rows))) {
+        return rc;
+    }
     for (int q = 0; q < nt; ++q)
         firk_interp(C, P->N, mesh.data(), x.data(), t[q], u + (size_t)q * C.M, du ? du + (size_t)q * C.M : nullptr);
     return 0;

@@ -81,8 +81,14 @@ def test_firk_method_codes_match_header():
     assert bvp.RadauIIa5.order == 9 and bvp.LobattoIIIb3.order == 4 and bvp.RadauIIa7().stages == 7
     o = bvp.make_opts(bvp.LobattoIIIa4(defect_threshold=0.2), abstol=1e-7)
     assert o.method == 204 and o.nlsolve_stages == 3 and o.defect_threshold == 0.2 and o.abstol == 1e-7
-    with pytest.raises(NotImplementedError):
-        bvp.RadauIIa3(nested_nlsolve=True)
+    nested = bvp.RadauIIa3(nested_nlsolve=True, nested_nlsolve_kwargs={"abstol": 1e-10})
+    assert nested.code == _abi.FIRK_NESTED + _abi.RADAU_IIA(3) and _abi.is_firk_nested(nested.code)
+    assert _abi.firk_stages(nested.code) == 3 and bvp.RadauIIa3().code == _abi.RADAU_IIA(3)
+    o = bvp.make_opts(nested, abstol=1e-7)
+    assert o.method == 1103 and o.nest_abstol == 1e-10
+    assert bvp.make_opts(bvp.RadauIIa3(nested_nlsolve=True), abstol=1e-7).nest_abstol == 0.0
+    with pytest.raises(ValueError):
+        bvp.RadauIIa3(nested_nlsolve=True, nested_nlsolve_kwargs={"maxiters": 5})
 
 
 def test_struct_layouts(tmp_path):

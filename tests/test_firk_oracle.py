@@ -354,3 +354,95 @@ def test_interior_standard_rows_rejected():
     spec = cases.pendulum(0.05)
     with pytest.raises(RuntimeError):
         O.solve(O.opts(103), spec)
+
+
+# --------------------------------------------------------------- nested form
+# method code + 1000: nested_nlsolve = true (FIRK/firk.jl:114-288)
+NESTED = 1000
+
+
+@pytest.mark.parametrize("code", [102, 103, 105, 204, 303, 404], ids=_name)
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_nested_jacobian_vs_finite_differences(code, name):
+    """The nested Jacobian (implicit-function derivative of the inner solves) against
+    central differences of the nested residual, whose inner solves stop at
+    eps^(4/5): agreement to 1e-5 relative."""
+    rng = np.random.default_rng(7)
+    spec = cases.random_state(CASES[name](), rng, scale=0.1)
+    J, F = O.firk_dense_system(code + NESTED, spec)
+    assert np.allclose(F, O.firk_residual(code + NESTED, spec), rtol=0, atol=0)
+    y0 = spec.full_guess().copy()
+    m = spec.full_mesh()
+    n = y0.size
+    Jfd = np.zeros((n, n))
+    eps = 1e-6
+    for j in range(n):
+        yp, ym = y0.copy(), y0.copy()
+        yp[j] += eps
+        ym[j] -= eps
+        Jfd[:, j] = (O.firk_residual(code + NESTED, spec.with_state(m, yp)) -
+                     O.firk_residual(code + NESTED, spec.with_state(m, ym))) / (2 * eps)
+    assert np.max(np.abs(J - Jfd)) < 1e-5 * max(1.0, np.max(np.abs(J)))
+    d = O.firk_newton_step(code + NESTED, spec)
+    assert np.max(np.abs(d - np.linalg.solve(J, -F))) <= 1e-10 * max(1.0, np.max(np.abs(d)))
+
+
+def test_nested_residual_is_expanded_continuity_at_solved_stages():
+    """The nested residual equals the expanded form's continuity rows at the stages the
+    inner solves return, and those stages zero the expanded stage rows."""
+    rng = np.random.default_rng(8)
+    spec = cases.random_state(cases.bratu(2.0, N=6), rng, scale=0.1)
+    code = 103
+    Fn = O.firk_residual(code + NESTED, spec)
+    st = spec.with_state(spec.full_mesh(), spec.full_guess())
+    # the inner solves' stages through a nested solve with no outer step
+    sol = O.solve(O.opts(code + NESTED, adaptive=0, max_newton_iters=0, nlsolve_stages=3), spec)
+    Fe = O.firk_residual(code, st, sol.stages)
+    s, M, N = 3, 2, spec.N
+    nb = 1  # one bca row
+    col = Fe[nb:nb + N * (s + 1) * M].reshape(N, s + 1, M)
+    assert np.max(np.abs(col[:, 1:, :])) <= 3.1e-13
+    assert np.max(np.abs(col[:, 0, :].reshape(-1) - Fn[nb:nb + N * M])) <= 1e-15
+
+
+@pytest.mark.parametrize("code", ALL, ids=_name)
+def test_nested_and_expanded_reach_the_same_solution(code):
+    """Both forms solve the same discrete collocation equations: on a fixed mesh the
+    nested solve's nodes equal the expanded solve's to 1e-10 (Newton tolerance 1e-12;
+    the inner solves stop at eps^(4/5))."""
+    spec = cases.bratu(2.5, N=12)
+    a = O.solve(O.opts(code, adaptive=0, newton_abstol=1e-12, nlsolve_stages=3), spec)
+    b = O.solve(O.opts(code + NESTED, adaptive=0, newton_abstol=1e-12, nlsolve_stages=3), spec)
+    assert a.retcode == b.retcode == 0
+    assert np.max(np.abs(a.u - b.u)) < 1e-10
+    assert np.max(np.abs(a.stages - b.stages)) < 1e-10
+
+
+@pytest.mark.parametrize("code", [102, 103, 105, 107, 203, 304, 404], ids=_name)
+def test_nested_interpolation_reference_case(code):
+    """FIRK test/nested/firk_basic_tests.jl:6-96: the expanded case's values with
+    nested_nlsolve = true (dt = 0.001, adaptive)."""
+    spec = _linear_interp_problem()
+    sol = O.solve(O.opts(code + NESTED, nlsolve_stages=3), spec)
+    assert sol.retcode == 0
+    st = spec.with_state(sol.t, sol.u.reshape(-1))
+    u, du = O.firk_interpolate(code + NESTED, st, [0.001, 0.04])
+    assert np.max(np.abs(u[0] - [0.998687464, -1.312035941])) < 1e-6
+    ua, dua = _interp_analytic(0.04)
+    assert np.max(np.abs(u[1] - ua)) < 1e-6 and np.max(np.abs(du[1] - dua)) < 1e-6
+
+
+@pytest.mark.parametrize("code", [103, 205, 404], ids=_name)
+def test_nested_adaptive_matches_expanded_meshes(code):
+    """Adaptive nested solves (whose transfer restates interp_eval! with its scaled step)
+    take the expanded solves' mesh passes to the same mesh size and nearly the same
+    nodes: the restart changes Newton's path, not the converged solutions the mesh
+    selection reads (which agree to the inner solves' eps^(4/5); redistribution
+    amplifies that to 4e-7 in the nodes here)."""
+    for lam in (0.5, 2.0):
+        spec = cases.bratu(lam, N=8)
+        a = O.solve(O.opts(code, abstol=1e-8, newton_abstol=1e-12, nlsolve_stages=3), spec)
+        b = O.solve(O.opts(code + NESTED, abstol=1e-8, newton_abstol=1e-12, nlsolve_stages=3), spec)
+        assert a.retcode == b.retcode == 0 and a.N == b.N and a.outer_iters == b.outer_iters
+        assert np.max(np.abs(a.t - b.t)) < 1e-5 and np.max(np.abs(a.u - b.u)) < 1e-5
+        assert b.defect_norm <= 1e-8
