@@ -414,7 +414,8 @@ def bench_firk(args, _abi, _lib, synthetic, learned):
     batch.close()
     ok = n_solved(res, abstol)
     value = ok / (t["solve_ms"] * 1e-3)
-    out = {"workload": f"bratu{n // 1024}k, FIRK method {code} ({s_} stages, expanded form): {n} Bratu BVPs, "
+    out = {"workload": f"bratu{n // 1024}k, FIRK method {code} ({s_} stages, {'nested' if code >= _abi.FIRK_NESTED else 'expanded'} "
+                       f"form): {n} Bratu BVPs, "
                        f"lambda~U(0.1,3.4), N~U{{64..1024}}, abstol {abstol}, Newton tolerance {args.newton_abstol}, "
                        "constant zero guess, FP64",
            "method": code, "value": value, "unit": "solves/s (device solve time)", "solve_ms": t["solve_ms"],
@@ -480,7 +481,8 @@ def main():
     ap.add_argument("--anchors", type=int, default=4,
                     help="network evaluations per instance (Hermite-interpolated to the nodes); 0: every node")
     ap.add_argument("--firk-method", type=int, default=103,
-                    help="FIRK method code of --workload firk (100 + s Radau IIA, 200/300/400 + s Lobatto IIIA/B/C)")
+                    help="FIRK method code of --workload firk (100 + s Radau IIA, 200/300/400 + s Lobatto IIIA/B/C; "
+                         "+ 1000: the nested form)")
     ap.add_argument("--workload", default="bratu", choices=["bratu", "lintanh", "learned", "singpert", "firk"],
                     help="bratu: 16,384 Bratu BVPs (MIRK4, FP64, learned warm start); lintanh: 2,048 n=32 "
                          "linear+tanh BVPs (MIRK6, FP32 factors + FP64 refinement); learned: warm start on "
