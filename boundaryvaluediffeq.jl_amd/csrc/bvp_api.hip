@@ -308,6 +308,10 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         }
         D.firk_tab = dtab;
         D.firk_nested = nested;
+        for (int i = 0; i < n; ++i)
+            if (inst[i].bc_kind == BVP_BC_STANDARD)
+                for (int k = 0; k < M; ++k)
+                    D.firk_irows |= inst[i].bc_t[k] != inst[i].t0 && inst[i].bc_t[k] != inst[i].t1;
         D.nest_tol = o->nest_abstol > 0 ? o->nest_abstol : 3.000213634488528e-13;  // eps^(4/5)
     }
     // options

@@ -85,10 +85,6 @@ int validate_opts(const bvp_opts *o, int32_t n, const bvp_instance *inst, int M,
             return fail(err, BVP_EINVAL, "unknown FIRK method %d (BVP_RADAU_IIA(1, 2, 3, 5, 7), "
                                          "BVP_LOBATTO_IIIA/B/C(2..5))", o->method);
         if (blk || M > 4) return fail(err, BVP_ENOTSUP, "FIRK methods run on the M <= 4 kernels");
-        for (int i = 0; i < n; ++i)
-            for (int k = 0; k < M; ++k)
-                if (inst[i].bc_t[k] != inst[i].t0 && inst[i].bc_t[k] != inst[i].t1)
-                    return fail(err, BVP_ENOTSUP, "instance %d: FIRK boundary rows must sit at t0 or t1", i);
         if (o->adaptive && !T.adaptive)
             return fail(err, BVP_ENOTSUP, "RadauIIa1, LobattoIIIb2 and LobattoIIIc2 do not adapt the mesh: "
                                           "set adaptive = 0");

@@ -258,6 +258,7 @@ __device__ inline void load_stages(const double *K, int k, double (&Kk)[S][M]) {
 // theorem, dK/dy_i = W^-1 [J_1; ...; J_s] -- the KJ of the condensation.
 template <int M> struct FirkNest {
     bool on;
+    bool irows;  // boundary rows inside the span may occur (EvalSol rows)
     double tol;
     double k0[M];
 };
@@ -267,6 +268,7 @@ __device__ inline FirkNest<M> firk_nest_of(const BatchDev &B, int b) {
     FirkNest<M> n;
     n.on = B.firk_nested != 0;
     n.tol = B.nest_tol;
+    n.irows = true;
 #pragma unroll
     for (int c = 0; c < M; ++c) n.k0[c] = n.on ? B.u0c[(size_t)b * M + c] : 0.0;
     return n;
@@ -321,14 +323,233 @@ __device__ inline void store_stages(double *K, int k, const double (&Kk)[S][M]) 
         for (int c = 0; c < M; ++c) K[((size_t)k * S + r) * M + c] = Kk[r][c];
 }
 
-// Boundary rows of the FIRK path: every row reads a node value (t0 or t1).
-template <int M>
-__device__ inline void firk_bc_rows(const Inst<M> &I, Shared<M> &sh, const double *mesh, const double *y, int N,
-                                    int lane) {
+// The quintic through value and slope at 0, h, h/2 (get_S_coeffs / s_constraints_block,
+// FIRK/adaptivity.jl:183-227), solved with partial pivoting as the 6 x 6 system of the
+// reference; bk = (y(0), y(h), y(h/2), y'(0), y'(h), y'(h/2)).
+__device__ inline void firk_quintic(double h, const double (&bk)[6], double (&coef)[6]) {
+    double A[6][6], b[6];
+    const double t[3] = {0.0, h, 0.5 * h};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double p = 1.0, pd = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            A[r][m] = p;
+            A[3 + r][m] = m * pd;
+            pd = p;
+            p *= t[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) b[r] = bk[r];
+    double X[6][1];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) X[r][0] = b[r];
+    lu_solve_small<6, 1>(A, X);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) coef[r] = X[r][0];
+}
+
+// LDS of the FIRK kernels: the wave kernels' area plus, per boundary row, the
+// linearisation of an interior row for the root solve ([2M + 2]: coefficients of
+// dy_iv and dy_iv+1, constant, flag; the flag is zero for a row read from a node).
+template <int M> struct FirkShared : Shared<M> {
+    double irow[M * (2 * M + 2)];
+};
+
+// ------------------------------------------------------------------ interior boundary rows
+// A STANDARD row inside the span reads EvalSol of the FIRK cache (FIRK/interpolation.jl:
+// 338-441) on interval j (interval(), Core/utils.jl:119-121: the left interval at an
+// interior node): the quintic through y_j, y_{j+1}, their slopes and a value/slope pair
+// of the stage polynomial, at tau = t - t_j.  Expanded: the pair at (tau h) in the stage
+// polynomial's variable (tau the absolute offset, as the reference writes it), stages of
+// the state; nested: the midpoint pair, stages the caller solved from zero (EvalSol's own
+// inner problem).  With g: the derivatives of the value, [M] wrt y_j, [M] wrt y_{j+1},
+// [S M] wrt K_j (expanded only; the nested inner solve sees y_j without its dual part).
+template <int M, int S, bool WANTG>
+__device__ inline double firk_evalsol(const Inst<M> &I, const FirkTab &T, bool expanded, double tj, double tj1,
+                                      double t, int comp, const double *yi, const double *yj,
+                                      const double (&Kk)[S][M], double (&g)[(S + 2) * M]) {
+    const double h = tj1 - tj, tau = t - tj;
+    const double th = (expanded ? tau : 0.5) * h;
+    double fi = 0.0, fj = 0.0, Ji[M], Jj[M], yic = 0.0, yjc = 0.0, kc[S];
+    {
+        Dual<M> Yd[M], Fd[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            Yd[c] = dconst<M>(yi[c]);
+            Yd[c].d[c] = 1.0;
+        }
+        f_eval<M, Dual<M>>(I.fam, I.P, tj, Yd, Fd);
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+            if (a == comp) {
+                fi = Fd[a].v;
+#pragma unroll
+                for (int c = 0; c < M; ++c) Ji[c] = Fd[a].d[c];
+            }
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            Yd[c] = dconst<M>(yj[c]);
+            Yd[c].d[c] = 1.0;
+        }
+        f_eval<M, Dual<M>>(I.fam, I.P, tj1, Yd, Fd);
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+            if (a == comp) {
+                fj = Fd[a].v;
+#pragma unroll
+                for (int c = 0; c < M; ++c) Jj[c] = Fd[a].d[c];
+            }
+    }
+#pragma unroll
+    for (int c = 0; c < M; ++c)
+        if (c == comp) {
+            yic = yi[c];
+            yjc = yj[c];
+#pragma unroll
+            for (int q = 0; q < S; ++q) kc[q] = Kk[q][c];
+        }
+    // stage polynomial of component comp and its derivatives wrt K_j.comp
+    double v = 0.0, zp = 0.0, p = 1.0, hk = 1.0, dz[S], dzp[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) { dz[q] = 0.0; dzp[q] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < S; ++q) a += T.Q[k][q] * kc[q];
+        const double coef = a / hk;
+        zp += (k + 1) * coef * p;
+#pragma unroll
+        for (int q = 0; q < S; ++q) dzp[q] += (k + 1) * (T.Q[k][q] / hk) * p;
+        p *= th;
+        v += coef * p;
+#pragma unroll
+        for (int q = 0; q < S; ++q) dz[q] += (T.Q[k][q] / hk) * p;
+        hk *= h;
+    }
+    const double bk[6] = {yic, yjc, yic + v, fi, fj, zp};
+    double cf[6];
+    firk_quintic(h, bk, cf);
+    double val = 0.0, tp = 1.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) { val += cf[m] * tp; tp *= tau; }
+    if constexpr (WANTG) {
+        // the quintic is linear in its six conditions: their weights w in the value solve
+        // A' w = (1, tau, ..., tau^5), A the constraint matrix of firk_quintic
+        double At[6][6], X[6][1], w[6];
+        const double tn[3] = {0.0, h, 0.5 * h};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double pw = 1.0, pd = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                At[m][r] = pw;
+                At[m][3 + r] = m * pd;
+                pd = pw;
+                pw *= tn[r];
+            }
+        }
+        double pp = 1.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) { X[m][0] = pp; pp *= tau; }
+        lu_solve_small<6, 1>(At, X);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) w[r] = X[r][0];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            g[c] = w[3] * Ji[c] + (c == comp ? w[0] + w[2] : 0.0);
+            g[M + c] = w[4] * Jj[c] + (c == comp ? w[1] : 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int c = 0; c < M; ++c)
+                g[2 * M + q * M + c] = (expanded && c == comp) ? w[2] * dz[q] + w[5] * dzp[q] : 0.0;
+    }
+    return val;
+}
+
+// Residual of the boundary row k inside the span (interval j) at (y + alpha dy,
+// K + alpha dK) (dy, dK may be null: alpha 0; K unused by the nested form).  WANTG: also
+// its linearisation for the root solve into ir: [M] coefficients of dy_j, [M] of dy_{j+1},
+// the constant, with the stage corrections eliminated through the interval's record lk
+// (dK = kg + KJ dy_j).
+template <int M, int S, bool WANTG>
+__device__ inline double firk_bc_interior(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest,
+                                          const double *mesh, const double *y, const double *K, const double *dy,
+                                          const double *dK, double alpha, int k, int j, const double *lk,
+                                          double *ir) {
+    const int comp = sel<M>(I.bcc, k);
+    double yi[M], yj[M], Kk[S][M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        yi[c] = y[(size_t)j * M + c];
+        yj[c] = y[(size_t)(j + 1) * M + c];
+        if (dy) {
+            yi[c] += alpha * dy[(size_t)j * M + c];
+            yj[c] += alpha * dy[(size_t)(j + 1) * M + c];
+        }
+    }
+    if (nest.on) {
+        FirkNest<M> z = nest;
+#pragma unroll
+        for (int c = 0; c < M; ++c) z.k0[c] = 0.0;
+        firk_inner<M, S>(I, T, z, mesh[j], mesh[j + 1] - mesh[j], yi, Kk);
+    } else {
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                const size_t q = ((size_t)j * S + r) * M + c;
+                Kk[r][c] = dK ? K[q] + alpha * dK[q] : K[q];
+            }
+    }
+    double g[(S + 2) * M];
+    const double v =
+        firk_evalsol<M, S, WANTG>(I, T, !nest.on, mesh[j], mesh[j + 1], sel<M>(I.bct, k), comp, yi, yj, Kk, g);
+    if constexpr (WANTG) {
+        constexpr int NN = S * M;
+        double cst = 0.0, ca[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) ca[c] = g[c];
+        if (!nest.on) {
+#pragma unroll
+            for (int q = 0; q < NN; ++q) {
+                const double gq = g[2 * M + q];
+                cst += gq * lk[q];
+#pragma unroll
+                for (int c = 0; c < M; ++c) ca[c] += gq * lk[NN + q * M + c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < M; ++c) { ir[c] = ca[c]; ir[M + c] = g[M + c]; }
+        ir[2 * M] = cst;
+    }
+    return v - sel<M>(I.bcv, k);
+}
+
+// Residual of boundary row k (L: where it sits) at (y + alpha dy, K + alpha dK).
+template <int M, int S>
+__device__ inline double firk_bc_row(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest, const double *mesh,
+                                     const double *y, const double *K, const double *dy, const double *dK,
+                                     double alpha, int k, const BcLoc &L) {
+    if (L.iv < 0 || !nest.irows) {
+        const size_t q = (size_t)L.node * M + sel<M>(I.bcc, k);
+        return (dy ? y[q] + alpha * dy[q] : y[q]) - sel<M>(I.bcv, k);
+    }
+    return firk_bc_interior<M, S, false>(I, T, nest, mesh, y, K, dy, dK, alpha, k, L.iv, nullptr, nullptr);
+}
+
+// Boundary rows of the FIRK path at (y, K): residuals and nodes in sh.
+template <int M, int S>
+__device__ inline void firk_bc_rows(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest, Shared<M> &sh,
+                                    const double *mesh, const double *y, const double *K, int N, int lane) {
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
         sh.bcnode[lane] = L.node;
-        sh.bcres[lane] = y[(size_t)L.node * M + sel<M>(I.bcc, lane)] - sel<M>(I.bcv, lane);
+        sh.bcres[lane] = firk_bc_row<M, S>(I, T, nest, mesh, y, K, nullptr, nullptr, 0.0, lane, L);
     }
     __syncthreads();
 }
@@ -396,7 +617,7 @@ __device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, const 
             rel = nr;
         }
     }
-    firk_bc_rows<M>(I, sh, mesh, y, N, lane);
+    firk_bc_rows<M, S>(I, T, nest, sh, mesh, y, K, N, lane);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const double a = fabs(sh.bcres[k]);
@@ -413,12 +634,29 @@ __device__ inline double firk_forward(const Inst<M> &I, const FirkTab &T, const 
 // Tree solve and back-substitution of the condensed system: dy at the nodes, then
 // dK = kg + KJ dy_i per interval.  dy: [N+1][M], dK: [N][S][M].  Returns false when
 // the root system is singular.
-template <int M, int S>
-__device__ inline bool firk_correction(const Inst<M> &I, Shared<M> &sh, int N, int lane, double *scr,
-                                       const double *loc, Rel<M> &rel, double *dy, double *dK, bool nested = false) {
+template <int M, int S, bool IR = true>
+__device__ inline bool firk_correction(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest, Shared<M> &sh,
+                                       const double *mesh, const double *y, const double *K, int N, int lane,
+                                       double *scr, const double *loc, Rel<M> &rel, double *dy, double *dK) {
+    const bool nested = nest.on;
     int P, start, len;
     chunk_of(N, lane, P, start, len);
-    if (!tree_solve<M, false>(I, sh, N, lane, P, scr, rel)) return false;
+    // the linearisation of rows inside the span (rows read from a node carry a zero flag)
+    double *irow = static_cast<FirkShared<M> &>(sh).irow;
+    if constexpr (IR) {
+        __syncthreads();  // the interval records are read across lanes
+        if (lane < M) {
+            const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+            double *ir = irow + lane * (2 * M + 2);
+            ir[2 * M + 1] = L.iv >= 0 ? 1.0 : 0.0;
+            if (L.iv >= 0)
+                firk_bc_interior<M, S, true>(I, T, nest, mesh, y, K, nullptr, nullptr, 0.0, lane, L.iv,
+                                             loc + (size_t)L.iv * kFirkLoc<M, S>, ir);
+        }
+        if (!tree_solve<M, false, true>(I, sh, N, lane, P, scr, rel, irow)) return false;
+    } else if (!tree_solve<M, false>(I, sh, N, lane, P, scr, rel)) {
+        return false;
+    }
     if (len > 0) {
         double ya[M], yn[M];
 #pragma unroll
@@ -498,9 +736,7 @@ __device__ inline double firk_trial_sumsq(const Inst<M> &I, const FirkTab &T, co
     }
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
-        const int comp = sel<M>(I.bcc, lane);
-        const size_t q = (size_t)L.node * M + comp;
-        const double v = (y[q] + alpha * dy[q]) - sel<M>(I.bcv, lane);
+        const double v = firk_bc_row<M, S>(I, T, nest, mesh, y, K, dy, dK, alpha, lane, L);
         ss += v * v;
     }
     bad = !(fabs(ss) <= 1.7976931348623157e308);
@@ -545,7 +781,7 @@ struct FirkNewtonOut {
 // plain, then with the BackTracking line search, each from the same start, stopping
 // when max|F| <= tol (checked before every step).  Work areas: dy [cap][M], dK, the
 // start copy y0 [cap][M], K0, loc.
-template <int M, int S>
+template <int M, int S, bool IR = true>
 __device__ inline FirkNewtonOut firk_newton(const BatchDev &B, const Inst<M> &I, const FirkTab &T,
                                             const FirkNest<M> &nest, Shared<M> &sh, const double *mesh, double *y,
                                             double *K, int N, int lane, double *scr, double *loc, double *dy,
@@ -567,7 +803,7 @@ __device__ inline FirkNewtonOut firk_newton(const BatchDev &B, const Inst<M> &I,
             if (!(fabs(fn) <= 1.7976931348623157e308)) { out.ret = BVP_RET_UNSTABLE; break; }
             if (fn <= B.newton_tol) { out.ret = BVP_RET_SUCCESS; break; }
             if (it >= B.max_newton) { out.ret = BVP_RET_MAXITERS; break; }
-            if (sing || !firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK, nest.on)) {
+            if (sing || !firk_correction<M, S, IR>(I, T, nest, sh, mesh, y, K, N, lane, scr, loc, rel, dy, dK)) {
                 out.ret = BVP_RET_FAILURE;
                 break;
             }
@@ -667,32 +903,6 @@ __device__ inline double firk_defect_pass(const Inst<M> &I, const FirkTab &T, co
     return anybad ? NAN : dmx;
 }
 
-// The quintic through value and slope at 0, h, h/2 (get_S_coeffs / s_constraints_block,
-// FIRK/adaptivity.jl:183-227), solved with partial pivoting as the 6 x 6 system of the
-// reference; bk = (y(0), y(h), y(h/2), y'(0), y'(h), y'(h/2)).
-__device__ inline void firk_quintic(double h, const double (&bk)[6], double (&coef)[6]) {
-    double A[6][6], b[6];
-    const double t[3] = {0.0, h, 0.5 * h};
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        double p = 1.0, pd = 0.0;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            A[r][m] = p;
-            A[3 + r][m] = m * pd;
-            pd = p;
-            p *= t[r];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 6; ++r) b[r] = bk[r];
-    double X[6][1];
-#pragma unroll
-    for (int r = 0; r < 6; ++r) X[r][0] = b[r];
-    lu_solve_small<6, 1>(A, X);
-#pragma unroll
-    for (int r = 0; r < 6; ++r) coef[r] = X[r][0];
-}
 
 // interpolant! of FIRKExpandInterpolation / FIRKNestedInterpolation
 // (FIRK/interpolation.jl:127-316), and the node values of interp_eval!
@@ -748,7 +958,7 @@ __device__ inline void firk_extend(const double *y, double *K, int N, int lane) 
 // the transfer of the solution (the old quintic interpolant at the new nodes, stages as
 // extend_y sets them for a new problem), or halving with a zero restart when Newton or
 // the defect threshold failed.
-template <int M, int S, bool NEST>
+template <int M, int S, bool NEST, bool IR>
 __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, int b, Shared<M> &sh, int lane,
                                            double *scr, double *aux) {
     Inst<M> I;
@@ -757,7 +967,8 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
     Bufs<M> bf = bufs_of<M>(B, b);
     double *K = firk_stages_of<M, S>(B, b);
     FirkNest<M> nest = firk_nest_of<M>(B, b);
-    nest.on = NEST;  // a compile-time form: the expanded kernels carry no inner-solve code
+    nest.on = NEST;  // compile-time forms: the expanded kernels carry no inner-solve code,
+    nest.irows = IR;  // and kernels for batches without interior rows no EvalSol code
     const size_t cap = (size_t)B.cap;
     double *dy = aux;                          // [cap][M]
     double *dK = dy + cap * M;                 // [cap][S][M]
@@ -787,7 +998,7 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
         const double *mesh = bf.mesh[cur];
         double *y = bf.y_of(cur);
         double *mesh_o = bf.mesh[1 - cur], *y_o = bf.y_of(1 - cur);
-        FirkNewtonOut nl = firk_newton<M, S>(B, I, T, nest, sh, mesh, y, K, N, lane, scr, loc, dy, dK, y0, K0);
+        FirkNewtonOut nl = firk_newton<M, S, IR>(B, I, T, nest, sh, mesh, y, K, N, lane, scr, loc, dy, dK, y0, K0);
         iters += nl.iters;
         rn = nl.fn;
         rn_valid = true;
@@ -865,10 +1076,11 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
 }
 
 // ------------------------------------------------------------------ kernels
-// Persistent FIRK solve: each wave pulls instances from the queue.  NEST: the nested form.
-template <int M, int S, bool NEST>
+// Persistent FIRK solve: each wave pulls instances from the queue.  NEST: the nested form;
+// IR: STANDARD rows inside the span may occur.
+template <int M, int S, bool NEST, bool IR>
 __device__ __forceinline__ void firk_solve_queue(const BatchDev &B) {
-    __shared__ Shared<M> sh;
+    __shared__ FirkShared<M> sh;
     const int lane = threadIdx.x;
     const FirkTab &T = *reinterpret_cast<const FirkTab *>(B.firk_tab);
     double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
@@ -876,19 +1088,19 @@ __device__ __forceinline__ void firk_solve_queue(const BatchDev &B) {
     for (;;) {
         const int item = next_item<M>(B.counter, sh, lane);
         if (item >= B.n) break;
-        firk_solve_instance<M, S, NEST>(B, T, B.order[item], sh, lane, scr, aux);
+        firk_solve_instance<M, S, NEST, IR>(B, T, B.order[item], sh, lane, scr, aux);
     }
 }
 
 // Two register-allocation variants the autotuner chooses between: one wave per SIMD
 // (few spills) and two waves per SIMD (twice the waves to hide latency, more spills).
-template <int M, int S, bool NEST>
+template <int M, int S, bool NEST, bool IR = false>
 __global__ __launch_bounds__(64, 1) void k_firk_solve(BatchDev B) {
-    firk_solve_queue<M, S, NEST>(B);
+    firk_solve_queue<M, S, NEST, IR>(B);
 }
 template <int M, int S, bool NEST>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_firk_solve_w2(BatchDev B) {
-    firk_solve_queue<M, S, NEST>(B);
+    firk_solve_queue<M, S, NEST, false>(B);
 }
 
 // Component kernels at the packed state (mesh, guess and the packed stages init_K; the
@@ -900,7 +1112,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 //   OP_DEFECT      out0: [N][M] defect vectors (interval offset), out1: norm
 template <int M, int S>
 __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
-    __shared__ Shared<M> sh;
+    __shared__ FirkShared<M> sh;
     const int lane = threadIdx.x;
     const FirkTab &T = *reinterpret_cast<const FirkTab *>(B.firk_tab);
     double *scr = as_global(B.scratch + (size_t)blockIdx.x * B.scratch_per_slot);
@@ -956,7 +1168,7 @@ __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
                 int ia = 0, ib = 0;
                 for (int k = 0; k < M; ++k) {
                     const BcLoc L = bc_locate<M>(I, k, mesh, N);
-                    const double v = y[(size_t)L.node * M + sel<M>(I.bcc, k)] - sel<M>(I.bcv, k);
+                    const double v = firk_bc_row<M, S>(I, T, nest, mesh, y, K, nullptr, nullptr, 0.0, k, L);
                     if (I.bc_kind == BVP_BC_TWO_POINT && sel<M>(I.bct, k) != I.t0)
                         out[nbca + (size_t)N * rowlen + (ib++)] = v;
                     else
@@ -967,7 +1179,7 @@ __global__ __launch_bounds__(64, 1) void k_firk_component(BatchDev B, int op) {
             Rel<M> rel;
             bool sing = false;
             firk_forward<M, S>(I, T, nest, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
-            const bool ok = !sing && firk_correction<M, S>(I, sh, N, lane, scr, loc, rel, dy, dK, nest.on);
+            const bool ok = !sing && firk_correction<M, S>(I, T, nest, sh, mesh, y, K, N, lane, scr, loc, rel, dy, dK);
             double *out = B.out0 + xo;
             for (int q = lane; q < (N + 1) * M; q += WAVE) {
                 const int i = q / M, c = q % M;
@@ -1040,7 +1252,11 @@ __global__ __launch_bounds__(256) void k_firk_gather(BatchDev B, const long long
 
 template <int M, int S, int WPE>
 hipError_t firk_launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
-    if (B.firk_nested) {
+    if (B.firk_irows) {
+        // batches with STANDARD rows inside the span: the one-wave-per-SIMD kernels only
+        if (B.firk_nested) k_firk_solve<M, S, true, true><<<blocks, 64, 0, s>>>(B);
+        else k_firk_solve<M, S, false, true><<<blocks, 64, 0, s>>>(B);
+    } else if (B.firk_nested) {
         if constexpr (WPE == 2) k_firk_solve_w2<M, S, true><<<blocks, 64, 0, s>>>(B);
         else k_firk_solve<M, S, true><<<blocks, 64, 0, s>>>(B);
     } else {

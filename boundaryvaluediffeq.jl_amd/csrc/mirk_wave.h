@@ -94,6 +94,7 @@ struct BatchDev {
     const double *init_K;
     int firk_nested;          // the nested form: u0c holds each instance's inner-solve start k0
     double nest_tol;          // its inner tolerance
+    int firk_irows;           // some STANDARD boundary row lies inside the span (EvalSol rows)
 
     // options
     int adaptive, max_sub, max_newton, stages, transfer_mode;

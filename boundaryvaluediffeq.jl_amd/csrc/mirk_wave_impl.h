@@ -811,10 +811,16 @@ __device__ inline void forward_dual(const Inst<M> &I, const double *mesh, const 
 // end-to-end relation), tree down-sweep.  Leaves in sh.lo[l] / sh.hi[l] the
 // corrections at the first and last node of lane l's chunk.  Returns false (all
 // lanes) when the root system is singular.  scr holds the chunk records (read by the
-// walk to an interior boundary node), F32 their storage precision.
-template <int M, bool F32>
+// walk to an interior boundary node), F32 their storage precision.  ROW2: irow holds,
+// per boundary row, [2M + 2] doubles: a row linear in the corrections of two adjacent
+// nodes (sh.bcnode and the next; coefficients, constant) where its last entry is
+// nonzero, else the row reads its node; rowfn() writes irow after the up-sweep.
+struct NoRows {
+    __device__ void operator()() const {}
+};
+template <int M, bool F32, bool ROW2 = false, class RowFn = NoRows>
 __device__ __forceinline__ bool tree_solve(const Inst<M> &I, Shared<M> &sh, int N, int lane, int P, double *scr,
-                                           Rel<M> &rel) {
+                                           Rel<M> &rel, const double *irow = nullptr, RowFn rowfn = RowFn()) {
     int D = 0;
     while ((1 << D) < P) ++D;
     double *tree = sh.tree;  // tree records stay on chip (read back level by level in the down-sweep)
@@ -834,6 +840,10 @@ __device__ __forceinline__ bool tree_solve(const Inst<M> &I, Shared<M> &sh, int 
         }
         __syncthreads();
     }
+    if constexpr (ROW2) {
+        rowfn();
+        __syncthreads();
+    }
     // ---- root: 2M x 2M system of the boundary rows and the end-to-end relation
     if (lane == 0) {
         constexpr int NN = 2 * M;
@@ -844,21 +854,10 @@ __device__ __forceinline__ bool tree_solve(const Inst<M> &I, Shared<M> &sh, int 
             for (int c = 0; c < NN; ++c) A[i][c] = 0.0;
             b[i] = 0.0;
         }
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-            const int node = sh.bcnode[k];
-            const int comp = sel<M>(I.bcc, k);
-            b[k] = -sh.bcres[k];
-            if (I.bc_kind == BVP_BC_TWO_POINT || node == 0 || node == N) {
-#pragma unroll
-                for (int c = 0; c < M; ++c) {
-                    A[k][c] = (node == 0 && c == comp) ? 1.0 : 0.0;
-                    A[k][M + c] = (node == N && c == comp) ? 1.0 : 0.0;
-                }
-            } else {
-                // Interior node: express dy_node affinely in (dy_0, dy_N) by walking the
-                // elimination records from the root to the node (columns: dy_0 | dy_N | 1).
-                constexpr int W = 2 * M + 1;
+        // dy_node affinely in (dy_0, dy_N) (columns: dy_0 | dy_N | 1), by walking the
+        // elimination records from the root to the node (identity rows at nodes 0 and N)
+        constexpr int W = 2 * M + 1;
+        auto walk = [&](int node, double (&mt)[M][W]) {
                 double mlo[M][W], mhi[M][W];
 #pragma unroll
                 for (int i = 0; i < M; ++i)
@@ -869,7 +868,6 @@ __device__ __forceinline__ bool tree_solve(const Inst<M> &I, Shared<M> &sh, int 
                     }
                 int l = 0;
                 bool found = false;
-                double mt[M][W];
                 for (int d = D - 1; d >= 0 && !found; --d) {
                     const int span = 1 << d;
                     if (l + span >= P) continue;
@@ -945,6 +943,36 @@ __device__ __forceinline__ bool tree_solve(const Inst<M> &I, Shared<M> &sh, int 
                             for (int c = 0; c < W; ++c) mt[i][c] = mn[i][c];
                     }
                 }
+        };
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int node = sh.bcnode[k];
+            const int comp = sel<M>(I.bcc, k);
+            b[k] = -sh.bcres[k];
+            if (ROW2 && irow[k * (W + 1) + W] != 0.0) {
+                // a row linear in dy_node and dy_node+1: coefficients irow[k][0..M) and
+                // [M..2M), constant irow[k][2M]
+                const double *ir = irow + k * (W + 1);
+                double ma[M][W], mb[M][W];
+                walk(node, ma);
+                walk(node + 1, mb);
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int i = 0; i < M; ++i) v += ir[i] * ma[i][c] + ir[M + i] * mb[i][c];
+                    if (c < 2 * M) A[k][c] = v;
+                    else b[k] -= v + ir[2 * M];
+                }
+            } else if (I.bc_kind == BVP_BC_TWO_POINT || node == 0 || node == N) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) {
+                    A[k][c] = (node == 0 && c == comp) ? 1.0 : 0.0;
+                    A[k][M + c] = (node == N && c == comp) ? 1.0 : 0.0;
+                }
+            } else {
+                double mt[M][W];
+                walk(node, mt);
 #pragma unroll
                 for (int i = 0; i < M; ++i) {
                     if (i == comp) {

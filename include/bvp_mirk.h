@@ -52,9 +52,12 @@ extern "C" {
  * solves each interval's stage equations from the fixed start K0 (u0 per stage, or the
  * mean of the guess, FIRK/utils.jl:41-83) by NewtonRaphson to bvp_opts.nest_abstol.
  * RadauIIa1, LobattoIIIb2 and LobattoIIIc2 do not adapt the
- * mesh (FIRK/algorithms.jl:411): they need adaptive = 0.  On the GPU: M = 1..4, boundary
- * rows at t0 or t1, DefectControl, FP64, the NewtonRaphson stages of the polyalgorithm
- * (nlsolve_stages bits 0 and 1). */
+ * mesh (FIRK/algorithms.jl:411): they need adaptive = 0.  A STANDARD row inside the span
+ * reads EvalSol of the FIRK cache (FIRK/interpolation.jl:338-441): the quintic of its
+ * interval, whose midpoint pair the expanded form takes at (t - t_j) h in the stage
+ * polynomial's variable, as the reference does, and the nested form at the midpoint from
+ * stages solved from zero.  On the GPU: M = 1..4, DefectControl, FP64, the NewtonRaphson
+ * stages of the polyalgorithm (nlsolve_stages bits 0 and 1). */
 #define BVP_FIRK_NESTED     1000  /* added to a FIRK code: the nested form (below) */
 #define BVP_RADAU_IIA(s)    (100 + (s))
 #define BVP_LOBATTO_IIIA(s) (200 + (s))
@@ -331,8 +334,8 @@ int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_inst
 /* ---- FIRK components.  The state of instance i is its (mesh, guess) and its stage
  * values: `stages` holds N_i*s*M doubles per instance (interval, then stage, then
  * component; instances back to back), or is NULL for the stages the reference
- * starts from, K_ij = y_{i+1} (extend_y, FIRK/firk.jl:76-83).  M = 1..4; boundary
- * rows at t0 or t1.  Slot sizes below use n_x = M*(N*(s+1)+1), the length of the
+ * starts from, K_ij = y_{i+1} (extend_y, FIRK/firk.jl:76-83).  M = 1..4.  Slot sizes
+ * below use n_x = M*(N*(s+1)+1), the length of the
  * expanded unknown vector.  With a nested code (+ BVP_FIRK_NESTED) `stages` is not
  * read (the inner solves give them, from K0, at the default inner tolerance), and
  * residual and correction have the node layout, n_x = M*(N+1): [bc; Phi] or

@@ -90,6 +90,8 @@ def lib():
         L.ref_firk_newton_step.argtypes = [C.c_int, C.POINTER(_Instance), _dp, _dp]
         L.ref_firk_defect.argtypes = [C.c_int, C.POINTER(_Instance), _dp, _dp, _dp]
         L.ref_firk_interpolate.argtypes = [C.c_int, C.POINTER(_Instance), _dp, C.c_int, _dp, _dp, _dp]
+        L.ref_firk_evalsol.argtypes = [C.c_int, C.POINTER(_Instance), C.c_double, C.c_double, C.c_double, C.c_int,
+                                       _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -357,6 +359,22 @@ def firk_interpolate(code, spec, t, stages=None):
     _check(lib().ref_firk_interpolate(code, C.byref(P.inst), _d(st), len(t), _d(t), _d(u), _d(du)),
            "firk_interpolate")
     return u[:len(t) * spec.M].reshape(-1, spec.M), du[:len(t) * spec.M].reshape(-1, spec.M)
+
+
+def firk_evalsol(code, spec, tj, tj1, t, comp, yi, yj, K):
+    """EvalSol of a FIRK cache (FIRK/interpolation.jl:338-441) on the interval [tj, tj1] at t,
+    component comp, from nodes yi, yj and stages K (s, M).  Returns (value, g): g holds the
+    derivatives wrt y_j (M), y_{j+1} (M) and K_j (s M); the nested form's stages carry none."""
+    s, M = code % 100, spec.M
+    P = _Pinned(spec)
+    yi = np.ascontiguousarray(yi, dtype=np.float64).reshape(M)
+    yj = np.ascontiguousarray(yj, dtype=np.float64).reshape(M)
+    K = np.ascontiguousarray(K, dtype=np.float64).reshape(s * M)
+    v = np.zeros(1)
+    g = np.zeros((s + 2) * M)
+    _check(lib().ref_firk_evalsol(code, C.byref(P.inst), tj, tj1, t, comp, _d(yi), _d(yj), _d(K), _d(v), _d(g)),
+           "firk_evalsol")
+    return float(v[0]), g
 
 
 def firk_unpack(code, spec, x):

@@ -1242,9 +1242,9 @@ int solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
 //   defect .............. FIRK/adaptivity.jl:349-405, 581-606 (eval_q)
 //   mesh_selector! ...... FIRK/adaptivity.jl:247-300 (the MIRK rules, rho = 1)
 //   interpolation ....... FIRK/adaptivity.jl:8-48, 183-227; FIRK/interpolation.jl:229-316
-// A STANDARD boundary row must sit at t0 or t1 here (EvalSol's end-point shortcut,
-// FIRK/interpolation.jl:331-332); the interior form of FIRK/interpolation.jl:333-368
-// is not restated.
+// STANDARD boundary rows at t0 or t1 read the node (EvalSol's end-point shortcut,
+// FIRK/interpolation.jl:344-345); rows inside the span evaluate EvalSol's quintic
+// (firk_evalsol, FIRK/interpolation.jl:338-441).
 // The solution transfer after a mesh redistribution restates what interp_eval!
 // computes for each new node (the quintic interpolant of the old solution,
 // FIRK/adaptivity.jl:8-48) and places the nodes as extend_y does for a new problem
@@ -1458,10 +1458,14 @@ struct FEval {
     std::vector<double> col;  // per interval (s+1) M: Phi_i, then G_i1..G_is
     std::vector<double> Y;    // per interval s M: stage points y_i + h sum_j a_rj K_j
     std::vector<double> bc;   // per BC row, input order
+    std::vector<double> bcg;  // per BC row (s+2) M: an interior row's derivatives (y_iv, y_iv+1, K_iv)
 };
 
 // Phi! of the expanded form (FIRK/collocation.jl:59-93), in the reference's order
 // of operations: Y_r = y_i + h (K a_r), G_r = f(Y_r) - K_r, Phi = (y_{i+1} - y_i) - h (K b)
+double firk_evalsol(const FCtx &C, const double *mesh, int j, double t, int comp, const double *yi, const double *yj,
+                    const double *K, bool expanded, double *g);
+
 void firk_evaluate(const FCtx &C, int N, const double *mesh, const double *x, const std::vector<BcRow> &rows,
                    FEval &E) {
     const int M = C.M, s = C.s();
@@ -1489,7 +1493,15 @@ void firk_evaluate(const FCtx &C, int N, const double *mesh, const double *x, co
         }
     }
     E.bc.assign(rows.size(), 0.0);
-    for (size_t k = 0; k < rows.size(); ++k) E.bc[k] = x[C.yoff(rows[k].node) + rows[k].comp] - C.P.bc_val[k];
+    E.bcg.assign(rows.size() * (s + 2) * M, 0.0);
+    for (size_t k = 0; k < rows.size(); ++k) {
+        const BcRow &r = rows[k];
+        const double v = r.direct ? x[C.yoff(r.node) + r.comp]
+                                  : firk_evalsol(C, mesh, r.iv, C.P.bc_t[k], r.comp, x + C.yoff(r.iv),
+                                                 x + C.yoff(r.iv + 1), x + C.koff(r.iv, 0), true,
+                                                 &E.bcg[k * (s + 2) * M]);
+        E.bc[k] = v - C.P.bc_val[k];
+    }
 }
 
 double firk_max_abs(const FEval &E) {
@@ -1526,7 +1538,7 @@ void firk_jacobian(const FCtx &C, int N, const double *mesh, const std::vector<B
     int r = 0;
     auto bc_rows = [&](int node) {
         for (size_t k = 0; k < rows.size(); ++k)
-            if (rows[k].node == node) {
+            if (rows[k].direct && rows[k].node == node) {
                 J.push_back({r, (int)C.yoff(node) + rows[k].comp, 1.0});
                 rhs.push_back(-E.bc[k]);
                 ++r;
@@ -1537,6 +1549,18 @@ void firk_jacobian(const FCtx &C, int N, const double *mesh, const std::vector<B
     for (int i = 0; i < N; ++i) {
         const double h = mesh[i + 1] - mesh[i];
         const double *col = &E.col[(size_t)i * (s + 1) * M];
+        // interior STANDARD rows of interval i: derivatives of EvalSol wrt y_i, y_{i+1}, K_i
+        for (size_t k = 0; k < rows.size(); ++k) {
+            if (rows[k].direct || rows[k].iv != i) continue;
+            const double *g = &E.bcg[k * (s + 2) * M];
+            for (int c = 0; c < M; ++c) {
+                J.push_back({r, (int)C.yoff(i) + c, g[c]});
+                J.push_back({r, (int)C.yoff(i + 1) + c, g[M + c]});
+            }
+            for (int q = 0; q < s * M; ++q) J.push_back({r, (int)C.koff(i, 0) + q, g[2 * M + q]});
+            rhs.push_back(-E.bc[k]);
+            ++r;
+        }
         for (int a = 0; a < M; ++a) {  // Phi_i
             J.push_back({r, (int)C.yoff(i) + a, -1.0});
             for (int j = 0; j < s; ++j) J.push_back({r, (int)C.koff(i, j) + a, -h * C.T.b[j]});
@@ -1790,6 +1814,66 @@ void quintic(double h, const double *bk, double *coef) {
     }
 }
 
+// EvalSol of a FIRK cache at a point t inside interval j (FIRK/interpolation.jl:338-385
+// expanded, 398-441 nested), component comp: the quintic through y_j, y_{j+1}, their
+// slopes f and a third value/slope pair from the stage polynomial, evaluated at
+// tau = t - t_j.  The expanded form takes that pair at (tau h) in the stage polynomial's
+// variable (tau the absolute offset, as the reference writes it) from the state's stages;
+// the nested form takes it at the midpoint, from stages the caller solved from zero (the
+// inner problem's u0 there).  g (optional): the derivatives of the value, [M] wrt y_j,
+// [M] wrt y_{j+1}, [s M] wrt K_j (zero in the nested form, whose inner solve sees y_j
+// without its dual part, nodual_value).
+double firk_evalsol(const FCtx &C, const double *mesh, int j, double t, int comp, const double *yi, const double *yj,
+                    const double *K, bool expanded, double *g) {
+    const int M = C.M, s = C.s();
+    const double h = mesh[j + 1] - mesh[j], tau = t - mesh[j];
+    const double th = (expanded ? tau : 0.5) * h;
+    std::vector<double> fi(M), fj(M);
+    f_eval(C.P, mesh[j], yi, fi.data());
+    f_eval(C.P, mesh[j + 1], yj, fj.data());
+    // stage polynomial of component comp (get_q_coeffs_interp / apply_q) and its
+    // derivatives wrt the stages K_j.comp
+    double v = 0, zp = 0, p = 1, dz[7] = {0}, dzp[7] = {0};
+    for (int k = 0; k < s; ++k) {
+        const double hk = std::pow(h, (double)k);
+        double a = 0;
+        for (int q = 0; q < s; ++q) a += C.T.Q[k][q] * K[q * M + comp];
+        const double coef = a / hk;
+        zp += (k + 1) * coef * p;
+        for (int q = 0; q < s; ++q) dzp[q] += (k + 1) * (C.T.Q[k][q] / hk) * p;
+        p *= th;
+        v += coef * p;
+        for (int q = 0; q < s; ++q) dz[q] += (C.T.Q[k][q] / hk) * p;
+    }
+    const double z = yi[comp] + v;
+    const double bk[6] = {yi[comp], yj[comp], z, fi[comp], fj[comp], zp};
+    double cf[6], val = 0;
+    quintic(h, bk, cf);
+    for (int m = 0; m < 6; ++m) val += cf[m] * std::pow(tau, (double)m);
+    if (g) {
+        // weights of the six conditions in the value (the quintic is linear in them)
+        double w[6];
+        for (int r = 0; r < 6; ++r) {
+            double e[6] = {0, 0, 0, 0, 0, 0}, ce[6];
+            e[r] = 1.0;
+            quintic(h, e, ce);
+            w[r] = 0;
+            for (int m = 0; m < 6; ++m) w[r] += ce[m] * std::pow(tau, (double)m);
+        }
+        std::vector<double> Ji((size_t)M * M), Jj((size_t)M * M);
+        J_eval(C.P, mesh[j], yi, Ji.data());
+        J_eval(C.P, mesh[j + 1], yj, Jj.data());
+        std::fill(g, g + (size_t)(s + 2) * M, 0.0);
+        for (int c = 0; c < M; ++c) {
+            g[c] = w[3] * Ji[(size_t)comp * M + c] + (c == comp ? w[0] + w[2] : 0.0);
+            g[M + c] = w[4] * Jj[(size_t)comp * M + c] + (c == comp ? w[1] : 0.0);
+        }
+        if (expanded)
+            for (int q = 0; q < s; ++q) g[2 * M + q * M + comp] = w[2] * dz[q] + w[5] * dzp[q];
+    }
+    return val;
+}
+
 // interpolant! of FIRKExpandInterpolation (FIRK/interpolation.jl:229-316), also
 // interp_eval! of the transfer (FIRK/adaptivity.jl:8-48) without its h scaling
 void firk_interp(const FCtx &C, int N, const double *mesh, const double *x, double t, double *u, double *du) {
@@ -1818,8 +1902,6 @@ void firk_interp(const FCtx &C, int N, const double *mesh, const double *x, doub
 int firk_check(const bvp_instance &P, int code, FTab &T) {
     if (!firk_tableau(code, T)) return BVP_EINVAL;
     if (check_instance(P)) return BVP_EINVAL;
-    for (int k = 0; k < P.n_bc; ++k)
-        if (P.bc_t[k] != P.t0 && P.bc_t[k] != P.t1) return BVP_ENOTSUP;
     return 0;
 }
 
@@ -2026,6 +2108,7 @@ struct FNEval {
     std::vector<double> phi;  // [N][M]
     std::vector<double> K;    // [N][s][M] converged stages
     std::vector<double> bc;
+    std::vector<double> bcg;  // per BC row (s+2) M: an interior row's derivatives (y_iv, y_iv+1, -)
 };
 
 void firk_nested_evaluate(const FCtx &C, int N, const double *mesh, const double *y, const std::vector<double> &k0,
@@ -2044,7 +2127,22 @@ void firk_nested_evaluate(const FCtx &C, int N, const double *mesh, const double
         }
     }
     E.bc.assign(rows.size(), 0.0);
-    for (size_t k = 0; k < rows.size(); ++k) E.bc[k] = y[(size_t)rows[k].node * M + rows[k].comp] - C.P.bc_val[k];
+    E.bcg.assign(rows.size() * (s + 2) * M, 0.0);
+    std::vector<double> zero(M, 0.0), Kz((size_t)s * M);
+    for (size_t k = 0; k < rows.size(); ++k) {
+        const BcRow &r = rows[k];
+        double v;
+        if (r.direct) {
+            v = y[(size_t)r.node * M + r.comp];
+        } else {
+            // EvalSol's own inner solve, from zero (FIRK/interpolation.jl:432-434)
+            const int i = r.iv;
+            firk_inner_solve(C, mesh[i], mesh[i + 1] - mesh[i], y + (size_t)i * M, zero, Kz.data());
+            v = firk_evalsol(C, mesh, i, C.P.bc_t[k], r.comp, y + (size_t)i * M, y + (size_t)(i + 1) * M, Kz.data(),
+                             false, &E.bcg[k * (s + 2) * M]);
+        }
+        E.bc[k] = v - C.P.bc_val[k];
+    }
 }
 
 double firk_nested_max_abs(const FNEval &E) {
@@ -2070,7 +2168,7 @@ void firk_nested_jacobian(const FCtx &C, int N, const double *mesh, const double
     int r = 0;
     auto bc_rows = [&](int node) {
         for (size_t k = 0; k < rows.size(); ++k)
-            if (rows[k].node == node) {
+            if (rows[k].direct && rows[k].node == node) {
                 J.push_back({r, node * M + rows[k].comp, 1.0});
                 rhs.push_back(-E.bc[k]);
                 ++r;
@@ -2080,6 +2178,16 @@ void firk_nested_jacobian(const FCtx &C, int N, const double *mesh, const double
     std::vector<double> W, G, Jh;
     for (int i = 0; i < N; ++i) {
         const double h = mesh[i + 1] - mesh[i];
+        for (size_t k = 0; k < rows.size(); ++k) {  // interior STANDARD rows of interval i
+            if (rows[k].direct || rows[k].iv != i) continue;
+            const double *g = &E.bcg[k * (s + 2) * M];
+            for (int c = 0; c < M; ++c) {
+                J.push_back({r, i * M + c, g[c]});
+                J.push_back({r, (i + 1) * M + c, g[M + c]});
+            }
+            rhs.push_back(-E.bc[k]);
+            ++r;
+        }
         firk_stage_system(C, mesh[i], h, y + (size_t)i * M, &E.K[(size_t)i * s * M], W, G, Jh);
         dense_lu_solve(n, M, W, Jh);  // Jh <- KJ
         for (int a = 0; a < M; ++a) {
@@ -2221,7 +2329,8 @@ int firk_nested_solve_one(const bvp_instance &P, const bvp_opts &o, bvp_result &
             nl.ret = firk_nested_newton_stage(C, N, mesh.data(), y, k0, rows, tol, o.max_newton_iters, st == 1, nl, E);
             if (nl.ret == BVP_RET_SUCCESS) break;
         }
-        eval_valid = true;
+        eval_valid = !first;  // E holds the last stage's state (none ran: no NewtonRaphson stage asked for)
+        if (first) firk_nested_evaluate(C, N, mesh.data(), y.data(), k0, rows, E);
         newton_iters += nl.iters;
         if (nl.stage == 1) fb |= 1;
         nl_ret = nl.ret;
@@ -2314,6 +2423,25 @@ int firk_nested_state(const FCtx &C, std::vector<double> &mesh, std::vector<doub
 int solve_any(const bvp_instance &P, const bvp_opts &o, bvp_result &res) {
     if (o.method >= 1000) return firk_nested_solve_one(P, o, res);
     return o.method >= 100 ? firk_solve_one(P, o, res) : solve_one(P, o, res);
+}
+
+// banded row order of firk_jacobian / firk_nested_jacobian -> the reference's row order
+// ([bc; collocation] or [bca; collocation; bcb]); per_iv collocation rows per interval
+std::vector<int> firk_row_map(const std::vector<BcRow> &rows, int N, int per_iv, bool tp) {
+    std::vector<int> bc_ref(rows.size());
+    int r = 0;
+    for (size_t k = 0; k < rows.size(); ++k) if (!tp || !rows[k].side_b) bc_ref[k] = r++;
+    const int top = r;
+    r += N * per_iv;
+    if (tp) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) bc_ref[k] = r++;
+    std::vector<int> map;
+    for (size_t k = 0; k < rows.size(); ++k) if (rows[k].direct && rows[k].node == 0) map.push_back(bc_ref[k]);
+    for (int i = 0; i < N; ++i) {
+        for (size_t k = 0; k < rows.size(); ++k) if (!rows[k].direct && rows[k].iv == i) map.push_back(bc_ref[k]);
+        for (int q = 0; q < per_iv; ++q) map.push_back(top + i * per_iv + q);
+    }
+    for (size_t k = 0; k < rows.size(); ++k) if (rows[k].direct && rows[k].node == N) map.push_back(bc_ref[k]);
+    return map;
 }
 
 }  // namespace
@@ -2643,64 +2771,45 @@ int ref_firk_dense_system(int code, const bvp_instance *P, const double *stages,
     int rc = firk_check(*P, code, T);
     if (rc) return rc;
     FCtx C(*P, T);
+    std::vector<Trip> Jt;
+    std::vector<double> rhs;
+    std::vector<BcRow> rows;
+    int n, per_iv;
     if (firk_is_nested(code)) {
         // node layout: J is M (N+1) square, rows in the reference's order
         std::vector<double> mesh, y, k0;
-        std::vector<BcRow> rows;
         FNEval E;
         if ((rc = firk_nested_state(C, mesh, y, k0, rows, E))) return rc;
         flatten_residual_phi(*P, rows, E.phi, E.bc, F);
-        std::vector<Trip> Jt;
-        std::vector<double> rhs;
         firk_nested_jacobian(C, P->N, mesh.data(), y.data(), rows, E, Jt, rhs);
-        const int M = C.M, N = P->N, n = M * (N + 1);
-        const bool tp = P->bc_kind == BVP_BC_TWO_POINT;
-        std::vector<int> bc_ref(rows.size());
-        int r = 0;
-        for (size_t k = 0; k < rows.size(); ++k) if (!tp || !rows[k].side_b) bc_ref[k] = r++;
-        const int top = r;
-        r += N * M;
-        if (tp) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) bc_ref[k] = r++;
-        std::vector<int> map(n);
-        r = 0;
-        for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == 0) map[r++] = bc_ref[k];
-        for (int i = 0; i < N * M; ++i) map[r++] = top + i;
-        for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == N) map[r++] = bc_ref[k];
-        std::memset(J, 0, sizeof(double) * (size_t)n * n);
-        for (const Trip &t : Jt) J[(size_t)map[t.r] * n + t.c] += t.v;
-        return 0;
+        n = C.M * (P->N + 1);
+        per_iv = C.M;
+    } else {
+        std::vector<double> mesh, x;
+        if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
+        FEval E;
+        firk_evaluate(C, P->N, mesh.data(), x.data(), rows, E);
+        firk_flatten(C, rows, E, F);
+        firk_jacobian(C, P->N, mesh.data(), rows, E, Jt, rhs);
+        n = (int)C.nx(P->N);
+        per_iv = (T.s + 1) * C.M;
     }
-    std::vector<double> mesh, x;
-    std::vector<BcRow> rows;
-    if ((rc = firk_state(C, stages, mesh, x, rows))) return rc;
-    FEval E;
-    firk_evaluate(C, P->N, mesh.data(), x.data(), rows, E);
-    firk_flatten(C, rows, E, F);
-    std::vector<Trip> Jt;
-    std::vector<double> rhs;
-    firk_jacobian(C, P->N, mesh.data(), rows, E, Jt, rhs);
-    // banded row order -> reference row order
-    const int n = (int)C.nx(P->N), M = C.M, s = T.s, N = P->N;
-    const bool tp = P->bc_kind == BVP_BC_TWO_POINT;
-    std::vector<int> nb0, nbN;  // BC rows (input index) on node 0 / node N, banded order
-    for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == 0) nb0.push_back((int)k);
-    for (size_t k = 0; k < rows.size(); ++k) if (rows[k].node == N) nbN.push_back((int)k);
-    std::vector<int> bc_ref(rows.size());  // input index -> reference row
-    {
-        int r = 0;
-        for (size_t k = 0; k < rows.size(); ++k) if (!tp || !rows[k].side_b) bc_ref[k] = r++;
-        r += N * (s + 1) * M;
-        if (tp) for (size_t k = 0; k < rows.size(); ++k) if (rows[k].side_b) bc_ref[k] = r++;
-    }
-    const int nbc_top = tp ? (int)std::count_if(rows.begin(), rows.end(), [](const BcRow &b) { return !b.side_b; })
-                           : (int)rows.size();
-    std::vector<int> map(n);
-    int r = 0;
-    for (int k : nb0) map[r++] = bc_ref[k];
-    for (int i = 0; i < N * (s + 1) * M; ++i) map[r++] = nbc_top + i;
-    for (int k : nbN) map[r++] = bc_ref[k];
+    const std::vector<int> map = firk_row_map(rows, P->N, per_iv, P->bc_kind == BVP_BC_TWO_POINT);
+    if ((int)map.size() != n) return 1;
     std::memset(J, 0, sizeof(double) * (size_t)n * n);
     for (const Trip &t : Jt) J[(size_t)map[t.r] * n + t.c] += t.v;
+    return 0;
+}
+
+// EvalSol of interval [tj, tj1] at t (firk_evalsol), for the checks of its derivatives:
+// value and g ((s+2) M: y_j, y_{j+1}, K_j)
+int ref_firk_evalsol(int code, const bvp_instance *P, double tj, double tj1, double t, int comp, const double *yi,
+                     const double *yj, const double *K, double *value, double *g) {
+    FTab T;
+    if (!firk_tableau(code, T) || comp < 0 || comp >= P->M) return BVP_EINVAL;
+    FCtx C(*P, T);
+    const double mesh[2] = {tj, tj1};
+    *value = firk_evalsol(C, mesh, 0, t, comp, yi, yj, K, !firk_is_nested(code), g);
     return 0;
 }
 

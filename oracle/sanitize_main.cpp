@@ -31,6 +31,8 @@ int ref_firk_newton_step(int code, const bvp_instance *P, const double *stages, 
 int ref_firk_defect(int code, const bvp_instance *P, const double *stages, double *errors, double *defect_norm);
 int ref_firk_interpolate(int code, const bvp_instance *P, const double *stages, int nt, const double *t, double *u,
                          double *du);
+int ref_firk_evalsol(int code, const bvp_instance *P, double tj, double tj1, double t, int comp, const double *yi,
+                     const double *yj, const double *K, double *value, double *g);
 }
 
 namespace {
@@ -127,17 +129,19 @@ void exercise(Problem &q, const char *name) {
     }
 }
 
-// the FIRK entry points (expanded form) for every method on an end-point-BC problem
+// the FIRK entry points (expanded and nested forms) for every method
 void exercise_firk(Problem &q, const char *name) {
     const bvp_instance &P = q.inst;
     const int M = P.M, N = P.N;
-    const int codes[] = {101, 102, 103, 105, 107, 202, 203, 204, 205, 302, 303, 304, 305, 402, 403, 404, 405};
-    for (int code : codes) {
+    const int bases[] = {101, 102, 103, 105, 107, 202, 203, 204, 205, 302, 303, 304, 305, 402, 403, 404, 405};
+    for (int nest : {0, 1000})
+    for (int base : bases) {
+        const int code = base + nest;
         int ints[3];
         double tab[113];
         expect(ref_firk_tableau(code, ints, tab) == 0, name);
         const int s = ints[0];
-        const size_t nx = (size_t)M * ((size_t)N * (s + 1) + 1);
+        const size_t nx = nest ? (size_t)M * (N + 1) : (size_t)M * ((size_t)N * (s + 1) + 1);
         std::vector<double> K((size_t)N * s * M, 0.1), F(nx), J(nx * nx), d(nx), err((size_t)N * M);
         expect(ref_firk_residual(code, &P, nullptr, F.data()) == 0, name);
         expect(ref_firk_residual(code, &P, K.data(), F.data()) == 0, name);
@@ -195,11 +199,15 @@ int main() {
     exercise_firk(ps[0], "firk bratu");
     Problem tu0 = make(BVP_FAMILY_SCALAR_TU, 1, 0, 1, {}, BVP_BC_STANDARD, {0.0}, {0}, {1.0}, 9);
     exercise_firk(tu0, "firk scalar_tu");
-    {
-        std::vector<double> F(64);
-        expect(ref_firk_residual(103, &ps[4].inst, nullptr, F.data()) == BVP_ENOTSUP, "firk interior row");
-    }
+    exercise_firk(ps[4], "firk scalar_tu interior row");
+    exercise_firk(ps[1], "firk pendulum interior row");
     exercise_firk(ps[5], "firk linear_tanh3");
+    {
+        double v, g[16];
+        const double yi[2] = {0.1, 0.2}, yj[2] = {0.3, 0.4}, K[10] = {0};
+        expect(ref_firk_evalsol(105, &ps[1].inst, 0.5, 0.6, 0.55, 1, yi, yj, K, &v, g) == 0, "firk evalsol");
+        expect(ref_firk_evalsol(105, &ps[1].inst, 0.5, 0.6, 0.55, 2, yi, yj, K, &v, g) == BVP_EINVAL, "evalsol comp");
+    }
 
     // invalid inputs: error codes, no out-of-bounds access
     Problem bad = ps[0];

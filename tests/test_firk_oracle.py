@@ -348,12 +348,146 @@ def test_singpert_adaptive_refines(entry):
     assert abs(u[0, 0] - entry["u0"]) < 1e-4
 
 
-def test_interior_standard_rows_rejected():
-    """Interior STANDARD rows need the reference's EvalSol at an interior point, which is
-    not restated for FIRK: the oracle refuses them."""
-    spec = cases.pendulum(0.05)
-    with pytest.raises(RuntimeError):
-        O.solve(O.opts(103), spec)
+# --------------------------------------------------------------- interior STANDARD rows
+# EvalSol at a point inside the span (FIRK/interpolation.jl:338-441)
+def _pend(dt):
+    """The reference's FIRK pendulum (test/expanded/firk_basic_tests.jl:5-22): rows at pi/4
+    (inside the span) and pi/2."""
+    return cases.pendulum(dt)
+
+
+@pytest.mark.parametrize("code", [102, 103, 105, 204, 304, 404], ids=_name)
+@pytest.mark.parametrize("nested", [False, True], ids=["expanded", "nested"])
+def test_evalsol_derivatives_vs_finite_differences(code, nested):
+    """EvalSol's derivative row (wrt y_j, y_{j+1} and, expanded, K_j; the nested form's
+    stages are held) against central differences of its value."""
+    rng = np.random.default_rng(11 + code)
+    c = code + (NESTED if nested else 0)
+    spec = _pend(0.07)
+    s, M = code % 100, 2
+    yi, yj, K = rng.standard_normal(M), rng.standard_normal(M), rng.standard_normal((s, M))
+    tj, tj1, t = 0.7, 0.78, 0.7 + 0.03
+    for comp in (0, 1):
+        v, g = O.firk_evalsol(c, spec, tj, tj1, t, comp, yi, yj, K)
+        x0 = np.concatenate([yi, yj, K.reshape(-1)])
+        fd = np.zeros_like(x0)
+        for q in range(x0.size):
+            xp, xm = x0.copy(), x0.copy()
+            xp[q] += 1e-6
+            xm[q] -= 1e-6
+            vp = O.firk_evalsol(c, spec, tj, tj1, t, comp, xp[:M], xp[M:2 * M], xp[2 * M:].reshape(s, M))[0]
+            vm = O.firk_evalsol(c, spec, tj, tj1, t, comp, xm[:M], xm[M:2 * M], xm[2 * M:].reshape(s, M))[0]
+            fd[q] = (vp - vm) / 2e-6
+        if nested:
+            assert np.all(g[2 * M:] == 0)
+            fd[2 * M:] = 0.0
+        assert np.max(np.abs(g - fd)) < 1e-7 * max(1.0, np.max(np.abs(g))), (comp, g, fd)
+
+
+@pytest.mark.parametrize("code", [103, 105, 204], ids=_name)
+def test_evalsol_special_points(code):
+    """At a mesh node inside the span EvalSol takes the interval to its left (interval(),
+    Core/utils.jl:119-121) and its quintic returns that node's value; with h = 1 the
+    expanded form's stage-polynomial point (tau h) is the midpoint, so the row reads the
+    dense output; the nested form's quintic is the dense output on any mesh."""
+    rng = np.random.default_rng(code)
+    s, M = code % 100, 2
+    spec = cases.linear([[0, 1], [1, 0]], tspan=(0.0, 2.0), bc=[(0.5, 0, 1.0), (2.0, 0, 0.0)], dt=1.0,
+                        u0=[1.0, 0.0], standard=True)
+    spec = cases.random_state(spec, rng, scale=0.5)
+    K = rng.standard_normal((spec.N, s, M)) * 0.3
+    F = O.firk_residual(code, spec, K)
+    u, _ = O.firk_interpolate(code, spec, [0.5], K)
+    assert abs(F[0] - (u[0, 0] - 1.0)) < 1e-13
+    # nested: the row at a point of any interval equals the dense output there
+    spec2 = cases.random_state(cases.linear([[0, 1], [1, 0]], tspan=(0.0, 1.0), bc=[(0.37, 1, 0.5), (1.0, 0, 0.0)],
+                                            dt=0.1, u0=[1.0, 0.0], standard=True), rng, scale=0.5)
+    Fn = O.firk_residual(code + NESTED, spec2)
+    un, _ = O.firk_interpolate(code + NESTED, spec2, [0.37])
+    assert abs(Fn[0] - (un[0, 1] - 0.5)) < 1e-11
+    # a row exactly on an interior node: the left interval's quintic at tau = h
+    spec3 = cases.random_state(cases.linear([[0, 1], [1, 0]], tspan=(0.0, 1.0), bc=[(0.5, 0, 0.0), (1.0, 0, 0.0)],
+                                            dt=0.125, u0=[1.0, 0.0], standard=True), rng, scale=0.5)
+    K3 = rng.standard_normal((spec3.N, s, M)) * 0.3
+    F3 = O.firk_residual(code, spec3, K3)
+    y3 = spec3.full_guess().reshape(-1, M)
+    assert abs(F3[0] - y3[4, 0]) < 1e-12
+
+
+@pytest.mark.parametrize("code", [102, 103, 204, 404], ids=_name)
+def test_interior_row_jacobian_vs_finite_differences(code):
+    """The expanded Newton matrix with a row at pi/4 (inside an interval, dt = 0.07, and
+    on a node, dt = pi/64) against central differences of the residual; the banded
+    solve against a dense one."""
+    rng = np.random.default_rng(21 + code)
+    for dt in (0.07, math.pi / 64):
+        spec = cases.random_state(_pend(dt), rng, scale=0.1)
+        K = rng.standard_normal((spec.N, code % 100, 2)) * 0.1
+        J, F = O.firk_dense_system(code, spec, K)
+        x0 = np.zeros(J.shape[1])
+        y0, m = spec.full_guess().copy(), spec.full_mesh()
+        s, M = code % 100, 2
+        for i in range(spec.N + 1):
+            x0[i * (s + 1) * M:i * (s + 1) * M + M] = y0[i * M:(i + 1) * M]
+            if i < spec.N:
+                x0[i * (s + 1) * M + M:(i + 1) * (s + 1) * M] = K[i].reshape(-1)
+
+        def res(x):
+            y, Kx = O.firk_unpack(code, spec, x)
+            return O.firk_residual(code, spec.with_state(m, y.reshape(-1)), Kx)
+
+        Jfd = np.zeros_like(J)
+        for q in range(x0.size):
+            xp, xm = x0.copy(), x0.copy()
+            xp[q] += 1e-6
+            xm[q] -= 1e-6
+            Jfd[:, q] = (res(xp) - res(xm)) / 2e-6
+        assert np.max(np.abs(J - Jfd)) < 1e-6 * max(1.0, np.max(np.abs(J)))
+        d = O.firk_newton_step(code, spec, K)
+        assert np.max(np.abs(d - np.linalg.solve(J, -F))) <= 1e-9 * max(1.0, np.max(np.abs(d)))
+
+
+def test_nested_interior_row_jacobian():
+    """Nested form with a row at pi/4: the collocation rows' Jacobian against central
+    differences; the interior row's against central differences of EvalSol with its
+    stages held (K from zero, nodual_value), which is what the reference differentiates."""
+    code = 103 + NESTED
+    rng = np.random.default_rng(31)
+    spec = cases.random_state(_pend(0.07), rng, scale=0.1)
+    J, F = O.firk_dense_system(code, spec)
+    y0, m = spec.full_guess().copy(), spec.full_mesh()
+    Jfd = np.zeros_like(J)
+    for q in range(y0.size):
+        yp, ym = y0.copy(), y0.copy()
+        yp[q] += 1e-6
+        ym[q] -= 1e-6
+        Jfd[:, q] = (O.firk_residual(code, spec.with_state(m, yp)) - O.firk_residual(code, spec.with_state(m, ym))) / 2e-6
+    assert np.max(np.abs(J[1:] - Jfd[1:])) < 1e-5 * max(1.0, np.max(np.abs(J)))
+    # row 0 (u(pi/4)): only y_j, y_{j+1} of its interval, and not the full derivative
+    j = int(np.searchsorted(m, math.pi / 4)) - 1
+    nz = np.nonzero(J[0])[0]
+    assert set(nz) <= set(range(j * 2, j * 2 + 4))
+    d = O.firk_newton_step(code, spec)
+    assert np.max(np.abs(d - np.linalg.solve(J, -F))) <= 1e-9 * max(1.0, np.max(np.abs(d)))
+
+
+@pytest.mark.parametrize("code", ALL, ids=_name)
+@pytest.mark.parametrize("nested", [False, True], ids=["expanded", "nested"])
+def test_reference_pendulum_solves(code, nested):
+    """The reference's FIRK pendulum runs (test/expanded/firk_basic_tests.jl:5-51,
+    test/nested/firk_pendulum_tests.jl): NewtonRaphson, dt = 0.005 (0.05 for Radau IIA
+    5 and 7), adaptive except RadauIIa1, LobattoIIIb2, LobattoIIIc2.  The reference checks
+    only that they run; here every run returns Success with u(pi/4) = -pi/2 and
+    u(pi/2) = pi/2 held to the Newton tolerance."""
+    dt = 0.05 if code in (105, 107) else 0.005
+    adaptive = 0 if code in (101, 302, 402) else 1
+    c = code + (NESTED if nested else 0)
+    spec = _pend(dt)
+    sol = O.solve(O.opts(c, adaptive=adaptive, nlsolve_stages=1), spec)
+    assert sol.retcode == 0, sol.retcode
+    st = spec.with_state(sol.t, sol.u.reshape(-1))
+    F = O.firk_residual(c, st, None if nested else sol.stages)
+    assert abs(F[0]) <= 1e-6 and abs(F[1]) <= 1e-6
 
 
 # --------------------------------------------------------------- nested form

@@ -228,8 +228,6 @@ def test_firk_plan_results_and_limits(ctx):
         assert np.array_equal(x.u, y.u) and np.array_equal(x.stages, y.stages)
     ref = [O.solve(_oopts(103, abstol=1e-8, newton_abstol=1e-12), s) for s in specs]
     _compare(103, r1, ref, specs)
-    with pytest.raises(_lib.BVPError):  # interior STANDARD row
-        _lib.solve_batch(ctx, _opts(103), [cases.pendulum(0.05)])
     with pytest.raises(_lib.BVPError):  # RadauIIa1 does not adapt
         _lib.solve_batch(ctx, _opts(101), [cases.bratu(1.0, N=8)])
     with pytest.raises(_lib.BVPError):  # TrustRegion stage

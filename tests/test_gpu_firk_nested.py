@@ -72,10 +72,11 @@ def _compare(nc, got, ref, specs, tol=1e-9, abstol=None):
             assert ek <= tol, ek
         else:
             st = s.with_state(g.t, g.u.reshape(-1))
-            d = O.firk_newton_step(nc, st)
+            kst = None if nc >= NESTED else g.stages
+            d = O.firk_newton_step(nc, st, kst)
             err = float(np.max(np.abs(d)) / max(1.0, np.max(np.abs(g.u))))
             if abstol is not None:
-                assert O.firk_defect(nc, st)[1] <= abstol
+                assert O.firk_defect(nc, st, kst)[1] <= abstol
         assert err <= tol, err
 
 
@@ -154,3 +155,44 @@ def test_nested_user_api(ctx):
     for t in (0.04, 0.37, 0.9):
         assert np.max(np.abs(sol(t) - ex(t))) <= 1e-8
     assert sol.stages.shape == (len(sol.t) - 1, 5, 2)
+
+
+# --------------------------------------------------------------- interior STANDARD rows
+@pytest.mark.parametrize("code", [102, 103, 105, 204, 304, 404], ids=_name)
+@pytest.mark.parametrize("nested", [False, True], ids=["expanded", "nested"])
+def test_interior_row_components_match_oracle(ctx, code, nested):
+    """A row at pi/4 (inside an interval, dt = 0.07; on a node, dt = pi/64) and one at
+    pi/2 (the README pendulum, FIRK test/expanded/firk_basic_tests.jl:5-22): residual,
+    Newton step (the row's linearisation through EvalSol, the stage corrections eliminated
+    through the interval record) and defect against the oracle."""
+    import math
+    _say("firk interior rows components", _name(code), nested)
+    c = code + (NESTED if nested else 0)
+    rng = np.random.default_rng(600 + code)
+    for dt in (0.07, math.pi / 64):
+        spec = cases.random_state(cases.pendulum(dt), rng, scale=0.1)
+        K = None if nested else [rng.standard_normal((spec.N, code % 100, 2)) * 0.1]
+        (g,) = _lib.firk_residual(ctx, c, [spec], K)
+        ref = O.firk_residual(c, spec, None if nested else K[0])
+        assert np.max(np.abs(g - ref)) <= 1e-11 * max(1.0, np.max(np.abs(ref))), np.max(np.abs(g - ref))
+        (d,) = _lib.firk_newton_step(ctx, c, [spec], K)
+        rd = O.firk_newton_step(c, spec, None if nested else K[0])
+        assert np.max(np.abs(d - rd)) <= 1e-9 * max(1.0, np.max(np.abs(rd))), np.max(np.abs(d - rd))
+
+
+@pytest.mark.parametrize("code", [102, 103, 105, 107, 203, 204, 304, 305, 403, 404], ids=_name)
+@pytest.mark.parametrize("nested", [False, True], ids=["expanded", "nested"])
+def test_reference_pendulum_solves_match_oracle(ctx, code, nested):
+    """The reference's FIRK pendulum runs (test/expanded/firk_basic_tests.jl:5-51,
+    test/nested/firk_pendulum_tests.jl): NewtonRaphson, dt = 0.005 (0.05 for Radau IIA 5
+    and 7), adaptive, Newton tolerance 1e-12; same return code, mesh size, outer and
+    Newton iterations as the oracle, nodes within 1e-9."""
+    _say("firk pendulum solve", _name(code), nested)
+    c = code + (NESTED if nested else 0)
+    dt = 0.05 if code in (105, 107) else 0.005
+    spec = cases.pendulum(dt)
+    o = _opts(c, nlsolve_stages=_abi.STAGE_NEWTON, newton_abstol=1e-12)
+    (g,) = _lib.solve_batch(ctx, o, [spec])
+    r = O.solve(_oopts(c, nlsolve_stages=1, newton_abstol=1e-12), spec)
+    _compare(c, [g], [r], [spec])
+    assert g.retcode == 0 and g.newton_iters == r.newton_iters
