@@ -308,6 +308,7 @@ int batch_build(bvp_ctx *ctx, const bvp_opts *o, int32_t n, const bvp_instance *
         }
         D.firk_tab = dtab;
         D.firk_nested = nested;
+        D.firk_irows = (o->nlsolve_stages & BVP_STAGE_TRUST_REGION) != 0;  // the stage lives in those kernels
         for (int i = 0; i < n; ++i)
             if (inst[i].bc_kind == BVP_BC_STANDARD)
                 for (int k = 0; k < M; ++k)

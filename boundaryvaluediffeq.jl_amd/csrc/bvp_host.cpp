@@ -91,8 +91,17 @@ int validate_opts(const bvp_opts *o, int32_t n, const bvp_instance *inst, int M,
         if (o->controller != BVP_CONTROL_DEFECT)
             return fail(err, BVP_ENOTSUP, "FIRK methods use DefectControl");
         if (o->warm_start != BVP_WARM_NONE) return fail(err, BVP_ENOTSUP, "no learned warm start for FIRK methods");
-        if (!(o->nlsolve_stages & 3) || (o->nlsolve_stages & ~3))
-            return fail(err, BVP_ENOTSUP, "FIRK methods run the NewtonRaphson stages (nlsolve_stages 1, 2 or 3)");
+        if (!(o->nlsolve_stages & 7) || (o->nlsolve_stages & ~7))
+            return fail(err, BVP_EINVAL, "nlsolve_stages must be a non-empty subset of bits 0-2");
+        if (o->nlsolve_stages & BVP_STAGE_TRUST_REGION) {
+            if (o->method >= BVP_FIRK_NESTED)
+                return fail(err, BVP_ENOTSUP, "the TrustRegion stage runs on the expanded FIRK form only");
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < M; ++k)
+                    if (inst[i].bc_t[k] != inst[i].t0 && inst[i].bc_t[k] != inst[i].t1)
+                        return fail(err, BVP_ENOTSUP,
+                                    "instance %d: the FIRK TrustRegion stage needs boundary rows at t0 or t1", i);
+        }
     } else if (o->method != BVP_MIRK4 && o->method != BVP_MIRK6) {
         return fail(err, BVP_EINVAL, "method must be 4, 6 or a FIRK method code");
     }

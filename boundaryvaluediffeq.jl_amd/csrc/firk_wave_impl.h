@@ -777,6 +777,262 @@ struct FirkNewtonOut {
     double fn;
 };
 
+// ------------------------------------------------------------------ trust region (expanded form)
+// Products with the expanded Jacobian for the TrustRegion stage, rows
+//   Phi_i = y_{i+1} - y_i - h sum_j b_j K_ij,  G_ir = f(t_i + c_r h, Y_ir) - K_ir,
+//   Y_ir = y_i + h sum_j a_rj K_ij,  boundary rows y_node.comp - value
+// (the Jacobian of FIRK/collocation.jl:59-93 that the oracle assembles), stage
+// Jacobians J_r evaluated at (y, K).  firk_jt_f: g = J'F into (gy [N+1][M], gK [N][S][M]).
+template <int M, int S>
+__device__ inline void firk_jt_f(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
+                                 const double *K, int N, int lane, double *gy, double *gK) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    // pass 1: each interval's terms on y_i and K_i (node N: zero)
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
+        load_stages<M, S>(K, k, Kk);
+        firk_interval<M, S, true>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            double v = -Phi[c];
+#pragma unroll
+            for (int r = 0; r < S; ++r)
+#pragma unroll
+                for (int a = 0; a < M; ++a) v += J[r][a][c] * G[r][a];
+            gy[(size_t)k * M + c] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                double v = -h * T.b[j] * Phi[c] - G[j][c];
+#pragma unroll
+                for (int r = 0; r < S; ++r)
+#pragma unroll
+                    for (int a = 0; a < M; ++a) v += h * T.A[r][j] * J[r][a][c] * G[r][a];
+                gK[((size_t)k * S + j) * M + c] = v;
+            }
+    }
+    if (lane == P - 1)
+#pragma unroll
+        for (int c = 0; c < M; ++c) gy[(size_t)N * M + c] = 0.0;
+    __syncthreads();
+    // pass 2: Phi_i on y_{i+1}
+    for (int k = start; k < start + len; ++k) {
+        const double h = mesh[k + 1] - mesh[k];
+        double Kk[S][M];
+        load_stages<M, S>(K, k, Kk);
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < S; ++j) a += Kk[j][c] * T.b[j];
+            gy[(size_t)(k + 1) * M + c] += -h * a + (y[(size_t)(k + 1) * M + c] - y[(size_t)k * M + c]);
+        }
+    }
+    __syncthreads();
+    if (lane == 0)
+        for (int r = 0; r < M; ++r) {
+            const BcLoc L = bc_locate<M>(I, r, mesh, N);
+            const size_t q = (size_t)L.node * M + sel<M>(I.bcc, r);
+            gy[q] += y[q] - sel<M>(I.bcv, r);
+        }
+    __syncthreads();
+}
+
+// |J g|^2 for g = (gy, gK), in all lanes
+template <int M, int S>
+__device__ inline double firk_jac_norm2(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
+                                        const double *K, const double *gy, const double *gK, int N, int lane) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double acc = 0.0;
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M], gk[S][M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) { yi[c] = y[(size_t)k * M + c]; yj[c] = y[(size_t)(k + 1) * M + c]; }
+        load_stages<M, S>(K, k, Kk);
+        load_stages<M, S>(gK, k, gk);
+        firk_interval<M, S, true>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+            double v = gy[(size_t)(k + 1) * M + a] - gy[(size_t)k * M + a];
+#pragma unroll
+            for (int j = 0; j < S; ++j) v -= h * T.b[j] * gk[j][a];
+            acc += v * v;
+        }
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                double v = -gk[r][a];
+#pragma unroll
+                for (int c = 0; c < M; ++c) {
+                    double w = gy[(size_t)k * M + c];
+#pragma unroll
+                    for (int j = 0; j < S; ++j) w += h * T.A[r][j] * gk[j][c];
+                    v += J[r][a][c] * w;
+                }
+                acc += v * v;
+            }
+    }
+    if (lane == 0)
+        for (int r = 0; r < M; ++r) {
+            const BcLoc L = bc_locate<M>(I, r, mesh, N);
+            const double v = gy[(size_t)L.node * M + sel<M>(I.bcc, r)];
+            acc += v * v;
+        }
+    const double s = uniform(wave_sum(acc));
+    __syncthreads();
+    return s;
+}
+
+// |F|^2 and max|F| at (y + a dy + b gy, K + a dK + b gK), in all lanes (NaN if not finite)
+template <int M, int S>
+__device__ inline double firk_trial2(const Inst<M> &I, const FirkTab &T, const double *mesh, const double *y,
+                                     const double *K, const double *dy, const double *dK, double a, const double *gy,
+                                     const double *gK, double b, int N, int lane, double &mx) {
+    int P, start, len;
+    chunk_of(N, lane, P, start, len);
+    double ss = 0.0, m = 0.0;
+    for (int k = start; k < start + len; ++k) {
+        const double ti = mesh[k], h = mesh[k + 1] - ti;
+        double yi[M], yj[M], Kk[S][M], G[S][M], Phi[M], J[S][M][M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const size_t p = (size_t)k * M + c, q = (size_t)(k + 1) * M + c;
+            yi[c] = y[p] + a * dy[p] + b * gy[p];
+            yj[c] = y[q] + a * dy[q] + b * gy[q];
+        }
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                const size_t q = ((size_t)k * S + r) * M + c;
+                Kk[r][c] = K[q] + a * dK[q] + b * gK[q];
+            }
+        firk_interval<M, S, false>(I, T, ti, h, yi, yj, Kk, G, Phi, J);
+#pragma unroll
+        for (int c = 0; c < M; ++c) { ss += Phi[c] * Phi[c]; m = fmax(m, fabs(Phi[c])); }
+#pragma unroll
+        for (int r = 0; r < S; ++r)
+#pragma unroll
+            for (int c = 0; c < M; ++c) { ss += G[r][c] * G[r][c]; m = fmax(m, fabs(G[r][c])); }
+    }
+    if (lane < M) {
+        const BcLoc L = bc_locate<M>(I, lane, mesh, N);
+        const size_t q = (size_t)L.node * M + sel<M>(I.bcc, lane);
+        const double v = y[q] + a * dy[q] + b * gy[q] - sel<M>(I.bcv, lane);
+        ss += v * v;
+        m = fmax(m, fabs(v));
+    }
+    const bool bad = !(fabs(ss) <= 1.7976931348623157e308);
+    const bool anybad = wave_any(bad);
+    ss = uniform(wave_sum(ss));
+    mx = anybad ? NAN : uniform(wave_max(m));
+    __syncthreads();
+    return anybad ? NAN : ss;
+}
+
+// The TrustRegion stage on the expanded system (dogleg between the Newton step and the
+// Cauchy point along J'F, initial radius max(|F|, max x - min x) / 11, the "Simple"
+// radius update; the restatement the oracle follows, Core/default_internal_solve.jl:
+// 37-43).  Boundary rows read nodes (the host refuses rows inside the span here).
+// gy, gK: work areas for J'F.
+template <int M, int S>
+__device__ inline int firk_trust_region(const BatchDev &B, const Inst<M> &I, const FirkTab &T,
+                                        const FirkNest<M> &nest, Shared<M> &sh, const double *mesh, double *y,
+                                        double *K, int N, int lane, double *scr, double *loc, double *dy, double *dK,
+                                        double *gy, double *gK, FirkNewtonOut &out) {
+    double mx;
+    double ss = firk_trial2<M, S>(I, T, mesh, y, K, y, K, 0.0, y, K, 0.0, N, lane, mx);
+    double xmax = -INFINITY, xmin = INFINITY;
+    for (int q = lane; q < (N + 1) * M; q += WAVE) { xmax = fmax(xmax, y[q]); xmin = fmin(xmin, y[q]); }
+    for (int q = lane; q < N * S * M; q += WAVE) { xmax = fmax(xmax, K[q]); xmin = fmin(xmin, K[q]); }
+    xmax = uniform(wave_max(xmax));
+    xmin = uniform(-wave_max(-xmin));
+    __syncthreads();
+    double radius = fmax(sqrt(ss), xmax - xmin) / 11.0;
+    int shrinks = 0;
+    bool fresh = true;
+    double dn2 = 0.0, l2 = 0.0, gdn = 0.0, jg2 = 0.0;
+    for (int it = 0;; ++it) {
+        out.fn = mx;
+        if (!(mx <= 1.7976931348623157e308)) return BVP_RET_UNSTABLE;
+        if (mx <= B.newton_tol) return BVP_RET_SUCCESS;
+        if (it >= B.max_newton) return BVP_RET_MAXITERS;
+        if (fresh) {
+            Rel<M> rel;
+            bool sing = false;
+            firk_forward<M, S>(I, T, nest, sh, mesh, y, K, N, lane, scr, loc, rel, sing);
+            if (sing || !firk_correction<M, S, false>(I, T, nest, sh, mesh, y, K, N, lane, scr, loc, rel, dy, dK))
+                return BVP_RET_FAILURE;
+            firk_jt_f<M, S>(I, T, mesh, y, K, N, lane, gy, gK);
+            double a = 0.0, b = 0.0, c = 0.0;
+            for (int q = lane; q < (N + 1) * M; q += WAVE) {
+                a += dy[q] * dy[q];
+                b += gy[q] * gy[q];
+                c += gy[q] * dy[q];
+            }
+            for (int q = lane; q < N * S * M; q += WAVE) {
+                a += dK[q] * dK[q];
+                b += gK[q] * gK[q];
+                c += gK[q] * dK[q];
+            }
+            dn2 = uniform(wave_sum(a));
+            l2 = uniform(wave_sum(b));
+            gdn = uniform(wave_sum(c));
+            __syncthreads();
+            jg2 = firk_jac_norm2<M, S>(I, T, mesh, y, K, gy, gK, N, lane);
+            fresh = false;
+        }
+        out.iters++;
+        // step d = cn dn + cg g and the predicted reduction of the linear model
+        double cn, cg, pred;
+        if (sqrt(dn2) <= radius) {
+            cn = 1.0; cg = 0.0; pred = ss;
+        } else {
+            const double lg = sqrt(l2), dcauchy = l2 * lg / jg2;
+            if (dcauchy >= radius) {
+                cn = 0.0; cg = -radius / lg;
+                pred = 2.0 * radius * lg - radius * radius * jg2 / l2;
+            } else {
+                const double beta = l2 / jg2;
+                const double A2 = dn2 + 2.0 * beta * gdn + beta * beta * l2;
+                const double B2 = -2.0 * (beta * gdn + beta * beta * l2);
+                const double C2 = dcauchy * dcauchy - radius * radius;
+                const double tau = (-B2 + sqrt(fmax(0.0, B2 * B2 - 4.0 * A2 * C2))) / (2.0 * A2);
+                cn = tau; cg = -(1.0 - tau) * beta;
+                pred = ss - (1.0 - tau) * (1.0 - tau) * (ss - l2 * l2 / jg2);
+            }
+        }
+        double mxt;
+        const double sst = firk_trial2<M, S>(I, T, mesh, y, K, dy, dK, cn, gy, gK, cg, N, lane, mxt);
+        const double rho = (ss - sst) / pred;
+        if (rho > 1e-4) {
+            for (int q = lane; q < (N + 1) * M; q += WAVE) y[q] += cn * dy[q] + cg * gy[q];
+            for (int q = lane; q < N * S * M; q += WAVE) K[q] += cn * dK[q] + cg * gK[q];
+            __syncthreads();
+            ss = sst;
+            mx = mxt;
+            fresh = true;
+        }
+        if (rho < 0.25) {
+            radius *= 0.25;
+            if (++shrinks > 32) { out.fn = mx; return BVP_RET_FAILURE; }
+        } else {
+            shrinks = 0;
+            if (rho > 0.75) radius *= 2.0;
+        }
+    }
+}
+
+
 // The NewtonRaphson stages of the polyalgorithm (Core/default_internal_solve.jl:31-45):
 // plain, then with the BackTracking line search, each from the same start, stopping
 // when max|F| <= tol (checked before every step).  Work areas: dy [cap][M], dK, the
@@ -787,14 +1043,23 @@ __device__ inline FirkNewtonOut firk_newton(const BatchDev &B, const Inst<M> &I,
                                             double *K, int N, int lane, double *scr, double *loc, double *dy,
                                             double *dK, double *y0, double *K0) {
     FirkNewtonOut out{BVP_RET_MAXITERS, 0, -1, NAN};
-    const int stages = B.stages & 3;
+    // the TrustRegion stage: compiled into the general kernels (IR) only
+    const int stages = B.stages & (IR ? 7 : 3);
     bool first = true;
-    if ((stages & 3) == 3) firk_copy_state<M, S>(y0, K0, y, K, N, lane);
-    for (int st = 0; st < 2; ++st) {
+    if (__popc(stages) > 1) firk_copy_state<M, S>(y0, K0, y, K, N, lane);
+    for (int st = 0; st < 3; ++st) {
         if (!(stages & (1 << st))) continue;
         if (!first) firk_copy_state<M, S>(y, K, y0, K0, N, lane);
         first = false;
         out.stage = st;
+        if constexpr (IR) {
+            if (st == 2 && !nest.on) {
+                // the last stage: the start copy is free for J'F
+                out.ret = firk_trust_region<M, S>(B, I, T, nest, sh, mesh, y, K, N, lane, scr, loc, dy, dK, y0, K0,
+                                                  out);
+                break;
+            }
+        }
         for (int it = 0;; ++it) {
             Rel<M> rel;
             bool sing = false;
@@ -1004,6 +1269,7 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
         rn_valid = true;
         cnt_it += (long long)N * nl.iters;
         if (nl.stage == 1) fb |= 1;
+        if (nl.stage == 2) fb |= 16;  // the TrustRegion stage ran
         nl_ret = nl.ret;
         double error_norm = 2.0 * B.abstol;
         info = nl.ret;
@@ -1077,7 +1343,7 @@ __device__ inline void firk_solve_instance(const BatchDev &B, const FirkTab &T, 
 
 // ------------------------------------------------------------------ kernels
 // Persistent FIRK solve: each wave pulls instances from the queue.  NEST: the nested form;
-// IR: STANDARD rows inside the span may occur.
+// IR: the general kernels (STANDARD rows inside the span, the TrustRegion stage).
 template <int M, int S, bool NEST, bool IR>
 __device__ __forceinline__ void firk_solve_queue(const BatchDev &B) {
     __shared__ FirkShared<M> sh;

@@ -94,7 +94,8 @@ struct BatchDev {
     const double *init_K;
     int firk_nested;          // the nested form: u0c holds each instance's inner-solve start k0
     double nest_tol;          // its inner tolerance
-    int firk_irows;           // some STANDARD boundary row lies inside the span (EvalSol rows)
+    int firk_irows;           // the general FIRK kernels: some STANDARD row lies inside the span
+                              // (EvalSol rows), or the TrustRegion stage is asked for
 
     // options
     int adaptive, max_sub, max_newton, stages, transfer_mode;

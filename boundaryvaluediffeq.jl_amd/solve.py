@@ -156,9 +156,9 @@ class _FIRK:
     values and stages; nested_nlsolve = true solves for the node values only, each
     interval's stages from an inner NewtonRaphson (lib/BoundaryValueDiffEqFIRK/src/
     collocation.jl:118-205), whose tolerance nested_nlsolve_kwargs["abstol"] sets
-    (default eps^(4/5)).  On the GPU the outer nonlinear solver runs the NewtonRaphson
-    stages of the polyalgorithm (plain, then with BackTracking); nlsolve = None selects
-    both."""
+    (default eps^(4/5)).  On the GPU nlsolve = None runs the NewtonRaphson stages of the
+    polyalgorithm (plain, then with BackTracking); a nlsolve that names the TrustRegion
+    stage adds it (expanded form, boundary rows at the ends)."""
     code = 0
     stages = 0
     order = 0

@@ -56,8 +56,9 @@ extern "C" {
  * reads EvalSol of the FIRK cache (FIRK/interpolation.jl:338-441): the quintic of its
  * interval, whose midpoint pair the expanded form takes at (t - t_j) h in the stage
  * polynomial's variable, as the reference does, and the nested form at the midpoint from
- * stages solved from zero.  On the GPU: M = 1..4, DefectControl, FP64, the NewtonRaphson
- * stages of the polyalgorithm (nlsolve_stages bits 0 and 1). */
+ * stages solved from zero.  On the GPU: M = 1..4, DefectControl, FP64; nlsolve_stages
+ * selects the stages of the polyalgorithm, the TrustRegion stage (bit 2) on the
+ * expanded form with boundary rows at t0 or t1. */
 #define BVP_FIRK_NESTED     1000  /* added to a FIRK code: the nested form (below) */
 #define BVP_RADAU_IIA(s)    (100 + (s))
 #define BVP_LOBATTO_IIIA(s) (200 + (s))

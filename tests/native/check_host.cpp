@@ -258,7 +258,12 @@ int main() {
         o.nlsolve_stages = 3;
         expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_OK, "radau5 options");
         o.nlsolve_stages = 7;
-        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_ENOTSUP, "firk trust region");
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_OK, "firk trust region");
+        o.method = 103 + BVP_FIRK_NESTED;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_ENOTSUP, "nested trust region");
+        o.method = 103;
+        o.nlsolve_stages = 8;
+        expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_EINVAL, "firk stage bits");
         o.nlsolve_stages = 3;
         o.method = 101;
         expect(bvphost::validate_opts(&o, 1, a.data(), 2, false, err) == BVP_ENOTSUP, "radau1 adaptive");

@@ -230,10 +230,14 @@ def test_firk_plan_results_and_limits(ctx):
     _compare(103, r1, ref, specs)
     with pytest.raises(_lib.BVPError):  # RadauIIa1 does not adapt
         _lib.solve_batch(ctx, _opts(101), [cases.bratu(1.0, N=8)])
-    with pytest.raises(_lib.BVPError):  # TrustRegion stage
-        o = _opts(103)
+    with pytest.raises(_lib.BVPError):  # TrustRegion stage: expanded form only
+        o = _opts(103 + _abi.FIRK_NESTED)
         o.nlsolve_stages = _abi.STAGES_DEFAULT
         _lib.solve_batch(ctx, o, [cases.bratu(1.0, N=8)])
+    with pytest.raises(_lib.BVPError):  # ... and boundary rows at the ends
+        o = _opts(103)
+        o.nlsolve_stages = _abi.STAGES_DEFAULT
+        _lib.solve_batch(ctx, o, [cases.pendulum(0.05)])
     with pytest.raises(_lib.BVPError):  # FIRK codes go through bvp_firk_*
         _lib.residual(ctx, 103, [cases.bratu(1.0, N=8)])
 
@@ -258,3 +262,23 @@ def test_user_api_solve_and_dense_output(ctx):
         assert abs(sol(0.5, idxs=0) - ua(0.5)[0]) < 1e-6
         if isinstance(alg, bvp.RadauIIa5):
             assert sol.stages.shape == (len(sol.t) - 1, 5, 2)
+
+
+@pytest.mark.parametrize("code", [102, 103, 105, 204, 304, 404], ids=_name)
+def test_firk_trust_region_matches_oracle(ctx, code):
+    """The TrustRegion stage of the polyalgorithm on the expanded form: alone (nlsolve_stages
+    = 4) on a fixed mesh and through the adaptive outer loop, and last after the
+    NewtonRaphson stages with an iteration cap that makes every stage fail; Bratu instances
+    and a linear problem; same return codes, iteration counts and stages as the oracle,
+    nodes within 1e-9."""
+    _say("firk trust region", _name(code))
+    specs = [cases.bratu(lam, N=n) for lam, n in ((0.7, 12), (2.0, 20), (3.2, 16))]
+    specs.append(cases.linear([[0, 1], [0, 0]], bc=([(0, 5.0)], [(0, 0.0)]), dt=0.2, u0=[5.0, -3.5]))
+    for kw in (dict(adaptive=0, nlsolve_stages=4), dict(nlsolve_stages=4, abstol=1e-7),
+               dict(adaptive=0, nlsolve_stages=7, max_newton_iters=2)):
+        o = _opts(code, newton_abstol=1e-12, **kw)
+        got = _lib.solve_batch(ctx, o, specs)
+        ref = [O.solve(_oopts(code, newton_abstol=1e-12, **kw), s) for s in specs]
+        _compare(code, got, ref, specs)
+        for g, r in zip(got, ref):
+            assert g.newton_iters == r.newton_iters and g.fallback_used == r.fallback_used
