@@ -136,9 +136,13 @@ __device__ inline bool lu_solve_loop(double (&W)[NN][NN], double (&R)[NN][NR]) {
     return true;
 }
 
+// Local systems up to 12 unknowns run fully unrolled (the arrays at fixed offsets: in
+// registers, the overflow spilled at known slots); above, the rolled loops over private
+// arrays spill less.  Measured at M = 2, s = 5 (10 unknowns) on 16,384 Bratu BVPs:
+// 121 ms rolled, 88.5 ms unrolled.
 template <int NN, int NR>
 __device__ inline bool lu_solve(double (&W)[NN][NN], double (&R)[NN][NR]) {
-    if constexpr (NN <= 8) return lu_solve_small<NN, NR>(W, R);
+    if constexpr (NN <= 12) return lu_solve_small<NN, NR>(W, R);
     else return lu_solve_loop<NN, NR>(W, R);
 }
 
