@@ -11,7 +11,7 @@ LIB = OUT / "libbvp_mirk.so"
 SOURCES = ["bvp_api.hip", "bvp_host.cpp", "mirk_wave.hip", "wave_m1.hip", "wave_m2.hip", "wave_m2v.hip", "wave_m2_bratu.hip", "wave_m2_singpert.hip", "wave_m3.hip", "wave_m4.hip", "wave_m6_mirk4.hip", "wave_m6_mirk6.hip", "wave_ge_m12.hip", "wave_ge_m34.hip",
            "wave_f32_m2.hip", "wave_f32_m2v.hip", "wave_f32_bratu.hip", "wave_f32_singpert.hip", "wave_f32_m134.hip",
            "block_dispatch.hip", "block_m8.hip", "block_m16.hip", "block_m32.hip", "mlp.hip",
-           "firk_dispatch.hip", "firk_m1.hip", "firk_m2.hip", "firk_m3.hip", "firk_m4.hip"]
+           "firk_dispatch.hip", "firk_m1.hip", "firk_m2.hip", "firk_m3.hip", "firk_m4.hip", "firk_m6a.hip", "firk_m6b.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BVP_OFFLOAD_ARCH", "gfx950")

@@ -84,7 +84,8 @@ int validate_opts(const bvp_opts *o, int32_t n, const bvp_instance *inst, int M,
         if (!firk_tableau(o->method, T))
             return fail(err, BVP_EINVAL, "unknown FIRK method %d (BVP_RADAU_IIA(1, 2, 3, 5, 7), "
                                          "BVP_LOBATTO_IIIA/B/C(2..5))", o->method);
-        if (blk || M > 4) return fail(err, BVP_ENOTSUP, "FIRK methods run on the M <= 4 kernels");
+        if (blk || (M > 4 && M != 6))
+            return fail(err, BVP_ENOTSUP, "FIRK methods run on the wave kernels (M = 1..4, 6)");
         if (o->adaptive && !T.adaptive)
             return fail(err, BVP_ENOTSUP, "RadauIIa1, LobattoIIIb2 and LobattoIIIc2 do not adapt the mesh: "
                                           "set adaptive = 0");

@@ -17,6 +17,7 @@ FIRK_DECL_M(1)
 FIRK_DECL_M(2)
 FIRK_DECL_M(3)
 FIRK_DECL_M(4)
+FIRK_DECL_M(6)
 #undef FIRK_DECL_M
 #undef FIRK_DECL
 }  // namespace bvpdev
@@ -24,13 +25,14 @@ FIRK_DECL_M(4)
 using namespace bvpdev;
 
 bool firk_supports(int M, int S) {
-    return M >= 1 && M <= 4 && (S == 1 || S == 2 || S == 3 || S == 4 || S == 5 || S == 7);
+    return ((M >= 1 && M <= 4) || M == 6) && (S == 1 || S == 2 || S == 3 || S == 4 || S == 5 || S == 7);
 }
 
 // one case per (M, S): CALL is the launcher call with MM, SS in scope
 #define FIRK_SWITCH(CALL)                                                                       \
     switch (B.M * 16 + S) {                                                                     \
     FIRK_CASES_M(1, CALL) FIRK_CASES_M(2, CALL) FIRK_CASES_M(3, CALL) FIRK_CASES_M(4, CALL)   \
+    FIRK_CASES_M(6, CALL)                                                                       \
     default: return hipErrorInvalidValue;                                                       \
     }
 #define FIRK_CASE(MM_, SS_, CALL)   \
@@ -65,7 +67,7 @@ int firk_blocks_per_cu(int M, int S, int variant, int nested) {
         return variant == 1 ? firk_occupancy_t<MM, SS, 2>(nested) : firk_occupancy_t<MM, SS, 1>(nested);
 #define FIRK_OCC_M(MM) FIRK_OCC(MM, 1) FIRK_OCC(MM, 2) FIRK_OCC(MM, 3) FIRK_OCC(MM, 4) FIRK_OCC(MM, 5) FIRK_OCC(MM, 7)
     switch (M * 16 + S) {
-        FIRK_OCC_M(1) FIRK_OCC_M(2) FIRK_OCC_M(3) FIRK_OCC_M(4)
+        FIRK_OCC_M(1) FIRK_OCC_M(2) FIRK_OCC_M(3) FIRK_OCC_M(4) FIRK_OCC_M(6)
     default: return 4;
     }
 #undef FIRK_OCC_M

@@ -1527,10 +1527,11 @@ hipError_t firk_launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
         if (B.firk_nested) k_firk_solve<M, S, true, true><<<blocks, 64, 0, s>>>(B);
         else k_firk_solve<M, S, false, true><<<blocks, 64, 0, s>>>(B);
     } else if (B.firk_nested) {
-        if constexpr (WPE == 2) k_firk_solve_w2<M, S, true><<<blocks, 64, 0, s>>>(B);
+        // M = 6: the one-wave-per-SIMD kernels serve both variants
+        if constexpr (WPE == 2 && M <= 4) k_firk_solve_w2<M, S, true><<<blocks, 64, 0, s>>>(B);
         else k_firk_solve<M, S, true><<<blocks, 64, 0, s>>>(B);
     } else {
-        if constexpr (WPE == 2) k_firk_solve_w2<M, S, false><<<blocks, 64, 0, s>>>(B);
+        if constexpr (WPE == 2 && M <= 4) k_firk_solve_w2<M, S, false><<<blocks, 64, 0, s>>>(B);
         else k_firk_solve<M, S, false><<<blocks, 64, 0, s>>>(B);
     }
     return hipGetLastError();
@@ -1554,11 +1555,11 @@ int firk_occupancy_t(int nested) {
     int nb = 0;
     hipError_t e;
     if (nested) {
-        if constexpr (WPE == 2)
+        if constexpr (WPE == 2 && M <= 4)
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S, true>, 64, 0);
         else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S, true>, 64, 0);
     } else {
-        if constexpr (WPE == 2)
+        if constexpr (WPE == 2 && M <= 4)
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S, false>, 64, 0);
         else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S, false>, 64, 0);
     }

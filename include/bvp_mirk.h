@@ -56,7 +56,7 @@ extern "C" {
  * reads EvalSol of the FIRK cache (FIRK/interpolation.jl:338-441): the quintic of its
  * interval, whose midpoint pair the expanded form takes at (t - t_j) h in the stage
  * polynomial's variable, as the reference does, and the nested form at the midpoint from
- * stages solved from zero.  On the GPU: M = 1..4, DefectControl, FP64; nlsolve_stages
+ * stages solved from zero.  On the GPU: M = 1..4 and 6, DefectControl, FP64; nlsolve_stages
  * selects the stages of the polyalgorithm, the TrustRegion stage (bit 2) on the
  * expanded form with boundary rows at t0 or t1. */
 #define BVP_FIRK_NESTED     1000  /* added to a FIRK code: the nested form (below) */
@@ -335,7 +335,7 @@ int bvp_mirk_interpolate(bvp_ctx *ctx, int32_t method, int32_t n, const bvp_inst
 /* ---- FIRK components.  The state of instance i is its (mesh, guess) and its stage
  * values: `stages` holds N_i*s*M doubles per instance (interval, then stage, then
  * component; instances back to back), or is NULL for the stages the reference
- * starts from, K_ij = y_{i+1} (extend_y, FIRK/firk.jl:76-83).  M = 1..4.  Slot sizes
+ * starts from, K_ij = y_{i+1} (extend_y, FIRK/firk.jl:76-83).  M = 1..4, 6.  Slot sizes
  * below use n_x = M*(N*(s+1)+1), the length of the
  * expanded unknown vector.  With a nested code (+ BVP_FIRK_NESTED) `stages` is not
  * read (the inner solves give them, from K0, at the default inner tolerance), and

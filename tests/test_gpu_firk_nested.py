@@ -196,3 +196,19 @@ def test_reference_pendulum_solves_match_oracle(ctx, code, nested):
     r = O.solve(_oopts(c, nlsolve_stages=1, newton_abstol=1e-12), spec)
     _compare(c, [g], [r], [spec])
     assert g.retcode == 0 and g.newton_iters == r.newton_iters
+
+
+# --------------------------------------------------------------- state dimension 6
+@pytest.mark.parametrize("code", [103, 105, 204, 105 + NESTED], ids=lambda c: _nname(c - NESTED) if c > NESTED
+                         else _name(c))
+def test_swirling_flow_matches_oracle(ctx, code):
+    """FIRK test/expanded/firk_basic_tests.jl:159-188: Swirling Flow III (M = 6, eps = 0.01,
+    zero guess, dt = 0.01), adaptive at abstol 1e-6; same return code, mesh size and
+    iterations as the oracle, nodes within 1e-9."""
+    _say("firk swirling flow", code)
+    spec = cases.swirl(0.01, dt=0.01)
+    o = _opts(code, newton_abstol=1e-12)
+    (g,) = _lib.solve_batch(ctx, o, [spec])
+    r = O.solve(_oopts(code, newton_abstol=1e-12), spec)
+    _compare(code, [g], [r], [spec], abstol=1e-6)
+    assert g.retcode == 0 and g.newton_iters == r.newton_iters
