@@ -282,3 +282,19 @@ def test_firk_trust_region_matches_oracle(ctx, code):
         _compare(code, got, ref, specs)
         for g, r in zip(got, ref):
             assert g.newton_iters == r.newton_iters and g.fallback_used == r.fallback_used
+
+
+@pytest.mark.parametrize("code", [202, 203, 204, 205, 303, 304, 305, 403, 404, 405, 102, 103, 105, 107], ids=_name)
+def test_reference_singular_bvp_case(ctx, code):
+    """FIRK test/expanded/singular_bvp_tests.jl:36-97 as the reference runs it (the problem
+    passes no singular_term, so the ODE is y1' = y2, y2' = -y1): y(0) = 1, y(1) = sin(1),
+    u0 = [1, 0], dt = 0.01, default options; Success, and the oracle's return code, mesh size
+    and nodes (within 1e-9)."""
+    import math
+    _say("firk singular-bvp case", _name(code))
+    spec = cases.linear([[0, 1], [-1, 0]], tspan=(0.0, 1.0), bc=([(0, 1.0)], [(0, math.sin(1.0))]), dt=0.01,
+                        u0=[1.0, 0.0])
+    (g,) = _lib.solve_batch(ctx, _opts(code, newton_abstol=1e-12), [spec])
+    r = O.solve(_oopts(code, newton_abstol=1e-12), spec)
+    _compare(code, [g], [r], [spec])
+    assert g.retcode == 0
