@@ -580,3 +580,21 @@ def test_nested_adaptive_matches_expanded_meshes(code):
         assert a.retcode == b.retcode == 0 and a.N == b.N and a.outer_iters == b.outer_iters
         assert np.max(np.abs(a.t - b.t)) < 1e-5 and np.max(np.abs(a.u - b.u)) < 1e-5
         assert b.defect_norm <= 1e-8
+
+
+@pytest.mark.parametrize("code", [102, 103, 105, 204, 304, 404], ids=_name)
+def test_trust_region_reaches_the_newton_solution(code):
+    """The TrustRegion stage alone (nlsolve_stages = 4) reaches the collocation solution
+    NewtonRaphson reaches on the same fixed mesh (Bratu lambda = 2 and 3.2 from zero), to
+    1e-10; on a linear problem, where NewtonRaphson takes one step, the trust region starts
+    inside a radius of max(|F|, max x - min x) / 11 and needs more, to the same solution."""
+    for lam in (2.0, 3.2):
+        spec = cases.bratu(lam, N=16)
+        a = O.solve(O.opts(code, adaptive=0, newton_abstol=1e-12, nlsolve_stages=1), spec)
+        b = O.solve(O.opts(code, adaptive=0, newton_abstol=1e-12, nlsolve_stages=4), spec)
+        assert a.retcode == b.retcode == 0 and (b.fallback_used & 16)
+        assert np.max(np.abs(a.u - b.u)) < 1e-10 and np.max(np.abs(a.stages - b.stages)) < 1e-10
+    spec = cases.linear([[0, 1], [0, 0]], bc=([(0, 5.0)], [(0, 0.0)]), dt=0.2, u0=[5.0, -3.5])
+    a = O.solve(O.opts(code, adaptive=0, newton_abstol=1e-12, nlsolve_stages=1), spec)
+    b = O.solve(O.opts(code, adaptive=0, newton_abstol=1e-12, nlsolve_stages=4), spec)
+    assert a.newton_iters == 1 < b.newton_iters and np.max(np.abs(a.u - b.u)) < 1e-10
