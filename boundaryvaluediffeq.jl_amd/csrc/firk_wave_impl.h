@@ -1520,6 +1520,11 @@ __global__ __launch_bounds__(256) void k_firk_gather(BatchDev B, const long long
     for (int q = threadIdx.x; q < N * S * M; q += blockDim.x) out[o + q] = K[q];
 }
 
+// The two-waves-per-SIMD register variant where the compiler reaches that occupancy
+// (M = 1; M = 2 with s <= 3; M = 3 with s <= 2); elsewhere both variants launch the
+// one-wave kernels.
+template <int M, int S> constexpr bool kFirkW2 = M == 1 || (M <= 3 && M * S <= 6);
+
 template <int M, int S, int WPE>
 hipError_t firk_launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
     if (B.firk_irows) {
@@ -1527,11 +1532,10 @@ hipError_t firk_launch_solve_t(const BatchDev &B, int blocks, hipStream_t s) {
         if (B.firk_nested) k_firk_solve<M, S, true, true><<<blocks, 64, 0, s>>>(B);
         else k_firk_solve<M, S, false, true><<<blocks, 64, 0, s>>>(B);
     } else if (B.firk_nested) {
-        // M = 6: the one-wave-per-SIMD kernels serve both variants
-        if constexpr (WPE == 2 && M <= 4) k_firk_solve_w2<M, S, true><<<blocks, 64, 0, s>>>(B);
+        if constexpr (WPE == 2 && kFirkW2<M, S>) k_firk_solve_w2<M, S, true><<<blocks, 64, 0, s>>>(B);
         else k_firk_solve<M, S, true><<<blocks, 64, 0, s>>>(B);
     } else {
-        if constexpr (WPE == 2 && M <= 4) k_firk_solve_w2<M, S, false><<<blocks, 64, 0, s>>>(B);
+        if constexpr (WPE == 2 && kFirkW2<M, S>) k_firk_solve_w2<M, S, false><<<blocks, 64, 0, s>>>(B);
         else k_firk_solve<M, S, false><<<blocks, 64, 0, s>>>(B);
     }
     return hipGetLastError();
@@ -1555,11 +1559,11 @@ int firk_occupancy_t(int nested) {
     int nb = 0;
     hipError_t e;
     if (nested) {
-        if constexpr (WPE == 2 && M <= 4)
+        if constexpr (WPE == 2 && kFirkW2<M, S>)
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S, true>, 64, 0);
         else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S, true>, 64, 0);
     } else {
-        if constexpr (WPE == 2 && M <= 4)
+        if constexpr (WPE == 2 && kFirkW2<M, S>)
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve_w2<M, S, false>, 64, 0);
         else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_firk_solve<M, S, false>, 64, 0);
     }
