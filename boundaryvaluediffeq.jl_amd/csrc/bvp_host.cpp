@@ -97,11 +97,6 @@ int validate_opts(const bvp_opts *o, int32_t n, const bvp_instance *inst, int M,
         if (o->nlsolve_stages & BVP_STAGE_TRUST_REGION) {
             if (o->method >= BVP_FIRK_NESTED)
                 return fail(err, BVP_ENOTSUP, "the TrustRegion stage runs on the expanded FIRK form only");
-            for (int i = 0; i < n; ++i)
-                for (int k = 0; k < M; ++k)
-                    if (inst[i].bc_t[k] != inst[i].t0 && inst[i].bc_t[k] != inst[i].t1)
-                        return fail(err, BVP_ENOTSUP,
-                                    "instance %d: the FIRK TrustRegion stage needs boundary rows at t0 or t1", i);
         }
     } else if (o->method != BVP_MIRK4 && o->method != BVP_MIRK6) {
         return fail(err, BVP_EINVAL, "method must be 4, 6 or a FIRK method code");

@@ -484,7 +484,8 @@ template <int M, int S, bool WANTG>
 __device__ inline double firk_bc_interior(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest,
                                           const double *mesh, const double *y, const double *K, const double *dy,
                                           const double *dK, double alpha, int k, int j, const double *lk,
-                                          double *ir) {
+                                          double *ir, const double *dy2 = nullptr, const double *dK2 = nullptr,
+                                          double beta = 0.0, double *graw = nullptr) {
     const int comp = sel<M>(I.bcc, k);
     double yi[M], yj[M], Kk[S][M];
 #pragma unroll
@@ -494,6 +495,10 @@ __device__ inline double firk_bc_interior(const Inst<M> &I, const FirkTab &T, co
         if (dy) {
             yi[c] += alpha * dy[(size_t)j * M + c];
             yj[c] += alpha * dy[(size_t)(j + 1) * M + c];
+        }
+        if (dy2) {
+            yi[c] += beta * dy2[(size_t)j * M + c];
+            yj[c] += beta * dy2[(size_t)(j + 1) * M + c];
         }
     }
     if (nest.on) {
@@ -507,13 +512,18 @@ __device__ inline double firk_bc_interior(const Inst<M> &I, const FirkTab &T, co
 #pragma unroll
             for (int c = 0; c < M; ++c) {
                 const size_t q = ((size_t)j * S + r) * M + c;
-                Kk[r][c] = dK ? K[q] + alpha * dK[q] : K[q];
+                Kk[r][c] = (dK ? K[q] + alpha * dK[q] : K[q]) + (dK2 ? beta * dK2[q] : 0.0);
             }
     }
     double g[(S + 2) * M];
     const double v =
         firk_evalsol<M, S, WANTG>(I, T, !nest.on, mesh[j], mesh[j + 1], sel<M>(I.bct, k), comp, yi, yj, Kk, g);
     if constexpr (WANTG) {
+        if (graw)
+#pragma unroll
+            for (int q = 0; q < (S + 2) * M; ++q) graw[q] = g[q];
+    }
+    if (WANTG && ir) {
         constexpr int NN = S * M;
         double cst = 0.0, ca[M];
 #pragma unroll
@@ -538,12 +548,14 @@ __device__ inline double firk_bc_interior(const Inst<M> &I, const FirkTab &T, co
 template <int M, int S>
 __device__ inline double firk_bc_row(const Inst<M> &I, const FirkTab &T, const FirkNest<M> &nest, const double *mesh,
                                      const double *y, const double *K, const double *dy, const double *dK,
-                                     double alpha, int k, const BcLoc &L) {
+                                     double alpha, int k, const BcLoc &L, const double *dy2 = nullptr,
+                                     const double *dK2 = nullptr, double beta = 0.0) {
     if (L.iv < 0 || !nest.irows) {
         const size_t q = (size_t)L.node * M + sel<M>(I.bcc, k);
-        return (dy ? y[q] + alpha * dy[q] : y[q]) - sel<M>(I.bcv, k);
+        return (dy ? y[q] + alpha * dy[q] : y[q]) + (dy2 ? beta * dy2[q] : 0.0) - sel<M>(I.bcv, k);
     }
-    return firk_bc_interior<M, S, false>(I, T, nest, mesh, y, K, dy, dK, alpha, k, L.iv, nullptr, nullptr);
+    return firk_bc_interior<M, S, false>(I, T, nest, mesh, y, K, dy, dK, alpha, k, L.iv, nullptr, nullptr, dy2, dK2,
+                                         beta);
 }
 
 // Boundary rows of the FIRK path at (y, K): residuals and nodes in sh.
@@ -782,6 +794,14 @@ struct FirkNewtonOut {
 };
 
 // ------------------------------------------------------------------ trust region (expanded form)
+// the expanded form with boundary rows anywhere in the span
+template <int M> __device__ inline FirkNest<M> firk_expanded_rows() {
+    FirkNest<M> n{};
+    n.on = false;
+    n.irows = true;
+    return n;
+}
+
 // Products with the expanded Jacobian for the TrustRegion stage, rows
 //   Phi_i = y_{i+1} - y_i - h sum_j b_j K_ij,  G_ir = f(t_i + c_r h, Y_ir) - K_ir,
 //   Y_ir = y_i + h sum_j a_rj K_ij,  boundary rows y_node.comp - value
@@ -842,8 +862,22 @@ __device__ inline void firk_jt_f(const Inst<M> &I, const FirkTab &T, const doubl
     if (lane == 0)
         for (int r = 0; r < M; ++r) {
             const BcLoc L = bc_locate<M>(I, r, mesh, N);
-            const size_t q = (size_t)L.node * M + sel<M>(I.bcc, r);
-            gy[q] += y[q] - sel<M>(I.bcv, r);
+            if (L.iv < 0) {
+                const size_t q = (size_t)L.node * M + sel<M>(I.bcc, r);
+                gy[q] += y[q] - sel<M>(I.bcv, r);
+            } else {
+                // a row inside the span: its EvalSol derivatives on y_j, y_{j+1}, K_j
+                const int j = L.iv;
+                double g[(S + 2) * M];
+                const double v = firk_bc_interior<M, S, true>(I, T, firk_expanded_rows<M>(), mesh, y, K, nullptr,
+                                                              nullptr, 0.0, r, j, nullptr, nullptr, nullptr, nullptr,
+                                                              0.0, g);
+                for (int c = 0; c < M; ++c) {
+                    gy[(size_t)j * M + c] += v * g[c];
+                    gy[(size_t)(j + 1) * M + c] += v * g[M + c];
+                }
+                for (int q = 0; q < S * M; ++q) gK[(size_t)j * S * M + q] += v * g[2 * M + q];
+            }
         }
     __syncthreads();
 }
@@ -888,7 +922,18 @@ __device__ inline double firk_jac_norm2(const Inst<M> &I, const FirkTab &T, cons
     if (lane == 0)
         for (int r = 0; r < M; ++r) {
             const BcLoc L = bc_locate<M>(I, r, mesh, N);
-            const double v = gy[(size_t)L.node * M + sel<M>(I.bcc, r)];
+            double v;
+            if (L.iv < 0) {
+                v = gy[(size_t)L.node * M + sel<M>(I.bcc, r)];
+            } else {
+                const int j = L.iv;
+                double g[(S + 2) * M];
+                firk_bc_interior<M, S, true>(I, T, firk_expanded_rows<M>(), mesh, y, K, nullptr, nullptr, 0.0, r, j,
+                                             nullptr, nullptr, nullptr, nullptr, 0.0, g);
+                v = 0.0;
+                for (int c = 0; c < M; ++c) v += g[c] * gy[(size_t)j * M + c] + g[M + c] * gy[(size_t)(j + 1) * M + c];
+                for (int q = 0; q < S * M; ++q) v += g[2 * M + q] * gK[(size_t)j * S * M + q];
+            }
             acc += v * v;
         }
     const double s = uniform(wave_sum(acc));
@@ -930,8 +975,8 @@ __device__ inline double firk_trial2(const Inst<M> &I, const FirkTab &T, const d
     }
     if (lane < M) {
         const BcLoc L = bc_locate<M>(I, lane, mesh, N);
-        const size_t q = (size_t)L.node * M + sel<M>(I.bcc, lane);
-        const double v = y[q] + a * dy[q] + b * gy[q] - sel<M>(I.bcv, lane);
+        const double v =
+            firk_bc_row<M, S>(I, T, firk_expanded_rows<M>(), mesh, y, K, dy, dK, a, lane, L, gy, gK, b);
         ss += v * v;
         m = fmax(m, fabs(v));
     }
@@ -946,7 +991,7 @@ __device__ inline double firk_trial2(const Inst<M> &I, const FirkTab &T, const d
 // The TrustRegion stage on the expanded system (dogleg between the Newton step and the
 // Cauchy point along J'F, initial radius max(|F|, max x - min x) / 11, the "Simple"
 // radius update; the restatement the oracle follows, Core/default_internal_solve.jl:
-// 37-43).  Boundary rows read nodes (the host refuses rows inside the span here).
+// 37-43).  Boundary rows anywhere in the span.
 // gy, gK: work areas for J'F.
 template <int M, int S>
 __device__ inline int firk_trust_region(const BatchDev &B, const Inst<M> &I, const FirkTab &T,

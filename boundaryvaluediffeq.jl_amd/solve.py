@@ -158,7 +158,7 @@ class _FIRK:
     collocation.jl:118-205), whose tolerance nested_nlsolve_kwargs["abstol"] sets
     (default eps^(4/5)).  On the GPU nlsolve = None runs the NewtonRaphson stages of the
     polyalgorithm (plain, then with BackTracking); a nlsolve that names the TrustRegion
-    stage adds it (expanded form, boundary rows at the ends)."""
+    stage adds it (expanded form)."""
     code = 0
     stages = 0
     order = 0

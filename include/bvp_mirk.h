@@ -58,7 +58,7 @@ extern "C" {
  * polynomial's variable, as the reference does, and the nested form at the midpoint from
  * stages solved from zero.  On the GPU: M = 1..4 and 6, DefectControl, FP64; nlsolve_stages
  * selects the stages of the polyalgorithm, the TrustRegion stage (bit 2) on the
- * expanded form with boundary rows at t0 or t1. */
+ * expanded form. */
 #define BVP_FIRK_NESTED     1000  /* added to a FIRK code: the nested form (below) */
 #define BVP_RADAU_IIA(s)    (100 + (s))
 #define BVP_LOBATTO_IIIA(s) (200 + (s))
